@@ -1,0 +1,177 @@
+"""Benchmark: batched DTW over 100,000 random-walk pairs on one MI355X.
+
+Workload (the six-measure 100k-pair configuration): 100,000 pairs of
+independent seeded random-walk trajectories (seed 0), lengths uniform on
+20..200, FP64 coordinates resident in HBM (plus the FP32 copy).  One step =
+one call of the engine for DTW over all 100,000 pairs (planner + persistent
+DP kernel), inputs already on the device.  `value` is pairs/s over the timed
+steps.  The other five measures are timed the same way and reported under
+"measures".
+
+roofline: the DP kernel is bound by VALU FP32 issue.  Algorithmic FLOP per DP
+cell (FMA = 2): DTW 9, Frechet 9, LCSS 9, EDR 10, ERP 11; Hausdorff 24 per
+(point, segment) evaluation.  achieved = FLOP per launch / mean kernel time
+(HIP events on the launch stream); peak = 157.3 TFLOP/s (MI355X vector FP32).
+
+cpu_baseline: the FP64 full-matrix restatement of the DP (oracle/), one pair
+per task, OpenMP over all host cores, on the same pairs.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_FP32_TFLOPS = 157.3
+FLOP_PER_CELL = {"dtw": 9, "frechet": 9, "lcss": 9, "edr": 10, "erp": 11, "hausdorff": 24}
+EPS = 0.0025
+
+
+def cells_of(measure, la, lb):
+    la = la.astype(np.float64)
+    lb = lb.astype(np.float64)
+    if measure == "hausdorff":  # point-segment evaluations, both directions
+        return float(np.sum(la * np.maximum(lb - 1, 0) + lb * np.maximum(la - 1, 0)))
+    return float(np.sum(la * lb))
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--pairs", type=int, default=100_000)
+    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--measures", default="dtw,edr,lcss,erp,frechet,hausdorff",
+                    help="measures reported under 'measures' (dtw is always the headline)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work at most")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design note only "
+                                   "(see DESIGN.md), so --gpus must be 1"}))
+        sys.exit(2)
+
+    import torch
+
+    from trajsimimeasures_amd import engine
+    from trajsimimeasures_amd.data import lengths_uniform, random_walks
+
+    t0 = time.time()
+    rng = np.random.default_rng(0)
+    ts = random_walks(rng, lengths_uniform(rng, 2 * args.pairs, 20, 200))
+    pa = np.arange(0, 2 * args.pairs, 2)
+    pb = pa + 1
+    gpoint = ts.xy.mean(0)
+    gen_s = time.time() - t0
+    print(f"[bench] generated {args.pairs} pairs ({ts.xy.shape[0]} points) in {gen_s:.1f}s", file=sys.stderr,
+          flush=True)
+
+    dev = torch.device("cuda:0")
+    dset = engine.DeviceTrajSet(ts, device=dev)
+    pa_d = torch.as_tensor(pa, dtype=torch.int32, device=dev)
+    pb_d = torch.as_tensor(pb, dtype=torch.int32, device=dev)
+    out = torch.empty(args.pairs, dtype=torch.float64, device=dev)
+    la, lb = ts.lengths[pa], ts.lengths[pb]
+
+    def step(measure):
+        engine.pairs(measure, dset, pa_d, pb_d, eps=EPS, gpoint=gpoint, lanes=args.lanes, validate=False, out=out)
+
+    def timed(measure, steps, warmup):
+        for _ in range(warmup):
+            step(measure)
+        torch.cuda.synchronize()
+        with engine.KernelTimer() as kt:
+            t = time.perf_counter()
+            for _ in range(steps):
+                step(measure)
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t
+        return el, kt.ms / max(kt.launches, 1), kt.launches
+
+    measures = [m for m in args.measures.split(",") if m]
+    report = {}
+    for m in measures:
+        steps = args.steps if m == "dtw" else max(3, args.steps // 4)
+        el, kms, nl = timed(m, steps, args.warmup)
+        cells = cells_of(m, la, lb)
+        flops = cells * FLOP_PER_CELL[m]
+        report[m] = {
+            "pairs_per_s": args.pairs * steps / el,
+            "ms_per_step": 1e3 * el / steps,
+            "kernel_ms": kms,
+            "cells_per_s": cells / (kms * 1e-3),
+            "roofline_frac": flops / (kms * 1e-3) / (PEAK_FP32_TFLOPS * 1e12),
+        }
+        print(f"[bench] {m}: {report[m]}", file=sys.stderr, flush=True)
+
+    el, kms, nl = timed("dtw", args.steps, args.warmup)
+    cells = cells_of("dtw", la, lb)
+    achieved = cells * FLOP_PER_CELL["dtw"] / (kms * 1e-3) / 1e12
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle import oracle as O
+        threads = O.max_threads()
+        n = 2000
+        while True:
+            t = time.perf_counter()
+            O.pairs("dtw", ts.xy, ts.offsets, pa[:n], pb[:n], threads=threads)
+            dt = time.perf_counter() - t
+            if n >= args.pairs or dt * 2 * (1 + (n < args.pairs)) > args.cpu_budget or dt > args.cpu_budget / 4:
+                break
+            n = min(args.pairs, n * 4)
+        cpu = {"value": n / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
+               "sample": f"first {n} of the {args.pairs} pairs, DTW, FP64 full-matrix restatement of the "
+                         f"traj_dist recurrence (oracle/tsm_oracle.c, gcc -O2), one pair per OpenMP task, "
+                         f"{threads} threads on {cpu_model()}; {n / dt * cells_of('dtw', la[:n], lb[:n]) / n:.3e} cells/s",
+               "seconds": dt}
+        print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
+    line = {
+        "metric": "DTW pairs/s (100k random-walk pairs, lengths 20-200)",
+        "value": args.pairs * args.steps / el,
+        "unit": "pairs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic seeded random walks (seed 0), FP64 coordinates resident in HBM",
+        "config": {"workload": "100,000 random-walk pairs, lengths uniform 20..200, DTW (six-measure config)",
+                   "pairs": args.pairs, "cells": cells, "lanes": args.lanes or "auto"},
+        "cells_per_s": cells / (kms * 1e-3),
+        "kernel_ms": kms,
+        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "note": "9 algorithmic FP32 FLOP per DTW cell over mean dp_kernel duration (HIP events)"},
+        "cpu_baseline": cpu,
+        "measures": report,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

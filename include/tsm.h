@@ -1,0 +1,135 @@
+/*
+ * tsm.h -- C ABI of libtsm.so, the MI355X (gfx950) engine for batched
+ * trajectory-similarity dynamic programs: DTW, EDR, LCSS, ERP, discrete
+ * Frechet and Hausdorff.
+ *
+ * The reference exposes one Python function per measure over zero-padded
+ * batches (core/dtw.py:56, core/edr.py:55, core/lcss.py:56, core/erp.py:94,
+ * core/discret_frechet.py:53, core/hausdorff.py:97), each of which copies
+ * the batch to the device, launches one Numba kernel and copies the
+ * distances back.  This ABI replaces those bodies: trajectories travel as a
+ * packed point buffer plus an offsets array, pairs as index arrays, and one
+ * call computes a whole batch on one stream.  Python binds it with ctypes
+ * (trajsimimeasures_amd/_lib.py); INTEGRATION.md shows the binding.
+ *
+ * Conventions
+ *   - Every function returns TSM_OK (0) or a negative TSM_E* code; the
+ *     message of the last failure on the calling thread is tsm_last_error().
+ *   - "device" pointers are HIP device memory on the current device;
+ *     "host" pointers are ordinary host memory.
+ *   - Device-pointer entry points are stream-ordered on `stream`
+ *     (a hipStream_t, NULL = the null stream) and do not synchronise, except
+ *     when the library's workspace has to grow (a hipMalloc).
+ *   - Results are float64 like the reference's (core/dtw.py:70,76); EDR and
+ *     LCSS also report the integer DP count.
+ *   - Trajectory lengths must be >= 1.
+ */
+#ifndef TSM_H
+#define TSM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TSM_ABI_VERSION 1
+
+/* Measure ids. */
+#define TSM_DTW 0       /* core/dtw.py:10-77 */
+#define TSM_EDR 1       /* core/edr.py:10-76 */
+#define TSM_LCSS 2      /* core/lcss.py:12-77 */
+#define TSM_ERP 3       /* core/erp.py:10-120 */
+#define TSM_FRECHET 4   /* core/discret_frechet.py:9-74 */
+#define TSM_HAUSDORFF 5 /* core/hausdorff.py:12-123 */
+
+/* Error codes. */
+#define TSM_OK 0
+#define TSM_E_ARG (-1)     /* invalid argument */
+#define TSM_E_HIP (-2)     /* HIP runtime error */
+#define TSM_E_NOMEM (-3)   /* device allocation failed */
+#define TSM_E_UNSUP (-4)   /* unsupported combination */
+
+/* EDR/LCSS match conventions. */
+#define TSM_EPS_SQRT 0    /* sqrt(dx^2+dy^2) < eps: traj_dist, core_cpu/stedr.pyx:11-12,36 */
+#define TSM_EPS_SQUARED 1 /* dx^2+dy^2 < eps*eps: reference GPU, core/edr.py:22,42 */
+
+/* Arithmetic of the DP. */
+#define TSM_PREC_FP32 0 /* default: FP32 DP (EDR/LCSS match tests always use FP64 coordinates) */
+#define TSM_PREC_FP64 1 /* FP64 DP end to end (DTW, ERP, Frechet): the drift-check rerun */
+
+/* A packed trajectory set resident on the device. */
+typedef struct tsm_trajset {
+    const float *xy32;      /* device [npts][2] FP32 copy of xy64 (tsm_xy_to_fp32) */
+    const double *xy64;     /* device [npts][2] FP64 coordinates */
+    const int64_t *offsets; /* device [n + 1], offsets[0] = 0 */
+    int64_t n;              /* number of trajectories */
+    int64_t npts;           /* number of points, offsets[n] */
+    int64_t max_len;        /* longest trajectory (sizes scratch; must be >= every length) */
+} tsm_trajset;
+
+/* Per-call parameters; tsm_params_default() fills the defaults. */
+typedef struct tsm_params {
+    double eps;     /* EDR/LCSS threshold (core/edr.py:55 `eps`) */
+    int eps_mode;   /* TSM_EPS_SQRT (default) or TSM_EPS_SQUARED */
+    double gx, gy;  /* ERP gap point g (core/erp.py:94 `gpoint`, default (0,0) erp.py:103) */
+    int lanes;      /* lanes per pair: 0 = autotuned, else 4, 8 or 16 (threads_per_traj hint) */
+    int precision;  /* TSM_PREC_FP32 or TSM_PREC_FP64 */
+} tsm_params;
+
+/* Library / device setup (reference: numba.cuda context, implicit). */
+int tsm_init(int device);
+const char *tsm_last_error(void);
+int tsm_abi_version(void);
+void tsm_params_default(tsm_params *p);
+
+/* FP64 -> FP32 copy of a packed point buffer (2*npts values), stream-ordered. */
+int tsm_xy_to_fp32(const double *xy64, float *xy32, int64_t npts, void *stream);
+
+/*
+ * Explicit pairs: pair k is (pa[k], pb[k]) of `set`, pa as t1 and pb as t2
+ * (core/dtw.py:56 `t1, len1, t2, len2`).  pa, pb: device int32 [npairs].
+ * out: device float64 [npairs].  counts: device int32 [npairs] or NULL
+ * (EDR/LCSS only).  Replaces the kernel launch + copies of core/dtw.py:61-77,
+ * core/edr.py:59-76, core/lcss.py:60-77, core/erp.py:98-120,
+ * core/discret_frechet.py:57-74, core/hausdorff.py:101-123.
+ */
+int tsm_pairs(int measure, const tsm_trajset *set, const int32_t *pa, const int32_t *pb,
+              int64_t npairs, const tsm_params *prm, double *out, int32_t *counts, void *stream);
+
+/*
+ * Query x database grid: pair k = q * nd + d is (q0 + q, d0 + d) for
+ * q < nq, d < nd, all indices into `set`.  Replaces the repeat/tile index
+ * arrays and per-batch gathers of test_trajsimi_time_DQ.py:148-173.
+ * out: device float64 [nq * nd], row-major.
+ */
+int tsm_pairs_grid(int measure, const tsm_trajset *set, int64_t q0, int64_t nq, int64_t d0, int64_t nd,
+                   const tsm_params *prm, double *out, int32_t *counts, void *stream);
+
+/*
+ * Upper triangle of the all-pairs matrix over trajectories [0, n) of `set`:
+ * pair k enumerates (i, j), i < j, row-major, like
+ * itertools.combinations(range(n), 2) in test_clustering_time.py:219.
+ * out: device float64 [n*(n-1)/2].
+ */
+int tsm_pairs_triu(int measure, const tsm_trajset *set, int64_t n, const tsm_params *prm, double *out,
+                   int32_t *counts, void *stream);
+
+/*
+ * Host-memory convenience: uploads xy64 [npts][2] / offsets [n+1] and the
+ * pair lists, runs tsm_pairs on the null stream, copies results back and
+ * synchronises.  This is the body of the reference's per-batch functions.
+ */
+int tsm_pairs_host(int measure, const double *xy64, const int64_t *offsets, int64_t n, int64_t npts,
+                   const int32_t *pa, const int32_t *pb, int64_t npairs, const tsm_params *prm,
+                   double *out, int32_t *counts);
+
+/* Kernel timing of the DP / Hausdorff kernels (HIP events on the launch
+ * stream), for roofline reporting.  No reference counterpart. */
+int tsm_profile_enable(int on);
+int tsm_profile_read(double *total_ms, int64_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TSM_H */

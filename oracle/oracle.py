@@ -1,0 +1,119 @@
+"""ctypes front of the FP64 CPU oracle (oracle/tsm_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: the checker for tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg.  The product (trajsimimeasures_amd) never
+imports this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtsm_oracle.so")
+
+DTW, EDR, LCSS, ERP, FRECHET, HAUSDORFF = 0, 1, 2, 3, 4, 5
+NAMES = {"dtw": DTW, "edr": EDR, "lcss": LCSS, "erp": ERP, "frechet": FRECHET, "hausdorff": HAUSDORFF}
+
+_lib = None
+_P = ctypes.c_void_p
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc -O2 -ffp-contract=off)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        d = ctypes.c_double
+        i64 = ctypes.c_int64
+        L.tsmo_eps_threshold.argtypes = [d, ctypes.c_int]
+        L.tsmo_eps_threshold.restype = d
+        for name in ("tsmo_dtw", "tsmo_frechet", "tsmo_hausdorff"):
+            f = getattr(L, name)
+            f.argtypes = [_P, i64, _P, i64]
+            f.restype = d
+        for name in ("tsmo_edr", "tsmo_lcss"):
+            f = getattr(L, name)
+            f.argtypes = [_P, i64, _P, i64, d]
+            f.restype = d
+        for name in ("tsmo_edr_count", "tsmo_lcss_count"):
+            f = getattr(L, name)
+            f.argtypes = [_P, i64, _P, i64, d]
+            f.restype = i64
+        L.tsmo_erp.argtypes = [_P, i64, _P, i64, _P]
+        L.tsmo_erp.restype = d
+        L.tsmo_pairs.argtypes = [ctypes.c_int, _P, _P, _P, _P, i64, d, _P, _P, _P, ctypes.c_int]
+        L.tsmo_pairs.restype = ctypes.c_int
+        L.tsmo_max_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def eps_threshold(eps: float, mode: str = "sqrt") -> float:
+    """Squared-distance threshold equivalent to the EDR/LCSS match test."""
+    return lib().tsmo_eps_threshold(float(eps), 0 if mode == "sqrt" else 1)
+
+
+def _arr(t):
+    a = np.ascontiguousarray(np.asarray(t, dtype=np.float64).reshape(-1, 2))
+    return a
+
+
+def single(measure: str, a, b, eps: float = 0.0, eps_mode: str = "sqrt", gpoint=(0.0, 0.0)) -> float:
+    """One pair, reference orientation (a = t1)."""
+    a, b = _arr(a), _arr(b)
+    L = lib()
+    m = NAMES[measure]
+    if m == DTW:
+        return L.tsmo_dtw(a.ctypes.data, len(a), b.ctypes.data, len(b))
+    if m == FRECHET:
+        return L.tsmo_frechet(a.ctypes.data, len(a), b.ctypes.data, len(b))
+    if m == HAUSDORFF:
+        return L.tsmo_hausdorff(a.ctypes.data, len(a), b.ctypes.data, len(b))
+    if m == ERP:
+        g = np.ascontiguousarray(np.asarray(gpoint, dtype=np.float64).reshape(2))
+        return L.tsmo_erp(a.ctypes.data, len(a), b.ctypes.data, len(b), g.ctypes.data)
+    thr = eps_threshold(eps, eps_mode)
+    if m == EDR:
+        return L.tsmo_edr(a.ctypes.data, len(a), b.ctypes.data, len(b), thr)
+    return L.tsmo_lcss(a.ctypes.data, len(a), b.ctypes.data, len(b), thr)
+
+
+def count(measure: str, a, b, eps: float, eps_mode: str = "sqrt") -> int:
+    a, b = _arr(a), _arr(b)
+    thr = eps_threshold(eps, eps_mode)
+    f = lib().tsmo_edr_count if measure == "edr" else lib().tsmo_lcss_count
+    return int(f(a.ctypes.data, len(a), b.ctypes.data, len(b), thr))
+
+
+def pairs(measure: str, xy, offsets, pa, pb, eps: float = 0.0, eps_mode: str = "sqrt", gpoint=(0.0, 0.0),
+          threads: int = 0):
+    """Batched oracle over a packed set; returns (values float64, counts int64)."""
+    xy = np.ascontiguousarray(np.asarray(xy, dtype=np.float64).reshape(-1, 2))
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    pa = np.ascontiguousarray(pa, dtype=np.int64)
+    pb = np.ascontiguousarray(pb, dtype=np.int64)
+    n = pa.size
+    out = np.empty(n, dtype=np.float64)
+    cnt = np.empty(n, dtype=np.int64)
+    g = np.ascontiguousarray(np.asarray(gpoint, dtype=np.float64).reshape(2))
+    thr = eps_threshold(eps, eps_mode)
+    rc = lib().tsmo_pairs(NAMES[measure], xy.ctypes.data, offsets.ctypes.data, pa.ctypes.data, pb.ctypes.data, n,
+                          thr, g.ctypes.data, out.ctypes.data, cnt.ctypes.data, int(threads))
+    if rc != 0:
+        raise ValueError(f"oracle rejected measure {measure!r}")
+    return out, cnt
+
+
+def max_threads() -> int:
+    return lib().tsmo_max_threads()

@@ -1,0 +1,282 @@
+/*
+ * tsm_oracle.c -- FP64 CPU oracle for the six trajectory-similarity measures.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: the product path
+ * (trajsimimeasures_amd, libtsm.so) never links, loads or calls it.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's CPU-baseline leg use it.
+ *
+ * Each measure is a textbook full-matrix (n1+1) x (n2+1) dynamic program in
+ * float64, the shape of the traj_dist Cython routines the reference uses as
+ * its CPU path (utilities/method_helper.py:45-50, traj_dist==1.15), with the
+ * boundary conditions and quirks of the reference's GPU kernels in core/{dtw,edr,...}.py,
+ * which this engine is a drop-in for:
+ *
+ *   DTW       core/dtw.py:27-53             inf boundary, D = d + min3
+ *   EDR       core/edr.py:29-52             ZERO boundary, min(diag+sub, up+1, left+1), / max(n1,n2)
+ *   LCSS      core/lcss.py:31-53            zero boundary, match -> diag+1 else max(up,left), 1 - c/min(n1,n2)
+ *   ERP       core/erp.py:29-91             constant-sum boundary D[i][0]=Sa, D[0][j]=Sb
+ *   Frechet   core/discret_frechet.py:26-50 inf boundary, D = max(d, min3)
+ *   Hausdorff core/hausdorff.py:12-69,121   point-to-segment, squared distances, sqrt at the end
+ *
+ * Distances are sqrt(dx*dx + dy*dy) evaluated without FMA contraction
+ * (core/utils.py:8-17); build with -ffp-contract=off.
+ *
+ * The EDR/LCSS match test is supplied as a squared-distance threshold `thr`:
+ * a cell matches iff dx*dx + dy*dy < thr.  tsmo_eps_threshold() turns an
+ * epsilon into that threshold for both conventions:
+ *   mode 0 ("sqrt", traj_dist / core_cpu/stedr.pyx:11-12,36):  sqrt(d2) < eps
+ *   mode 1 ("squared", reference GPU, core/edr.py:22,42):       d2 < eps*eps
+ * For mode 0 the threshold is the least double t with sqrt(t) >= eps, so
+ * d2 < t  <=>  sqrt(d2) < eps for every double d2 (sqrt is correctly rounded
+ * and monotone).
+ *
+ * The batched entry tsmo_pairs() runs one pair per task with OpenMP over
+ * contiguous slices, like the reference CPU driver's slicing
+ * (utilities/tool_funcs.py:195-208, test_trajsimi_time.py:280-316).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+enum { TSMO_DTW = 0, TSMO_EDR = 1, TSMO_LCSS = 2, TSMO_ERP = 3, TSMO_FRECHET = 4, TSMO_HAUSDORFF = 5 };
+
+static inline double sqd(const double *p, const double *q) {
+    double dx = p[0] - q[0];
+    double dy = p[1] - q[1];
+    return dx * dx + dy * dy;
+}
+static inline double eucl(const double *p, const double *q) { return sqrt(sqd(p, q)); }
+static inline double min2(double a, double b) { return a < b ? a : b; }
+static inline double max2(double a, double b) { return a > b ? a : b; }
+static inline double min3(double a, double b, double c) { return min2(min2(a, b), c); }
+
+/* Least double t with sqrt(t) >= eps (mode 0), or eps*eps (mode 1). */
+double tsmo_eps_threshold(double eps, int mode) {
+    if (mode == 1) return eps * eps;
+    if (!(eps > 0.0)) return 0.0; /* sqrt(d2) < eps impossible for eps <= 0 */
+    if (isinf(eps)) return INFINITY;
+    double t = eps * eps;
+    while (sqrt(t) >= eps) t = nextafter(t, 0.0);
+    while (sqrt(t) < eps) t = nextafter(t, INFINITY);
+    return t;
+}
+
+/* core/dtw.py:27-53 -- D[0][0]=0, D[i][0]=D[0][j]=inf, D=d+min3. */
+double tsmo_dtw(const double *a, int64_t n1, const double *b, int64_t n2) {
+    const int64_t w = n2 + 1;
+    double *D = (double *)malloc(sizeof(double) * (size_t)(n1 + 1) * (size_t)w);
+    for (int64_t j = 0; j <= n2; ++j) D[j] = INFINITY;
+    D[0] = 0.0;
+    for (int64_t i = 1; i <= n1; ++i) {
+        D[i * w] = INFINITY;
+        for (int64_t j = 1; j <= n2; ++j) {
+            double d = eucl(a + 2 * (i - 1), b + 2 * (j - 1));
+            D[i * w + j] = d + min3(D[(i - 1) * w + j - 1], D[(i - 1) * w + j], D[i * w + j - 1]);
+        }
+    }
+    double r = D[n1 * w + n2];
+    free(D);
+    return r;
+}
+
+/* core/discret_frechet.py:26-50 -- same boundary as DTW, D = max(d, min3). */
+double tsmo_frechet(const double *a, int64_t n1, const double *b, int64_t n2) {
+    const int64_t w = n2 + 1;
+    double *D = (double *)malloc(sizeof(double) * (size_t)(n1 + 1) * (size_t)w);
+    for (int64_t j = 0; j <= n2; ++j) D[j] = INFINITY;
+    D[0] = 0.0;
+    for (int64_t i = 1; i <= n1; ++i) {
+        D[i * w] = INFINITY;
+        for (int64_t j = 1; j <= n2; ++j) {
+            double d = eucl(a + 2 * (i - 1), b + 2 * (j - 1));
+            D[i * w + j] = max2(d, min3(D[(i - 1) * w + j - 1], D[(i - 1) * w + j], D[i * w + j - 1]));
+        }
+    }
+    double r = D[n1 * w + n2];
+    free(D);
+    return r;
+}
+
+/* core/edr.py:29-52 -- zero boundary; returns the integer count C[n1][n2]. */
+int64_t tsmo_edr_count(const double *a, int64_t n1, const double *b, int64_t n2, double thr) {
+    const int64_t w = n2 + 1;
+    int64_t *C = (int64_t *)calloc((size_t)(n1 + 1) * (size_t)w, sizeof(int64_t));
+    for (int64_t i = 1; i <= n1; ++i) {
+        for (int64_t j = 1; j <= n2; ++j) {
+            int64_t sub = sqd(a + 2 * (i - 1), b + 2 * (j - 1)) < thr ? 0 : 1;
+            int64_t v = C[(i - 1) * w + j - 1] + sub;
+            int64_t up = C[(i - 1) * w + j] + 1;
+            int64_t left = C[i * w + j - 1] + 1;
+            if (up < v) v = up;
+            if (left < v) v = left;
+            C[i * w + j] = v;
+        }
+    }
+    int64_t r = C[n1 * w + n2];
+    free(C);
+    return r;
+}
+
+/* core/lcss.py:31-53 -- zero boundary; returns the integer count C[n1][n2]. */
+int64_t tsmo_lcss_count(const double *a, int64_t n1, const double *b, int64_t n2, double thr) {
+    const int64_t w = n2 + 1;
+    int64_t *C = (int64_t *)calloc((size_t)(n1 + 1) * (size_t)w, sizeof(int64_t));
+    for (int64_t i = 1; i <= n1; ++i) {
+        for (int64_t j = 1; j <= n2; ++j) {
+            int64_t v;
+            if (sqd(a + 2 * (i - 1), b + 2 * (j - 1)) < thr) {
+                v = C[(i - 1) * w + j - 1] + 1;
+            } else {
+                int64_t up = C[(i - 1) * w + j], left = C[i * w + j - 1];
+                v = up > left ? up : left;
+            }
+            C[i * w + j] = v;
+        }
+    }
+    int64_t r = C[n1 * w + n2];
+    free(C);
+    return r;
+}
+
+double tsmo_edr(const double *a, int64_t n1, const double *b, int64_t n2, double thr) {
+    return (double)tsmo_edr_count(a, n1, b, n2, thr) / (double)(n1 > n2 ? n1 : n2); /* core/edr.py:52 */
+}
+
+double tsmo_lcss(const double *a, int64_t n1, const double *b, int64_t n2, double thr) {
+    return 1.0 - (double)tsmo_lcss_count(a, n1, b, n2, thr) / (double)(n1 < n2 ? n1 : n2); /* core/lcss.py:53 */
+}
+
+/* core/erp.py:29-91 -- gap distances to g, constant boundary sums Sa, Sb
+ * (erp.py:44-49 serial sums, erp.py:62-78 boundary use). */
+double tsmo_erp(const double *a, int64_t n1, const double *b, int64_t n2, const double *g) {
+    const int64_t w = n2 + 1;
+    double *ga = (double *)malloc(sizeof(double) * (size_t)n1);
+    double *gb = (double *)malloc(sizeof(double) * (size_t)n2);
+    double sa = 0.0, sb = 0.0;
+    for (int64_t i = 0; i < n1; ++i) { ga[i] = eucl(g, a + 2 * i); }
+    for (int64_t j = 0; j < n2; ++j) { gb[j] = eucl(g, b + 2 * j); }
+    for (int64_t i = 0; i < n1; ++i) sa += ga[i];
+    for (int64_t j = 0; j < n2; ++j) sb += gb[j];
+    double *D = (double *)malloc(sizeof(double) * (size_t)(n1 + 1) * (size_t)w);
+    D[0] = 0.0;
+    for (int64_t j = 1; j <= n2; ++j) D[j] = sb;
+    for (int64_t i = 1; i <= n1; ++i) {
+        D[i * w] = sa;
+        for (int64_t j = 1; j <= n2; ++j) {
+            double left = D[i * w + j - 1] + gb[j - 1];
+            double up = D[(i - 1) * w + j] + ga[i - 1];
+            double diag = D[(i - 1) * w + j - 1] + eucl(a + 2 * (i - 1), b + 2 * (j - 1));
+            D[i * w + j] = min3(left, up, diag);
+        }
+    }
+    double r = D[n1 * w + n2];
+    free(D);
+    free(ga);
+    free(gb);
+    return r;
+}
+
+/* core/hausdorff.py:12-45 -- squared point-to-segment distance. */
+static double p2s_sq(const double *p, const double *s1, const double *s2, double dps1, double dps2) {
+    double segl = sqd(s1, s2);
+    if (s1[0] == s2[0] && s1[1] == s2[1]) return dps1;
+    double xd = s2[0] - s1[0];
+    double yd = s2[1] - s1[1];
+    double u1 = ((p[0] - s1[0]) * xd) + ((p[1] - s1[1]) * yd);
+    double u = u1 / segl;
+    if (u < 0.00001 || u > 1.0) return min2(dps1, dps2);
+    double ix = s1[0] + u * xd;
+    double iy = s1[1] + u * yd;
+    double dx = p[0] - ix, dy = p[1] - iy;
+    return dx * dx + dy * dy;
+}
+
+/* core/hausdorff.py:48-69 -- squared point-to-polyline distance; 0 when the
+ * polyline has a single point (no segment), as the reference returns. */
+static double p2t_sq(const double *p, const double *t, int64_t n) {
+    double dist = 0.0, dps1 = 0.0, dps2 = 0.0;
+    for (int64_t i = 0; i + 1 < n; ++i) {
+        if (i == 0) {
+            dps1 = sqd(p, t);
+            dps2 = sqd(p, t + 2);
+        } else {
+            dps1 = dps2;
+            dps2 = sqd(p, t + 2 * (i + 1));
+        }
+        double ps = p2s_sq(p, t + 2 * i, t + 2 * (i + 1), dps1, dps2);
+        if (i == 0 || ps < dist) dist = ps;
+    }
+    return dist;
+}
+
+/* core/hausdorff.py:97-123 -- sqrt of the max over both directions, padding
+ * excluded (the reference reduces over uninitialised padding, :114,121). */
+double tsmo_hausdorff(const double *a, int64_t n1, const double *b, int64_t n2) {
+    double m = 0.0;
+    for (int64_t i = 0; i < n1; ++i) m = max2(m, p2t_sq(a + 2 * i, b, n2));
+    for (int64_t j = 0; j < n2; ++j) m = max2(m, p2t_sq(b + 2 * j, a, n1));
+    return sqrt(m);
+}
+
+/* One pair of the batched oracle; count_out receives the EDR/LCSS count. */
+static double one_pair(int measure, const double *a, int64_t n1, const double *b, int64_t n2,
+                       double thr, const double *g, int64_t *count_out) {
+    switch (measure) {
+    case TSMO_DTW: return tsmo_dtw(a, n1, b, n2);
+    case TSMO_EDR: {
+        int64_t c = tsmo_edr_count(a, n1, b, n2, thr);
+        *count_out = c;
+        return (double)c / (double)(n1 > n2 ? n1 : n2);
+    }
+    case TSMO_LCSS: {
+        int64_t c = tsmo_lcss_count(a, n1, b, n2, thr);
+        *count_out = c;
+        return 1.0 - (double)c / (double)(n1 < n2 ? n1 : n2);
+    }
+    case TSMO_ERP: return tsmo_erp(a, n1, b, n2, g);
+    case TSMO_FRECHET: return tsmo_frechet(a, n1, b, n2);
+    case TSMO_HAUSDORFF: return tsmo_hausdorff(a, n1, b, n2);
+    default: return NAN;
+    }
+}
+
+/*
+ * Batched oracle over a packed trajectory set: xy is float64 [npts, 2],
+ * offsets int64 [ntraj + 1]; pair k is (pa[k], pb[k]) with pa as t1.
+ * out[k] receives the measure value, counts[k] (if non-NULL) the EDR/LCSS
+ * count.  nthreads <= 0 uses the OpenMP default.  Returns 0, or -1 on a bad
+ * measure id.
+ */
+int tsmo_pairs(int measure, const double *xy, const int64_t *offsets, const int64_t *pa,
+               const int64_t *pb, int64_t npairs, double thr, const double *g, double *out,
+               int64_t *counts, int nthreads) {
+    if (measure < 0 || measure > 5) return -1;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    int64_t k;
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < npairs; ++k) {
+        int64_t ia = pa[k], ib = pb[k];
+        int64_t c = -1;
+        const double *a = xy + 2 * offsets[ia];
+        const double *b = xy + 2 * offsets[ib];
+        out[k] = one_pair(measure, a, offsets[ia + 1] - offsets[ia], b, offsets[ib + 1] - offsets[ib], thr, g, &c);
+        if (counts) counts[k] = c;
+    }
+    return 0;
+}
+
+int tsmo_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -1,0 +1,144 @@
+"""CPU-side checks: the C-ABI library loads and exports every function
+include/tsm.h declares, argument errors come back as error codes without a
+device, and the data layer (packing, generator) behaves."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from trajsimimeasures_amd import _lib
+from trajsimimeasures_amd.data import (TrajSet, lengths_lognormal, lengths_uniform, lengths_uniform_with_tail,
+                                       pack_list, pack_padded, random_walks)
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "tsm.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(tsm_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_function():
+    L = _lib.lib()
+    names = header_functions()
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(L, n), n
+    assert set(names) == set(_lib.EXPORTS)
+    assert L.tsm_abi_version() == 1
+
+
+def test_params_default():
+    p = _lib.ParamsC()
+    _lib.lib().tsm_params_default(ctypes.byref(p))
+    assert (p.eps, p.eps_mode, p.gx, p.gy, p.lanes, p.precision) == (0.0, 0, 0.0, 0.0, 0, 0)
+
+
+def test_argument_errors_without_device():
+    L = _lib.lib()
+    s = _lib.TrajSetC(0, 0, 0, 5, 5, 1)
+    prm = _lib.params()
+    rc = L.tsm_pairs_triu(99, ctypes.byref(s), 3, ctypes.byref(prm), None, None, None)
+    assert rc == -1 and b"unknown measure" in L.tsm_last_error()
+    rc = L.tsm_pairs_triu(0, ctypes.byref(s), 6, ctypes.byref(prm), None, None, None)
+    assert rc == -1 and b"triangle" in L.tsm_last_error()
+    rc = L.tsm_pairs(0, None, None, None, 1, ctypes.byref(prm), None, None, None)
+    assert rc == -1
+    rc = L.tsm_pairs(0, ctypes.byref(s), None, None, 1, ctypes.byref(prm), None, None, None)
+    assert rc == -1 and b"null pair" in L.tsm_last_error()
+    rc = L.tsm_pairs_grid(0, ctypes.byref(s), 0, 5, 0, 5, ctypes.byref(prm), None, None, None)
+    assert rc == -1
+    xy = np.zeros((3, 2))
+    offs = np.array([0, 2, 4], dtype=np.int64)  # does not end at npts
+    pa = np.zeros(1, np.int32)
+    out = np.zeros(1)
+    rc = L.tsm_pairs_host(0, xy.ctypes.data, offs.ctypes.data, 2, 3, pa.ctypes.data, pa.ctypes.data, 1,
+                          ctypes.byref(prm), out.ctypes.data, None)
+    assert rc == -1 and b"offsets" in L.tsm_last_error()
+    offs = np.array([0, 0, 3], dtype=np.int64)  # empty trajectory
+    rc = L.tsm_pairs_host(0, xy.ctypes.data, offs.ctypes.data, 2, 3, pa.ctypes.data, pa.ctypes.data, 1,
+                          ctypes.byref(prm), out.ctypes.data, None)
+    assert rc == -1 and b"empty" in L.tsm_last_error()
+    with pytest.raises(_lib.TsmError):
+        _lib.check(-1)
+
+
+def test_params_validation():
+    with pytest.raises(ValueError):
+        _lib.params(eps_mode="cube")
+    with pytest.raises(ValueError):
+        _lib.params(precision="fp16")
+    with pytest.raises(ValueError):
+        _lib.params(gpoint=[1, 2, 3])
+    with pytest.raises(ValueError):
+        _lib.measure_id("cdds")
+    assert _lib.measure_id("dfrechet") == _lib.FRECHET
+
+
+def test_pack_padded_roundtrip():
+    rng = np.random.default_rng(0)
+    lens = np.array([3, 1, 5])
+    t = np.zeros((3, 5, 2))
+    for i, n in enumerate(lens):
+        t[i, :n] = rng.random((n, 2))
+    ts = pack_padded(t, lens)
+    assert ts.n == 3 and list(ts.lengths) == [3, 1, 5]
+    for i, n in enumerate(lens):
+        assert np.array_equal(ts.traj(i), t[i, :n])
+    with pytest.raises(ValueError):
+        pack_padded(t, [3, 1, 6])
+    with pytest.raises(ValueError):
+        pack_padded(t, [3, 0, 5])
+    sub = ts.subset([2, 0])
+    assert np.array_equal(sub.traj(0), ts.traj(2)) and np.array_equal(sub.traj(1), ts.traj(0))
+    cat = TrajSet.concat([ts, sub])
+    assert cat.n == 5 and np.array_equal(cat.traj(4), ts.traj(0))
+    pl = pack_list([ts.traj(1), ts.traj(2)])
+    assert list(pl.lengths) == [1, 5]
+
+
+def test_random_walk_generator():
+    rng = np.random.default_rng(0)
+    lens = lengths_uniform(rng, 500, 20, 200)
+    assert lens.min() >= 20 and lens.max() <= 200
+    ts = random_walks(rng, lens)
+    assert ts.n == 500 and np.array_equal(ts.lengths, lens)
+    assert ts.xy.min() >= 0.0 and ts.xy.max() <= 1.0
+    # reproducible from the seed, independent of chunking
+    r1 = np.random.default_rng(7)
+    a = random_walks(r1, lengths_uniform(r1, 50, 1, 40))
+    r2 = np.random.default_rng(7)
+    b = random_walks(r2, lengths_uniform(r2, 50, 1, 40), chunk=7)
+    assert np.array_equal(a.xy, b.xy)
+    # step lengths: lognormal with mean 0.002 (clipping only shortens steps)
+    steps = np.linalg.norm(np.diff(ts.xy, axis=0), axis=1)
+    inner = np.ones(steps.size, bool)
+    inner[ts.offsets[1:-1] - 1] = False
+    assert 0.0017 < steps[inner].mean() < 0.0021
+
+
+def test_length_samplers():
+    rng = np.random.default_rng(0)
+    ln = lengths_lognormal(rng, 20000)
+    assert ln.min() >= 10 and ln.max() <= 1000 and 55 <= np.median(ln) <= 65
+    lt = lengths_uniform_with_tail(rng, 20000)
+    tail = lt >= 1000
+    assert 0.015 < tail.mean() < 0.025 and lt[~tail].max() <= 500 and lt.max() <= 2000
+
+
+def test_dropin_rejects_shape_mismatch():
+    from trajsimimeasures_amd import core
+    with pytest.raises(AssertionError):
+        core.dtw(np.zeros((2, 3, 2)), [1, 1], np.zeros((2, 4, 2)), [1, 1])
+    with pytest.raises(AssertionError):
+        core.edr(np.zeros((2, 3, 2)), [1, 1], np.zeros((3, 3, 2)), [1, 1, 1], 0.1)
+
+
+def test_pad_lists_to_array():
+    from trajsimimeasures_amd.core import pad_lists_to_array
+    arr = pad_lists_to_array([[[1, 2]], [[3, 4], [5, 6]]])
+    assert arr.shape == (2, 2, 2) and arr.dtype == np.float64
+    assert np.array_equal(arr[0], [[1, 2], [0, 0]])

@@ -1,0 +1,170 @@
+"""Pins the FP64 oracle (oracle/tsm_oracle.c) against what the mathematics
+fixes: brute-force enumeration over explicit alignments, closed forms, hand
+cases that separate the reference's boundary variants from the textbook
+ones, invariants, and the EDR/LCSS threshold equivalence.  The reference
+ships no golden vectors for this path (its "tests" are timing drivers), so
+these checks are the oracle's anchor."""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests import bruteforce as BF
+
+
+def rnd(rng, n, scale=1.0):
+    return rng.random((n, 2)) * scale
+
+
+@pytest.fixture(scope="module")
+def rng():
+    return np.random.default_rng(1234)
+
+
+# ---------------------------------------------------------------- brute force
+
+@pytest.mark.parametrize("n1,n2", [(1, 1), (1, 4), (3, 1), (2, 3), (4, 4), (5, 3)])
+def test_dtw_frechet_bruteforce(rng, n1, n2):
+    for _ in range(5):
+        a, b = rnd(rng, n1), rnd(rng, n2)
+        assert O.single("dtw", a, b) == pytest.approx(BF.dtw(a, b), rel=1e-12, abs=1e-15)
+        assert O.single("frechet", a, b) == BF.frechet(a, b)  # a max/min of the same distances: exact
+
+
+@pytest.mark.parametrize("n1,n2", [(1, 1), (1, 3), (3, 2), (3, 3), (4, 3)])
+def test_edr_lcss_bruteforce(rng, n1, n2):
+    for eps in (0.05, 0.3, 0.6):
+        thr = O.eps_threshold(eps)
+        match = lambda p, q: (p[0] - q[0]) ** 2 + (p[1] - q[1]) ** 2 < thr
+        for _ in range(4):
+            a, b = rnd(rng, n1), rnd(rng, n2)
+            assert O.count("edr", a, b, eps) == BF.edr_count(a, b, match)
+            assert O.count("lcss", a, b, eps) == BF.lcss_count(a, b, match)
+
+
+@pytest.mark.parametrize("n1,n2", [(1, 1), (2, 1), (1, 3), (3, 3), (4, 2)])
+def test_erp_bruteforce(rng, n1, n2):
+    for _ in range(4):
+        a, b = rnd(rng, n1), rnd(rng, n2)
+        g = rng.random(2)
+        assert O.single("erp", a, b, gpoint=g) == pytest.approx(BF.erp(a, b, g), rel=1e-12)
+
+
+def test_hausdorff_textbook(rng):
+    # agrees with the textbook point-to-segment Hausdorff whenever no
+    # projection parameter lands in (0, 1e-5), where the reference takes the
+    # nearer endpoint instead
+    for n1, n2 in [(2, 2), (3, 5), (6, 4), (1, 4), (4, 1), (1, 1)]:
+        for _ in range(5):
+            a, b = rnd(rng, n1), rnd(rng, n2)
+            assert O.single("hausdorff", a, b) == pytest.approx(BF.hausdorff(a, b), rel=1e-12, abs=1e-15)
+
+
+# ---------------------------------------------------------------- closed forms / hand cases
+
+def test_dtw_one_point_closed_form(rng):
+    p = rng.random((1, 2))
+    b = rnd(rng, 7)
+    s = 0.0
+    for q in b:  # D[1][j] = d_j + D[1][j-1]: the in-order running sum, bit for bit
+        s += math.sqrt((p[0, 0] - q[0]) ** 2 + (p[0, 1] - q[1]) ** 2)
+    assert O.single("dtw", p, b) == s
+    assert O.single("frechet", p, b) == max(math.sqrt((p[0, 0] - q[0]) ** 2 + (p[0, 1] - q[1]) ** 2) for q in b)
+
+
+def test_dtw_hand_case():
+    a = [(0, 0), (1, 0), (2, 0)]
+    b = [(0, 0), (2, 0)]
+    assert O.single("dtw", a, b) == 1.0
+    assert O.single("frechet", [(0, 0), (1, 0)], [(0, 1), (1, 1)]) == 1.0
+
+
+def test_identity_and_symmetry(rng):
+    a, b = rnd(rng, 9), rnd(rng, 6)
+    for m in ("dtw", "frechet", "erp", "hausdorff"):
+        assert O.single(m, a, a) == 0.0
+        assert O.single(m, a, b) == O.single(m, b, a)
+    for m in ("edr", "lcss"):
+        assert O.single(m, a, a, eps=1e-9) == 0.0
+        assert O.count(m, a, b, 0.2) == O.count(m, b, a, 0.2)
+
+
+def test_edr_lcss_extreme_eps(rng):
+    a, b = rnd(rng, 5), rnd(rng, 8)
+    # eps <= 0 never matches: zero-boundary EDR count = min(n1, n2) (the
+    # textbook boundary would give max); LCSS count = 0
+    assert O.count("edr", a, b, 0.0) == 5
+    assert O.single("edr", a, b, eps=0.0) == 5 / 8
+    assert O.single("lcss", a, b, eps=-1.0) == 1.0
+    # eps = inf always matches: EDR 0 (textbook would give |n1 - n2| / max), LCSS min(n1, n2)
+    assert O.single("edr", a, b, eps=math.inf) == 0.0
+    assert O.count("lcss", a, b, math.inf) == 5
+    assert O.single("lcss", a, b, eps=math.inf) == 0.0
+
+
+def test_edr_zero_boundary_hand_case():
+    # the second point of b matches a's only point after skipping b[0] for free
+    assert O.single("edr", [(5, 5)], [(0, 0), (5, 5)], eps=1.0) == 0.0
+
+
+def test_erp_closed_forms():
+    # one point each: min(ga + gb, d) = d by the triangle inequality
+    assert O.single("erp", [(0.3, 0.4)], [(0.6, 0.8)], gpoint=(0, 0)) == pytest.approx(0.5, rel=1e-15)
+    # constant-sum boundary D[i][0] = Sa: 3.0 (the cumulative boundary gives 0.0)
+    assert O.single("erp", [(0, 0), (3, 0)], [(3, 0)], gpoint=(0, 0)) == 3.0
+
+
+def test_erp_translation_invariance(rng):
+    a, b, g = rnd(rng, 6), rnd(rng, 4), rng.random(2)
+    t = np.array([0.25, -0.5])
+    assert O.single("erp", a + t, b + t, gpoint=g + t) == pytest.approx(O.single("erp", a, b, gpoint=g), rel=1e-12)
+
+
+def test_hausdorff_hand_cases():
+    # point-to-segment: 2.0 (point-to-point would give sqrt(5))
+    assert O.single("hausdorff", [(0, 0), (2, 0)], [(1, 1), (1, 2)]) == 2.0
+    # a projection parameter in (0, 1e-5) takes the nearer endpoint
+    p, s = [(5e-6, 1.0)], [(0.0, 0.0), (1.0, 0.0)]
+    assert O.single("hausdorff", p, s) == math.sqrt(5e-6 ** 2 + 1.0)
+    # a degenerate segment uses the distance to its first endpoint
+    assert O.single("hausdorff", [(0.0, 0.0)], [(3.0, 4.0), (3.0, 4.0)]) == 5.0
+    # one-point polylines contribute 0 (core/hausdorff.py:50-51)
+    assert O.single("hausdorff", [(0.0, 0.0)], [(3.0, 4.0)]) == 0.0
+
+
+# ---------------------------------------------------------------- epsilon threshold
+
+def test_eps_threshold_equivalence(rng):
+    for eps in np.concatenate([rng.random(200) * 0.01, [0.0025, 1e-300, 1.0, 3.0]]):
+        thr = O.eps_threshold(eps, "sqrt")
+        cands = [thr, np.nextafter(thr, 0), np.nextafter(thr, np.inf), eps * eps,
+                 np.nextafter(eps * eps, 0), np.nextafter(eps * eps, np.inf)]
+        for d2 in cands:
+            assert (d2 < thr) == (math.sqrt(d2) < eps), (eps, d2)
+        assert O.eps_threshold(eps, "squared") == eps * eps
+
+
+def test_eps_threshold_differs_from_square(rng):
+    # the sqrt convention is not the same test as d2 < eps*eps for every eps
+    eps = rng.random(2000) * 0.01
+    assert any(O.eps_threshold(e, "sqrt") != e * e for e in eps)
+
+
+# ---------------------------------------------------------------- batched entry
+
+def test_pairs_matches_single(rng):
+    from trajsimimeasures_amd.data import lengths_uniform, random_walks
+    ts = random_walks(rng, lengths_uniform(rng, 40, 1, 30))
+    pa = np.arange(0, 40, 2)
+    pb = pa + 1
+    g = ts.xy.mean(0)
+    for m in ("dtw", "edr", "lcss", "erp", "frechet", "hausdorff"):
+        vals, cnt = O.pairs(m, ts.xy, ts.offsets, pa, pb, eps=0.01, gpoint=g, threads=3)
+        vals1, _ = O.pairs(m, ts.xy, ts.offsets, pa, pb, eps=0.01, gpoint=g, threads=1)
+        assert np.array_equal(vals, vals1)
+        for k in range(pa.size):
+            ref = O.single(m, ts.traj(pa[k]), ts.traj(pb[k]), eps=0.01, gpoint=g)
+            assert vals[k] == ref
+            if m in ("edr", "lcss"):
+                assert cnt[k] == O.count(m, ts.traj(pa[k]), ts.traj(pb[k]), 0.01)

@@ -1,0 +1,80 @@
+"""Build libtsm.so (HIP, gfx950) in-tree.
+
+Each translation unit under csrc/ is compiled to an object in parallel, then
+linked into ``trajsimimeasures_amd/libtsm.so``.  ``python -m
+trajsimimeasures_amd.build`` runs it; ``__graft_entry__.build()`` calls it.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+import time
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(PKG, "csrc")
+ROOT = os.path.dirname(PKG)
+BUILD = os.path.join(ROOT, "_build")
+LIB = os.path.join(PKG, "libtsm.so")
+ARCH = os.environ.get("TSM_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+
+# -ffp-contract=off: FMAs only where the source asks for them (fmaf), so the
+# FP64 paths keep the oracle's rounding order.  The packed-FP32 feature is
+# switched off for every HIP source (v_pk_* FP32 forms are not used).
+CFLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Werror",
+    "-Wno-unused-command-line-argument",
+    f"--offload-arch={ARCH}",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+]
+
+
+def sources():
+    return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
+
+
+def _compile(src: str) -> tuple[str, float, str]:
+    obj = os.path.join(BUILD, os.path.basename(src)[:-4] + ".o")
+    t0 = time.time()
+    cmd = [HIPCC, *CFLAGS, "-c", src, "-o", obj]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    out = "\n".join(l for l in (p.stdout + p.stderr).splitlines()
+                    if "packed-fp32-ops' is not a recognized feature" not in l
+                    and "(ignoring feature)" not in l)
+    if p.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{' '.join(cmd)}\n{out}")
+    return obj, time.time() - t0, out
+
+
+def build(verbose: bool = True, jobs: int | None = None) -> str:
+    """Compile every HIP source for gfx950 and link libtsm.so; returns its path."""
+    os.makedirs(BUILD, exist_ok=True)
+    srcs = sources()
+    jobs = jobs or min(len(srcs), max(1, os.cpu_count() or 1))
+    t0 = time.time()
+    with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
+        results = list(ex.map(_compile, srcs))
+    objs = [r[0] for r in results]
+    tmp = LIB + ".tmp"
+    cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{p.stdout}{p.stderr}")
+    os.replace(tmp, LIB)
+    if verbose:
+        for src, (_, dt, out) in zip(srcs, results):
+            print(f"  {os.path.basename(src)}: {dt:.1f}s{(' ' + out) if out.strip() else ''}")
+        print(f"built {LIB} in {time.time() - t0:.1f}s")
+    return LIB
+
+
+if __name__ == "__main__":
+    try:
+        build()
+    except RuntimeError as e:
+        print(e, file=sys.stderr)
+        sys.exit(1)

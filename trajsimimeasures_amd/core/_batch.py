@@ -1,0 +1,52 @@
+"""Shared body of the drop-in per-measure functions: pack the padded batch,
+call libtsm.so's host entry point, return float64 [B] like the reference
+(core/dtw.py:56-77)."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from .. import _lib
+from ..data import pack_padded
+
+__all__ = ["run_padded", "lanes_hint"]
+
+
+def lanes_hint(threads_per_traj) -> int:
+    """The reference's block size per pair (config.py:45) is a hint here: 8
+    and 16 select that many lanes per pair, anything else the autotuner."""
+    try:
+        t = int(threads_per_traj)
+    except (TypeError, ValueError):
+        return 0
+    return t if t in (8, 16) else 0
+
+
+def run_padded(measure: int, t1, len1, t2, len2, *, eps=0.0, eps_mode="sqrt", gpoint=None, threads_per_traj=64,
+               precision="fp32", return_counts=False):
+    t1 = np.asarray(t1, dtype=np.float64)
+    t2 = np.asarray(t2, dtype=np.float64)
+    if t1.ndim != 3 or t2.ndim != 3:
+        raise ValueError("t1 and t2 must be [batch, seq, 2] arrays")
+    assert t1.shape[0] == t2.shape[0]
+    B = t1.shape[0]
+    if B == 0:
+        out = np.zeros(0, dtype=np.float64)
+        return (out, np.zeros(0, dtype=np.int32)) if return_counts else out
+    a = pack_padded(t1, len1)
+    b = pack_padded(t2, len2)
+    xy = np.ascontiguousarray(np.concatenate([a.xy, b.xy], axis=0))
+    if not np.all(np.isfinite(xy)):
+        raise ValueError("coordinates must be finite")
+    offs = np.concatenate([a.offsets, b.offsets[1:] + a.offsets[-1]]).astype(np.int64)
+    pa = np.arange(B, dtype=np.int32)
+    pb = pa + np.int32(B)
+    out = np.empty(B, dtype=np.float64)
+    cnt = np.empty(B, dtype=np.int32) if return_counts else None
+    prm = _lib.params(eps, eps_mode, gpoint, lanes_hint(threads_per_traj), precision)
+    L = _lib.lib()
+    _lib.check(L.tsm_pairs_host(measure, xy.ctypes.data, offs.ctypes.data, 2 * B, xy.shape[0], pa.ctypes.data,
+                                pb.ctypes.data, B, ctypes.byref(prm), out.ctypes.data,
+                                cnt.ctypes.data if cnt is not None else None))
+    return (out, cnt) if return_counts else out

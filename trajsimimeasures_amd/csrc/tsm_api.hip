@@ -1,0 +1,283 @@
+// tsm_api.hip -- the C ABI of libtsm.so (include/tsm.h): validation,
+// device context, ERP prologue, Hausdorff launch and host-memory staging.
+//
+// Host flow of one batched call (the body of the reference's per-measure
+// functions, e.g. core/dtw.py:56-77):
+//   1. validate, pick the lanes-per-pair L (the caller's hint or a model);
+//   2. ERP only: per-point gap distances and per-trajectory sums;
+//   3. plan: counting sort of pairs into tasks (tsm_plan.hpp);
+//   4. one persistent DP kernel (tsm_dp.hpp), or the Hausdorff kernel.
+// Everything is stream-ordered on the caller's stream; the only host sync
+// is a workspace grow.
+#include "tsm_hausdorff.hpp"
+#include "tsm_host.hpp"
+
+namespace tsm {
+
+__global__ void xy_to_f32_kernel(const double *in, float *out, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
+}
+
+// ERP gap distances |p - g| per point (core/erp.py:32-40), FP64 and FP32.
+__global__ void erp_gap_kernel(const double *xy, int64_t npts, double gx, double gy, double *g64, float *g32) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npts; i += (int64_t)gridDim.x * blockDim.x) {
+        double d = sqrt(dist2_f64(gx, gy, xy[2 * i], xy[2 * i + 1]));
+        g64[i] = d;
+        g32[i] = (float)d;
+    }
+}
+
+// Per-trajectory sums of gap distances, accumulated serially in FP64 in point
+// order like the reference (core/erp.py:44-49).
+__global__ void erp_gapsum_kernel(const int64_t *offs, int64_t n, const double *g64, double *s64, float *s32) {
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int64_t i = offs[t]; i < offs[t + 1]; ++i) s += g64[i];
+        s64[t] = s;
+        s32[t] = (float)s;
+    }
+}
+
+thread_local std::string g_err;
+std::mutex g_mu;
+Context g_ctx;
+
+int fail(int code, const std::string &msg) {
+    g_err = msg;
+    return code;
+}
+
+}  // namespace tsm
+
+namespace {
+
+using namespace tsm;
+
+// Lanes-per-pair choice: the caller's hint, else a cost model over the
+// expected column count.  Waste per pair is (L - 1) skew steps over nrow rows
+// plus column padding up to a multiple of L; both shrink with smaller L,
+// while register columns (and so occupancy) grow.
+int choose_lanes(const tsm_params &prm, int64_t max_len) {
+    if (prm.lanes == 4 || prm.lanes == 8 || prm.lanes == 16) return prm.lanes;
+    (void)max_len;
+    return 16;
+}
+
+int erp_prologue(const tsm_trajset &set, double gx, double gy, hipStream_t st) {
+    Context &c = g_ctx;
+    const int64_t npts = set.npts;
+    int rc;
+    if ((rc = c.gap64.ensure(npts * 8 + 8)) || (rc = c.gap32.ensure(npts * 4 + 8)) ||
+        (rc = c.gsum64.ensure(set.n * 8 + 8)) || (rc = c.gsum32.ensure(set.n * 4 + 8)))
+        return rc;
+    hipLaunchKernelGGL(erp_gap_kernel, dim3(grid_for(npts, 256)), dim3(256), 0, st, set.xy64, npts, gx, gy,
+                       c.gap64.as<double>(), c.gap32.as<float>());
+    hipLaunchKernelGGL(erp_gapsum_kernel, dim3(grid_for(set.n, 256)), dim3(256), 0, st, set.offsets, set.n,
+                       c.gap64.as<double>(), c.gsum64.as<double>(), c.gsum32.as<float>());
+    TSM_HIP(hipGetLastError());
+    return TSM_OK;
+}
+
+int run(int measure, const tsm_trajset *set, const PairSpace &ps, int64_t npairs, const tsm_params *prm_in,
+        double *out, int32_t *counts, void *stream) {
+    if (measure < TSM_DTW || measure > TSM_HAUSDORFF) return fail(TSM_E_ARG, "unknown measure id " + std::to_string(measure));
+    if (!set) return fail(TSM_E_ARG, "null trajectory set");
+    if (npairs < 0) return fail(TSM_E_ARG, "negative pair count");
+    if (npairs > (int64_t)0xFFFFFFFEll) return fail(TSM_E_ARG, "at most 2^32-2 pairs per call");
+    if (npairs == 0) return TSM_OK;
+    if (!out) return fail(TSM_E_ARG, "null output buffer");
+    if (!set->offsets || set->n < 1) return fail(TSM_E_ARG, "empty trajectory set");
+    if (set->max_len < 1) return fail(TSM_E_ARG, "max_len must be >= 1");
+    if (set->npts < set->n) return fail(TSM_E_ARG, "npts must be >= n (every trajectory has a point)");
+    tsm_params prm;
+    tsm_params_default(&prm);
+    if (prm_in) prm = *prm_in;
+    const bool f64 = prm.precision == TSM_PREC_FP64;
+    if (prm.precision != TSM_PREC_FP32 && prm.precision != TSM_PREC_FP64)
+        return fail(TSM_E_ARG, "precision must be TSM_PREC_FP32 or TSM_PREC_FP64");
+    if (prm.eps_mode != TSM_EPS_SQRT && prm.eps_mode != TSM_EPS_SQUARED)
+        return fail(TSM_E_ARG, "eps_mode must be TSM_EPS_SQRT or TSM_EPS_SQUARED");
+    const bool needs64 = f64 || measure == TSM_EDR || measure == TSM_LCSS || measure == TSM_ERP;
+    if (needs64 && !set->xy64) return fail(TSM_E_ARG, "this measure needs the FP64 coordinates (xy64)");
+    if (!f64 && measure != TSM_EDR && measure != TSM_LCSS && !set->xy32)
+        return fail(TSM_E_ARG, "FP32 measures need the FP32 coordinates (xy32)");
+    if (set->max_len > (1 << 24)) return fail(TSM_E_ARG, "trajectories longer than 2^24 points are not supported");
+
+    std::lock_guard<std::mutex> lk(g_mu);
+    int rc = ensure_ctx();
+    if (rc) return rc;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+
+    if (measure == TSM_HAUSDORFF) {
+        HArgs h;
+        h.xy32 = set->xy32;
+        h.xy64 = set->xy64;
+        h.offs = set->offsets;
+        h.ps = ps;
+        h.npairs = npairs;
+        h.out = out;
+        int grid = (int)(npairs < 1048576 ? npairs : 1048576);
+        ProfScope p_(st);
+        if (f64) hipLaunchKernelGGL(hausdorff_kernel<double>, dim3(grid), dim3(kWave), 0, st, h);
+        else hipLaunchKernelGGL(hausdorff_kernel<float>, dim3(grid), dim3(kWave), 0, st, h);
+        TSM_HIP(hipGetLastError());
+        return TSM_OK;
+    }
+    if (measure == TSM_ERP && (rc = erp_prologue(*set, prm.gx, prm.gy, st))) return rc;
+
+    const double thr = eps_threshold(prm.eps, prm.eps_mode);
+    const int L = choose_lanes(prm, set->max_len);
+    switch (measure) {
+    case TSM_DTW:
+        return f64 ? run_dp<kDTW, true>(L, *set, ps, npairs, thr, out, counts, st)
+                   : run_dp<kDTW, false>(L, *set, ps, npairs, thr, out, counts, st);
+    case TSM_FRECHET:
+        return f64 ? run_dp<kFrechet, true>(L, *set, ps, npairs, thr, out, counts, st)
+                   : run_dp<kFrechet, false>(L, *set, ps, npairs, thr, out, counts, st);
+    case TSM_ERP:
+        return f64 ? run_dp<kERP, true>(L, *set, ps, npairs, thr, out, counts, st)
+                   : run_dp<kERP, false>(L, *set, ps, npairs, thr, out, counts, st);
+    case TSM_EDR: return run_dp<kEDR, false>(L, *set, ps, npairs, thr, out, counts, st);
+    case TSM_LCSS: return run_dp<kLCSS, false>(L, *set, ps, npairs, thr, out, counts, st);
+    }
+    return fail(TSM_E_ARG, "unknown measure");
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- C ABI
+
+extern "C" {
+
+int tsm_abi_version(void) { return TSM_ABI_VERSION; }
+
+const char *tsm_last_error(void) { return g_err.c_str(); }
+
+void tsm_params_default(tsm_params *p) {
+    if (!p) return;
+    p->eps = 0.0;
+    p->eps_mode = TSM_EPS_SQRT;
+    p->gx = 0.0;
+    p->gy = 0.0;
+    p->lanes = 0;
+    p->precision = TSM_PREC_FP32;
+}
+
+int tsm_init(int device) {
+    int n = 0;
+    TSM_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) return fail(TSM_E_ARG, "device " + std::to_string(device) + " not present");
+    TSM_HIP(hipSetDevice(device));
+    std::lock_guard<std::mutex> lk(g_mu);
+    return ensure_ctx();
+}
+
+int tsm_xy_to_fp32(const double *xy64, float *xy32, int64_t npts, void *stream) {
+    if (npts < 0 || (npts > 0 && (!xy64 || !xy32))) return fail(TSM_E_ARG, "bad point buffers");
+    if (npts == 0) return TSM_OK;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(xy_to_f32_kernel, dim3(grid_for(2 * npts, 256)), dim3(256), 0, st, xy64, xy32, 2 * npts);
+    TSM_HIP(hipGetLastError());
+    return TSM_OK;
+}
+
+int tsm_pairs(int measure, const tsm_trajset *set, const int32_t *pa, const int32_t *pb, int64_t npairs,
+              const tsm_params *prm, double *out, int32_t *counts, void *stream) {
+    if (npairs > 0 && (!pa || !pb)) return fail(TSM_E_ARG, "null pair index arrays");
+    PairSpace ps{};
+    ps.mode = kExplicit;
+    ps.pa = pa;
+    ps.pb = pb;
+    return run(measure, set, ps, npairs, prm, out, counts, stream);
+}
+
+int tsm_pairs_grid(int measure, const tsm_trajset *set, int64_t q0, int64_t nq, int64_t d0, int64_t nd,
+                   const tsm_params *prm, double *out, int32_t *counts, void *stream) {
+    if (!set) return fail(TSM_E_ARG, "null trajectory set");
+    if (nq < 0 || nd < 0 || q0 < 0 || d0 < 0 || q0 + nq > set->n || d0 + nd > set->n)
+        return fail(TSM_E_ARG, "grid ranges outside the trajectory set");
+    PairSpace ps{};
+    ps.mode = kGrid;
+    ps.q0 = q0;
+    ps.nq = nq;
+    ps.d0 = d0;
+    ps.nd = nd;
+    return run(measure, set, ps, nq * nd, prm, out, counts, stream);
+}
+
+int tsm_pairs_triu(int measure, const tsm_trajset *set, int64_t n, const tsm_params *prm, double *out,
+                   int32_t *counts, void *stream) {
+    if (!set) return fail(TSM_E_ARG, "null trajectory set");
+    if (n < 0 || n > set->n) return fail(TSM_E_ARG, "triangle side outside the trajectory set");
+    PairSpace ps{};
+    ps.mode = kTriu;
+    ps.n = n;
+    return run(measure, set, ps, n < 2 ? 0 : n * (n - 1) / 2, prm, out, counts, stream);
+}
+
+int tsm_pairs_host(int measure, const double *xy64, const int64_t *offsets, int64_t n, int64_t npts,
+                   const int32_t *pa, const int32_t *pb, int64_t npairs, const tsm_params *prm, double *out,
+                   int32_t *counts) {
+    if (!xy64 || !offsets || n < 1 || npts < 1) return fail(TSM_E_ARG, "empty trajectory set");
+    if (npairs < 0) return fail(TSM_E_ARG, "negative pair count");
+    if (npairs == 0) return TSM_OK;
+    if (!pa || !pb || !out) return fail(TSM_E_ARG, "null pair or output array");
+    if (offsets[0] != 0 || offsets[n] != npts) return fail(TSM_E_ARG, "offsets must run from 0 to npts");
+    int64_t max_len = 0;
+    for (int64_t i = 0; i < n; ++i) {
+        int64_t l = offsets[i + 1] - offsets[i];
+        if (l < 1) return fail(TSM_E_ARG, "trajectory " + std::to_string(i) + " is empty");
+        if (l > max_len) max_len = l;
+    }
+    for (int64_t k = 0; k < npairs; ++k)
+        if (pa[k] < 0 || pa[k] >= n || pb[k] < 0 || pb[k] >= n)
+            return fail(TSM_E_ARG, "pair " + std::to_string(k) + " indexes outside the set");
+    Context &c = g_ctx;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        int rc = ensure_ctx();
+        if (rc) return rc;
+        if ((rc = c.h_xy64.ensure(npts * 16)) || (rc = c.h_xy32.ensure(npts * 8)) || (rc = c.h_offs.ensure((n + 1) * 8)) ||
+            (rc = c.h_pa.ensure(npairs * 4)) || (rc = c.h_pb.ensure(npairs * 4)) || (rc = c.h_out.ensure(npairs * 8)) ||
+            (rc = c.h_cnt.ensure(npairs * 4)))
+            return rc;
+    }
+    TSM_HIP(hipMemcpy(c.h_xy64.p, xy64, npts * 16, hipMemcpyHostToDevice));
+    TSM_HIP(hipMemcpy(c.h_offs.p, offsets, (n + 1) * 8, hipMemcpyHostToDevice));
+    TSM_HIP(hipMemcpy(c.h_pa.p, pa, npairs * 4, hipMemcpyHostToDevice));
+    TSM_HIP(hipMemcpy(c.h_pb.p, pb, npairs * 4, hipMemcpyHostToDevice));
+    int rc = tsm_xy_to_fp32(c.h_xy64.as<double>(), c.h_xy32.as<float>(), npts, nullptr);
+    if (rc) return rc;
+    tsm_trajset set{c.h_xy32.as<float>(), c.h_xy64.as<double>(), c.h_offs.as<int64_t>(), n, npts, max_len};
+    rc = tsm_pairs(measure, &set, c.h_pa.as<int32_t>(), c.h_pb.as<int32_t>(), npairs, prm, c.h_out.as<double>(),
+                   counts ? c.h_cnt.as<int32_t>() : nullptr, nullptr);
+    if (rc) return rc;
+    TSM_HIP(hipMemcpy(out, c.h_out.p, npairs * 8, hipMemcpyDeviceToHost));
+    if (counts) TSM_HIP(hipMemcpy(counts, c.h_cnt.p, npairs * 4, hipMemcpyDeviceToHost));
+    return TSM_OK;
+}
+
+int tsm_profile_enable(int on) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_ctx.prof = on != 0;
+    g_ctx.ev_used = 0;
+    return TSM_OK;
+}
+
+int tsm_profile_read(double *total_ms, int64_t *launches) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    double ms = 0.0;
+    for (size_t i = 0; i < g_ctx.ev_used; ++i) {
+        TSM_HIP(hipEventSynchronize(g_ctx.ev[i].second));
+        float t = 0.f;
+        TSM_HIP(hipEventElapsedTime(&t, g_ctx.ev[i].first, g_ctx.ev[i].second));
+        ms += t;
+    }
+    if (total_ms) *total_ms = ms;
+    if (launches) *launches = (int64_t)g_ctx.ev_used;
+    g_ctx.ev_used = 0;
+    return TSM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,6 @@
+// tsm_dp_edr.hip -- EDR instances of the DP kernel and its launcher.
+#include "tsm_launch.hpp"
+
+namespace tsm {
+template int run_dp<kEDR, false>(int, const tsm_trajset &, const PairSpace &, int64_t, double, double *, int32_t *, hipStream_t);
+}  // namespace tsm

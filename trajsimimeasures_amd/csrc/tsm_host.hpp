@@ -1,0 +1,134 @@
+// tsm_host.hpp -- host-side state shared by the translation units of
+// libtsm.so: the per-process device context, grow-only workspace buffers,
+// error reporting and launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/tsm.h"
+#include "tsm_device.hpp"
+
+namespace tsm {
+
+int fail(int code, const std::string &msg);
+
+#define TSM_HIP(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            return fail(TSM_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+// Grow-only device buffer.
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    inline int ensure(size_t bytes) {
+        if (bytes <= cap) return TSM_OK;
+        if (p) {
+            hipError_t e = hipFree(p);
+            p = nullptr;
+            cap = 0;
+            if (e != hipSuccess) return fail(TSM_E_HIP, std::string("hipFree: ") + hipGetErrorString(e));
+        }
+        size_t want = bytes + bytes / 4 + 256;
+        hipError_t e = hipMalloc(&p, want);
+        if (e != hipSuccess) {
+            p = nullptr;
+            (void)hipGetLastError();
+            return fail(TSM_E_NOMEM, "hipMalloc(" + std::to_string(want) + ") failed: " + hipGetErrorString(e));
+        }
+        cap = want;
+        return TSM_OK;
+    }
+    template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
+};
+
+struct Context {
+    int device = -1;
+    int num_cu = 0;
+    DevBuf hist, slots, wend, ntasks, counter, bnd;
+    DevBuf gap64, gap32, gsum64, gsum32;
+    // staging for tsm_pairs_host
+    DevBuf h_xy64, h_xy32, h_offs, h_pa, h_pb, h_out, h_cnt;
+    // profiling
+    bool prof = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    size_t ev_used = 0;
+};
+
+extern std::mutex g_mu;
+extern Context g_ctx;
+
+inline int ensure_ctx() {
+    int dev = 0;
+    TSM_HIP(hipGetDevice(&dev));
+    if (g_ctx.device == dev) return TSM_OK;
+    if (g_ctx.device >= 0) return fail(TSM_E_UNSUP, "libtsm serves one device per process");
+    hipDeviceProp_t prop;
+    TSM_HIP(hipGetDeviceProperties(&prop, dev));
+    g_ctx.device = dev;
+    g_ctx.num_cu = prop.multiProcessorCount;
+    return TSM_OK;
+}
+
+inline double eps_threshold(double eps, int mode) {
+    if (mode == TSM_EPS_SQUARED) return eps * eps;
+    if (!(eps > 0.0)) return 0.0;
+    if (std::isinf(eps)) return INFINITY;
+    double t = eps * eps;
+    while (std::sqrt(t) >= eps) t = std::nextafter(t, 0.0);
+    while (std::sqrt(t) < eps) t = std::nextafter(t, INFINITY);
+    return t;
+}
+
+inline int grid_for(int64_t n, int block) {
+    int64_t g = (n + block - 1) / block;
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    return (int)g;
+}
+
+// Resident waves of one persistent kernel instance on this device.
+template <typename K>
+int persistent_grid(K kernel) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWave, 0) != hipSuccess || per_cu < 1) {
+        (void)hipGetLastError();
+        per_cu = 4;
+    }
+    return per_cu * g_ctx.num_cu;
+}
+
+struct ProfScope {
+    hipEvent_t e1 = nullptr, e2 = nullptr;
+    hipStream_t s;
+    explicit ProfScope(hipStream_t st) : s(st) {
+        if (!g_ctx.prof) return;
+        if (g_ctx.ev_used == g_ctx.ev.size()) {
+            hipEvent_t a, b;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+            g_ctx.ev.push_back({a, b});
+        }
+        e1 = g_ctx.ev[g_ctx.ev_used].first;
+        e2 = g_ctx.ev[g_ctx.ev_used].second;
+        g_ctx.ev_used++;
+        (void)hipEventRecord(e1, s);
+    }
+    ~ProfScope() {
+        if (e2) (void)hipEventRecord(e2, s);
+    }
+};
+
+// DP launcher for one measure and precision, lanes-per-pair L in {8, 16};
+// defined in tsm_dp_<measure>.hip.
+template <int M, bool F64>
+int run_dp(int L, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr, double *out,
+           int32_t *counts, hipStream_t st);
+
+}  // namespace tsm

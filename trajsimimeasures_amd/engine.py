@@ -1,0 +1,139 @@
+"""Device-resident engine API over libtsm.so.
+
+``DeviceTrajSet`` uploads a packed ``TrajSet`` once (FP64 coordinates, the
+FP32 copy, offsets) into HBM through torch's allocator; ``pairs``, ``grid``
+and ``triu`` run a measure over explicit pairs, a query x database grid or
+the upper triangle of the all-pairs matrix, on torch's current stream, and
+return device tensors.  torch supplies device memory and the stream only; the
+compute is the HIP kernels of libtsm.so.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from .data import TrajSet
+
+__all__ = ["DeviceTrajSet", "pairs", "grid", "triu", "KernelTimer"]
+
+
+def _stream_ptr():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+class DeviceTrajSet:
+    """A TrajSet resident on the current HIP device."""
+
+    def __init__(self, ts: TrajSet, device="cuda"):
+        if ts.n < 1:
+            raise ValueError("empty trajectory set")
+        lens = ts.lengths
+        if lens.min() < 1:
+            raise ValueError("every trajectory needs at least one point")
+        if not np.all(np.isfinite(ts.xy)):
+            raise ValueError("coordinates must be finite")
+        self.host = ts
+        self.n = ts.n
+        self.npts = int(ts.offsets[-1])
+        self.max_len = int(lens.max())
+        self.lengths = torch.as_tensor(lens, dtype=torch.int64, device=device)
+        self.xy64 = torch.as_tensor(ts.xy, dtype=torch.float64, device=device).contiguous()
+        self.offsets = torch.as_tensor(ts.offsets, dtype=torch.int64, device=device).contiguous()
+        self.xy32 = torch.empty((self.npts, 2), dtype=torch.float32, device=device)
+        _lib.check(_lib.lib().tsm_xy_to_fp32(self.xy64.data_ptr(), self.xy32.data_ptr(), self.npts, _stream_ptr()))
+
+    def cstruct(self) -> _lib.TrajSetC:
+        return _lib.TrajSetC(self.xy32.data_ptr(), self.xy64.data_ptr(), self.offsets.data_ptr(), self.n,
+                             self.npts, self.max_len)
+
+
+def _outs(n: int, measure: int, device, want_counts: bool):
+    out = torch.empty(n, dtype=torch.float64, device=device)
+    cnt = None
+    if want_counts and measure in (_lib.EDR, _lib.LCSS):
+        cnt = torch.empty(n, dtype=torch.int32, device=device)
+    return out, cnt
+
+
+def pairs(measure, dset: DeviceTrajSet, pa, pb, *, eps=0.0, eps_mode="sqrt", gpoint=None, lanes=0,
+          precision="fp32", return_counts=False, validate=True, out=None):
+    """Measure over explicit pairs (pa[k], pb[k]) of ``dset`` (pa as t1).
+
+    validate=False skips the (synchronising) index range check; ``out`` may
+    supply a float64 device tensor of len(pa) to write into."""
+    m = _lib.measure_id(measure)
+    dev = dset.xy64.device
+    pa = torch.as_tensor(pa, dtype=torch.int32, device=dev).contiguous()
+    pb = torch.as_tensor(pb, dtype=torch.int32, device=dev).contiguous()
+    if pa.shape != pb.shape or pa.dim() != 1:
+        raise ValueError("pa and pb must be 1-D and of equal length")
+    n = pa.numel()
+    if validate and n and (int(pa.min()) < 0 or int(pb.min()) < 0 or int(pa.max()) >= dset.n
+                           or int(pb.max()) >= dset.n):
+        raise IndexError("pair index outside the trajectory set")
+    if out is None:
+        out, cnt = _outs(n, m, dev, return_counts)
+    else:
+        if out.dtype != torch.float64 or out.numel() != n or not out.is_contiguous() or out.device != dev:
+            raise ValueError("out must be a contiguous float64 tensor of len(pa) on the set's device")
+        cnt = None
+        if return_counts and m in (_lib.EDR, _lib.LCSS):
+            cnt = torch.empty(n, dtype=torch.int32, device=dev)
+    prm = _lib.params(eps, eps_mode, gpoint, lanes, precision)
+    s = dset.cstruct()
+    _lib.check(_lib.lib().tsm_pairs(m, ctypes.byref(s), pa.data_ptr(), pb.data_ptr(), n, ctypes.byref(prm),
+                                    out.data_ptr(), cnt.data_ptr() if cnt is not None else None, _stream_ptr()))
+    return (out, cnt) if return_counts else out
+
+
+def grid(measure, dset: DeviceTrajSet, q0: int, nq: int, d0: int, nd: int, *, eps=0.0, eps_mode="sqrt",
+         gpoint=None, lanes=0, precision="fp32", return_counts=False):
+    """Measure over the grid (q0 + q, d0 + d); returns [nq, nd] (row-major)."""
+    m = _lib.measure_id(measure)
+    dev = dset.xy64.device
+    out, cnt = _outs(nq * nd, m, dev, return_counts)
+    prm = _lib.params(eps, eps_mode, gpoint, lanes, precision)
+    s = dset.cstruct()
+    _lib.check(_lib.lib().tsm_pairs_grid(m, ctypes.byref(s), q0, nq, d0, nd, ctypes.byref(prm), out.data_ptr(),
+                                         cnt.data_ptr() if cnt is not None else None, _stream_ptr()))
+    out = out.view(nq, nd)
+    if return_counts:
+        return out, (cnt.view(nq, nd) if cnt is not None else None)
+    return out
+
+
+def triu(measure, dset: DeviceTrajSet, n: int | None = None, *, eps=0.0, eps_mode="sqrt", gpoint=None,
+         lanes=0, precision="fp32", return_counts=False):
+    """Upper triangle over trajectories [0, n): n*(n-1)/2 values in
+    itertools.combinations order."""
+    m = _lib.measure_id(measure)
+    n = dset.n if n is None else int(n)
+    dev = dset.xy64.device
+    npairs = n * (n - 1) // 2
+    out, cnt = _outs(npairs, m, dev, return_counts)
+    prm = _lib.params(eps, eps_mode, gpoint, lanes, precision)
+    s = dset.cstruct()
+    _lib.check(_lib.lib().tsm_pairs_triu(m, ctypes.byref(s), n, ctypes.byref(prm), out.data_ptr(),
+                                         cnt.data_ptr() if cnt is not None else None, _stream_ptr()))
+    return (out, cnt) if return_counts else out
+
+
+class KernelTimer:
+    """HIP-event timing of the DP / Hausdorff kernels inside libtsm.so, on the
+    stream they were launched on."""
+
+    def __enter__(self):
+        _lib.check(_lib.lib().tsm_profile_enable(1))
+        return self
+
+    def __exit__(self, *exc):
+        ms = ctypes.c_double(0.0)
+        n = ctypes.c_int64(0)
+        _lib.check(_lib.lib().tsm_profile_read(ctypes.byref(ms), ctypes.byref(n)))
+        _lib.check(_lib.lib().tsm_profile_enable(0))
+        self.ms = ms.value
+        self.launches = n.value
+        return False
