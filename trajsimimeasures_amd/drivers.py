@@ -1,0 +1,171 @@
+"""Callers of the similarity path: the reference's dispatch and collation
+helpers and its three GPU drivers, on the engine.
+
+  heuristic_fn_wrapper   utilities/method_helper.py:32-73
+  heuristic_colllate_fn  utilities/method_helper.py:142-153
+  pairwise               test_trajsimi_time.py:222-276   (pairs of trajectories)
+  query_db               test_trajsimi_time_DQ.py:136-181 (|Q| x |D| grid)
+  all_pairs              test_clustering_time.py:208-269  (upper triangle, symmetrised)
+
+The reference drivers pad every 512-pair batch on the host, gather it with
+fancy indexing and launch one kernel per batch.  Here a trajectory list is
+packed and uploaded once, pair addressing (explicit pairs, grid, triangle)
+happens on the device, and one engine call covers the whole workload.  Each
+driver returns the values and the reference's timing split
+[running, collate, compute] in seconds.
+"""
+from __future__ import annotations
+
+import time
+from functools import partial
+
+import numpy as np
+
+from . import _lib, core
+from .data import TrajSet, pack_list
+
+__all__ = ["heuristic_fn_wrapper", "heuristic_colllate_fn", "is_heuristic", "pairwise", "query_db", "all_pairs",
+           "symmetric_from_triu", "DEFAULT_EPS"]
+
+DEFAULT_EPS = 0.0025  # EDR/LCSS epsilon of the 100k-pair workload
+
+_GPU_FNS = {"dtw": core.dtw, "edr": core.edr, "erp": core.erp, "frechet": core.dfrechet,
+            "hausdorff": core.hausdorff, "lcss": core.lcss}
+_MEASURE = {"dtw": "dtw", "edr": "edr", "erp": "erp", "frechet": "frechet", "hausdorff": "hausdorff",
+            "lcss": "lcss"}
+
+
+def is_heuristic(fn_name: str) -> bool:
+    """utilities/method_helper.py:27-29."""
+    heurs = ['dtw', 'erp', 'frechet', 'hausdorff', 'edr', 'lcss', 'sar', 'cdds']
+    return any(h in fn_name for h in heurs)
+
+
+def heuristic_fn_wrapper(fn_name: str, eps: float = DEFAULT_EPS, threads_per_traj: int = 64):
+    """Name -> batch function with the reference's bound arguments
+    (method_helper.py:32-73): eps for edr/lcss, threads_per_traj for all.
+    The ``*_cpu`` names (traj_dist) and the spatio-temporal / symbolic
+    measures are not part of this engine."""
+    fn = _GPU_FNS.get(fn_name)
+    if fn is None:
+        raise NotImplementedError(f"{fn_name!r} is not served by this engine (GPU measures: {sorted(_GPU_FNS)})")
+    if fn_name in ("edr", "lcss"):
+        fn = partial(fn, eps=eps)
+    return partial(fn, threads_per_traj=threads_per_traj)
+
+
+def heuristic_colllate_fn(batch, is_pad: bool = True, quantize_fp32: bool = False):
+    """method_helper.py:142-153: a list of (traj, traj2) -> padded
+    (t1, len1, t2, len2).  The reference pads through float32 tensors
+    (tool_funcs.py:230-234), rounding coordinates to FP32; quantize_fp32=True
+    reproduces that, the default keeps FP64."""
+    lst_trajs, lst_trajs2 = zip(*batch)
+    len1 = [len(t) for t in lst_trajs]
+    len2 = [len(t) for t in lst_trajs2]
+    if not is_pad:
+        return lst_trajs, len1, lst_trajs2, len2
+    padded = core.pad_lists_to_array(list(lst_trajs) + list(lst_trajs2))
+    if quantize_fp32:
+        padded = padded.astype(np.float32).astype(np.float64)
+    return np.copy(padded[:len(len1)]), len1, np.copy(padded[len(len1):]), len2
+
+
+def _opts(method: str, eps, gpoint, lanes, precision):
+    if method not in _MEASURE:
+        raise NotImplementedError(f"{method!r} is not served by this engine")
+    return dict(eps=eps if method in ("edr", "lcss") else 0.0, gpoint=gpoint if method == "erp" else None,
+                lanes=lanes, precision=precision)
+
+
+def _device_set(ts: TrajSet):
+    import torch
+
+    from .engine import DeviceTrajSet
+    if not torch.cuda.is_available():
+        raise RuntimeError("the engine needs a HIP device")
+    return DeviceTrajSet(ts)
+
+
+def pairwise(method: str, pairs, *, eps: float = DEFAULT_EPS, gpoint=None, lanes: int = 0, precision="fp32"):
+    """Values of ``method`` for a list of (traj, traj2) pairs
+    (test_trajsimi_time.py:222-276).  Returns (list of floats,
+    [running, collate, compute] seconds)."""
+    import torch
+
+    from . import engine
+    t0 = time.time()
+    pairs = list(pairs)
+    if not pairs:
+        return [], [0.0, 0.0, 0.0]
+    a, b = zip(*pairs)
+    ts = pack_list(list(a) + list(b))
+    n = len(pairs)
+    dset = _device_set(ts)
+    collate = time.time() - t0
+    t1 = time.time()
+    pa = np.arange(n, dtype=np.int32)
+    out = engine.pairs(method, dset, pa, pa + n, validate=False, **_opts(method, eps, gpoint, lanes, precision))
+    torch.cuda.synchronize()
+    dists = out.cpu().numpy().tolist()
+    comp = time.time() - t1
+    return dists, [time.time() - t0, collate, comp]
+
+
+def query_db(method: str, Q, D, *, eps: float = DEFAULT_EPS, gpoint=None, lanes: int = 0, precision="fp32"):
+    """[|Q|, |D|] matrix of ``method`` between every query and every database
+    trajectory (test_trajsimi_time_DQ.py:136-181, whose flattened order is
+    query-major).  Returns (float64 ndarray, [running, collate, compute])."""
+    import torch
+
+    from . import engine
+    t0 = time.time()
+    Q, D = list(Q), list(D)
+    ts = pack_list(Q + D)
+    dset = _device_set(ts)
+    collate = time.time() - t0
+    t1 = time.time()
+    out = engine.grid(method, dset, 0, len(Q), len(Q), len(D), **_opts(method, eps, gpoint, lanes, precision))
+    torch.cuda.synchronize()
+    mat = out.cpu().numpy()
+    comp = time.time() - t1
+    return mat, [time.time() - t0, collate, comp]
+
+
+def symmetric_from_triu(vals: np.ndarray, n: int) -> np.ndarray:
+    """Row-major strict upper triangle (itertools.combinations order) ->
+    symmetric [n, n] matrix with a zero diagonal (test_clustering_time.py:256-269)."""
+    vals = np.asarray(vals, dtype=np.float64)
+    if vals.size != n * (n - 1) // 2:
+        raise ValueError("triangle size does not match n")
+    m = np.zeros((n, n), dtype=np.float64)
+    iu = np.triu_indices(n, 1)
+    m[iu] = vals
+    return m + m.T
+
+
+def all_pairs(method: str, trajs, *, eps: float = DEFAULT_EPS, gpoint=None, lanes: int = 0, precision="fp32"):
+    """Symmetric all-pairs matrix over ``trajs`` computed on the upper
+    triangle only (test_clustering_time.py:208-269).  Returns (float64
+    [n, n], [running, collate, compute])."""
+    import torch
+
+    from . import engine
+    t0 = time.time()
+    ts = trajs if isinstance(trajs, TrajSet) else pack_list(list(trajs))
+    n = ts.n
+    dset = _device_set(ts)
+    collate = time.time() - t0
+    t1 = time.time()
+    tri = engine.triu(method, dset, n, **_opts(method, eps, gpoint, lanes, precision))
+    torch.cuda.synchronize()
+    vals = tri.cpu().numpy()
+    comp = time.time() - t1
+    return symmetric_from_triu(vals, n), [time.time() - t0, collate, comp]
+
+
+def method_names():
+    return sorted(_GPU_FNS)
+
+
+def measure_id(name: str) -> int:
+    return _lib.measure_id(_MEASURE.get(name, name))
