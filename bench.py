@@ -64,6 +64,8 @@ def main():
                     help="measures reported under 'measures' (dtw is always the headline)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work at most")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run one plain 256 MiB device copy (calibrates HBM byte counters under rocprofv3)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -87,6 +89,12 @@ def main():
           flush=True)
 
     dev = torch.device("cuda:0")
+    if args.copy_probe:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     dset = engine.DeviceTrajSet(ts, device=dev)
     pa_d = torch.as_tensor(pa, dtype=torch.int32, device=dev)
     pb_d = torch.as_tensor(pb, dtype=torch.int32, device=dev)

@@ -73,7 +73,7 @@ typedef struct tsm_params {
     double eps;     /* EDR/LCSS threshold (core/edr.py:55 `eps`) */
     int eps_mode;   /* TSM_EPS_SQRT (default) or TSM_EPS_SQUARED */
     double gx, gy;  /* ERP gap point g (core/erp.py:94 `gpoint`, default (0,0) erp.py:103) */
-    int lanes;      /* lanes per pair: 0 = autotuned, else 4, 8 or 16 (threads_per_traj hint) */
+    int lanes;      /* lanes per pair: 0 = autotuned per column count, else 4 or 8 (threads_per_traj hint) */
     int precision;  /* TSM_PREC_FP32 or TSM_PREC_FP64 */
 } tsm_params;
 
@@ -123,6 +123,16 @@ int tsm_pairs_triu(int measure, const tsm_trajset *set, int64_t n, const tsm_par
 int tsm_pairs_host(int measure, const double *xy64, const int64_t *offsets, int64_t n, int64_t npts,
                    const int32_t *pa, const int32_t *pb, int64_t npairs, const tsm_params *prm,
                    double *out, int32_t *counts);
+
+/*
+ * Autotuner hook: lanes per pair (4 or 8) for each column count 1..n-1
+ * (entry 0 unused) of the DP kernels with FP32 (coord_bits 32) or FP64
+ * (coord_bits 64) coordinates; NULL restores the built-in cost model.  The
+ * Python autotuner (trajsimimeasures_amd/autotune.py) measures and sets it.
+ * No reference counterpart (the reference fixes 64 threads per pair,
+ * config.py:45).
+ */
+int tsm_tune_set_lanes(int coord_bits, const uint8_t *lanes_of_ncol, int n);
 
 /* Kernel timing of the DP / Hausdorff kernels (HIP events on the launch
  * stream), for roofline reporting.  No reference counterpart. */

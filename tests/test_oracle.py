@@ -168,3 +168,25 @@ def test_pairs_matches_single(rng):
             assert vals[k] == ref
             if m in ("edr", "lcss"):
                 assert cnt[k] == O.count(m, ts.traj(pa[k]), ts.traj(pb[k]), 0.01)
+
+
+def _edr_short_rule(a, b, thr):
+    """EDR count with the update D = diag if match else min3(diag, up, left) + 1,
+    which the engine uses; equal to the reference recurrence because unit-cost
+    edit distances satisfy diag <= min(up, left) + 1."""
+    n1, n2 = len(a), len(b)
+    D = np.zeros((n1 + 1, n2 + 1), dtype=np.int64)
+    for i in range(1, n1 + 1):
+        for j in range(1, n2 + 1):
+            d2 = (a[i - 1, 0] - b[j - 1, 0]) ** 2 + (a[i - 1, 1] - b[j - 1, 1]) ** 2
+            D[i, j] = D[i - 1, j - 1] if d2 < thr else min(D[i - 1, j - 1], D[i - 1, j], D[i, j - 1]) + 1
+    return int(D[n1, n2])
+
+
+def test_edr_short_update_rule(rng):
+    for _ in range(300):
+        n1, n2 = rng.integers(1, 12, 2)
+        a, b = rnd(rng, n1), rnd(rng, n2)
+        eps = rng.random() * 0.6
+        thr = O.eps_threshold(eps)
+        assert _edr_short_rule(a, b, thr) == O.count("edr", a, b, eps)

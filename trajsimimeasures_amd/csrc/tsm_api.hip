@@ -9,6 +9,8 @@
 //   4. one persistent DP kernel (tsm_dp.hpp), or the Hausdorff kernel.
 // Everything is stream-ordered on the caller's stream; the only host sync
 // is a workspace grow.
+#include <cstring>
+
 #include "tsm_hausdorff.hpp"
 #include "tsm_host.hpp"
 
@@ -19,23 +21,28 @@ __global__ void xy_to_f32_kernel(const double *in, float *out, int64_t n) {
         out[i] = (float)in[i];
 }
 
-// ERP gap distances |p - g| per point (core/erp.py:32-40), FP64 and FP32.
-__global__ void erp_gap_kernel(const double *xy, int64_t npts, double gx, double gy, double *g64, float *g32) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npts; i += (int64_t)gridDim.x * blockDim.x) {
-        double d = sqrt(dist2_f64(gx, gy, xy[2 * i], xy[2 * i + 1]));
-        g64[i] = d;
-        g32[i] = (float)d;
-    }
-}
-
-// Per-trajectory sums of gap distances, accumulated serially in FP64 in point
-// order like the reference (core/erp.py:44-49).
-__global__ void erp_gapsum_kernel(const int64_t *offs, int64_t n, const double *g64, double *s64, float *s32) {
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-        double s = 0.0;
-        for (int64_t i = offs[t]; i < offs[t + 1]; ++i) s += g64[i];
-        s64[t] = s;
-        s32[t] = (float)s;
+// ERP gap distances |p - g| per point (core/erp.py:32-40) in FP64 and FP32,
+// and their per-trajectory sums (core/erp.py:44-49): one wavefront per
+// trajectory, lanes over its points, an FP64 wave reduction for the sum.
+__global__ __launch_bounds__(256) void erp_gap_kernel(const double *xy, const int64_t *offs, int64_t n, double gx,
+                                                      double gy, double *g64, float *g32, double *s64, float *s32) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t waves = (int64_t)gridDim.x * (256 / kWave);
+    for (int64_t t = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kWave; t < n; t += waves) {
+        const int64_t o0 = offs[t], o1 = offs[t + 1];
+        double acc = 0.0;
+        for (int64_t i = o0 + lane; i < o1; i += kWave) {
+            const double d = sqrt(dist2_f64(gx, gy, xy[2 * i], xy[2 * i + 1]));
+            g64[i] = d;
+            g32[i] = (float)d;
+            acc += d;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+        if (lane == 0) {
+            s64[t] = acc;
+            s32[t] = (float)acc;
+        }
     }
 }
 
@@ -58,10 +65,10 @@ using namespace tsm;
 // expected column count.  Waste per pair is (L - 1) skew steps over nrow rows
 // plus column padding up to a multiple of L; both shrink with smaller L,
 // while register columns (and so occupancy) grow.
-int choose_lanes(const tsm_params &prm, int64_t max_len) {
-    if (prm.lanes == 4 || prm.lanes == 8 || prm.lanes == 16) return prm.lanes;
-    (void)max_len;
-    return 16;
+int choose_lanes(const tsm_params &prm) {
+    if (prm.lanes == 4 || prm.lanes == 8) return prm.lanes;
+    if (prm.lanes == 16) return 8;
+    return 0;
 }
 
 int erp_prologue(const tsm_trajset &set, double gx, double gy, hipStream_t st) {
@@ -71,10 +78,9 @@ int erp_prologue(const tsm_trajset &set, double gx, double gy, hipStream_t st) {
     if ((rc = c.gap64.ensure(npts * 8 + 8)) || (rc = c.gap32.ensure(npts * 4 + 8)) ||
         (rc = c.gsum64.ensure(set.n * 8 + 8)) || (rc = c.gsum32.ensure(set.n * 4 + 8)))
         return rc;
-    hipLaunchKernelGGL(erp_gap_kernel, dim3(grid_for(npts, 256)), dim3(256), 0, st, set.xy64, npts, gx, gy,
-                       c.gap64.as<double>(), c.gap32.as<float>());
-    hipLaunchKernelGGL(erp_gapsum_kernel, dim3(grid_for(set.n, 256)), dim3(256), 0, st, set.offsets, set.n,
-                       c.gap64.as<double>(), c.gsum64.as<double>(), c.gsum32.as<float>());
+    hipLaunchKernelGGL(erp_gap_kernel, dim3(grid_for(set.n * kWave, 256)), dim3(256), 0, st, set.xy64, set.offsets,
+                       set.n, gx, gy, c.gap64.as<double>(), c.gap32.as<float>(), c.gsum64.as<double>(),
+                       c.gsum32.as<float>());
     TSM_HIP(hipGetLastError());
     return TSM_OK;
 }
@@ -117,6 +123,16 @@ int run(int measure, const tsm_trajset *set, const PairSpace &ps, int64_t npairs
         h.ps = ps;
         h.npairs = npairs;
         h.out = out;
+        h.seg = nullptr;
+        h.rseg = nullptr;
+        if (!f64) {
+            if ((rc = g_ctx.seg.ensure(set->npts * sizeof(float4))) || (rc = g_ctx.rseg.ensure(set->npts * sizeof(float))))
+                return rc;
+            h.seg = g_ctx.seg.as<float4>();
+            h.rseg = g_ctx.rseg.as<float>();
+            hipLaunchKernelGGL(seg_prep_kernel, dim3(grid_for(set->npts, 256)), dim3(256), 0, st, set->xy32, set->npts,
+                               g_ctx.seg.as<float4>(), g_ctx.rseg.as<float>());
+        }
         int grid = (int)(npairs < 1048576 ? npairs : 1048576);
         ProfScope p_(st);
         if (f64) hipLaunchKernelGGL(hausdorff_kernel<double>, dim3(grid), dim3(kWave), 0, st, h);
@@ -127,7 +143,7 @@ int run(int measure, const tsm_trajset *set, const PairSpace &ps, int64_t npairs
     if (measure == TSM_ERP && (rc = erp_prologue(*set, prm.gx, prm.gy, st))) return rc;
 
     const double thr = eps_threshold(prm.eps, prm.eps_mode);
-    const int L = choose_lanes(prm, set->max_len);
+    const int L = choose_lanes(prm);
     switch (measure) {
     case TSM_DTW:
         return f64 ? run_dp<kDTW, true>(L, *set, ps, npairs, thr, out, counts, st)
@@ -255,6 +271,23 @@ int tsm_pairs_host(int measure, const double *xy64, const int64_t *offsets, int6
     if (rc) return rc;
     TSM_HIP(hipMemcpy(out, c.h_out.p, npairs * 8, hipMemcpyDeviceToHost));
     if (counts) TSM_HIP(hipMemcpy(counts, c.h_cnt.p, npairs * 4, hipMemcpyDeviceToHost));
+    return TSM_OK;
+}
+
+int tsm_tune_set_lanes(int coord_bits, const uint8_t *lanes_of_ncol, int n) {
+    if (coord_bits != 32 && coord_bits != 64) return fail(TSM_E_ARG, "coord_bits must be 32 or 64");
+    const int k = coord_bits == 64 ? 1 : 0;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!lanes_of_ncol) {
+        g_ctx.tuned[k] = false;
+        return TSM_OK;
+    }
+    if (n < 2 || n > 257) return fail(TSM_E_ARG, "lanes table must cover column counts 1..n-1 with 2 <= n <= 257");
+    for (int i = 1; i < n; ++i)
+        if (lanes_of_ncol[i] != 4 && lanes_of_ncol[i] != 8) return fail(TSM_E_ARG, "lanes must be 4 or 8");
+    std::memset(g_ctx.tuned_lanes[k], 0, sizeof(g_ctx.tuned_lanes[k]));
+    std::memcpy(g_ctx.tuned_lanes[k], lanes_of_ncol, (size_t)n);
+    g_ctx.tuned[k] = true;
     return TSM_OK;
 }
 

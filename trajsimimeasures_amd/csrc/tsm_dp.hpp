@@ -12,20 +12,24 @@
 // longer trajectory on the rows (streamed) and the shorter on the columns
 // (held in registers).
 //
-// Layout of the work: a "segment" of L lanes (L = 4, 8 or 16; 64/L segments
-// per wavefront) owns one pair.  Lane j of a segment holds W consecutive
-// columns (their points in VGPRs and the previous row of D for them) and
-// sweeps the rows in skewed order: at step s it computes row s - j.  The
-// value of D at the lane's left edge comes from lane j-1, computed one step
-// earlier, through one DPP row_shr:1 move; the diagonal is the previous
-// step's left value.  No DP matrix touches memory.  Pairs wider than L*W
-// columns are swept in column strips; the last lane of a strip writes its
-// right-edge column to a scratch vector which the next strip's first lane
-// reads back.
+// Layout of the work: a "segment" of L lanes (L = 4 or 8, so 16 or 8
+// segments per wavefront) owns one pair.  Lane j of a segment holds W
+// consecutive columns -- their points in VGPRs and the previous row of D for
+// them -- and sweeps the rows in skewed order: at step s it computes row
+// s - j.  The value of D at the lane's left edge comes from lane j-1,
+// computed one step earlier, through one DPP row_shr:1 move; the diagonal is
+// the previous step's left value.  No DP matrix touches memory.  The row
+// state is double-buffered across an unrolled pair of steps (even steps read
+// buffer A and write B, odd steps the reverse), so a row update needs no
+// register moves.  Pairs wider than L*W columns are swept in column strips;
+// the last lane of a strip writes its right-edge column to a scratch vector
+// which the next strip's first lane reads back.
 //
-// Scheduling: a planner sorts pairs by (W bucket, rows) with a counting sort,
-// packs them 64/L to a task (all segments of a task share W), and a
-// persistent kernel pulls tasks from an atomic counter, largest first.
+// Scheduling: every pair falls in a bucket b = (L, W) chosen per column
+// count by a table (cost model or measured autotuning); a planner sorts
+// pairs by (bucket, rows) with a counting sort and packs them 64/L to a task,
+// so all segments of a task share (L, W) and have similar row counts; one
+// persistent kernel pulls tasks from an atomic counter, largest W first.
 #pragma once
 
 #include <type_traits>
@@ -34,13 +38,30 @@
 
 namespace tsm {
 
-constexpr int kRowKeys = 1024;  // rows are bucketed as min(nrow, kRowKeys - 1) for the sort
 
-// Maximum register columns per lane for a segment width L.
-template <int L> struct WMax;
-template <> struct WMax<4> { static constexpr int value = 32; };
-template <> struct WMax<8> { static constexpr int value = 24; };
-template <> struct WMax<16> { static constexpr int value = 16; };
+constexpr int kRowKeys = 128;   // row bins per bucket: min(nrow, 511) / 4
+constexpr int kMaxSeg = 16;     // segments per wave at L = 4
+constexpr int kTabMax = 257;    // bucket table covers column counts 1..256
+
+// Register columns per lane: 24 with FP32 coordinates (TSM_WMAX32; measured
+// faster than 32, whose ~190 VGPRs halve occupancy) and 24 with FP64 ones.
+#ifndef TSM_WMAX32
+#define TSM_WMAX32 24
+#endif
+template <typename CT> struct WMaxOf { static constexpr int value = sizeof(CT) == 4 ? TSM_WMAX32 : 24; };
+
+// Buckets: b = 2 * (WM - W) + (L == 4 ? 1 : 0), i.e. W descending, L = 8 first.
+__host__ __device__ constexpr int bucket_of(int WM, int L, int W) { return 2 * (WM - W) + (L == 4 ? 1 : 0); }
+__host__ __device__ constexpr int bucket_L(int b) { return (b & 1) ? 4 : 8; }
+__host__ __device__ constexpr int bucket_W(int WM, int b) { return WM - b / 2; }
+
+// Column count -> bucket table, plus the bucket of pairs wider than the
+// table (swept in strips of L = 8, W = WM).
+struct BucketTable {
+    uint8_t b[kTabMax];  // b[ncol] for 1 <= ncol < kTabMax
+    int ncmax;           // largest column count the table covers without strips
+    int wide;            // bucket for ncol > ncmax
+};
 
 struct DPArgs {
     const float *xy32;
@@ -51,11 +72,11 @@ struct DPArgs {
     const float *gsum32;     // ERP: per-trajectory sum of gap distances
     const double *gsum64;
     PairSpace ps;
-    const uint32_t *slots;   // task-major pair slots, kEmptySlot = idle segment
-    const int64_t *wend;     // wend[k]: end of the task range of bucket W = WMAX - k
-    const int64_t *ntasks;   // device scalar
+    const uint32_t *slots;   // pair slots grouped by task, kEmptySlot = idle segment
+    const int64_t *meta;     // [0] ntasks, [1 + b] task end of bucket b, [1 + nb + b] slot base of bucket b
+    int nb;                  // number of buckets
     unsigned int *counter;   // task queue head
-    void *bnd;               // strip boundary scratch (AT), per wave and segment
+    void *bnd;               // strip boundary scratch (AT), kMaxSeg segments per wave
     int64_t bnd_stride;
     double thr;              // EDR/LCSS squared-distance threshold
     double *out;
@@ -119,9 +140,12 @@ template <bool F64> struct Rec<kEDR, F64> {
     static constexpr bool kGap = false;
     __device__ static AT top(AT) { return 0.0f; }
     __device__ static AT left0(AT) { return 0.0f; }
+    // min(diag + sub, up + 1, left + 1) with sub in {0, 1} equals
+    // (match ? diag : min3(diag, up, left) + 1), since unit-cost edit
+    // distances satisfy diag <= min(up, left) + 1.
     __device__ static AT cell(CT ax, CT ay, AT, CT bx, CT by, AT, AT diag, AT up, AT left, double thr) {
-        AT sub = dist2_f64(ax, ay, bx, by) < thr ? 0.0f : 1.0f;
-        return fminf(diag + sub, fminf(up, left) + 1.0f);
+        const AT v = min3f(diag, up, left) + 1.0f;
+        return dist2_f64(ax, ay, bx, by) < thr ? diag : v;
     }
     __device__ static double finish(AT v, int64_t na, int64_t nb, int32_t *cnt) {
         int32_t c = (int32_t)v;
@@ -167,19 +191,78 @@ template <typename AT> __device__ __forceinline__ AT gsum_at(const DPArgs &a, in
 template <> __device__ __forceinline__ float gsum_at<float>(const DPArgs &a, int64_t t) { return a.gsum32[t]; }
 template <> __device__ __forceinline__ double gsum_at<double>(const DPArgs &a, int64_t t) { return a.gsum64[t]; }
 
-// One task: 64/L segments, each sweeping one pair with W columns per lane.
-template <int M, bool F64, int L, int W>
-__device__ __forceinline__ void run_task(const DPArgs &a, int64_t task, int64_t wave_id) {
+// Lane shift by one within rows of 16 lanes; out-of-row sources read 0.
+__device__ __forceinline__ float shl_in(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x111, 0xF, 0xF, true));
+}
+__device__ __forceinline__ double shl_in(double v) { return shr1(v); }
+
+// Per-lane state of one strip sweep.
+template <typename CT, typename AT> struct RowPt {
+    CT x, y;
+    AT g;
+};
+
+// One step of the sweep: row r = s - j of this lane, reading the previous
+// row from `src` and writing the new one to `dst`.  `cur` holds the point of
+// row r, loaded one step earlier; `nxt` receives the point of the lane's
+// next row.  (Loading two steps ahead measured slower.)
+template <int M, bool F64, int W, bool kBnd>
+__device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t nrow_s, int rlast, int64_t orow,
+                                        bool lane0, typename Rec<M, F64>::AT left_fixed,
+                                        typename Rec<M, F64>::AT *bnd, bool write_bnd,
+                                        const typename Rec<M, F64>::CT (&bx)[W],
+                                        const typename Rec<M, F64>::CT (&by)[W],
+                                        const typename Rec<M, F64>::AT (&gb)[W],
+                                        const typename Rec<M, F64>::AT (&src)[W], typename Rec<M, F64>::AT (&dst)[W],
+                                        const RowPt<typename Rec<M, F64>::CT, typename Rec<M, F64>::AT> &cur,
+                                        RowPt<typename Rec<M, F64>::CT, typename Rec<M, F64>::AT> &nxt,
+                                        typename Rec<M, F64>::AT &lp_in, typename Rec<M, F64>::AT &lp_out,
+                                        typename Rec<M, F64>::AT cur_bl, typename Rec<M, F64>::AT &nxt_bl) {
     using R = Rec<M, F64>;
     using CT = typename R::CT;
     using AT = typename R::AT;
-    constexpr int S = kWave / L;
+    const AT recv = shl_in(src[W - 1]);  // lane j-1's last column of row s - j
+    // next row of this lane, kept inside the trajectory: a lane whose next
+    // row is out of range does not use the point, so any row will do
+    const uint32_t rn = min((uint32_t)(s + 1 - j), (uint32_t)rlast);
+    Pt<CT>::load(a, orow + rn, nxt.x, nxt.y);
+    if constexpr (R::kGap) nxt.g = gap_at<AT>(a, orow + rn);
+    if constexpr (kBnd) {
+        if (lane0) nxt_bl = bnd[rn];
+    }
+    const int r = s - j;
+    if ((uint32_t)r < nrow_s) {
+        AT left_in;
+        if constexpr (kBnd) left_in = lane0 ? cur_bl : recv;
+        else left_in = lane0 ? left_fixed : recv;
+        AT diag = lp_in;  // D[r][col0]: the corner value before the first row
+        lp_out = left_in;
+        AT lf = left_in;
+#pragma unroll
+        for (int c = 0; c < W; ++c) {
+            const AT v = R::cell(cur.x, cur.y, cur.g, bx[c], by[c], gb[c], diag, src[c], lf, a.thr);
+            diag = src[c];
+            dst[c] = v;
+            lf = v;
+        }
+        if (write_bnd) bnd[r] = lf;
+    }
+}
+
+// One task: 64/L segments, each sweeping one pair with W columns per lane.
+template <int M, bool F64, int L, int W>
+__device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t wave_id) {
+    using R = Rec<M, F64>;
+    using CT = typename R::CT;
+    using AT = typename R::AT;
     constexpr int kSpan = L * W;
     const int lane = threadIdx.x & (kWave - 1);
     const int seg = lane / L;
     const int j = lane % L;
+    const bool lane0 = j == 0;
 
-    const uint32_t slot = a.slots[task * S + seg];
+    const uint32_t slot = a.slots[slot0 + seg];
     const bool valid = slot != kEmptySlot;
     int64_t ia = 0, ib = 0;
     if (valid) decode_pair(a.ps, (int64_t)slot, ia, ib);
@@ -202,19 +285,19 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t task, int64_t 
     // a pair needing strips longer than the scratch column (max_len was
     // understated by the caller) is not computed; its result is NaN
     const bool fits = (ncol <= kSpan) || (nrow <= a.bnd_stride);
-    if (valid && !fits && j == 0) a.out[slot] = __builtin_nan("");
+    if (valid && !fits && lane0) a.out[slot] = __builtin_bit_cast(double, 0x7ff8000000000000ull);  // quiet NaN
     const int nstrips = (valid && fits) ? (ncol + kSpan - 1) / kSpan : 0;
-    const int maxrow = __builtin_amdgcn_readfirstlane(wave_max(nrow));
+    const int maxrow = __builtin_amdgcn_readfirstlane(wave_max(nstrips > 0 ? nrow : 0));
     const int maxstrips = __builtin_amdgcn_readfirstlane(wave_max(nstrips));
     const int nsteps = maxrow + L - 1;
-    AT *bnd = reinterpret_cast<AT *>(a.bnd) + (wave_id * S + seg) * a.bnd_stride;
+    AT *bnd = reinterpret_cast<AT *>(a.bnd) + (wave_id * kMaxSeg + seg) * a.bnd_stride;
     const int rlast = nrow > 0 ? nrow - 1 : 0;
 
     for (int strip = 0; strip < maxstrips; ++strip) {
         const bool in_strip = strip < nstrips;
         const int col0 = strip * kSpan + j * W;  // first column (0-based) of this lane
         CT bx[W], by[W];
-        AT gb[W], prev[W];
+        AT gb[W], A[W], B[W];
 #pragma unroll
         for (int c = 0; c < W; ++c) {
             int col = col0 + c;
@@ -222,65 +305,49 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t task, int64_t 
             Pt<CT>::load(a, ocol + col, bx[c], by[c]);
             if constexpr (R::kGap) gb[c] = gap_at<AT>(a, ocol + col);
             else gb[c] = 0;
-            prev[c] = topv;
+            A[c] = topv;
+            B[c] = topv;
         }
-        const AT corner0 = (col0 == 0) ? (AT)0 : topv;  // D[0][col0]
-        AT left_prev = corner0;
-        AT last = 0;
-        const bool read_bnd = (j == 0) && strip > 0;
+        AT lpA = (col0 == 0) ? (AT)0 : topv;  // D[0][col0]
+        AT lpB = lpA;
         const bool write_bnd = (j == L - 1) && (strip + 1 < nstrips);
-
-        const uint32_t nrow_s = in_strip ? (uint32_t)nrow : 0u;  // rows this lane sweeps in this strip
-        const AT left_fixed = strip == 0 ? leftv : (AT)0;           // D[i][0] for the first lane of strip 0
-        // a_r prefetch, one step ahead; indices clamped into the row trajectory
-        const int r0 = min(max(-j, 0), rlast);
-        CT ax, ay;
-        Pt<CT>::load(a, orow + r0, ax, ay);
-        AT ga = 0;
-        if constexpr (R::kGap) ga = gap_at<AT>(a, orow + r0);
-        AT bl = 0;
-        if (read_bnd) bl = bnd[r0];
-
-        for (int s = 0; s < nsteps; ++s) {
-            const AT recv = shr1(last);
-            const int rn = min(max(s + 1 - j, 0), rlast);
-            CT nx, ny;
-            Pt<CT>::load(a, orow + rn, nx, ny);
-            AT nga = 0;
-            if constexpr (R::kGap) nga = gap_at<AT>(a, orow + rn);
-            AT nbl = 0;
-            if (read_bnd) nbl = bnd[rn];
-
-            const int r = s - j;
-            if ((uint32_t)r < nrow_s) {
-                const AT left_in = (j == 0) ? (read_bnd ? bl : left_fixed) : recv;
-                AT diag = left_prev;  // D[r][col0]: corner0 before the first row
-                left_prev = left_in;
-                AT lf = left_in;
-#pragma unroll
-                for (int c = 0; c < W; ++c) {
-                    const AT up = prev[c];
-                    const AT v = R::cell(ax, ay, ga, bx[c], by[c], gb[c], diag, up, lf, a.thr);
-                    diag = up;
-                    prev[c] = v;
-                    lf = v;
-                }
-                last = lf;
-                if (write_bnd) bnd[r] = lf;
+        const uint32_t nrow_s = in_strip ? (uint32_t)nrow : 0u;
+        RowPt<CT, AT> p0, p1;  // points of the lane's rows at even / odd steps
+        const uint32_t r0 = min((uint32_t)(-j), (uint32_t)rlast);
+        Pt<CT>::load(a, orow + r0, p0.x, p0.y);
+        p0.g = 0;
+        p1.g = 0;
+        if constexpr (R::kGap) p0.g = gap_at<AT>(a, orow + r0);
+        AT bl0 = 0, bl1 = 0;
+        if (strip == 0) {
+            for (int s = 0; s < nsteps; s += 2) {
+                dp_step<M, F64, W, false>(a, s, j, nrow_s, rlast, orow, lane0, leftv, bnd, write_bnd, bx, by, gb, A,
+                                          B, p0, p1, lpA, lpB, bl0, bl1);
+                if (s + 1 < nsteps)
+                    dp_step<M, F64, W, false>(a, s + 1, j, nrow_s, rlast, orow, lane0, leftv, bnd, write_bnd, bx, by,
+                                              gb, B, A, p1, p0, lpB, lpA, bl1, bl0);
             }
-            ax = nx;
-            ay = ny;
-            ga = nga;
-            bl = nbl;
+        } else {
+            if (lane0) bl0 = bnd[r0];
+            for (int s = 0; s < nsteps; s += 2) {
+                dp_step<M, F64, W, true>(a, s, j, nrow_s, rlast, orow, lane0, leftv, bnd, write_bnd, bx, by, gb, A,
+                                         B, p0, p1, lpA, lpB, bl0, bl1);
+                if (s + 1 < nsteps)
+                    dp_step<M, F64, W, true>(a, s + 1, j, nrow_s, rlast, orow, lane0, leftv, bnd, write_bnd, bx, by,
+                                             gb, B, A, p1, p0, lpB, lpA, bl1, bl0);
+            }
         }
 
         if (in_strip && strip == nstrips - 1) {
             const int cend = ncol - 1 - strip * kSpan;  // last column within this strip
             if (j == cend / W) {
                 const int cr = cend % W;
-                AT v = prev[0];
+                // the lane's last row (nrow - 1) ran at step nrow - 1 + j:
+                // even steps write B, odd steps write A
+                const bool inB = ((nrow - 1 + j) & 1) == 0;
+                AT v = inB ? B[0] : A[0];
 #pragma unroll
-                for (int c = 1; c < W; ++c) v = (c == cr) ? prev[c] : v;
+                for (int c = 1; c < W; ++c) v = (c == cr) ? (inB ? B[c] : A[c]) : v;
                 int32_t cnt = 0;
                 double res = R::finish(v, na, nb, &cnt);
                 a.out[slot] = res;
@@ -291,24 +358,22 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t task, int64_t 
     }
 }
 
-template <int M, bool F64, int L, int W>
-__device__ __forceinline__ bool dispatch_w(const DPArgs &a, int64_t task, int64_t wave_id, int w) {
-    if constexpr (W == 0) {
-        return false;
-    } else {
-        if (w == W) {
-            run_task<M, F64, L, W>(a, task, wave_id);
-            return true;
+template <int M, bool F64, int WM, int b>
+__device__ __forceinline__ void dispatch_bucket(const DPArgs &a, int bk, int64_t slot0, int64_t wave_id) {
+    if constexpr (b < 2 * WM) {
+        if (bk == b) {
+            run_task<M, F64, bucket_L(b), bucket_W(WM, b)>(a, slot0, wave_id);
+            return;
         }
-        return dispatch_w<M, F64, L, W - 1>(a, task, wave_id, w);
+        dispatch_bucket<M, F64, WM, b + 1>(a, bk, slot0, wave_id);
     }
 }
 
 // Persistent kernel: each wavefront pulls tasks until the queue is drained.
-template <int M, bool F64, int L>
+template <int M, bool F64>
 __global__ __launch_bounds__(kWave) void dp_kernel(DPArgs a) {
-    constexpr int WM = WMax<L>::value;
-    const int64_t ntasks = *a.ntasks;
+    constexpr int WM = WMaxOf<typename Rec<M, F64>::CT>::value;
+    const int64_t ntasks = a.meta[0];
     const int64_t wave_id = blockIdx.x;
     const int lane = threadIdx.x;
     while (true) {
@@ -316,16 +381,17 @@ __global__ __launch_bounds__(kWave) void dp_kernel(DPArgs a) {
         if (lane == 0) t = atomicAdd(a.counter, 1u);
         t = __builtin_amdgcn_readfirstlane(t);
         if ((int64_t)t >= ntasks) break;
-        int w = WM;
-#pragma unroll 1
-        for (int k = 0; k < WM; ++k) {
-            if ((int64_t)t < a.wend[k]) {
-                w = WM - k;
-                break;
-            }
+        // bucket of task t: first b with task_end[b] > t (binary search)
+        int lo = 0, hi = a.nb - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (a.meta[1 + mid] > (int64_t)t) hi = mid;
+            else lo = mid + 1;
         }
-        w = __builtin_amdgcn_readfirstlane(w);
-        dispatch_w<M, F64, L, WM>(a, (int64_t)t, wave_id, w);
+        const int bk = __builtin_amdgcn_readfirstlane(lo);
+        const int64_t tstart = bk > 0 ? a.meta[bk] : 0;
+        const int64_t slot0 = a.meta[1 + a.nb + bk] + ((int64_t)t - tstart) * (kWave / bucket_L(bk));
+        dispatch_bucket<M, F64, WM, 0>(a, bk, slot0, wave_id);
     }
 }
 

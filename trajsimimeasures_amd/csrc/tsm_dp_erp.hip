@@ -1,7 +1,6 @@
-// tsm_dp_erp.hip -- ERP instances of the DP kernel and its launcher.
+// tsm_dp_erp.hip -- ERP instances of the DP kernel and its launcher, FP32 DP.
 #include "tsm_launch.hpp"
 
 namespace tsm {
 template int run_dp<kERP, false>(int, const tsm_trajset &, const PairSpace &, int64_t, double, double *, int32_t *, hipStream_t);
-template int run_dp<kERP, true>(int, const tsm_trajset &, const PairSpace &, int64_t, double, double *, int32_t *, hipStream_t);
 }  // namespace tsm

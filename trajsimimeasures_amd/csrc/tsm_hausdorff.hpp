@@ -11,9 +11,11 @@
 // excluded from the max.
 //
 // Mapping: one wavefront per pair.  Lanes take 64 points of one side at a
-// time; the other side's segments are wave-uniform, so their endpoints come
-// through scalar loads and every lane keeps a running minimum in a VGPR.  A
-// wave max-reduction and one square root finish the pair.
+// time; the other side's segments are wave-uniform, so their endpoints and
+// per-segment constants (dx, dy, 1/|s|^2, computed once per call by
+// seg_prep_kernel) come through scalar loads and every lane keeps a running
+// minimum in a VGPR.  A wave max-reduction and one square root finish the
+// pair.
 #pragma once
 
 #include "tsm_device.hpp"
@@ -23,6 +25,8 @@ namespace tsm {
 struct HArgs {
     const float *xy32;
     const double *xy64;
+    const float4 *seg;  // FP32 path: per point i, segment (p_i, p_i+1): (x_i+1, y_i+1, dx, dy)
+    const float *rseg;  // FP32 path: 1/|s|^2, or 0 for a degenerate segment
     const int64_t *offs;
     PairSpace ps;
     int64_t npairs;
@@ -82,6 +86,69 @@ __device__ __forceinline__ T p2t_sq(T px, T py, const T *__restrict__ Y, int ny)
     return best;
 }
 
+// FP32 form with precomputed segment constants.  G[k] = (x, y) of the
+// segment's end point s2 and (dx, dy) = s2 - s1; R[k] = 1/|s2 - s1|^2, or 0
+// for a degenerate segment, so that u = 0 < 1e-5 selects
+// min(dps1, dps2) = dps1, the reference's value for it.  The projection
+// residual is formed as (p - s1) - u (s2 - s1) with FMAs.  Both arrays are
+// read with wave-uniform addresses (scalar loads).
+struct SegState {
+    float ex, ey, dps1, best;
+};
+
+__device__ __forceinline__ void p2s_step(float px, float py, float4 g, float r, SegState &st) {
+    const float fx = px - g.x, fy = py - g.y;
+    const float dps2 = fmaf(fy, fy, fx * fx);
+    const float u = fmaf(st.ey, g.w, st.ex * g.z) * r;
+    const bool far = __builtin_amdgcn_fmed3f(u, 0.00001f, 1.0f) != u;  // u < 1e-5 or u > 1
+    const float gx = fmaf(-u, g.z, st.ex), gy = fmaf(-u, g.w, st.ey);
+    const float dp = fmaf(gy, gy, gx * gx);
+    const float dist = far ? fminf(st.dps1, dps2) : dp;
+    st.best = fminf(st.best, dist);
+    st.ex = fx;
+    st.ey = fy;
+    st.dps1 = dps2;
+}
+
+__device__ __forceinline__ float p2t_sq_f32(float px, float py, const float *__restrict__ Y,
+                                            const float4 *__restrict__ G, const float *__restrict__ R, int ny) {
+    SegState st;
+    st.ex = px - Y[0];
+    st.ey = py - Y[1];
+    st.dps1 = fmaf(st.ey, st.ey, st.ex * st.ex);
+    st.best = __builtin_inff();
+    const int nseg = ny - 1;
+    int k = 0;
+    // G and R are indexed by the wave-uniform k: read them as scalars
+    const int *Gi = reinterpret_cast<const int *>(G);
+    auto uni = [](int v) { return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(v)); };
+    auto seg_at = [&](int q) { return make_float4(uni(Gi[4 * q]), uni(Gi[4 * q + 1]), uni(Gi[4 * q + 2]), uni(Gi[4 * q + 3])); };
+    for (; k + 2 <= nseg; k += 2) {
+        p2s_step(px, py, seg_at(k), R[k], st);
+        p2s_step(px, py, seg_at(k + 1), R[k + 1], st);
+    }
+    if (k < nseg) p2s_step(px, py, seg_at(k), R[k], st);
+    return st.best;
+}
+
+// Per point i of every trajectory: segment (p_i, p_i+1) as G[i] = (x_i+1,
+// y_i+1, dx, dy) and R[i] = 1/|s|^2 (0 if degenerate); the last point of a
+// trajectory gets zeros (never read).
+__global__ void seg_prep_kernel(const float *xy, int64_t npts, float4 *G, float *R) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npts; i += (int64_t)gridDim.x * blockDim.x) {
+        float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
+        float r = 0.f;
+        if (i + 1 < npts) {
+            const float x1 = xy[2 * i], y1 = xy[2 * i + 1], x2 = xy[2 * i + 2], y2 = xy[2 * i + 3];
+            const float xd = x2 - x1, yd = y2 - y1;
+            g = make_float4(x2, y2, xd, yd);
+            r = (x1 == x2 && y1 == y2) ? 0.f : 1.0f / fmaf(yd, yd, xd * xd);
+        }
+        G[i] = g;
+        R[i] = r;
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kWave) void hausdorff_kernel(HArgs a) {
     const int lane = threadIdx.x;
@@ -103,7 +170,12 @@ __global__ __launch_bounds__(kWave) void hausdorff_kernel(HArgs a) {
                 int i = base + lane;
                 const bool ok = i < nx;
                 i = ok ? i : nx - 1;
-                const T d = p2t_sq<T>(X[2 * i], X[2 * i + 1], Y, ny);
+                T d;
+                if constexpr (sizeof(T) == 4)
+                    d = p2t_sq_f32(X[2 * i], X[2 * i + 1], Y, a.seg + (dir == 0 ? ob : oa),
+                                   a.rseg + (dir == 0 ? ob : oa), ny);
+                else
+                    d = p2t_sq<T>(X[2 * i], X[2 * i + 1], Y, ny);
                 if (ok) m = d > m ? d : m;
             }
         }

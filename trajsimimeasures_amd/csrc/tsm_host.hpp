@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -44,6 +45,8 @@ struct DevBuf {
             return fail(TSM_E_NOMEM, "hipMalloc(" + std::to_string(want) + ") failed: " + hipGetErrorString(e));
         }
         cap = want;
+        e = hipMemset(p, 0, want);  // fresh buffers start zeroed (the plan histogram relies on it)
+        if (e != hipSuccess) return fail(TSM_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
         return TSM_OK;
     }
     template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
@@ -52,10 +55,13 @@ struct DevBuf {
 struct Context {
     int device = -1;
     int num_cu = 0;
-    DevBuf hist, slots, wend, ntasks, counter, bnd;
-    DevBuf gap64, gap32, gsum64, gsum32;
+    DevBuf hist, cursor, slots, meta, counter, bnd;
+    DevBuf gap64, gap32, gsum64, gsum32, seg, rseg;
     // staging for tsm_pairs_host
     DevBuf h_xy64, h_xy32, h_offs, h_pa, h_pb, h_out, h_cnt;
+    // autotuned lanes per column count (index 0: FP32 coordinates, 1: FP64)
+    uint8_t tuned_lanes[2][258] = {};
+    bool tuned[2] = {false, false};
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -94,15 +100,20 @@ inline int grid_for(int64_t n, int block) {
     return (int)g;
 }
 
-// Resident waves of one persistent kernel instance on this device.
-template <typename K>
-int persistent_grid(K kernel) {
-    int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kWave, 0) != hipSuccess || per_cu < 1) {
-        (void)hipGetLastError();
-        per_cu = 4;
-    }
-    return per_cu * g_ctx.num_cu;
+// Grid of a persistent kernel: waves_per_cu resident waves per CU (the
+// TSM_WAVES_PER_CU environment variable overrides it; measured, not the
+// occupancy limit, since too many concurrent task variants per CU cost more
+// than they hide), or fewer when there is less work.  Waves that start
+// after the queue has drained exit at once.
+inline int persistent_grid(int64_t max_useful, int waves_per_cu) {
+    static const int env = [] {
+        const char *e = std::getenv("TSM_WAVES_PER_CU");
+        return e ? std::atoi(e) : 0;
+    }();
+    if (env > 0) waves_per_cu = env;
+    int64_t g = (int64_t)g_ctx.num_cu * waves_per_cu;
+    if (g > max_useful) g = max_useful;
+    return (int)(g < 1 ? 1 : g);
 }
 
 struct ProfScope {
@@ -125,10 +136,12 @@ struct ProfScope {
     }
 };
 
-// DP launcher for one measure and precision, lanes-per-pair L in {8, 16};
-// defined in tsm_dp_<measure>.hip.
+struct BucketTable;
+
+// DP launcher for one measure and precision; lanes_hint 0 (table), 4 or 8.
+// Defined in tsm_dp_<measure>.hip.
 template <int M, bool F64>
-int run_dp(int L, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr, double *out,
+int run_dp(int lanes_hint, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr, double *out,
            int32_t *counts, hipStream_t st);
 
 }  // namespace tsm

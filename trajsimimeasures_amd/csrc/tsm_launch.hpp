@@ -1,6 +1,6 @@
-// tsm_launch.hpp -- host launch sequence of the DP path: plan kernels, then
-// the persistent DP kernel.  Included by the per-measure translation units,
-// which instantiate run_dp for their measure.
+// tsm_launch.hpp -- host launch sequence of the DP path: bucket table, plan
+// kernels, then the persistent DP kernel.  Included by the per-measure
+// translation units, which instantiate run_dp for their measure.
 #pragma once
 
 #include "tsm_host.hpp"
@@ -8,28 +8,57 @@
 
 namespace tsm {
 
-template <int M, bool F64, int L>
-int run_dp_L(const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr, double *out,
-             int32_t *counts, hipStream_t st) {
-    constexpr int WM = WMax<L>::value;
-    constexpr int S = kWave / L;
+// Column count -> (L, W) bucket.  Priority: the caller's lanes hint, then
+// the autotuned lanes table (tsm_tune_set_lanes), then a cost model.  Per
+// pair, a segment of L lanes takes (nrow + L - 1) steps of (c0 + c1 * W)
+// issue slots for 64/L pairs at once, so the per-pair cost is
+// L * (nrow + L - 1) * (c0 + c1 * W) / 64 with c0 ~ c1 (per-step bookkeeping
+// ~ one cell): L = 4 wins whenever its W fits in registers.
+inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint) {
+    BucketTable t{};
+    t.ncmax = 8 * WM < kTabMax - 1 ? 8 * WM : kTabMax - 1;
+    t.wide = bucket_of(WM, 8, WM);
+    const int k = coord_bits == 64 ? 1 : 0;
+    const uint8_t *tuned = g_ctx.tuned[k] ? g_ctx.tuned_lanes[k] : nullptr;
+    for (int nc = 1; nc <= t.ncmax; ++nc) {
+        const int w4 = (nc + 3) / 4, w8 = (nc + 7) / 8;
+        int L;
+        if (w4 > WM) L = 8;
+        else if (lanes_hint == 4 || lanes_hint == 8) L = lanes_hint;
+        else if (tuned && (tuned[nc] == 4 || tuned[nc] == 8)) L = tuned[nc];
+        else {
+            const double rows = nc > 32 ? nc : 32;
+            const double c4 = 4.0 * (rows + 3) * (1.0 + w4), c8 = 8.0 * (rows + 7) * (1.0 + w8);
+            L = c4 <= c8 ? 4 : 8;
+        }
+        t.b[nc] = (uint8_t)bucket_of(WM, L, (nc + L - 1) / L);
+    }
+    t.b[0] = t.b[1];
+    return t;
+}
+
+template <int M, bool F64>
+int run_dp(int lanes_hint, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr, double *out,
+           int32_t *counts, hipStream_t st) {
+    using CT = typename Rec<M, F64>::CT;
     using AT = typename Rec<M, F64>::AT;
+    constexpr int WM = WMaxOf<CT>::value;
+    constexpr int NB = 2 * WM;
+    static_assert(NB <= kMaxBuckets, "bucket count");
     Context &c = g_ctx;
-    const int64_t nbins = (int64_t)WM * kRowKeys;
-    const int64_t nslots = npairs + (int64_t)WM * S;
+    const BucketTable tab = bucket_table(WM, sizeof(CT) * 8, lanes_hint);
+    const int64_t nslots = npairs + (int64_t)NB * kMaxSeg;
     int rc;
-    if ((rc = c.hist.ensure(nbins * 4)) || (rc = c.slots.ensure(nslots * 4)) || (rc = c.wend.ensure(WM * 8)) ||
-        (rc = c.ntasks.ensure(8)) || (rc = c.counter.ensure(4)))
+    if ((rc = c.hist.ensure(kMaxBins * 4)) || (rc = c.cursor.ensure(kMaxBins * 4)) ||
+        (rc = c.slots.ensure(nslots * 4)) || (rc = c.meta.ensure((1 + 2 * NB) * 8)) || (rc = c.counter.ensure(4)))
         return rc;
 
-    auto kern = dp_kernel<M, F64, L>;
-    int grid = persistent_grid(kern);
-    const int64_t max_tasks = (npairs + S - 1) / S + WM;
-    if (grid > max_tasks) grid = (int)max_tasks;
+    auto kern = dp_kernel<M, F64>;
+    const int grid = persistent_grid((npairs + 3) / 4 + NB, 8);
     int64_t bnd_stride = 0;
-    if (set.max_len > (int64_t)L * WM) {  // some pair may need column strips
+    if (set.max_len > tab.ncmax) {  // some pair may need column strips
         bnd_stride = set.max_len;
-        if ((rc = c.bnd.ensure((size_t)grid * S * bnd_stride * sizeof(AT)))) return rc;
+        if ((rc = c.bnd.ensure((size_t)grid * kMaxSeg * bnd_stride * sizeof(AT)))) return rc;
     }
 
     PlanArgs pa;
@@ -37,16 +66,17 @@ int run_dp_L(const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double
     pa.offs = set.offsets;
     pa.npairs = npairs;
     pa.hist = c.hist.as<uint32_t>();
+    pa.cursor = c.cursor.as<uint32_t>();
     pa.slots = c.slots.as<uint32_t>();
-    pa.wend = c.wend.as<int64_t>();
-    pa.ntasks = c.ntasks.as<int64_t>();
+    pa.meta = c.meta.as<int64_t>();
     pa.counter = c.counter.as<unsigned int>();
+    pa.nb = NB;
+    pa.tab = tab;
 
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(nbins, 256)), dim3(256), 0, st, pa.hist, nbins, 0u);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(nslots, 256)), dim3(256), 0, st, pa.slots, nslots, kEmptySlot);
-    hipLaunchKernelGGL(plan_hist_kernel<L>, dim3(grid_for(npairs, 256)), dim3(256), 0, st, pa);
-    hipLaunchKernelGGL(plan_scan_kernel<L>, dim3(1), dim3(1024), 0, st, pa);
-    hipLaunchKernelGGL(plan_scatter_kernel<L>, dim3(grid_for(npairs, 256)), dim3(256), 0, st, pa);
+    const int pblocks = (int)((npairs + kPairsPerBlock - 1) / kPairsPerBlock);
+    hipLaunchKernelGGL(plan_hist_kernel, dim3(pblocks), dim3(kPlanBlock), 0, st, pa);
+    hipLaunchKernelGGL(plan_scan_kernel, dim3(1), dim3(kScanThreads), 0, st, pa);
+    hipLaunchKernelGGL(plan_scatter_kernel, dim3(pblocks), dim3(kPlanBlock), 0, st, pa);
     TSM_HIP(hipGetLastError());
 
     DPArgs a;
@@ -59,8 +89,8 @@ int run_dp_L(const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double
     a.gsum64 = c.gsum64.as<double>();
     a.ps = ps;
     a.slots = pa.slots;
-    a.wend = pa.wend;
-    a.ntasks = pa.ntasks;
+    a.meta = pa.meta;
+    a.nb = NB;
     a.counter = pa.counter;
     a.bnd = c.bnd.p;
     a.bnd_stride = bnd_stride;
@@ -73,15 +103,6 @@ int run_dp_L(const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double
     }
     TSM_HIP(hipGetLastError());
     return TSM_OK;
-}
-
-template <int M, bool F64>
-int run_dp(int L, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr, double *out,
-           int32_t *counts, hipStream_t st) {
-    switch (L) {
-    case 8: return run_dp_L<M, F64, 8>(set, ps, npairs, thr, out, counts, st);
-    default: return run_dp_L<M, F64, 16>(set, ps, npairs, thr, out, counts, st);
-    }
 }
 
 }  // namespace tsm
