@@ -155,6 +155,15 @@ def main():
                "seconds": dt}
         print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
+    traffic = None
+    try:  # HBM bytes per DTW launch from the committed rocprofv3 counter runs
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            traffic = json.load(f)["dp_kernel_dtw_fp32_100k_pairs"]["traffic_bytes"]
+    except (OSError, KeyError, ValueError):
+        traffic = None
+    if args.pairs != 100_000:
+        traffic = None
+
     line = {
         "metric": "DTW pairs/s (100k random-walk pairs, lengths 20-200)",
         "value": args.pairs * args.steps / el,
@@ -173,7 +182,7 @@ def main():
         "cells_per_s": cells / (kms * 1e-3),
         "kernel_ms": kms,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
                      "note": "9 algorithmic FP32 FLOP per DTW cell over mean dp_kernel duration (HIP events)"},
         "cpu_baseline": cpu,
         "measures": report,

@@ -25,7 +25,7 @@ from . import _lib, core
 from .data import TrajSet, pack_list
 
 __all__ = ["heuristic_fn_wrapper", "heuristic_colllate_fn", "is_heuristic", "pairwise", "query_db", "all_pairs",
-           "symmetric_from_triu", "DEFAULT_EPS"]
+           "symmetric_from_triu", "fp64_drift", "DEFAULT_EPS"]
 
 DEFAULT_EPS = 0.0025  # EDR/LCSS epsilon of the 100k-pair workload
 
@@ -169,3 +169,22 @@ def method_names():
 
 def measure_id(name: str) -> int:
     return _lib.measure_id(_MEASURE.get(name, name))
+
+
+def fp64_drift(method: str, dset, pa, pb, *, gpoint=None, lanes: int = 0):
+    """Reruns pairs (pa[k], pb[k]) in FP64 on the GPU and returns
+    (fp32 values, fp64 values, max relative drift).  The drift check of the
+    cumulative-sum measures (DTW, ERP) on long trajectories."""
+    import torch
+
+    from . import engine
+    if method not in ("dtw", "erp", "frechet", "hausdorff"):
+        raise ValueError("FP64 reruns apply to dtw, erp, frechet and hausdorff")
+    g = gpoint if method == "erp" else None
+    v32 = engine.pairs(method, dset, pa, pb, gpoint=g, lanes=lanes, validate=False)
+    v64 = engine.pairs(method, dset, pa, pb, gpoint=g, lanes=lanes, precision="fp64", validate=False)
+    torch.cuda.synchronize()
+    a, b = v32.cpu().numpy(), v64.cpu().numpy()
+    den = np.maximum(np.abs(b), 1e-300)
+    drift = float(np.max(np.abs(a - b) / den)) if a.size else 0.0
+    return a, b, drift
