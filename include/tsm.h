@@ -134,6 +134,28 @@ int tsm_pairs_host(int measure, const double *xy64, const int64_t *offsets, int6
  */
 int tsm_tune_set_lanes(int coord_bits, const uint8_t *lanes_of_ncol, int n);
 
+/* Resident waves per CU of the persistent DP kernel (0 = default 8); the
+ * autotuner measures and sets it.  No reference counterpart. */
+int tsm_tune_set_waves_per_cu(int waves_per_cu);
+
+/*
+ * hipGraph capture of one explicit-pairs batch (ERP prologue, planner and
+ * persistent DP kernel, or the Hausdorff kernels) on a non-null stream.  The
+ * capture call runs the batch once eagerly (sizing the graph's private
+ * workspace and producing results), then records the same launches; each
+ * tsm_graph_launch replays them on the same device buffers, whose contents
+ * (coordinates, pair indices) may change between launches as long as sizes
+ * and lengths stay within those captured.  For the reference's repeated
+ * fixed-size batches (effi_batch_size_gpu = 512, config.py:43) and serving
+ * loops.  The graph holds no memsets.
+ */
+typedef struct tsm_graph tsm_graph;
+int tsm_graph_capture_pairs(int measure, const tsm_trajset *set, const int32_t *pa, const int32_t *pb,
+                            int64_t npairs, const tsm_params *prm, double *out, int32_t *counts, void *stream,
+                            tsm_graph **graph);
+int tsm_graph_launch(tsm_graph *graph, void *stream);
+int tsm_graph_destroy(tsm_graph *graph);
+
 /* Kernel timing of the DP / Hausdorff kernels (HIP events on the launch
  * stream), for roofline reporting.  No reference counterpart. */
 int tsm_profile_enable(int on);

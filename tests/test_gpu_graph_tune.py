@@ -1,0 +1,65 @@
+"""hipGraph capture/replay of a batch and the autotuner, on the device."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.workloads import EPS, uniform_pairs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("measure", ["dtw", "edr", "erp", "hausdorff"])
+def test_graph_replays_match_eager(gpu, measure):
+    import torch
+
+    from trajsimimeasures_amd import engine
+    print(f"[case] graph replay {measure}", flush=True)
+    ts, pa, pb = uniform_pairs(4000, seed=31)
+    d = engine.DeviceTrajSet(ts)
+    g = engine.PairsGraph(measure, d, pa, pb, eps=EPS, gpoint=(0.5, 0.5))
+    eager = engine.pairs(measure, d, pa, pb, eps=EPS, gpoint=(0.5, 0.5)).cpu().numpy()
+    for rep in range(3):
+        g.out.fill_(-1.0)
+        g.stream.synchronize()
+        torch.cuda.synchronize()
+        g.launch()
+        g.stream.synchronize()
+        got = g.out.cpu().numpy()
+        assert np.array_equal(got, eager), (measure, rep)
+        x = torch.randn(1 << 20, device="cuda").sum()  # eager work between replays
+        torch.cuda.synchronize()
+        del x
+    # refill the pair list in place: replays follow the new contents
+    perm = np.random.default_rng(3).permutation(pa.size)
+    g.pa.copy_(torch.as_tensor(pb[perm], dtype=torch.int32))
+    g.pb.copy_(torch.as_tensor(pa[perm], dtype=torch.int32))
+    torch.cuda.synchronize()
+    g.launch()
+    g.stream.synchronize()
+    ref, _ = O.pairs(measure, ts.xy, ts.offsets, pb[perm], pa[perm], eps=EPS, gpoint=(0.5, 0.5))
+    got = g.out.cpu().numpy()
+    if measure == "edr":
+        assert np.array_equal(got, ref)
+    else:
+        assert np.max(np.abs(got - ref) / np.maximum(ref, 1e-300)) <= 1e-5
+    g.close()
+
+
+def test_autotune_small(gpu):
+    from trajsimimeasures_amd import autotune, engine
+    t = autotune.tune(npairs_per_bucket=1024, waves=(4, 8), verbose=False)
+    assert set(t["lanes_fp32"][1:97]) <= {4, 8} and t["waves_per_cu"] in (4, 8)
+    ts, pa, pb = uniform_pairs(1500, seed=2)
+    d = engine.DeviceTrajSet(ts)
+    for m in ("dtw", "lcss"):
+        got = engine.pairs(m, d, pa, pb, eps=EPS).cpu().numpy()
+        ref, _ = O.pairs(m, ts.xy, ts.offsets, pa, pb, eps=EPS)
+        assert np.max(np.abs(got - ref) / np.maximum(ref, 1e-300)) <= 1e-5
+    # back to the shipped (or built-in) table
+    import ctypes
+
+    from trajsimimeasures_amd import _lib
+    _lib.check(_lib.lib().tsm_tune_set_lanes(32, None, 0))
+    _lib.check(_lib.lib().tsm_tune_set_lanes(64, None, 0))
+    _lib.check(_lib.lib().tsm_tune_set_waves_per_cu(0))
+    autotune.load_shipped()

@@ -26,6 +26,7 @@ ERRORS = {-1: "TSM_E_ARG", -2: "TSM_E_HIP", -3: "TSM_E_NOMEM", -4: "TSM_E_UNSUP"
 # Every function include/tsm.h declares; the load check verifies all exist.
 EXPORTS = ("tsm_init", "tsm_last_error", "tsm_abi_version", "tsm_params_default", "tsm_xy_to_fp32",
            "tsm_pairs", "tsm_pairs_grid", "tsm_pairs_triu", "tsm_pairs_host", "tsm_tune_set_lanes",
+           "tsm_tune_set_waves_per_cu", "tsm_graph_capture_pairs", "tsm_graph_launch", "tsm_graph_destroy",
            "tsm_profile_enable", "tsm_profile_read")
 
 
@@ -65,6 +66,11 @@ def _declare(L):
     L.tsm_pairs_triu.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _I64, ctypes.POINTER(ParamsC), _P, _P, _P]
     L.tsm_pairs_host.argtypes = [ctypes.c_int, _P, _P, _I64, _I64, _P, _P, _I64, ctypes.POINTER(ParamsC), _P, _P]
     L.tsm_tune_set_lanes.argtypes = [ctypes.c_int, _P, ctypes.c_int]
+    L.tsm_tune_set_waves_per_cu.argtypes = [ctypes.c_int]
+    L.tsm_graph_capture_pairs.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _P, _P, _I64,
+                                          ctypes.POINTER(ParamsC), _P, _P, _P, ctypes.POINTER(ctypes.c_void_p)]
+    L.tsm_graph_launch.argtypes = [_P, _P]
+    L.tsm_graph_destroy.argtypes = [_P]
     L.tsm_profile_enable.argtypes = [ctypes.c_int]
     L.tsm_profile_read.argtypes = [ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
     for name in EXPORTS:
@@ -87,6 +93,9 @@ def lib():
                 raise ImportError(f"{LIB_PATH} lacks {missing}")
             _declare(L)
             _lib = L
+            # the launch table measured on MI355X (autotune.py), if shipped
+            from . import autotune
+            autotune.load_shipped()
     return _lib
 
 

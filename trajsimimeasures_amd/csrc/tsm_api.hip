@@ -71,8 +71,7 @@ int choose_lanes(const tsm_params &prm) {
     return 0;
 }
 
-int erp_prologue(const tsm_trajset &set, double gx, double gy, hipStream_t st) {
-    Context &c = g_ctx;
+int erp_prologue(Workspace &c, const tsm_trajset &set, double gx, double gy, hipStream_t st) {
     const int64_t npts = set.npts;
     int rc;
     if ((rc = c.gap64.ensure(npts * 8 + 8)) || (rc = c.gap32.ensure(npts * 4 + 8)) ||
@@ -85,8 +84,10 @@ int erp_prologue(const tsm_trajset &set, double gx, double gy, hipStream_t st) {
     return TSM_OK;
 }
 
-int run(int measure, const tsm_trajset *set, const PairSpace &ps, int64_t npairs, const tsm_params *prm_in,
-        double *out, int32_t *counts, void *stream) {
+// Argument checks shared by every entry point; fills prm with defaults
+// overridden by prm_in.
+int validate(int measure, const tsm_trajset *set, int64_t npairs, const tsm_params *prm_in, const double *out,
+             tsm_params &prm) {
     if (measure < TSM_DTW || measure > TSM_HAUSDORFF) return fail(TSM_E_ARG, "unknown measure id " + std::to_string(measure));
     if (!set) return fail(TSM_E_ARG, "null trajectory set");
     if (npairs < 0) return fail(TSM_E_ARG, "negative pair count");
@@ -96,7 +97,6 @@ int run(int measure, const tsm_trajset *set, const PairSpace &ps, int64_t npairs
     if (!set->offsets || set->n < 1) return fail(TSM_E_ARG, "empty trajectory set");
     if (set->max_len < 1) return fail(TSM_E_ARG, "max_len must be >= 1");
     if (set->npts < set->n) return fail(TSM_E_ARG, "npts must be >= n (every trajectory has a point)");
-    tsm_params prm;
     tsm_params_default(&prm);
     if (prm_in) prm = *prm_in;
     const bool f64 = prm.precision == TSM_PREC_FP64;
@@ -109,12 +109,14 @@ int run(int measure, const tsm_trajset *set, const PairSpace &ps, int64_t npairs
     if (!f64 && measure != TSM_EDR && measure != TSM_LCSS && !set->xy32)
         return fail(TSM_E_ARG, "FP32 measures need the FP32 coordinates (xy32)");
     if (set->max_len > (1 << 24)) return fail(TSM_E_ARG, "trajectories longer than 2^24 points are not supported");
+    return TSM_OK;
+}
 
-    std::lock_guard<std::mutex> lk(g_mu);
-    int rc = ensure_ctx();
-    if (rc) return rc;
-    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-
+// The launch sequence of one call on workspace w (g_mu held).
+int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &ps, int64_t npairs,
+           const tsm_params &prm, double *out, int32_t *counts, hipStream_t st) {
+    const bool f64 = prm.precision == TSM_PREC_FP64;
+    int rc;
     if (measure == TSM_HAUSDORFF) {
         HArgs h;
         h.xy32 = set->xy32;
@@ -126,12 +128,12 @@ int run(int measure, const tsm_trajset *set, const PairSpace &ps, int64_t npairs
         h.seg = nullptr;
         h.rseg = nullptr;
         if (!f64) {
-            if ((rc = g_ctx.seg.ensure(set->npts * sizeof(float4))) || (rc = g_ctx.rseg.ensure(set->npts * sizeof(float))))
+            if ((rc = w.seg.ensure(set->npts * sizeof(float4))) || (rc = w.rseg.ensure(set->npts * sizeof(float))))
                 return rc;
-            h.seg = g_ctx.seg.as<float4>();
-            h.rseg = g_ctx.rseg.as<float>();
+            h.seg = w.seg.as<float4>();
+            h.rseg = w.rseg.as<float>();
             hipLaunchKernelGGL(seg_prep_kernel, dim3(grid_for(set->npts, 256)), dim3(256), 0, st, set->xy32, set->npts,
-                               g_ctx.seg.as<float4>(), g_ctx.rseg.as<float>());
+                               w.seg.as<float4>(), w.rseg.as<float>());
         }
         int grid = (int)(npairs < 1048576 ? npairs : 1048576);
         ProfScope p_(st);
@@ -140,24 +142,34 @@ int run(int measure, const tsm_trajset *set, const PairSpace &ps, int64_t npairs
         TSM_HIP(hipGetLastError());
         return TSM_OK;
     }
-    if (measure == TSM_ERP && (rc = erp_prologue(*set, prm.gx, prm.gy, st))) return rc;
+    if (measure == TSM_ERP && (rc = erp_prologue(w, *set, prm.gx, prm.gy, st))) return rc;
 
     const double thr = eps_threshold(prm.eps, prm.eps_mode);
     const int L = choose_lanes(prm);
     switch (measure) {
     case TSM_DTW:
-        return f64 ? run_dp<kDTW, true>(L, *set, ps, npairs, thr, out, counts, st)
-                   : run_dp<kDTW, false>(L, *set, ps, npairs, thr, out, counts, st);
+        return f64 ? run_dp<kDTW, true>(w, L, *set, ps, npairs, thr, out, counts, st)
+                   : run_dp<kDTW, false>(w, L, *set, ps, npairs, thr, out, counts, st);
     case TSM_FRECHET:
-        return f64 ? run_dp<kFrechet, true>(L, *set, ps, npairs, thr, out, counts, st)
-                   : run_dp<kFrechet, false>(L, *set, ps, npairs, thr, out, counts, st);
+        return f64 ? run_dp<kFrechet, true>(w, L, *set, ps, npairs, thr, out, counts, st)
+                   : run_dp<kFrechet, false>(w, L, *set, ps, npairs, thr, out, counts, st);
     case TSM_ERP:
-        return f64 ? run_dp<kERP, true>(L, *set, ps, npairs, thr, out, counts, st)
-                   : run_dp<kERP, false>(L, *set, ps, npairs, thr, out, counts, st);
-    case TSM_EDR: return run_dp<kEDR, false>(L, *set, ps, npairs, thr, out, counts, st);
-    case TSM_LCSS: return run_dp<kLCSS, false>(L, *set, ps, npairs, thr, out, counts, st);
+        return f64 ? run_dp<kERP, true>(w, L, *set, ps, npairs, thr, out, counts, st)
+                   : run_dp<kERP, false>(w, L, *set, ps, npairs, thr, out, counts, st);
+    case TSM_EDR: return run_dp<kEDR, false>(w, L, *set, ps, npairs, thr, out, counts, st);
+    case TSM_LCSS: return run_dp<kLCSS, false>(w, L, *set, ps, npairs, thr, out, counts, st);
     }
     return fail(TSM_E_ARG, "unknown measure");
+}
+
+int run(int measure, const tsm_trajset *set, const PairSpace &ps, int64_t npairs, const tsm_params *prm_in,
+        double *out, int32_t *counts, void *stream) {
+    tsm_params prm;
+    int rc = validate(measure, set, npairs, prm_in, out, prm);
+    if (rc || npairs == 0) return rc;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((rc = ensure_ctx())) return rc;
+    return launch(g_ctx.ws, measure, set, ps, npairs, prm, out, counts, reinterpret_cast<hipStream_t>(stream));
 }
 
 }  // namespace
@@ -288,6 +300,87 @@ int tsm_tune_set_lanes(int coord_bits, const uint8_t *lanes_of_ncol, int n) {
     std::memset(g_ctx.tuned_lanes[k], 0, sizeof(g_ctx.tuned_lanes[k]));
     std::memcpy(g_ctx.tuned_lanes[k], lanes_of_ncol, (size_t)n);
     g_ctx.tuned[k] = true;
+    return TSM_OK;
+}
+
+int tsm_tune_set_waves_per_cu(int waves_per_cu) {
+    if (waves_per_cu < 0 || waves_per_cu > 32) return fail(TSM_E_ARG, "waves_per_cu must be in 0..32 (0 = default 8)");
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_ctx.waves_per_cu = waves_per_cu == 0 ? 8 : waves_per_cu;
+    return TSM_OK;
+}
+
+}  // extern "C"
+
+// A captured batch: the planner and kernel launches of one call, replayable
+// on the same device buffers, with a private workspace.
+struct tsm_graph {
+    tsm::Workspace ws;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int tsm_graph_capture_pairs(int measure, const tsm_trajset *set, const int32_t *pa, const int32_t *pb,
+                            int64_t npairs, const tsm_params *prm_in, double *out, int32_t *counts, void *stream,
+                            tsm_graph **graph) {
+    if (!graph) return fail(TSM_E_ARG, "null graph handle");
+    *graph = nullptr;
+    if (!stream) return fail(TSM_E_ARG, "graph capture needs a non-null stream");
+    if (npairs < 1) return fail(TSM_E_ARG, "graph capture needs at least one pair");
+    if (!pa || !pb) return fail(TSM_E_ARG, "null pair index arrays");
+    tsm_params prm;
+    int rc = validate(measure, set, npairs, prm_in, out, prm);
+    if (rc) return rc;
+    PairSpace ps{};
+    ps.mode = kExplicit;
+    ps.pa = pa;
+    ps.pb = pb;
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    std::lock_guard<std::mutex> lk(g_mu);
+    if ((rc = ensure_ctx())) return rc;
+    tsm_graph *g = new tsm_graph();
+    // first run outside the capture: sizes the private workspace (and
+    // computes the results once)
+    if ((rc = launch(g->ws, measure, set, ps, npairs, prm, out, counts, st))) {
+        g->ws.release();
+        delete g;
+        return rc;
+    }
+    hipError_t e = hipStreamSynchronize(st);
+    if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        g_ctx.capturing = true;
+        rc = launch(g->ws, measure, set, ps, npairs, prm, out, counts, st);
+        g_ctx.capturing = false;
+        hipError_t e2 = hipStreamEndCapture(st, &g->graph);
+        if (rc == TSM_OK && e2 != hipSuccess) e = e2;
+    }
+    if (e == hipSuccess && rc == TSM_OK) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (e != hipSuccess || rc != TSM_OK) {
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        g->ws.release();
+        delete g;
+        return rc != TSM_OK ? rc : fail(TSM_E_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+    }
+    *graph = g;
+    return TSM_OK;
+}
+
+int tsm_graph_launch(tsm_graph *graph, void *stream) {
+    if (!graph || !graph->exec) return fail(TSM_E_ARG, "null graph");
+    TSM_HIP(hipGraphLaunch(graph->exec, reinterpret_cast<hipStream_t>(stream)));
+    return TSM_OK;
+}
+
+int tsm_graph_destroy(tsm_graph *graph) {
+    if (!graph) return TSM_OK;
+    if (graph->exec) (void)hipGraphExecDestroy(graph->exec);
+    if (graph->graph) (void)hipGraphDestroy(graph->graph);
+    graph->ws.release();
+    delete graph;
     return TSM_OK;
 }
 

@@ -50,18 +50,36 @@ struct DevBuf {
         return TSM_OK;
     }
     template <typename T> T *as() const { return reinterpret_cast<T *>(p); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
+    }
+};
+
+// Device scratch of one call sequence: the plan, the strip boundary columns
+// and the per-measure prologue arrays.  The context owns the shared one;
+// each captured graph owns its own, so replays never see reallocations.
+struct Workspace {
+    DevBuf hist, cursor, slots, meta, counter, bnd;
+    DevBuf gap64, gap32, gsum64, gsum32, seg, rseg;
+    void release() {
+        for (DevBuf *b : {&hist, &cursor, &slots, &meta, &counter, &bnd, &gap64, &gap32, &gsum64, &gsum32, &seg, &rseg})
+            b->release();
+    }
 };
 
 struct Context {
     int device = -1;
     int num_cu = 0;
-    DevBuf hist, cursor, slots, meta, counter, bnd;
-    DevBuf gap64, gap32, gsum64, gsum32, seg, rseg;
+    Workspace ws;
     // staging for tsm_pairs_host
     DevBuf h_xy64, h_xy32, h_offs, h_pa, h_pb, h_out, h_cnt;
     // autotuned lanes per column count (index 0: FP32 coordinates, 1: FP64)
     uint8_t tuned_lanes[2][258] = {};
     bool tuned[2] = {false, false};
+    int waves_per_cu = 8;  // resident waves per CU of the persistent DP kernel (autotuned)
+    bool capturing = false;  // inside a graph capture: no profiling events
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -100,17 +118,18 @@ inline int grid_for(int64_t n, int block) {
     return (int)g;
 }
 
-// Grid of a persistent kernel: waves_per_cu resident waves per CU (the
-// TSM_WAVES_PER_CU environment variable overrides it; measured, not the
-// occupancy limit, since too many concurrent task variants per CU cost more
-// than they hide), or fewer when there is less work.  Waves that start
-// after the queue has drained exit at once.
-inline int persistent_grid(int64_t max_useful, int waves_per_cu) {
+// Grid of a persistent kernel: waves_per_cu waves per CU (autotuned, 8 by
+// default; the TSM_WAVES_PER_CU environment variable overrides it), or fewer
+// when there is less work.  More waves than the registers allow wait for a
+// slot and exit at once when they find the queue drained.  Measured: beyond
+// 2 resident waves per SIMD the extra waves add memory-wait time on the row
+// loads faster than they hide it.
+inline int persistent_grid(int64_t max_useful) {
     static const int env = [] {
         const char *e = std::getenv("TSM_WAVES_PER_CU");
         return e ? std::atoi(e) : 0;
     }();
-    if (env > 0) waves_per_cu = env;
+    const int waves_per_cu = env > 0 ? env : g_ctx.waves_per_cu;
     int64_t g = (int64_t)g_ctx.num_cu * waves_per_cu;
     if (g > max_useful) g = max_useful;
     return (int)(g < 1 ? 1 : g);
@@ -120,7 +139,7 @@ struct ProfScope {
     hipEvent_t e1 = nullptr, e2 = nullptr;
     hipStream_t s;
     explicit ProfScope(hipStream_t st) : s(st) {
-        if (!g_ctx.prof) return;
+        if (!g_ctx.prof || g_ctx.capturing) return;
         if (g_ctx.ev_used == g_ctx.ev.size()) {
             hipEvent_t a, b;
             if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
@@ -141,7 +160,7 @@ struct BucketTable;
 // DP launcher for one measure and precision; lanes_hint 0 (table), 4 or 8.
 // Defined in tsm_dp_<measure>.hip.
 template <int M, bool F64>
-int run_dp(int lanes_hint, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr, double *out,
-           int32_t *counts, hipStream_t st);
+int run_dp(Workspace &w, int lanes_hint, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr,
+           double *out, int32_t *counts, hipStream_t st);
 
 }  // namespace tsm

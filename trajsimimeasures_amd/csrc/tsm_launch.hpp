@@ -38,14 +38,13 @@ inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint) {
 }
 
 template <int M, bool F64>
-int run_dp(int lanes_hint, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr, double *out,
-           int32_t *counts, hipStream_t st) {
+int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr,
+           double *out, int32_t *counts, hipStream_t st) {
     using CT = typename Rec<M, F64>::CT;
     using AT = typename Rec<M, F64>::AT;
     constexpr int WM = WMaxOf<CT>::value;
     constexpr int NB = 2 * WM;
     static_assert(NB <= kMaxBuckets, "bucket count");
-    Context &c = g_ctx;
     const BucketTable tab = bucket_table(WM, sizeof(CT) * 8, lanes_hint);
     const int64_t nslots = npairs + (int64_t)NB * kMaxSeg;
     int rc;
@@ -54,7 +53,7 @@ int run_dp(int lanes_hint, const tsm_trajset &set, const PairSpace &ps, int64_t 
         return rc;
 
     auto kern = dp_kernel<M, F64>;
-    const int grid = persistent_grid((npairs + 3) / 4 + NB, 8);
+    const int grid = persistent_grid((npairs + 3) / 4 + NB);
     int64_t bnd_stride = 0;
     if (set.max_len > tab.ncmax) {  // some pair may need column strips
         bnd_stride = set.max_len;

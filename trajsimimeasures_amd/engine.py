@@ -17,7 +17,7 @@ import torch
 from . import _lib
 from .data import TrajSet
 
-__all__ = ["DeviceTrajSet", "pairs", "grid", "triu", "KernelTimer"]
+__all__ = ["DeviceTrajSet", "pairs", "grid", "triu", "KernelTimer", "PairsGraph"]
 
 
 def _stream_ptr():
@@ -137,3 +137,53 @@ class KernelTimer:
         self.ms = ms.value
         self.launches = n.value
         return False
+
+
+class PairsGraph:
+    """A batch of explicit pairs captured once into a hipGraph (planner +
+    persistent kernel launches) and replayed with ``launch()``.  The device
+    tensors ``pa``, ``pb``, ``out`` and the set's coordinates may be refilled
+    in place between launches; their sizes may not change.  Capture and
+    replay run on a dedicated stream (default priority) unless one is
+    given."""
+
+    def __init__(self, measure, dset: DeviceTrajSet, pa, pb, *, eps=0.0, eps_mode="sqrt", gpoint=None, lanes=0,
+                 precision="fp32", stream=None):
+        m = _lib.measure_id(measure)
+        dev = dset.xy64.device
+        self.dset = dset
+        self.pa = torch.as_tensor(pa, dtype=torch.int32, device=dev).contiguous()
+        self.pb = torch.as_tensor(pb, dtype=torch.int32, device=dev).contiguous()
+        n = self.pa.numel()
+        if self.pb.numel() != n or n < 1:
+            raise ValueError("pa and pb must be non-empty and of equal length")
+        self.out = torch.empty(n, dtype=torch.float64, device=dev)
+        self.counts = torch.empty(n, dtype=torch.int32, device=dev) if m in (_lib.EDR, _lib.LCSS) else None
+        self.stream = stream if stream is not None else torch.cuda.Stream(device=dev)
+        self._prm = _lib.params(eps, eps_mode, gpoint, lanes, precision)
+        self._set = dset.cstruct()
+        self._h = ctypes.c_void_p()
+        torch.cuda.current_stream().synchronize()
+        _lib.check(_lib.lib().tsm_graph_capture_pairs(
+            m, ctypes.byref(self._set), self.pa.data_ptr(), self.pb.data_ptr(), n, ctypes.byref(self._prm),
+            self.out.data_ptr(), self.counts.data_ptr() if self.counts is not None else None,
+            ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(self._h)))
+
+    def launch(self):
+        """Replay the batch on the graph's stream; results land in self.out."""
+        if not self._h:
+            raise RuntimeError("graph already closed")
+        _lib.check(_lib.lib().tsm_graph_launch(self._h, ctypes.c_void_p(self.stream.cuda_stream)))
+        return self.out
+
+    def close(self):
+        if self._h:
+            self.stream.synchronize()
+            _lib.check(_lib.lib().tsm_graph_destroy(self._h))
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
