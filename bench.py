@@ -69,8 +69,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; multi-GPU sharding is a design note only "
-                                   "(see DESIGN.md), so --gpus must be 1"}))
+        print(json.dumps({"error": "this engine runs on one GPU: pairs would shard across GPUs by contiguous "
+                                   "ranges of the work list, which is not built, so --gpus must be 1"}))
         sys.exit(2)
 
     import torch

@@ -10,7 +10,7 @@
  * distances back.  This ABI replaces those bodies: trajectories travel as a
  * packed point buffer plus an offsets array, pairs as index arrays, and one
  * call computes a whole batch on one stream.  Python binds it with ctypes
- * (trajsimimeasures_amd/_lib.py); INTEGRATION.md shows the binding.
+ * (trajsimimeasures_amd/_lib.py).
  *
  * Conventions
  *   - Every function returns TSM_OK (0) or a negative TSM_E* code; the

@@ -1,0 +1,220 @@
+"""GPU tests of the learned kNN filter kernels (include/tsm_nn.h): the BF16
+MFMA GEMM against an FP64 product of the same BF16 values, the casts, the
+top-k selection against a sort, the resampler, and the encoder's forward and
+backward passes against the FP64 numpy restatement in tests/nn_ref.py."""
+import numpy as np
+import pytest
+
+from tests import nn_ref as R
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf16_round(a: np.ndarray) -> np.ndarray:
+    """Round FP32 values to the nearest BF16 (ties to even), as float64."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (37, 90, 40), (130, 17, 136), (1, 300, 384), (200, 129, 8)])
+def test_gemm_nt_matches_fp64_product(gpu, M, N, K):
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    rng = np.random.default_rng(M * 1000 + N + K)
+    a = rng.standard_normal((M, K)).astype(np.float32)
+    b = rng.standard_normal((N, K)).astype(np.float32)
+    c0 = rng.standard_normal((M, N)).astype(np.float32)
+    bias = rng.standard_normal(N).astype(np.float32)
+    A = F._bf16(torch.tensor(a, device="cuda"))
+    B = F._bf16(torch.tensor(b, device="cuda"))
+    for alpha, beta, use_bias in ((1.0, 0.0, False), (-2.0, 0.5, True)):
+        C = torch.tensor(c0, device="cuda")
+        F.gemm_nt(A, B, C, alpha=alpha, beta=beta, bias=torch.tensor(bias, device="cuda") if use_bias else None)
+        ab, bb = _bf16_round(a), _bf16_round(b)
+        ref = alpha * ab @ bb.T + beta * c0.astype(np.float64) + (bias if use_bias else 0.0)
+        # FP32 accumulation of exact BF16 products: error bounded by K ulps of the absolute sum
+        bound = 2e-6 * K * (np.abs(ab) @ np.abs(bb).T) * abs(alpha) + 1e-5
+        err = np.abs(C.cpu().numpy() - ref)
+        assert np.all(err <= bound), float(err.max())
+
+
+def test_cast_bf16_rounds_to_nearest_even(gpu):
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    rng = np.random.default_rng(3)
+    a = (rng.standard_normal((33, 70)) * 10.0 ** rng.integers(-3, 3, (33, 70))).astype(np.float32)
+    # exact ties between two BF16 values round to the even one
+    a[0, :4] = np.array([0x3F808000, 0x3F818000, 0xBF808000, 0x3F80C000], dtype=np.uint32).view(np.float32)
+    t = torch.tensor(a, device="cuda")
+    want = t.to(torch.bfloat16).view(torch.int16).cpu().numpy()
+    got = F._bf16(t).cpu().numpy()
+    assert np.array_equal(got, want)
+    got_t = F._bf16(t, transpose=True).cpu().numpy()
+    assert np.array_equal(got_t, want.T)
+
+
+@pytest.mark.parametrize("dtype", ["float32", "float64"])
+@pytest.mark.parametrize("rows,cols,k", [(3, 1000, 10), (5, 70000, 200), (2, 257, 257), (4, 5000, 1)])
+def test_topk_smallest_matches_sort(gpu, dtype, rows, cols, k):
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    rng = np.random.default_rng(cols + k)
+    v = rng.standard_normal((rows, cols)).astype(dtype)
+    v[0, ::7] = v[0, 3]           # many ties
+    v[1 % rows, :50] = -np.abs(v[1 % rows, :50])
+    idx, val = F.topk_smallest(torch.tensor(v, device="cuda"), k)
+    idx, val = idx.cpu().numpy(), val.cpu().numpy()
+    for r in range(rows):
+        assert len(set(idx[r].tolist())) == k
+        assert np.array_equal(v[r, idx[r]], val[r])
+        assert np.array_equal(np.sort(val[r]), np.sort(v[r])[:k])
+
+
+def test_resample_matches_numpy(gpu):
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    from trajsimimeasures_amd.data import lengths_lognormal, random_walks
+    from trajsimimeasures_amd.engine import DeviceTrajSet
+    rng = np.random.default_rng(11)
+    lens = lengths_lognormal(rng, 300)
+    lens[:3] = [1, 2, 16]
+    ts = random_walks(rng, lens)
+    dset = DeviceTrajSet(ts)
+    enc = F.Encoder()
+    ids = torch.tensor(rng.permutation(300)[:120].astype(np.int32), device="cuda")
+    X, lenf = enc.resample(dset, ids, 120)
+    X, lenf = X.cpu().numpy(), lenf.cpu().numpy()
+    for r, i in enumerate(ids.cpu().numpy()):
+        tr = ts.xy[ts.offsets[i]:ts.offsets[i + 1]]
+        n = len(tr)
+        s = np.linspace(0.0, n - 1, F.T)
+        want = np.stack([np.interp(s, np.arange(n), tr[:, 0]), np.interp(s, np.arange(n), tr[:, 1])], 1)
+        assert np.allclose(X[r], (want - 0.5) * 2.0, rtol=0, atol=2e-7)
+        assert abs(lenf[r] - (np.log(n) - 4.0)) < 1e-6
+
+
+def _cos(a, b):
+    return float(np.dot(a.ravel(), b.ravel()) / (np.linalg.norm(a) * np.linalg.norm(b) + 1e-300))
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / (np.linalg.norm(b) + 1e-300))
+
+
+def test_encoder_forward_and_gradients_match_fp64_reference(gpu):
+    """Forward embedding, ranking-loss gradient and every parameter gradient
+    of the HIP encoder against the FP64 reference on the same inputs.  The
+    GEMMs take BF16 operands, so agreement is to about 1%; a wrong gate,
+    index or transposed weight is off by order one."""
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    from trajsimimeasures_amd.data import lengths_lognormal, random_walks
+    from trajsimimeasures_amd.engine import DeviceTrajSet
+    rng = np.random.default_rng(21)
+    B, K, P, Q0 = 2, 15, 3, 6
+    n = B * (1 + K)
+    ts = random_walks(rng, lengths_lognormal(rng, n))
+    dset = DeviceTrajSet(ts)
+    p = F.init_params(4)
+    v = F._views(torch.from_numpy(p))
+    v["W_o"][:] = torch.from_numpy(rng.uniform(-0.1, 0.1, (F.D, F.H)).astype(np.float32))
+    v["b_o"][:] = torch.from_numpy(rng.uniform(-0.1, 0.1, F.D).astype(np.float32))
+    enc = F.Encoder(p)
+    X, lenf = enc.resample(dset, None, n)
+    E = enc.forward(X, lenf, keep=True)
+    loss, dE = F.rank_loss(E, B, K, P=P, Q0=Q0, tau=0.5)
+    grad = enc.backward(dE).cpu().numpy()
+    torch.cuda.synchronize()
+
+    pr = R.unpack(p.astype(np.float64))
+    S = enc.S.cpu().numpy().astype(np.float64)
+    Xr = X.cpu().numpy().astype(np.float64)
+    Er, act = R.forward(pr, Xr, lenf.cpu().numpy().astype(np.float64), S)
+    Eg = E.cpu().numpy().astype(np.float64)
+    skip = Xr.reshape(n, -1) @ S.T
+    assert _rel(Eg - skip, Er - skip) < 1e-2
+
+    # loss and dE: FP32 kernel vs FP64 on the GPU's own embeddings
+    lr_, dEr, _ = R.rank_loss(Eg, B, K, P, Q0, 0.5)
+    assert abs(float(loss.mean()) - lr_) <= 1e-4 * abs(lr_) + 1e-7
+    assert _rel(dE.cpu().numpy(), dEr) < 1e-4
+
+    # backward with the same dE through the FP64 activations
+    gr = R.pack(R.backward(pr, act, dE.cpu().numpy().astype(np.float64)))
+    gg = grad.astype(np.float64)
+    o = 0
+    for name, shape in F.LAYOUT:
+        m = int(np.prod(shape))
+        a, b = gg[o:o + m], gr[o:o + m]
+        o += m
+        assert _cos(a, b) > 0.999, (name, _cos(a, b))
+        assert _rel(a, b) < 5e-2, (name, _rel(a, b))
+
+
+def test_adam_step_matches_formula(gpu):
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    rng = np.random.default_rng(2)
+    p0 = rng.standard_normal(1000).astype(np.float32)
+    g = [rng.standard_normal(1000).astype(np.float32) for _ in range(3)]
+    pt = torch.tensor(p0, device="cuda")
+    opt = F.Adam(1000, "cuda", lr=0.01)
+    for gi in g:
+        opt.update(pt, torch.tensor(gi, device="cuda"))
+    m = np.zeros(1000)
+    v = np.zeros(1000)
+    p = p0.astype(np.float64)
+    for t, gi in enumerate(g, 1):
+        m = 0.9 * m + 0.1 * gi
+        v = 0.999 * v + 0.001 * gi.astype(np.float64) ** 2
+        p -= 0.01 * (m / (1 - 0.9 ** t)) / (np.sqrt(v / (1 - 0.999 ** t)) + 1e-8)
+    assert np.allclose(pt.cpu().numpy(), p, rtol=1e-5, atol=1e-6)
+
+
+def test_split_bf16_search_distances(gpu):
+    """|e_d|^2 - 2 e_q . e_d through the hi/lo split GEMM agrees with the
+    FP64 squared distance minus |e_q|^2 to about 2^-16 of the scale."""
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    rng = np.random.default_rng(5)
+    Eq = rng.standard_normal((40, F.D)).astype(np.float32)
+    Ed = rng.standard_normal((300, F.D)).astype(np.float32)
+    norm = torch.empty(300, dtype=torch.float32, device="cuda")
+    Bm = F.FilterIndex._split(torch.tensor(Ed, device="cuda"), False, norm)
+    Am = F.FilterIndex._split(torch.tensor(Eq, device="cuda"), True)
+    dist = torch.empty((40, 300), dtype=torch.float32, device="cuda")
+    F.gemm_nt(Am, Bm, dist, alpha=-2.0, bias=norm)
+    eq, ed = Eq.astype(np.float64), Ed.astype(np.float64)
+    want = ((eq[:, None] - ed[None]) ** 2).sum(-1) - (eq ** 2).sum(1)[:, None]
+    scale = (eq ** 2).sum(1)[:, None] + (ed ** 2).sum(1)[None]
+    assert np.all(np.abs(dist.cpu().numpy() - want) <= 5e-5 * scale)
+
+
+def test_filtered_knn_small_recall(gpu):
+    """End to end on a small set: the untrained filter (resampled-L2
+    embedding) shortlists candidates whose exact DTW re-rank recovers most of
+    the exact 10-NN; with C = database size the recall is exactly 1."""
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    from trajsimimeasures_amd.data import lengths_lognormal, random_walks
+    from trajsimimeasures_amd.engine import DeviceTrajSet
+    rng = np.random.default_rng(8)
+    nq, nd = 20, 4000
+    ts = random_walks(rng, lengths_lognormal(rng, nq + nd))
+    dset = DeviceTrajSet(ts)
+    truth, _ = F.exact_knn(dset, 0, nq, nq, nd, k=10)
+    enc = F.Encoder()
+    index = F.FilterIndex(enc, dset, torch.arange(nq, nq + nd, dtype=torch.int32, device="cuda"))
+    found, _ = F.filtered_knn(index, 0, nq, k=10, C=nd)
+    assert F.recall_at(found, truth) == 1.0
+    found, _ = F.filtered_knn(index, 0, nq, k=10, C=200)
+    assert F.recall_at(found, truth) > 0.8

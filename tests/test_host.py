@@ -15,8 +15,8 @@ from trajsimimeasures_amd.data import (TrajSet, lengths_lognormal, lengths_unifo
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def header_functions():
-    txt = open(os.path.join(ROOT, "include", "tsm.h")).read()
+def header_functions(name="tsm.h"):
+    txt = open(os.path.join(ROOT, "include", name)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
     return sorted(set(re.findall(r"\b(tsm_[a-z0-9_]+)\s*\(", txt)))
 
@@ -29,6 +29,10 @@ def test_library_exports_every_declared_function():
         assert hasattr(L, n), n
     assert set(names) == set(_lib.EXPORTS)
     assert L.tsm_abi_version() == 1
+    nn = header_functions("tsm_nn.h")
+    for n in nn:
+        assert hasattr(L, n), n
+    assert set(nn) == set(_lib.NN_EXPORTS)
 
 
 def test_params_default():
@@ -107,12 +111,28 @@ def test_random_walk_generator():
     ts = random_walks(rng, lens)
     assert ts.n == 500 and np.array_equal(ts.lengths, lens)
     assert ts.xy.min() >= 0.0 and ts.xy.max() <= 1.0
-    # reproducible from the seed, independent of chunking
+    # reproducible from the seed, and equal to a point-by-point walk over the
+    # same draws (starts, headings, turns, steps in that order)
     r1 = np.random.default_rng(7)
-    a = random_walks(r1, lengths_uniform(r1, 50, 1, 40))
+    lens7 = lengths_uniform(r1, 50, 1, 40)
+    a = random_walks(r1, lens7)
     r2 = np.random.default_rng(7)
-    b = random_walks(r2, lengths_uniform(r2, 50, 1, 40), chunk=7)
-    assert np.array_equal(a.xy, b.xy)
+    lengths_uniform(r2, 50, 1, 40)
+    p0 = r2.random((50, 2))
+    th0 = r2.random(50) * (2.0 * np.pi)
+    turn = r2.normal(0.0, 0.3, size=int(lens7.sum()))
+    step = np.exp(r2.normal(np.log(0.002) - 0.125, 0.5, size=int(lens7.sum())))
+    o = 0
+    for t in range(50):
+        x, y, cs = p0[t, 0], p0[t, 1], 0.0
+        assert a.xy[o, 0] == x and a.xy[o, 1] == y
+        for k in range(1, int(lens7[t])):
+            cs += turn[o + k]
+            th = th0[t] + cs
+            x = min(max(x + step[o + k] * np.cos(th), 0.0), 1.0)
+            y = min(max(y + step[o + k] * np.sin(th), 0.0), 1.0)
+            assert a.xy[o + k, 0] == x and a.xy[o + k, 1] == y
+        o += int(lens7[t])
     # step lengths: lognormal with mean 0.002 (clipping only shortens steps)
     steps = np.linalg.norm(np.diff(ts.xy, axis=0), axis=1)
     inner = np.ones(steps.size, bool)

@@ -28,6 +28,11 @@ EXPORTS = ("tsm_init", "tsm_last_error", "tsm_abi_version", "tsm_params_default"
            "tsm_pairs", "tsm_pairs_grid", "tsm_pairs_triu", "tsm_pairs_host", "tsm_tune_set_lanes",
            "tsm_tune_set_waves_per_cu", "tsm_graph_capture_pairs", "tsm_graph_launch", "tsm_graph_destroy",
            "tsm_profile_enable", "tsm_profile_read")
+# include/tsm_nn.h (argument types are declared by filter.py)
+NN_EXPORTS = ("tsm_nn_gemm_nt", "tsm_nn_cast_bf16", "tsm_nn_resample", "tsm_nn_conv_fwd", "tsm_nn_conv_bwd",
+              "tsm_nn_gru_h0", "tsm_nn_gru_h0_bwd", "tsm_nn_gru_step_fwd", "tsm_nn_gru_step_bwd", "tsm_nn_colsum",
+              "tsm_nn_rank_loss", "tsm_nn_adam", "tsm_nn_rownorm_bf16", "tsm_nn_topk_f32", "tsm_nn_topk_f64",
+              "tsm_nn_linear_f32", "tsm_nn_split_bf16")
 
 
 class TsmError(RuntimeError):
@@ -88,7 +93,7 @@ def lib():
             if not os.path.exists(LIB_PATH):
                 raise ImportError(f"libtsm.so not built at {LIB_PATH}; run python -m trajsimimeasures_amd.build")
             L = ctypes.CDLL(LIB_PATH)
-            missing = [n for n in EXPORTS if not hasattr(L, n)]
+            missing = [n for n in EXPORTS + NN_EXPORTS if not hasattr(L, n)]
             if missing:
                 raise ImportError(f"{LIB_PATH} lacks {missing}")
             _declare(L)

@@ -1,0 +1,691 @@
+// tsm_nn.hip -- kernels of the learned kNN filter: a trajectory encoder
+// (resampling, Conv1d(2->64, k=3) + ReLU, GRU(64->128), linear head plus a
+// fixed linear skip of the resampled points) with forward and backward
+// passes, its Adam update, and the embedding search (squared L2 distances
+// through an MFMA GEMM, then per-row top-k selection).
+//
+// GEMMs are BF16 x BF16 -> FP32 on the gfx950 matrix cores
+// (v_mfma_f32_16x16x32_bf16): C[M,N] = alpha * A[M,K] . B[N,K]^T + beta * C
+// (+ bias[N]), both operands row-major with K contiguous, so every lane's
+// fragment (8 consecutive K values of one row) is one 16-byte load.  A wave
+// owns a 32x32 output tile (2x2 MFMA tiles), a 256-thread block 64x64.
+//
+// Gate order and equations of the GRU follow the usual convention:
+//   r = sig(Gx_r + Gh_r), z = sig(Gx_z + Gh_z), n = tanh(Gx_n + r * Gh_n),
+//   h' = (1 - z) * n + z * h,  Gx = x W_ih^T + b_ih,  Gh = h W_hh^T + b_hh.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "../../include/tsm.h"
+#include "../../include/tsm_nn.h"
+#include "tsm_host.hpp"
+
+namespace tsm {
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+__device__ __forceinline__ float bf2f(uint16_t v) { return __builtin_bit_cast(float, (uint32_t)v << 16); }
+// round-to-nearest-even FP32 -> BF16 (inputs are finite)
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// ---------------------------------------------------------------- GEMM
+
+struct GemmArgs {
+    int M, N, K;
+    const uint16_t *A;  // [M, lda] bf16, K contiguous
+    int lda;
+    const uint16_t *B;  // [N, ldb] bf16, K contiguous
+    int ldb;
+    float *C;           // [M, ldc] fp32
+    int ldc;
+    float alpha, beta;
+    const float *bias;  // [N] or null
+};
+
+__device__ __forceinline__ bf16x8 load_frag(const uint16_t *base, int ld, int row, int rows, int k, int K) {
+    // 8 consecutive K values of `row`; zero outside the matrix (K % 8 == 0)
+    if (row < rows && k < K) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(base + (int64_t)row * ld + k);
+        return __builtin_bit_cast(bf16x8, v);
+    }
+    return __builtin_bit_cast(bf16x8, make_uint4(0, 0, 0, 0));
+}
+
+__global__ __launch_bounds__(256) void gemm_nt_bf16_kernel(GemmArgs g) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int m0 = blockIdx.y * 64 + (wave >> 1) * 32;
+    const int n0 = blockIdx.x * 64 + (wave & 1) * 32;
+    const int r16 = lane & 15, kb = (lane >> 4) * 8;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < g.K; k0 += 32) {
+        const bf16x8 a0 = load_frag(g.A, g.lda, m0 + r16, g.M, k0 + kb, g.K);
+        const bf16x8 a1 = load_frag(g.A, g.lda, m0 + 16 + r16, g.M, k0 + kb, g.K);
+        const bf16x8 b0 = load_frag(g.B, g.ldb, n0 + r16, g.N, k0 + kb, g.K);
+        const bf16x8 b1 = load_frag(g.B, g.ldb, n0 + 16 + r16, g.N, k0 + kb, g.K);
+        acc[0][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b0, acc[0][0], 0, 0, 0);
+        acc[0][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a0, b1, acc[0][1], 0, 0, 0);
+        acc[1][0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b0, acc[1][0], 0, 0, 0);
+        acc[1][1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a1, b1, acc[1][1], 0, 0, 0);
+    }
+    // D layout: lane holds column (lane & 15), rows 4 * (lane >> 4) + i
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int col = n0 + 16 * j + r16;
+            if (col >= g.N) continue;
+            const float bb = g.bias ? g.bias[col] : 0.f;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = m0 + 16 * i + 4 * (lane >> 4) + e;
+                if (row >= g.M) continue;
+                float *c = g.C + (int64_t)row * g.ldc + col;
+                const float v = g.alpha * acc[i][j][e] + bb;
+                *c = g.beta == 0.f ? v : v + g.beta * *c;
+            }
+        }
+}
+
+// FP32 [rows, cols] (ld_src) -> BF16, optionally transposed: dst[c][r].
+__global__ void cast_bf16_kernel(const float *src, int rows, int cols, int ld_src, uint16_t *dst, int ld_dst,
+                                 int transpose) {
+    const int64_t total = (int64_t)rows * cols;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / cols), c = (int)(i % cols);
+        const uint16_t v = f2bf(src[(int64_t)r * ld_src + c]);
+        if (transpose) dst[(int64_t)c * ld_dst + r] = v;
+        else dst[(int64_t)r * ld_dst + c] = v;
+    }
+}
+
+
+// ---------------------------------------------------------------- encoder kernels
+
+// Resample every trajectory to T points by linear interpolation along the
+// point index, centre to [-1, 1]: X[n][t] = 2 * (p(t / (T-1) * (len-1)) - 0.5).
+// lenf[n] = log(len) - 4.
+__global__ void resample_kernel(const double *xy, const int64_t *offs, const int32_t *ids, int n, int T, float *X,
+                                float *lenf) {
+    const int64_t total = (int64_t)n * T;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int r = (int)(i / T), t = (int)(i % T);
+        const int64_t tr = ids ? ids[r] : r;
+        const int64_t o = offs[tr];
+        const int64_t len = offs[tr + 1] - o;
+        const double s = T > 1 ? (double)t / (double)(T - 1) * (double)(len - 1) : 0.0;
+        int64_t lo = (int64_t)floor(s);
+        if (lo > len - 1) lo = len - 1;
+        const int64_t hi = lo + 1 < len ? lo + 1 : len - 1;
+        const double f = s - (double)lo;
+        const double x = xy[2 * (o + lo)] * (1.0 - f) + xy[2 * (o + hi)] * f;
+        const double y = xy[2 * (o + lo) + 1] * (1.0 - f) + xy[2 * (o + hi) + 1] * f;
+        X[2 * i] = (float)((x - 0.5) * 2.0);
+        X[2 * i + 1] = (float)((y - 0.5) * 2.0);
+        if (t == 0) lenf[r] = (float)(log((double)len) - 4.0);
+    }
+}
+
+// Conv1d(2 -> C, kernel 3, zero padding 1) + ReLU over [n, T] sequences.
+// Wc[c][i][k], bc[c]; H (fp32) and Hb (bf16) are [n*T, C].
+__global__ void conv_fwd_kernel(const float *X, int n, int T, int C, const float *Wc, const float *bc, float *H,
+                                uint16_t *Hb) {
+    const int64_t total = (int64_t)n * T * C;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int c = (int)(i % C);
+        const int64_t nt = i / C;
+        const int t = (int)(nt % T);
+        float a = bc[c];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int tt = t + k - 1;
+            if (tt < 0 || tt >= T) continue;
+            const float *xp = X + 2 * (nt - t + tt);
+            a = fmaf(Wc[(c * 2 + 0) * 3 + k], xp[0], a);
+            a = fmaf(Wc[(c * 2 + 1) * 3 + k], xp[1], a);
+        }
+        a = a > 0.f ? a : 0.f;
+        H[i] = a;
+        Hb[i] = f2bf(a);
+    }
+}
+
+// dWc[c][i][k] = sum dpre[n,t,c] X[n,t+k-1,i], dbc[c] = sum dpre, with
+// dpre = dH * (H > 0).  One block per channel; the reduction is in fixed
+// order within a block, so results are deterministic.
+__global__ __launch_bounds__(256) void conv_bwd_kernel(const float *X, const float *H, const float *dH, int n, int T,
+                                                       int C, float *dWc, float *dbc) {
+    const int c = blockIdx.x;
+    float acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    const int64_t total = (int64_t)n * T;
+    for (int64_t nt = threadIdx.x; nt < total; nt += blockDim.x) {
+        const int64_t i = nt * C + c;
+        const float g = H[i] > 0.f ? dH[i] : 0.f;
+        if (g == 0.f) continue;
+        const int t = (int)(nt % T);
+        acc[6] += g;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int tt = t + k - 1;
+            if (tt < 0 || tt >= T) continue;
+            const float *xp = X + 2 * (nt - t + tt);
+            acc[0 * 3 + k] = fmaf(g, xp[0], acc[0 * 3 + k]);
+            acc[1 * 3 + k] = fmaf(g, xp[1], acc[1 * 3 + k]);
+        }
+    }
+    __shared__ float sh[7][256];
+#pragma unroll
+    for (int q = 0; q < 7; ++q) sh[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o)
+#pragma unroll
+            for (int q = 0; q < 7; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) dWc[c * 6 + threadIdx.x] = sh[threadIdx.x][0];
+    if (threadIdx.x == 6) dbc[c] = sh[6][0];
+}
+
+__device__ __forceinline__ float sigm(float x) { return 1.f / (1.f + __expf(-x)); }
+
+// h0 = tanh(u * lenf); writes fp32 and bf16 copies.
+__global__ void gru_h0_kernel(const float *u, const float *lenf, int n, int Hd, float *h, uint16_t *hb) {
+    const int64_t total = (int64_t)n * Hd;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const float v = tanhf(u[i % Hd] * lenf[i / Hd]);
+        h[i] = v;
+        hb[i] = f2bf(v);
+    }
+}
+
+// One GRU step: Gx row stride ldgx (the [n, T, 3H] input projections at step
+// t), Gh [n, 3H] the recurrent projection of h.  Saves r, z, nn, Gh_n for the
+// backward pass and writes h' (fp32 + bf16).
+__global__ void gru_step_fwd_kernel(const float *Gx, int64_t ldgx, const float *Gh, const float *h, int n, int Hd,
+                                    float *r_s, float *z_s, float *n_s, float *ghn_s, float *hn, uint16_t *hnb) {
+    const int64_t total = (int64_t)n * Hd;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = i / Hd;
+        const int j = (int)(i % Hd);
+        const float *gx = Gx + row * ldgx;
+        const float *gh = Gh + row * 3 * Hd;
+        const float r = sigm(gx[j] + gh[j]);
+        const float z = sigm(gx[Hd + j] + gh[Hd + j]);
+        const float nn = tanhf(gx[2 * Hd + j] + r * gh[2 * Hd + j]);
+        const float hv = (1.f - z) * nn + z * h[i];
+        r_s[i] = r;
+        z_s[i] = z;
+        n_s[i] = nn;
+        ghn_s[i] = gh[2 * Hd + j];
+        hn[i] = hv;
+        hnb[i] = f2bf(hv);
+    }
+}
+
+// Backward of one GRU step.  dh: gradient w.r.t. h' (in), h: the step's
+// input state.  Writes dGx (pre-activation gradients of the input
+// projection, row stride lddgx), dGh [n, 3H] (fp32 + bf16) and
+// dhp = dh * z, the direct part of the gradient w.r.t. h.
+__global__ void gru_step_bwd_kernel(const float *dh, const float *h, const float *r_s, const float *z_s,
+                                    const float *n_s, const float *ghn_s, int n, int Hd, float *dGx, int64_t lddgx,
+                                    float *dGh, uint16_t *dGhb, float *dhp) {
+    const int64_t total = (int64_t)n * Hd;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = i / Hd;
+        const int j = (int)(i % Hd);
+        const float g = dh[i], r = r_s[i], z = z_s[i], nn = n_s[i], ghn = ghn_s[i];
+        const float dnn = g * (1.f - z);
+        const float dz = g * (h[i] - nn);
+        const float dan = dnn * (1.f - nn * nn);
+        const float dar = dan * ghn * r * (1.f - r);
+        const float daz = dz * z * (1.f - z);
+        float *gx = dGx + row * lddgx;
+        gx[j] = dar;
+        gx[Hd + j] = daz;
+        gx[2 * Hd + j] = dan;
+        float *gh = dGh + row * 3 * Hd;
+        uint16_t *ghb = dGhb + row * 3 * Hd;
+        const float dghn = dan * r;
+        gh[j] = dar;
+        gh[Hd + j] = daz;
+        gh[2 * Hd + j] = dghn;
+        ghb[j] = f2bf(dar);
+        ghb[Hd + j] = f2bf(daz);
+        ghb[2 * Hd + j] = f2bf(dghn);
+        dhp[i] = g * z;
+    }
+}
+
+// du[j] = sum_n dh0[n,j] * (1 - h0^2) * lenf[n]  (h0 = tanh(u * lenf)).
+__global__ __launch_bounds__(256) void gru_h0_bwd_kernel(const float *dh0, const float *h0, const float *lenf, int n,
+                                                         int Hd, float *du) {
+    const int j = blockIdx.x;
+    float acc = 0.f;
+    for (int r = threadIdx.x; r < n; r += blockDim.x) {
+        const float hv = h0[(int64_t)r * Hd + j];
+        acc = fmaf(dh0[(int64_t)r * Hd + j] * (1.f - hv * hv), lenf[r], acc);
+    }
+    __shared__ float sh[256];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) du[j] = sh[0];
+}
+
+// out[c] = sum over rows of M[r, c] (row stride ld); one block per column,
+// fixed reduction order.
+__global__ __launch_bounds__(256) void colsum_kernel(const float *M, int64_t rows, int cols, int64_t ld, float *out,
+                                                     float beta) {
+    const int c = blockIdx.x;
+    float acc = 0.f;
+    for (int64_t r = threadIdx.x; r < rows; r += blockDim.x) acc += M[r * ld + c];
+    __shared__ float sh[256];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) sh[threadIdx.x] += sh[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[c] = sh[0] + (beta == 0.f ? 0.f : beta * out[c]);
+}
+
+// Pairwise ranking loss over anchor lists.  Embeddings E [B*(1+K), D]: row
+// b*(1+K) is anchor b, the next K rows its candidates sorted by ascending
+// exact distance.  For positives p < P and negatives q >= Q0 (ranks),
+//   loss_b = mean softplus((d2_p - d2_q) / (tau * s_b)),  s_b = mean_k d2_k
+// (s_b held constant), d2_k = |e_k - e_b|^2.  Writes dE (mean over anchors)
+// and the per-anchor losses.  One block of 256 threads per anchor.
+__global__ __launch_bounds__(256) void rank_loss_kernel(const float *E, int B, int K, int D, int P, int Q0, float tau,
+                                                        float *dE, float *loss) {
+    const int b = blockIdx.x;
+    const float *ea = E + (int64_t)b * (1 + K) * D;
+    __shared__ float d2[1024];
+    __shared__ float g[1024];
+    __shared__ float red[256];
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        const float *ek = ea + (int64_t)(1 + k) * D;
+        float s = 0.f;
+        for (int q = 0; q < D; ++q) {
+            const float t = ek[q] - ea[q];
+            s = fmaf(t, t, s);
+        }
+        d2[k] = s;
+        g[k] = 0.f;
+    }
+    __syncthreads();
+    float part = 0.f;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) part += d2[k];
+    red[threadIdx.x] = part;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    const float scale = fmaxf(red[0] / (float)K, 1e-12f) * tau;
+    __syncthreads();
+    const int nneg = K - Q0;
+    const float inv = 1.f / ((float)P * (float)nneg * (float)B);
+    float lsum = 0.f;
+    // thread handles negatives q; loops over the P positives
+    for (int q = Q0 + threadIdx.x; q < K; q += blockDim.x) {
+        float gq = 0.f;
+        for (int p = 0; p < P; ++p) {
+            const float x = (d2[p] - d2[q]) / scale;
+            const float sp = x > 20.f ? x : log1pf(__expf(x));
+            lsum += sp;
+            const float s = sigm(x) * inv / scale;
+            gq -= s;
+            atomicAdd(&g[p], s);
+        }
+        g[q] = gq;
+    }
+    red[threadIdx.x] = lsum;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss[b] = red[0] / ((float)P * (float)nneg);
+    __syncthreads();
+    // dE_k = 2 (e_k - e_a) g_k ; dE_a = -sum_k dE_k
+    float *dea = dE + (int64_t)b * (1 + K) * D;
+    for (int q = threadIdx.x; q < D; q += blockDim.x) dea[q] = 0.f;
+    __syncthreads();
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        const float *ek = ea + (int64_t)(1 + k) * D;
+        float *dek = dea + (int64_t)(1 + k) * D;
+        for (int q = 0; q < D; ++q) dek[q] = 2.f * (ek[q] - ea[q]) * g[k];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < D; q += blockDim.x) {
+        float s = 0.f;
+        for (int k = 0; k < K; ++k) s += dea[(int64_t)(1 + k) * D + q];
+        dea[q] = -s;
+    }
+}
+
+// Adam on a flat FP32 parameter vector (bias-corrected, no weight decay).
+__global__ void adam_kernel(float *p, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2,
+                            float eps, float c1, float c2) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+// Row squared norms of a bf16 [n, D] matrix, in FP32.
+__global__ void rownorm_bf16_kernel(const uint16_t *E, int64_t n, int D, float *out) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int q = 0; q < D; ++q) {
+            const float v = bf2f(E[r * D + q]);
+            s = fmaf(v, v, s);
+        }
+        out[r] = s;
+    }
+}
+
+
+// out[n, Dout] = X[n, K] . W[Dout, K]^T + beta * out, in FP32 (the fixed skip
+// map of the encoder: precision matters more than speed for it).
+__global__ void linear_f32_kernel(const float *X, int64_t n, int K, const float *W, int Dout, float *out, float beta) {
+    const int64_t total = n * Dout;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = i / Dout;
+        const int c = (int)(i % Dout);
+        const float *x = X + r * K;
+        const float *w = W + (int64_t)c * K;
+        float a = 0.f;
+        for (int k = 0; k < K; ++k) a = fmaf(x[k], w[k], a);
+        out[i] = beta == 0.f ? a : a + beta * out[i];
+    }
+}
+
+// Split FP32 rows E [n, D] into BF16 hi and lo = bf16(e - hi) parts laid
+// out as [hi, lo, hi] (database side) or [hi, hi, lo] (query side) in
+// out [n, 3D], and the FP32 squared row norms of E.
+__global__ void split_bf16_kernel(const float *E, int64_t n, int D, int query_side, uint16_t *out, float *norm) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const float *e = E + r * D;
+        uint16_t *o = out + r * 3 * D;
+        float s = 0.f;
+        for (int q = 0; q < D; ++q) {
+            const float v = e[q];
+            s = fmaf(v, v, s);
+            const uint16_t hi = f2bf(v);
+            const uint16_t lo = f2bf(v - bf2f(hi));
+            o[q] = hi;
+            o[D + q] = query_side ? hi : lo;
+            o[2 * D + q] = query_side ? lo : hi;
+        }
+        if (norm) norm[r] = s;
+    }
+}
+
+// ---------------------------------------------------------------- top-k
+
+// Order-preserving unsigned keys of floating-point values.
+__device__ __forceinline__ uint32_t fkey(float f) {
+    const uint32_t u = __builtin_bit_cast(uint32_t, f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ uint64_t fkey(double f) {
+    const uint64_t u = __builtin_bit_cast(uint64_t, f);
+    return (u & 0x8000000000000000ull) ? ~u : (u | 0x8000000000000000ull);
+}
+
+// k smallest entries of each row of V [rows, cols] (row stride ld): one
+// block per row; radix select on 8-bit digits of the order-preserving key
+// finds the k-th smallest key, then the row is gathered (all keys below it,
+// then keys equal to it up to k).  Output order within a row is unspecified.
+template <typename T>
+__global__ __launch_bounds__(256) void topk_kernel(const T *V, int64_t rows, int64_t cols, int64_t ld, int k,
+                                                   int32_t *out_idx, T *out_val) {
+    using K_t = decltype(fkey((T)0));
+    constexpr int kBits = sizeof(K_t) * 8;
+    __shared__ uint32_t hist[256];
+    __shared__ K_t prefix_s;
+    __shared__ int remain_s;
+    __shared__ int n_less, n_eq;
+    const int64_t row = blockIdx.x;
+    const T *v = V + row * ld;
+    K_t prefix = 0, mask = 0;
+    int remain = k;
+    for (int shift = kBits - 8; shift >= 0; shift -= 8) {
+        hist[threadIdx.x] = 0;
+        __syncthreads();
+        for (int64_t c = threadIdx.x; c < cols; c += blockDim.x) {
+            const K_t key = fkey(v[c]);
+            if ((key & mask) == prefix) atomicAdd(&hist[(uint32_t)(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            int cum = 0, d = 0;
+            for (; d < 256; ++d) {
+                if (cum + (int)hist[d] >= remain) break;
+                cum += (int)hist[d];
+            }
+            if (d > 255) d = 255;
+            prefix_s = prefix | ((K_t)d << shift);
+            remain_s = remain - cum;
+        }
+        __syncthreads();
+        prefix = prefix_s;
+        remain = remain_s;
+        mask |= ((K_t)255) << shift;
+        __syncthreads();
+    }
+    // prefix is the k-th smallest key; (k - remain) keys are strictly below it
+    if (threadIdx.x == 0) {
+        n_less = 0;
+        n_eq = 0;
+    }
+    __syncthreads();
+    const int below = k - remain;
+    int32_t *oi = out_idx + row * (int64_t)k;
+    T *ov = out_val ? out_val + row * (int64_t)k : nullptr;
+    for (int64_t c = threadIdx.x; c < cols; c += blockDim.x) {
+        const K_t key = fkey(v[c]);
+        int slot = -1;
+        if (key < prefix) slot = atomicAdd(&n_less, 1);
+        else if (key == prefix) {
+            const int e = atomicAdd(&n_eq, 1);
+            if (e < remain) slot = below + e;
+        }
+        if (slot >= 0 && slot < k) {
+            oi[slot] = (int32_t)c;
+            if (ov) ov[slot] = v[c];
+        }
+    }
+}
+
+}  // namespace tsm
+
+// ---------------------------------------------------------------- C ABI (part)
+
+namespace {
+using namespace tsm;
+inline int grid1(int64_t n, int b = 256) {
+    int64_t g = (n + b - 1) / b;
+    if (g < 1) g = 1;
+    if (g > 65536) g = 65536;
+    return (int)g;
+}
+}  // namespace
+
+extern "C" {
+
+int tsm_nn_gemm_nt(int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb, float *C, int ldc,
+                   float alpha, float beta, const float *bias, void *stream) {
+    if (M < 0 || N < 0 || K < 0 || (K % 8) != 0) return fail(TSM_E_ARG, "gemm: K must be a multiple of 8");
+    if (M == 0 || N == 0) return TSM_OK;
+    if (!A || !B || !C || lda < K || ldb < K || ldc < N || (lda % 8) || (ldb % 8) ||
+        (reinterpret_cast<uintptr_t>(A) % 16) || (reinterpret_cast<uintptr_t>(B) % 16))
+        return fail(TSM_E_ARG, "gemm: bad operands (16-byte aligned, leading dimensions multiples of 8)");
+    GemmArgs g{M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, bias};
+    dim3 grid((N + 63) / 64, (M + 63) / 64);
+    hipLaunchKernelGGL(gemm_nt_bf16_kernel, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), g);
+    TSM_HIP(hipGetLastError());
+    return TSM_OK;
+}
+
+int tsm_nn_cast_bf16(const float *src, int rows, int cols, int ld_src, uint16_t *dst, int ld_dst, int transpose,
+                     void *stream) {
+    if (rows < 0 || cols < 0 || !src || !dst) return fail(TSM_E_ARG, "cast: bad arguments");
+    if (ld_src < cols || ld_dst < (transpose ? rows : cols)) return fail(TSM_E_ARG, "cast: bad leading dimension");
+    if ((int64_t)rows * cols == 0) return TSM_OK;
+    hipLaunchKernelGGL(cast_bf16_kernel, dim3(grid1((int64_t)rows * cols)), dim3(256), 0,
+                       reinterpret_cast<hipStream_t>(stream), src, rows, cols, ld_src, dst, ld_dst, transpose);
+    TSM_HIP(hipGetLastError());
+    return TSM_OK;
+}
+
+
+#define TSM_ST reinterpret_cast<hipStream_t>(stream)
+#define TSM_LAUNCHED()            \
+    do {                          \
+        TSM_HIP(hipGetLastError()); \
+        return TSM_OK;            \
+    } while (0)
+
+int tsm_nn_resample(const double *xy64, const int64_t *offsets, const int32_t *ids, int n, int T, float *X,
+                    float *lenf, void *stream) {
+    if (n < 0 || T < 2 || !xy64 || !offsets || !X || !lenf) return fail(TSM_E_ARG, "resample: bad arguments");
+    if (n == 0) return TSM_OK;
+    hipLaunchKernelGGL(resample_kernel, dim3(grid1((int64_t)n * T)), dim3(256), 0, TSM_ST, xy64, offsets, ids, n, T, X,
+                       lenf);
+    TSM_LAUNCHED();
+}
+
+int tsm_nn_conv_fwd(const float *X, int n, int T, int C, const float *Wc, const float *bc, float *H, uint16_t *Hb,
+                    void *stream) {
+    if (n < 0 || T < 1 || C < 1 || !X || !Wc || !bc || !H || !Hb) return fail(TSM_E_ARG, "conv_fwd: bad arguments");
+    if (n == 0) return TSM_OK;
+    hipLaunchKernelGGL(conv_fwd_kernel, dim3(grid1((int64_t)n * T * C)), dim3(256), 0, TSM_ST, X, n, T, C, Wc, bc, H,
+                       Hb);
+    TSM_LAUNCHED();
+}
+
+int tsm_nn_conv_bwd(const float *X, const float *H, const float *dH, int n, int T, int C, float *dWc, float *dbc,
+                    void *stream) {
+    if (n < 1 || T < 1 || C < 1 || !X || !H || !dH || !dWc || !dbc) return fail(TSM_E_ARG, "conv_bwd: bad arguments");
+    hipLaunchKernelGGL(conv_bwd_kernel, dim3(C), dim3(256), 0, TSM_ST, X, H, dH, n, T, C, dWc, dbc);
+    TSM_LAUNCHED();
+}
+
+int tsm_nn_gru_h0(const float *u, const float *lenf, int n, int Hd, float *h, uint16_t *hb, void *stream) {
+    if (n < 0 || Hd < 1 || !u || !lenf || !h || !hb) return fail(TSM_E_ARG, "gru_h0: bad arguments");
+    if (n == 0) return TSM_OK;
+    hipLaunchKernelGGL(gru_h0_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, TSM_ST, u, lenf, n, Hd, h, hb);
+    TSM_LAUNCHED();
+}
+
+int tsm_nn_gru_step_fwd(const float *Gx, int64_t ldgx, const float *Gh, const float *h, int n, int Hd, float *r,
+                        float *z, float *nn, float *ghn, float *hn, uint16_t *hnb, void *stream) {
+    if (n < 0 || Hd < 1 || ldgx < 3 * Hd) return fail(TSM_E_ARG, "gru_step_fwd: bad arguments");
+    if (n == 0) return TSM_OK;
+    hipLaunchKernelGGL(gru_step_fwd_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, TSM_ST, Gx, ldgx, Gh, h, n, Hd,
+                       r, z, nn, ghn, hn, hnb);
+    TSM_LAUNCHED();
+}
+
+int tsm_nn_gru_step_bwd(const float *dh, const float *h, const float *r, const float *z, const float *nn,
+                        const float *ghn, int n, int Hd, float *dGx, int64_t lddgx, float *dGh, uint16_t *dGhb,
+                        float *dhp, void *stream) {
+    if (n < 0 || Hd < 1 || lddgx < 3 * Hd) return fail(TSM_E_ARG, "gru_step_bwd: bad arguments");
+    if (n == 0) return TSM_OK;
+    hipLaunchKernelGGL(gru_step_bwd_kernel, dim3(grid1((int64_t)n * Hd)), dim3(256), 0, TSM_ST, dh, h, r, z, nn, ghn,
+                       n, Hd, dGx, lddgx, dGh, dGhb, dhp);
+    TSM_LAUNCHED();
+}
+
+int tsm_nn_gru_h0_bwd(const float *dh0, const float *h0, const float *lenf, int n, int Hd, float *du, void *stream) {
+    if (n < 1 || Hd < 1) return fail(TSM_E_ARG, "gru_h0_bwd: bad arguments");
+    hipLaunchKernelGGL(gru_h0_bwd_kernel, dim3(Hd), dim3(256), 0, TSM_ST, dh0, h0, lenf, n, Hd, du);
+    TSM_LAUNCHED();
+}
+
+int tsm_nn_colsum(const float *M, int64_t rows, int cols, int64_t ld, float *out, float beta, void *stream) {
+    if (rows < 0 || cols < 1 || ld < cols || !M || !out) return fail(TSM_E_ARG, "colsum: bad arguments");
+    hipLaunchKernelGGL(colsum_kernel, dim3(cols), dim3(256), 0, TSM_ST, M, rows, cols, ld, out, beta);
+    TSM_LAUNCHED();
+}
+
+int tsm_nn_rank_loss(const float *E, int B, int K, int D, int P, int Q0, float tau, float *dE, float *loss,
+                     void *stream) {
+    if (B < 1 || K < 2 || K > 1024 || D < 1 || P < 1 || Q0 < P || Q0 >= K || !(tau > 0.f) || !E || !dE || !loss)
+        return fail(TSM_E_ARG, "rank_loss: need 1 <= P <= Q0 < K <= 1024, tau > 0");
+    hipLaunchKernelGGL(rank_loss_kernel, dim3(B), dim3(256), 0, TSM_ST, E, B, K, D, P, Q0, tau, dE, loss);
+    TSM_LAUNCHED();
+}
+
+int tsm_nn_adam(float *p, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2, float eps,
+                int64_t step, void *stream) {
+    if (n < 0 || step < 1 || !p || !g || !m || !v) return fail(TSM_E_ARG, "adam: bad arguments");
+    if (n == 0) return TSM_OK;
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(adam_kernel, dim3(grid1(n)), dim3(256), 0, TSM_ST, p, g, m, v, n, lr, b1, b2, eps, c1, c2);
+    TSM_LAUNCHED();
+}
+
+int tsm_nn_rownorm_bf16(const uint16_t *E, int64_t n, int D, float *out, void *stream) {
+    if (n < 0 || D < 1 || !E || !out) return fail(TSM_E_ARG, "rownorm: bad arguments");
+    if (n == 0) return TSM_OK;
+    hipLaunchKernelGGL(rownorm_bf16_kernel, dim3(grid1(n)), dim3(256), 0, TSM_ST, E, n, D, out);
+    TSM_LAUNCHED();
+}
+
+int tsm_nn_topk_f32(const float *V, int64_t rows, int64_t cols, int64_t ld, int k, int32_t *idx, float *val,
+                    void *stream) {
+    if (rows < 0 || k < 1 || k > cols || ld < cols || !V || !idx) return fail(TSM_E_ARG, "topk: need 1 <= k <= cols");
+    if (rows == 0) return TSM_OK;
+    if (rows > 0x7fffffff) return fail(TSM_E_ARG, "topk: too many rows");
+    hipLaunchKernelGGL(topk_kernel<float>, dim3((unsigned)rows), dim3(256), 0, TSM_ST, V, rows, cols, ld, k, idx, val);
+    TSM_LAUNCHED();
+}
+
+int tsm_nn_topk_f64(const double *V, int64_t rows, int64_t cols, int64_t ld, int k, int32_t *idx, double *val,
+                    void *stream) {
+    if (rows < 0 || k < 1 || k > cols || ld < cols || !V || !idx) return fail(TSM_E_ARG, "topk: need 1 <= k <= cols");
+    if (rows == 0) return TSM_OK;
+    if (rows > 0x7fffffff) return fail(TSM_E_ARG, "topk: too many rows");
+    hipLaunchKernelGGL(topk_kernel<double>, dim3((unsigned)rows), dim3(256), 0, TSM_ST, V, rows, cols, ld, k, idx, val);
+    TSM_LAUNCHED();
+}
+
+
+int tsm_nn_linear_f32(const float *X, int64_t n, int K, const float *W, int Dout, float *out, float beta,
+                      void *stream) {
+    if (n < 0 || K < 1 || Dout < 1 || !X || !W || !out) return fail(TSM_E_ARG, "linear: bad arguments");
+    if (n == 0) return TSM_OK;
+    hipLaunchKernelGGL(linear_f32_kernel, dim3(grid1(n * Dout)), dim3(256), 0, TSM_ST, X, n, K, W, Dout, out, beta);
+    TSM_LAUNCHED();
+}
+
+int tsm_nn_split_bf16(const float *E, int64_t n, int D, int query_side, uint16_t *out, float *norm, void *stream) {
+    if (n < 0 || D < 1 || !E || !out) return fail(TSM_E_ARG, "split: bad arguments");
+    if (n == 0) return TSM_OK;
+    hipLaunchKernelGGL(split_bf16_kernel, dim3(grid1(n)), dim3(256), 0, TSM_ST, E, n, D, query_side, out, norm);
+    TSM_LAUNCHED();
+}
+
+}  // extern "C"

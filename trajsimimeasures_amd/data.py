@@ -137,8 +137,14 @@ STEP_SIGMA = 0.5
 TURN_SIGMA = 0.3
 
 
-def random_walks(rng: np.random.Generator, lengths, chunk: int = 65536) -> TrajSet:
-    """Seeded random-walk trajectories with the given lengths (see module doc)."""
+def random_walks(rng: np.random.Generator, lengths) -> TrajSet:
+    """Seeded random-walk trajectories with the given lengths (see module doc).
+
+    All random draws happen up front in a fixed order (start points, initial
+    headings, one turn and one step per point), then the walks advance one
+    point index at a time over the trajectories still that long (length-
+    sorted, so each index touches a prefix), so the cost is linear in the
+    number of points whatever the length spread."""
     lens = np.asarray(lengths, dtype=np.int64).reshape(-1)
     if lens.size and lens.min() < 1:
         raise ValueError("lengths must be >= 1")
@@ -152,30 +158,27 @@ def random_walks(rng: np.random.Generator, lengths, chunk: int = 65536) -> TrajS
     mu = np.log(STEP_MEAN) - 0.5 * STEP_SIGMA ** 2
     step = np.exp(rng.normal(mu, STEP_SIGMA, size=total))
     xy = np.empty((total, 2), dtype=np.float64)
-    for c0 in range(0, n, chunk):
-        c1 = min(n, c0 + chunk)
-        cl = lens[c0:c1]
-        L = int(cl.max())
-        m = c1 - c0
-        # positions within the chunk's padded [m, L] grid
-        base = offs[c0:c1]
-        col = np.arange(L)[None, :]
-        valid = col < cl[:, None]
-        idx = np.where(valid, base[:, None] + col, 0)
-        # heading: theta_k = theta_0 + sum_{1..k} turn (turn of point 0 unused)
-        tr = np.where(valid, turn[idx], 0.0)
-        tr[:, 0] = 0.0
-        theta = th0[c0:c1, None] + np.cumsum(tr, axis=1)
-        st = np.where(valid, step[idx], 0.0)
-        dx = st * np.cos(theta)
-        dy = st * np.sin(theta)
-        px = np.empty((m, L))
-        py = np.empty((m, L))
-        px[:, 0] = p0[c0:c1, 0]
-        py[:, 0] = p0[c0:c1, 1]
-        for k in range(1, L):
-            px[:, k] = np.clip(px[:, k - 1] + dx[:, k], 0.0, 1.0)
-            py[:, k] = np.clip(py[:, k - 1] + dy[:, k], 0.0, 1.0)
-        xy[idx[valid], 0] = px[valid]
-        xy[idx[valid], 1] = py[valid]
+    if n == 0:
+        return TrajSet(xy, offs)
+    order = np.argsort(-lens, kind="stable")
+    slen = lens[order]
+    base = offs[:-1][order]
+    px = p0[order, 0].copy()
+    py = p0[order, 1].copy()
+    th = th0[order]
+    cs = np.zeros(n)                     # running sum of turns since point 0
+    xy[base, 0] = px
+    xy[base, 1] = py
+    # number of trajectories longer than k, for every k
+    active = np.searchsorted(-slen, -np.arange(int(slen[0])), side="left")
+    for k in range(1, int(slen[0])):
+        m = int(active[k])
+        idx = base[:m] + k
+        cs[:m] += turn[idx]
+        theta = th[:m] + cs[:m]
+        st = step[idx]
+        px[:m] = np.clip(px[:m] + st * np.cos(theta), 0.0, 1.0)
+        py[:m] = np.clip(py[:m] + st * np.sin(theta), 0.0, 1.0)
+        xy[idx, 0] = px[:m]
+        xy[idx, 1] = py[:m]
     return TrajSet(xy, offs)

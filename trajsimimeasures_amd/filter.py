@@ -1,0 +1,415 @@
+"""Learned kNN filter for DTW: a trajectory encoder trained on distances the
+exact DP kernel produces, used to shortlist candidates that the exact kernel
+then re-ranks.
+
+Encoder (all kernels in csrc/tsm_nn.hip, reached through include/tsm_nn.h):
+  x  = trajectory resampled to T = 16 points along its index, centred to [-1, 1]
+  h1 = ReLU(Conv1d(2 -> 64, kernel 3, padding 1)(x))
+  h0 = tanh(u * (log(len) - 4));  hT = GRU(64 -> 128)(h1, h0)
+  e  = W_o hT + b_o + S x_flat          (128-d; S a fixed orthonormal 32 -> 128 map)
+With W_o = 0 the embedding is S x_flat, whose L2 distance is the resampled
+trajectories' L2 distance: the non-learned method.  Training learns the
+correction W_o hT + b_o.
+
+Weights are FP32 masters with BF16 copies for the GEMMs (FP32 accumulation
+on the matrix cores).  Training minimises a pairwise ranking loss on anchor
+lists: each anchor's candidates (its nearest by the current embedding) carry
+exact DTW labels from the DP kernel, and every true top-10 candidate is
+pushed closer than every candidate ranked 20th or worse.
+
+Search: distances |e_d|^2 - 2 e_q . e_d through the MFMA GEMM with the
+embeddings split into BF16 hi + lo parts (three products in one K = 384
+GEMM, ~16-bit mantissa), the C smallest per query by a radix select, then
+the exact DTW of those C pairs by the DP kernel and the k smallest of them.
+
+The reference's learned kNN path (test_knn_time.py:123-231) embeds with a
+PyTorch model, searches a faiss IVF index and has no exact re-rank; its
+training (nn/MLP_trajsimi_train.py:38-144) regresses embedding distances on
+ground-truth matrices computed by traj_dist (preprocessing/trajsimi.py:123-149).
+"""
+from __future__ import annotations
+
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _lib
+from .engine import DeviceTrajSet, KernelTimer, pairs as dp_pairs
+
+__all__ = ["Encoder", "FilterIndex", "exact_knn", "recall_at", "train_encoder", "hard_pairs"]
+
+T = 16
+C_CONV = 64
+H = 128
+D = 128
+G3 = 3 * H
+
+
+def _nn():
+    L = _lib.lib()
+    if not getattr(L, "_nn_declared", False):
+        P, I64 = ctypes.c_void_p, ctypes.c_int64
+        i, f = ctypes.c_int, ctypes.c_float
+        L.tsm_nn_gemm_nt.argtypes = [i, i, i, P, i, P, i, P, i, f, f, P, P]
+        L.tsm_nn_cast_bf16.argtypes = [P, i, i, i, P, i, i, P]
+        L.tsm_nn_resample.argtypes = [P, P, P, i, i, P, P, P]
+        L.tsm_nn_conv_fwd.argtypes = [P, i, i, i, P, P, P, P, P]
+        L.tsm_nn_conv_bwd.argtypes = [P, P, P, i, i, i, P, P, P]
+        L.tsm_nn_gru_h0.argtypes = [P, P, i, i, P, P, P]
+        L.tsm_nn_gru_h0_bwd.argtypes = [P, P, P, i, i, P, P]
+        L.tsm_nn_gru_step_fwd.argtypes = [P, I64, P, P, i, i, P, P, P, P, P, P, P]
+        L.tsm_nn_gru_step_bwd.argtypes = [P, P, P, P, P, P, i, i, P, I64, P, P, P, P]
+        L.tsm_nn_colsum.argtypes = [P, I64, i, I64, P, f, P]
+        L.tsm_nn_rank_loss.argtypes = [P, i, i, i, i, i, f, P, P, P]
+        L.tsm_nn_adam.argtypes = [P, P, P, P, I64, f, f, f, f, I64, P]
+        L.tsm_nn_rownorm_bf16.argtypes = [P, I64, i, P, P]
+        L.tsm_nn_topk_f32.argtypes = [P, I64, I64, I64, i, P, P, P]
+        L.tsm_nn_topk_f64.argtypes = [P, I64, I64, I64, i, P, P, P]
+        L.tsm_nn_linear_f32.argtypes = [P, I64, i, P, i, P, f, P]
+        L.tsm_nn_split_bf16.argtypes = [P, I64, i, i, P, P, P]
+        for n in ("gemm_nt", "cast_bf16", "resample", "conv_fwd", "conv_bwd", "gru_h0", "gru_h0_bwd",
+                  "gru_step_fwd", "gru_step_bwd", "colsum", "rank_loss", "adam", "rownorm_bf16", "topk_f32",
+                  "topk_f64", "linear_f32", "split_bf16"):
+            getattr(L, "tsm_nn_" + n).restype = ctypes.c_int
+        L._nn_declared = True
+    return L
+
+
+def _s():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _p(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _bf16(t: torch.Tensor, transpose: bool = False) -> torch.Tensor:
+    """FP32 [r, c] -> BF16 bits (uint16 in an int16 tensor), optionally [c, r]."""
+    r, c = t.shape
+    out = torch.empty((c, r) if transpose else (r, c), dtype=torch.int16, device=t.device)
+    _lib.check(_nn().tsm_nn_cast_bf16(_p(t), r, c, t.stride(0), _p(out), out.stride(0), int(transpose), _s()))
+    return out
+
+
+def gemm_nt(A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, alpha=1.0, beta=0.0, bias=None):
+    """C = alpha * A . B^T + beta * C (+ bias); A [M, K], B [N, K] bf16 bits, C fp32 [M, N]."""
+    M, K = A.shape
+    N = B.shape[0]
+    _lib.check(_nn().tsm_nn_gemm_nt(M, N, K, _p(A), A.stride(0), _p(B), B.stride(0), _p(C), C.stride(0),
+                                    float(alpha), float(beta), _p(bias), _s()))
+    return C
+
+
+def _skip_matrix(seed: int = 7) -> np.ndarray:
+    """Fixed 128 x 32 map with orthonormal columns scaled by sqrt(8 / T):
+    |S x - S y| = sqrt(8 / T) |x - y| for resampled trajectories x, y."""
+    g = np.random.default_rng(seed).standard_normal((D, 2 * T))
+    q, _ = np.linalg.qr(g)
+    return (q * np.sqrt(8.0 / T)).astype(np.float32)
+
+
+# Flat parameter layout: name -> shape.
+LAYOUT = (("Wc", (C_CONV, 2, 3)), ("bc", (C_CONV,)), ("W_ih", (G3, C_CONV)), ("b_ih", (G3,)),
+          ("W_hh", (G3, H)), ("b_hh", (G3,)), ("u", (H,)), ("W_o", (D, H)), ("b_o", (D,)))
+
+
+def _views(flat: torch.Tensor) -> dict:
+    out, o = {}, 0
+    for name, shape in LAYOUT:
+        n = int(np.prod(shape))
+        out[name] = flat[o:o + n].view(shape)
+        o += n
+    return out
+
+
+NPARAM = sum(int(np.prod(s)) for _, s in LAYOUT)
+
+
+def init_params(seed: int = 0) -> np.ndarray:
+    """PyTorch-style uniform fan-in initialisation; the head W_o, b_o starts
+    at zero so the untrained encoder is the resampled-L2 embedding."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(NPARAM, dtype=np.float32)
+    v = _views(torch.from_numpy(p))
+    kc = 1.0 / np.sqrt(6.0)
+    v["Wc"][:] = torch.from_numpy(rng.uniform(-kc, kc, (C_CONV, 2, 3)).astype(np.float32))
+    v["bc"][:] = torch.from_numpy(rng.uniform(-kc, kc, C_CONV).astype(np.float32))
+    kg = 1.0 / np.sqrt(H)
+    for name in ("W_ih", "b_ih", "W_hh", "b_hh"):
+        v[name][:] = torch.from_numpy(rng.uniform(-kg, kg, tuple(v[name].shape)).astype(np.float32))
+    v["u"][:] = torch.from_numpy((rng.standard_normal(H) * 0.1).astype(np.float32))
+    return p
+
+
+class Encoder:
+    """The encoder's parameters on the device, with forward and backward
+    passes made of libtsm.so kernels."""
+
+    def __init__(self, params: np.ndarray | None = None, device="cuda"):
+        self.device = torch.device(device)
+        p = init_params() if params is None else np.asarray(params, dtype=np.float32)
+        if p.size != NPARAM:
+            raise ValueError(f"expected {NPARAM} parameters, got {p.size}")
+        self.flat = torch.tensor(p, device=self.device)
+        self.w = _views(self.flat)
+        self.S = torch.tensor(_skip_matrix(), device=self.device)
+        self.refresh()
+
+    def refresh(self):
+        """Rebuild the BF16 weight copies after an update."""
+        w = self.w
+        self.W_ih_b = _bf16(w["W_ih"])               # [384, 64]  (B operand of x W_ih^T)
+        self.W_ihT_b = _bf16(w["W_ih"], True)        # [64, 384]  (B operand of dGx W_ih)
+        self.W_hh_b = _bf16(w["W_hh"])               # [384, 128]
+        self.W_hhT_b = _bf16(w["W_hh"], True)        # [128, 384]
+        self.W_o_b = _bf16(w["W_o"])                 # [128, 128]
+        self.W_oT_b = _bf16(w["W_o"], True)
+
+    def params(self) -> np.ndarray:
+        return self.flat.cpu().numpy().copy()
+
+    # -------------------------------------------------------------- forward
+    def resample(self, dset: DeviceTrajSet, ids: torch.Tensor | None, n: int):
+        X = torch.empty((n, T, 2), dtype=torch.float32, device=self.device)
+        lenf = torch.empty(n, dtype=torch.float32, device=self.device)
+        _lib.check(_nn().tsm_nn_resample(_p(dset.xy64), _p(dset.offsets), _p(ids), n, T, _p(X), _p(lenf), _s()))
+        return X, lenf
+
+    def forward(self, X: torch.Tensor, lenf: torch.Tensor, keep: bool = False):
+        """Embeddings [n, 128] (FP32) of resampled trajectories X [n, T, 2].
+        keep=True stores the activations for backward()."""
+        L, w, n, dev = _nn(), self.w, X.shape[0], self.device
+        s = _s()
+        H1 = torch.empty((n * T, C_CONV), dtype=torch.float32, device=dev)
+        H1b = torch.empty((n * T, C_CONV), dtype=torch.int16, device=dev)
+        _lib.check(L.tsm_nn_conv_fwd(_p(X), n, T, C_CONV, _p(w["Wc"]), _p(w["bc"]), _p(H1), _p(H1b), s))
+        Gx = torch.empty((n * T, G3), dtype=torch.float32, device=dev)
+        gemm_nt(H1b, self.W_ih_b, Gx, bias=w["b_ih"])
+        Hs = torch.empty((T + 1, n, H), dtype=torch.float32, device=dev)
+        Hb = torch.empty((n, H), dtype=torch.int16, device=dev)
+        _lib.check(L.tsm_nn_gru_h0(_p(w["u"]), _p(lenf), n, H, _p(Hs[0]), _p(Hb), s))
+        Gh = torch.empty((n, G3), dtype=torch.float32, device=dev)
+        if keep:
+            R, Z, NN, GHN = (torch.empty((T, n, H), dtype=torch.float32, device=dev) for _ in range(4))
+        else:
+            R, Z, NN, GHN = (torch.empty((1, n, H), dtype=torch.float32, device=dev) for _ in range(4))
+        for t in range(T):
+            gemm_nt(Hb, self.W_hh_b, Gh, bias=w["b_hh"])
+            k = t if keep else 0
+            gx_t = Gx.view(n, T, G3)[:, t, :]
+            _lib.check(L.tsm_nn_gru_step_fwd(_p(gx_t), T * G3, _p(Gh), _p(Hs[t]), n, H, _p(R[k]), _p(Z[k]), _p(NN[k]),
+                                             _p(GHN[k]), _p(Hs[t + 1]), _p(Hb), s))
+        E = torch.empty((n, D), dtype=torch.float32, device=dev)
+        gemm_nt(Hb, self.W_o_b, E, bias=w["b_o"])
+        _lib.check(L.tsm_nn_linear_f32(_p(X), n, 2 * T, _p(self.S), D, _p(E), 1.0, s))  # fixed skip map, FP32
+        if keep:
+            self._cache = dict(X=X, lenf=lenf, H1=H1, H1b=H1b, Hs=Hs, R=R, Z=Z, NN=NN, GHN=GHN, n=n)
+        return E
+
+    # -------------------------------------------------------------- backward
+    def backward(self, dE: torch.Tensor) -> torch.Tensor:
+        """Gradient (flat, FP32) of the loss w.r.t. the parameters, given dE
+        [n, 128] for the last forward(keep=True)."""
+        c, L, w, dev = self._cache, _nn(), self.w, self.device
+        n = c["n"]
+        s = _s()
+        grad = torch.zeros(NPARAM, dtype=torch.float32, device=dev)
+        g = _views(grad)
+        Hs = c["Hs"]
+        # head: dW_o = dE^T hT, db_o = sum dE, dh = dE W_o
+        dET = _bf16(dE, True)
+        gemm_nt(dET, _bf16(Hs[T], True), g["W_o"])
+        _lib.check(L.tsm_nn_colsum(_p(dE), n, D, D, _p(g["b_o"]), 0.0, s))
+        dh = torch.empty((n, H), dtype=torch.float32, device=dev)
+        gemm_nt(_bf16(dE), self.W_oT_b, dh)
+        dGx = torch.empty((n * T, G3), dtype=torch.float32, device=dev)
+        dGh = torch.empty((n, G3), dtype=torch.float32, device=dev)
+        dGhb = torch.empty((n, G3), dtype=torch.int16, device=dev)
+        for t in range(T - 1, -1, -1):
+            dgx_t = dGx.view(n, T, G3)[:, t, :]
+            _lib.check(L.tsm_nn_gru_step_bwd(_p(dh), _p(Hs[t]), _p(c["R"][t]), _p(c["Z"][t]), _p(c["NN"][t]),
+                                             _p(c["GHN"][t]), n, H, _p(dgx_t), T * G3, _p(dGh), _p(dGhb), _p(dh), s))
+            # dW_hh += dGh^T h_t ; db_hh += sum dGh ; dh = dh * z + dGh W_hh
+            gemm_nt(_bf16(dGh, True), _bf16(Hs[t], True), g["W_hh"], beta=1.0)
+            _lib.check(L.tsm_nn_colsum(_p(dGh), n, G3, G3, _p(g["b_hh"]), 1.0, s))
+            gemm_nt(dGhb, self.W_hhT_b, dh, beta=1.0)
+        _lib.check(L.tsm_nn_gru_h0_bwd(_p(dh), _p(Hs[0]), _p(c["lenf"]), n, H, _p(g["u"]), s))
+        # input projection: dW_ih = dGx^T h1, db_ih = sum dGx, dH1 = dGx W_ih
+        gemm_nt(_bf16(dGx, True), _bf16(c["H1"], True), g["W_ih"])
+        _lib.check(L.tsm_nn_colsum(_p(dGx), n * T, G3, G3, _p(g["b_ih"]), 0.0, s))
+        dH1 = torch.empty((n * T, C_CONV), dtype=torch.float32, device=dev)
+        gemm_nt(_bf16(dGx), self.W_ihT_b, dH1)
+        _lib.check(L.tsm_nn_conv_bwd(_p(c["X"]), _p(c["H1"]), _p(dH1), n, T, C_CONV, _p(g["Wc"]), _p(g["bc"]), s))
+        return grad
+
+    # -------------------------------------------------------------- embedding
+    def embed(self, dset: DeviceTrajSet, ids=None, batch: int = 65536) -> torch.Tensor:
+        """FP32 embeddings [n, 128] of trajectories ids (default: all)."""
+        if ids is None:
+            ids = torch.arange(dset.n, dtype=torch.int32, device=self.device)
+        ids = torch.as_tensor(ids, dtype=torch.int32, device=self.device)
+        out = torch.empty((ids.numel(), D), dtype=torch.float32, device=self.device)
+        for o in range(0, ids.numel(), batch):
+            sl = ids[o:o + batch]
+            X, lenf = self.resample(dset, sl, sl.numel())
+            out[o:o + sl.numel()] = self.forward(X, lenf)
+        return out
+
+
+class Adam:
+    def __init__(self, n: int, device, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+        self.m = torch.zeros(n, dtype=torch.float32, device=device)
+        self.v = torch.zeros(n, dtype=torch.float32, device=device)
+        self.lr, self.b1, self.b2, self.eps, self.step = lr, b1, b2, eps, 0
+
+    def update(self, p: torch.Tensor, g: torch.Tensor):
+        self.step += 1
+        _lib.check(_nn().tsm_nn_adam(_p(p), _p(g), _p(self.m), _p(self.v), p.numel(), self.lr, self.b1, self.b2,
+                                     self.eps, self.step, _s()))
+
+
+def rank_loss(E: torch.Tensor, B: int, K: int, P: int = 10, Q0: int = 20, tau: float = 0.05):
+    dE = torch.empty_like(E)
+    loss = torch.empty(B, dtype=torch.float32, device=E.device)
+    _lib.check(_nn().tsm_nn_rank_loss(_p(E), B, K, D, P, Q0, float(tau), _p(dE), _p(loss), _s()))
+    return loss, dE
+
+
+def topk_smallest(V: torch.Tensor, k: int):
+    """Indices (int32) and values of the k smallest entries of each row."""
+    rows, cols = V.shape
+    idx = torch.empty((rows, k), dtype=torch.int32, device=V.device)
+    val = torch.empty((rows, k), dtype=V.dtype, device=V.device)
+    f = _nn().tsm_nn_topk_f64 if V.dtype == torch.float64 else _nn().tsm_nn_topk_f32
+    _lib.check(f(_p(V), rows, cols, V.stride(0), k, _p(idx), _p(val), _s()))
+    return idx, val
+
+
+class FilterIndex:
+    """A database embedded once (the index build), searched per query batch:
+    BF16 hi/lo split embeddings so one K = 384 MFMA GEMM gives
+    |e_d|^2 - 2 e_q . e_d to ~16 mantissa bits."""
+
+    def __init__(self, enc: Encoder, dset: DeviceTrajSet, db_ids):
+        self.enc, self.dset = enc, dset
+        self.db_ids = torch.as_tensor(db_ids, dtype=torch.int32, device=enc.device)
+        E = enc.embed(dset, self.db_ids)
+        self.norm = torch.empty(E.shape[0], dtype=torch.float32, device=E.device)
+        self.B = self._split(E, query_side=False, norm=self.norm)
+
+    @staticmethod
+    def _split(E: torch.Tensor, query_side: bool, norm=None) -> torch.Tensor:
+        # database rows [hi, lo, hi], query rows [hi, hi, lo]: one K = 3D product
+        # gives hi.hi + hi.lo + lo.hi
+        out = torch.empty((E.shape[0], 3 * D), dtype=torch.int16, device=E.device)
+        _lib.check(_nn().tsm_nn_split_bf16(_p(E), E.shape[0], D, int(query_side), _p(out), _p(norm), _s()))
+        return out
+
+    def search(self, q_ids, C: int = 200, chunk: int = 1 << 20):
+        """Candidate database positions [nq, C] by embedding distance."""
+        Eq = self.enc.embed(self.dset, q_ids)
+        A = self._split(Eq, query_side=True)
+        nq, nd = A.shape[0], self.B.shape[0]
+        cand = torch.empty((nq, C), dtype=torch.int32, device=A.device)
+        rows = max(1, min(nq, chunk // max(nd // 1024, 1)))
+        dist = torch.empty((rows, nd), dtype=torch.float32, device=A.device)
+        for o in range(0, nq, rows):
+            r = min(rows, nq - o)
+            gemm_nt(A[o:o + r], self.B, dist[:r], alpha=-2.0, bias=self.norm)
+            idx, _ = topk_smallest(dist[:r], C)
+            cand[o:o + r] = idx
+        return cand
+
+
+def exact_knn(dset: DeviceTrajSet, q0: int, nq: int, d0: int, nd: int, k: int = 10, chunk_q: int = 64):
+    """Exact DTW k-nearest database positions [nq, k] (grid of the DP
+    kernel, then per-row selection), and the elapsed seconds."""
+    from .engine import grid
+    out = torch.empty((nq, k), dtype=torch.int32, device="cuda")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for o in range(0, nq, chunk_q):
+        r = min(chunk_q, nq - o)
+        dist = grid("dtw", dset, q0 + o, r, d0, nd)
+        idx, _ = topk_smallest(dist, k)
+        out[o:o + r] = idx
+        del dist
+    torch.cuda.synchronize()
+    return out, time.perf_counter() - t0
+
+
+def filtered_knn(index: FilterIndex, q0: int, nq: int, k: int = 10, C: int = 200):
+    """Embedding shortlist of C, exact DTW re-rank, k smallest: database
+    positions [nq, k] and the elapsed seconds (index build excluded)."""
+    dset = index.dset
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    qids = torch.arange(q0, q0 + nq, dtype=torch.int32, device="cuda")
+    cand = index.search(qids, C)
+    pa = qids.repeat_interleave(C)
+    pb = index.db_ids[cand.reshape(-1).long()]
+    d = dp_pairs("dtw", dset, pa, pb, validate=False).view(nq, C)
+    idx, _ = topk_smallest(d, k)
+    best = torch.gather(cand, 1, idx.long())
+    torch.cuda.synchronize()
+    return best, time.perf_counter() - t0
+
+
+def recall_at(found: torch.Tensor, truth: torch.Tensor) -> float:
+    f = found.cpu().numpy()
+    t = truth.cpu().numpy()
+    return float(np.mean([len(set(f[i]) & set(t[i])) / t.shape[1] for i in range(t.shape[0])]))
+
+
+def hard_pairs(enc: Encoder, dset: DeviceTrajSet, anchors, pool_ids, K: int = 200):
+    """For each anchor: its K nearest pool trajectories by the current
+    embedding (itself excluded), with exact DTW labels from the DP kernel,
+    sorted by label.  Returns (anchor ids, candidate ids [NA, K], labels)."""
+    anchors = torch.as_tensor(anchors, dtype=torch.int32, device="cuda")
+    pool = torch.as_tensor(pool_ids, dtype=torch.int32, device="cuda")
+    index = FilterIndex(enc, dset, pool)
+    cand = index.search(anchors, K + 1)
+    cid = pool[cand.long()]
+    # drop the anchor itself when it is in the pool, keep K
+    keep = torch.where(cid == anchors[:, None], torch.full_like(cid, -1), cid)
+    cids = torch.empty((anchors.numel(), K), dtype=torch.int32, device="cuda")
+    kk = keep.cpu().numpy()
+    out = np.empty((anchors.numel(), K), dtype=np.int32)
+    for i in range(kk.shape[0]):
+        row = kk[i][kk[i] >= 0][:K]
+        out[i] = row
+    cids.copy_(torch.from_numpy(out))
+    pa = anchors.repeat_interleave(K)
+    y = dp_pairs("dtw", dset, pa, cids.reshape(-1), validate=False).view(anchors.numel(), K)
+    order = torch.argsort(y, dim=1)
+    return anchors, torch.gather(cids, 1, order), torch.gather(y, 1, order)
+
+
+def train_encoder(enc: Encoder, dset: DeviceTrajSet, anchors, cands, labels, epochs: int = 20, batch: int = 32,
+                  lr: float = 1e-3, tau: float = 0.05, seed: int = 0, log=None):
+    """Adam on the ranking loss over anchor lists (cands sorted by label)."""
+    NA, K = cands.shape
+    opt = Adam(NPARAM, enc.device, lr=lr)
+    g = torch.Generator(device="cpu").manual_seed(seed)
+    hist = []
+    for ep in range(epochs):
+        perm = torch.randperm(NA, generator=g)
+        tot = 0.0
+        nb = 0
+        for o in range(0, NA - batch + 1, batch):
+            sel = perm[o:o + batch].to("cuda")
+            ids = torch.cat([anchors[sel][:, None], cands[sel]], dim=1).reshape(-1).contiguous()
+            X, lenf = enc.resample(dset, ids, ids.numel())
+            E = enc.forward(X, lenf, keep=True)
+            loss, dE = rank_loss(E, batch, K, tau=tau)
+            grad = enc.backward(dE)
+            opt.update(enc.flat, grad)
+            enc.refresh()
+            tot += float(loss.mean())
+            nb += 1
+        hist.append(tot / max(nb, 1))
+        if log:
+            log(f"epoch {ep}: loss {hist[-1]:.5f}")
+    return hist
