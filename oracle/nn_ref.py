@@ -1,7 +1,17 @@
-"""FP64 numpy restatement of the learned filter's encoder and ranking loss
-(forward and backward), the checker for the HIP kernels of
-trajsimimeasures_amd/filter.py.  Written from the equations in that module's
-docstring; pinned by finite differences in tests/test_nn_ref.py."""
+"""TEST INFRASTRUCTURE ONLY -- FP64 numpy restatement of the learned kNN
+filter's encoder and ranking loss, forward and backward: the checker for the
+HIP kernels of trajsimimeasures_amd/filter.py (csrc/tsm_nn.hip).  Only tests
+import it.
+
+Written from the encoder's equations (Conv1d(2->64, k=3, pad 1) + ReLU,
+GRU(64->128) with gate order r, z, n and h0 = tanh(u * lenf), head W_o h_T +
+b_o plus a fixed skip S x); the GRU convention is the common one
+(r = sig(.), z = sig(.), n = tanh(Gx_n + r * Gh_n), h' = (1-z) n + z h).
+The filter has no counterpart in the reference's kernels (its learned
+measures are PyTorch modules, e.g. nn/MLP.py), so nothing here follows a
+reference file.  Pinned by central finite differences of the loss through
+forward() against backward() (tests/test_nn_ref.py), and by the closed form
+of a zero head (embedding distance = resampled L2 distance)."""
 from __future__ import annotations
 
 import numpy as np
@@ -29,8 +39,22 @@ def sig(x):
     return 1.0 / (1.0 + np.exp(-x))
 
 
-def forward(p: dict, X: np.ndarray, lenf: np.ndarray, S: np.ndarray):
-    """X [n, T, 2], lenf [n] -> E [n, D] and the activations."""
+def bf16(a):
+    """Round to the nearest BF16 value (ties to even), kept as float64."""
+    u = np.ascontiguousarray(a, dtype=np.float64).astype(np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def _id(a):
+    return a
+
+
+def forward(p: dict, X: np.ndarray, lenf: np.ndarray, S: np.ndarray, round_bf16: bool = False):
+    """X [n, T, 2], lenf [n] -> E [n, D] and the activations.  round_bf16
+    rounds every matrix-product operand to BF16 first (the GEMM inputs of the
+    HIP kernels); products and everything else stay FP64."""
+    q = bf16 if round_bf16 else _id
     n = X.shape[0]
     pre = np.zeros((n, T, C))
     for k in range(3):
@@ -45,11 +69,11 @@ def forward(p: dict, X: np.ndarray, lenf: np.ndarray, S: np.ndarray):
         pre += np.einsum("nti,ci->ntc", xs, p["Wc"][:, :, k])
     pre += p["bc"]
     H1 = np.maximum(pre, 0.0)
-    Gx = H1 @ p["W_ih"].T + p["b_ih"]
+    Gx = q(H1) @ q(p["W_ih"]).T + p["b_ih"]
     h = np.tanh(lenf[:, None] * p["u"][None])
     hs, cache = [h], []
     for t in range(T):
-        Gh = h @ p["W_hh"].T + p["b_hh"]
+        Gh = q(h) @ q(p["W_hh"]).T + p["b_hh"]
         gx = Gx[:, t]
         r = sig(gx[:, :H] + Gh[:, :H])
         z = sig(gx[:, H:2 * H] + Gh[:, H:2 * H])
@@ -57,18 +81,18 @@ def forward(p: dict, X: np.ndarray, lenf: np.ndarray, S: np.ndarray):
         cache.append((r, z, nn, Gh[:, 2 * H:]))
         h = (1 - z) * nn + z * h
         hs.append(h)
-    E = h @ p["W_o"].T + p["b_o"] + X.reshape(n, -1) @ S.T
-    return E, dict(X=X, lenf=lenf, pre=pre, H1=H1, hs=hs, cache=cache)
+    E = q(h) @ q(p["W_o"]).T + p["b_o"] + X.reshape(n, -1) @ S.T
+    return E, dict(X=X, lenf=lenf, pre=pre, H1=H1, hs=hs, cache=cache, q=q)
 
 
 def backward(p: dict, act: dict, dE: np.ndarray) -> dict:
-    X, hs, cache, H1, pre = act["X"], act["hs"], act["cache"], act["H1"], act["pre"]
+    X, hs, cache, H1, pre, q = act["X"], act["hs"], act["cache"], act["H1"], act["pre"], act["q"]
     n = X.shape[0]
     g = {k: np.zeros_like(v) for k, v in p.items()}
     hT = hs[T]
-    g["W_o"] = dE.T @ hT
+    g["W_o"] = q(dE).T @ q(hT)
     g["b_o"] = dE.sum(0)
-    dh = dE @ p["W_o"]
+    dh = q(dE) @ q(p["W_o"])
     dGx = np.zeros((n, T, 3 * H))
     for t in range(T - 1, -1, -1):
         r, z, nn, ghn = cache[t]
@@ -80,15 +104,15 @@ def backward(p: dict, act: dict, dE: np.ndarray) -> dict:
         daz = dz * z * (1 - z)
         dGx[:, t] = np.concatenate([dar, daz, dan], 1)
         dGh = np.concatenate([dar, daz, dan * r], 1)
-        g["W_hh"] += dGh.T @ h
+        g["W_hh"] += q(dGh).T @ q(h)
         g["b_hh"] += dGh.sum(0)
-        dh = dh * z + dGh @ p["W_hh"]
+        dh = dh * z + q(dGh) @ q(p["W_hh"])
     h0 = hs[0]
     g["u"] = (dh * (1 - h0 ** 2) * act["lenf"][:, None]).sum(0)
     flat_dGx = dGx.reshape(n * T, -1)
-    g["W_ih"] = flat_dGx.T @ H1.reshape(n * T, -1)
+    g["W_ih"] = q(flat_dGx).T @ q(H1.reshape(n * T, -1))
     g["b_ih"] = flat_dGx.sum(0)
-    dH1 = (flat_dGx @ p["W_ih"]).reshape(n, T, C)
+    dH1 = (q(flat_dGx) @ q(p["W_ih"])).reshape(n, T, C)
     dpre = dH1 * (pre > 0)
     g["bc"] = dpre.sum((0, 1))
     for k in range(3):

@@ -48,7 +48,11 @@ def test_graph_replays_match_eager(gpu, measure):
 def test_autotune_small(gpu):
     from trajsimimeasures_amd import autotune, engine
     t = autotune.tune(npairs_per_bucket=1024, waves=(4, 8), verbose=False)
-    assert set(t["lanes_fp32"][1:97]) <= {4, 8} and t["waves_per_cu"] in (4, 8)
+    # a measured lanes table is kept only where it beats the built-in cost model
+    for key in ("lanes_fp32", "lanes_fp64"):
+        if key in t:
+            assert set(t[key][1:97]) <= {4, 8}
+    assert t["waves_per_cu"] in (4, 8)
     ts, pa, pb = uniform_pairs(1500, seed=2)
     d = engine.DeviceTrajSet(ts)
     for m in ("dtw", "lcss"):

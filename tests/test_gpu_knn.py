@@ -1,0 +1,62 @@
+"""The learned kNN filter at full size: 1,000 queries over 1,000,000 database
+random walks (lognormal lengths, median 60, clipped to [10, 1000]), k = 10,
+200 candidates per query re-ranked by the exact DTW kernel.  Shows what the
+trained network gains over the same filter without training (the resampled-
+L2 embedding), with the exact all-pairs DTW kNN as the judge of recall."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+NQ, ND, K, C = 1000, 1_000_000, 10, 200
+
+
+@pytest.fixture(scope="module")
+def knn(gpu):
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    from trajsimimeasures_amd.engine import DeviceTrajSet
+    ts = F.knn_workload(nq=NQ, nd=ND)
+    dset = DeviceTrajSet(ts)
+    truth, t_exact = F.exact_knn(dset, 0, NQ, NQ, ND, k=K)
+    db = torch.arange(NQ, NQ + ND, dtype=torch.int32, device="cuda")
+    return ts, dset, db, truth, t_exact
+
+
+def test_exact_knn_spot_check_against_oracle(knn):
+    """For a few queries the exact kNN's 10th distance (oracle, FP64) is no
+    larger than the oracle DTW to 3,000 other database trajectories, and the
+    10 found are the 10 smallest among found + others."""
+    from oracle import oracle as O
+    ts, _, _, truth, _ = knn
+    tr = truth.cpu().numpy()
+    rng = np.random.default_rng(0)
+    for q in (0, 17, 999):
+        others = rng.choice(ND, 3000, replace=False)
+        others = others[~np.isin(others, tr[q])]
+        ids = np.concatenate([tr[q], others]) + NQ
+        d, _ = O.pairs("dtw", ts.xy, ts.offsets, np.full(ids.size, q), ids)
+        assert d[:K].max() <= d[K:].min() * (1 + 1e-6)
+
+
+def test_trained_filter_gains_over_untrained(knn):
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    _, dset, db, truth, t_exact = knn
+    base = F.FilterIndex(F.Encoder(), dset, db)
+    found0, _ = F.filtered_knn(base, 0, NQ, k=K, C=C)
+    r0 = F.recall_at(found0, truth)
+    enc, info = F.train_filter(dset, db)
+    assert info["labels"] == 200_000 and info["loss_last"] < info["loss_first"]
+    index = F.FilterIndex(enc, dset, db)
+    F.filtered_knn(index, 0, NQ, k=K, C=C)
+    found, t_f = F.filtered_knn(index, 0, NQ, k=K, C=C)
+    r1 = F.recall_at(found, truth)
+    torch.cuda.synchronize()
+    print(f"recall@10 untrained {r0:.4f} trained {r1:.4f}; exact {NQ / t_exact:.0f} q/s, "
+          f"filtered {NQ / t_f:.0f} q/s", flush=True)
+    assert r1 >= 0.95
+    assert r1 > r0
+    assert t_exact / t_f >= 10.0

@@ -1,9 +1,9 @@
-"""Pins the FP64 reference of the encoder and its loss (tests/nn_ref.py) by
+"""Pins the FP64 reference of the encoder and its loss (oracle/nn_ref.py) by
 central finite differences: a dropped term, a wrong gate or a transposed
 weight in either the forward or the backward restatement breaks agreement."""
 import numpy as np
 
-from tests import nn_ref as R
+from oracle import nn_ref as R
 
 
 def _setup(seed=0, B=2, K=5):

@@ -36,9 +36,11 @@ import numpy as np
 import torch
 
 from . import _lib
-from .engine import DeviceTrajSet, KernelTimer, pairs as dp_pairs
+from .data import TrajSet, lengths_lognormal, random_walks
+from .engine import DeviceTrajSet, pairs as dp_pairs
 
-__all__ = ["Encoder", "FilterIndex", "exact_knn", "recall_at", "train_encoder", "hard_pairs"]
+__all__ = ["Encoder", "FilterIndex", "exact_knn", "filtered_knn", "recall_at", "train_encoder", "hard_pairs",
+           "train_filter", "knn_workload"]
 
 T = 16
 C_CONV = 64
@@ -388,8 +390,9 @@ def hard_pairs(enc: Encoder, dset: DeviceTrajSet, anchors, pool_ids, K: int = 20
 
 
 def train_encoder(enc: Encoder, dset: DeviceTrajSet, anchors, cands, labels, epochs: int = 20, batch: int = 32,
-                  lr: float = 1e-3, tau: float = 0.05, seed: int = 0, log=None):
-    """Adam on the ranking loss over anchor lists (cands sorted by label)."""
+                  lr: float = 1e-3, tau: float = 0.05, seed: int = 0, log=None, callback=None):
+    """Adam on the ranking loss over anchor lists (cands sorted by label).
+    ``callback(epoch)`` runs after every epoch; returns the mean loss per epoch."""
     NA, K = cands.shape
     opt = Adam(NPARAM, enc.device, lr=lr)
     g = torch.Generator(device="cpu").manual_seed(seed)
@@ -412,4 +415,40 @@ def train_encoder(enc: Encoder, dset: DeviceTrajSet, anchors, cands, labels, epo
         hist.append(tot / max(nb, 1))
         if log:
             log(f"epoch {ep}: loss {hist[-1]:.5f}")
+        if callback:
+            callback(ep)
     return hist
+
+
+def knn_workload(seed: int = 2024, nq: int = 1000, nd: int = 1_000_000) -> TrajSet:
+    """The kNN workload: nq query trajectories (ids [0, nq)) followed by nd
+    database trajectories (ids [nq, nq + nd)), random walks with lognormal
+    lengths (median 60, sigma 0.8, clipped to [10, 1000]), from one seed."""
+    rng = np.random.default_rng(seed)
+    return random_walks(rng, lengths_lognormal(rng, nq + nd))
+
+
+def train_filter(dset: DeviceTrajSet, db_ids, n_anchors: int = 1000, K: int = 200, epochs: int = 15,
+                 lr: float = 1e-3, tau: float = 0.05, batch: int = 32, seed: int = 0, log=None):
+    """The filter's training run, all on the device: n_anchors database
+    trajectories drawn at random, each with its K nearest database
+    trajectories by the untrained (resampled-L2) embedding, labelled with
+    exact DTW by the DP kernel (n_anchors * K distances), then ``epochs`` of
+    Adam on the ranking loss.  Returns (encoder, info dict)."""
+    db_ids = torch.as_tensor(db_ids, dtype=torch.int32, device="cuda")
+    if db_ids.numel() <= K:
+        raise ValueError("the database must hold more than K trajectories")
+    rng = np.random.default_rng(seed)
+    pick = torch.as_tensor(rng.choice(db_ids.numel(), n_anchors, replace=False), device="cuda")
+    anchors = db_ids[pick]
+    enc = Encoder()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    A, Cn, Y = hard_pairs(enc, dset, anchors, db_ids, K=K)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    hist = train_encoder(enc, dset, A, Cn, Y, epochs=epochs, batch=batch, lr=lr, tau=tau, seed=seed, log=log)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return enc, dict(labels=int(Cn.numel()), anchors=n_anchors, K=K, epochs=epochs, label_s=t1 - t0,
+                     train_s=t2 - t1, loss_first=hist[0], loss_last=hist[-1])
