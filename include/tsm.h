@@ -33,7 +33,7 @@
 extern "C" {
 #endif
 
-#define TSM_ABI_VERSION 1
+#define TSM_ABI_VERSION 2
 
 /* Measure ids. */
 #define TSM_DTW 0       /* core/dtw.py:10-77 */
@@ -66,6 +66,9 @@ typedef struct tsm_trajset {
     int64_t n;              /* number of trajectories */
     int64_t npts;           /* number of points, offsets[n] */
     int64_t max_len;        /* longest trajectory (sizes scratch; must be >= every length) */
+    double coord_max;       /* an upper bound on |x| and |y| over the set, or 0 if unknown.  EDR and
+                               LCSS use it to run their match test on the FP32 copy with a proven
+                               guard band (FP64 recheck inside the band); 0 keeps the test in FP64 */
 } tsm_trajset;
 
 /* Per-call parameters; tsm_params_default() fills the defaults. */

@@ -124,6 +124,27 @@ def test_eps_boundary_points(gpu):
             _check(m, ts, pa, pb, got, cnt, eps=eps, eps_mode=mode)
 
 
+@pytest.mark.parametrize("shift", [0.0, 5.0, 1000.0])
+def test_count_match_paths(gpu, shift):
+    """EDR and LCSS test matches on the FP32 copy against a guard band when
+    the set's coordinate bound allows it (shift 0: narrow band; shift 5: a
+    wide band with many FP64 rechecks) and in FP64 otherwise (shift 1000, or
+    coord_max = 0).  Every path is bit-exact against the oracle."""
+    from trajsimimeasures_amd.data import TrajSet
+    print(f"[case] count match paths shift={shift}", flush=True)
+    ts, pa, pb = uniform_pairs(2000, lo=5, hi=120, seed=31)
+    ts = TrajSet(ts.xy + shift, ts.offsets)
+    d = _dev(ts)
+    for m in ("edr", "lcss"):
+        for eps in (0.0025, 0.01):
+            got, cnt = _gpu(m, d, pa, pb, eps=eps)
+            _check(m, ts, pa, pb, got, cnt, eps=eps)
+            d.coord_max = 0.0
+            got64, cnt64 = _gpu(m, d, pa, pb, eps=eps)
+            d.coord_max = float(np.abs(ts.xy).max())
+            assert np.array_equal(cnt, cnt64) and np.array_equal(got, got64)
+
+
 @pytest.mark.parametrize("measure", ["dtw", "erp", "frechet", "hausdorff"])
 def test_fp64_mode(gpu, measure):
     print(f"[case] fp64 measure={measure}", flush=True)

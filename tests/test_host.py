@@ -28,7 +28,7 @@ def test_library_exports_every_declared_function():
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(_lib.EXPORTS)
-    assert L.tsm_abi_version() == 1
+    assert L.tsm_abi_version() == 2
     nn = header_functions("tsm_nn.h")
     for n in nn:
         assert hasattr(L, n), n
@@ -43,7 +43,7 @@ def test_params_default():
 
 def test_argument_errors_without_device():
     L = _lib.lib()
-    s = _lib.TrajSetC(0, 0, 0, 5, 5, 1)
+    s = _lib.TrajSetC(0, 0, 0, 5, 5, 1, 0.0)
     prm = _lib.params()
     rc = L.tsm_pairs_triu(99, ctypes.byref(s), 3, ctypes.byref(prm), None, None, None)
     assert rc == -1 and b"unknown measure" in L.tsm_last_error()

@@ -43,7 +43,8 @@ class TsmError(RuntimeError):
 
 class TrajSetC(ctypes.Structure):
     _fields_ = [("xy32", ctypes.c_void_p), ("xy64", ctypes.c_void_p), ("offsets", ctypes.c_void_p),
-                ("n", ctypes.c_int64), ("npts", ctypes.c_int64), ("max_len", ctypes.c_int64)]
+                ("n", ctypes.c_int64), ("npts", ctypes.c_int64), ("max_len", ctypes.c_int64),
+                ("coord_max", ctypes.c_double)]
 
 
 class ParamsC(ctypes.Structure):

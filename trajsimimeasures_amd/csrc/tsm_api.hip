@@ -9,6 +9,7 @@
 //   4. one persistent DP kernel (tsm_dp.hpp), or the Hausdorff kernel.
 // Everything is stream-ordered on the caller's stream; the only host sync
 // is a workspace grow.
+#include <cmath>
 #include <cstring>
 
 #include "tsm_hausdorff.hpp"
@@ -156,8 +157,18 @@ int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &p
     case TSM_ERP:
         return f64 ? run_dp<kERP, true>(w, L, *set, ps, npairs, thr, out, counts, st)
                    : run_dp<kERP, false>(w, L, *set, ps, npairs, thr, out, counts, st);
-    case TSM_EDR: return run_dp<kEDR, false>(w, L, *set, ps, npairs, thr, out, counts, st);
-    case TSM_LCSS: return run_dp<kLCSS, false>(w, L, *set, ps, npairs, thr, out, counts, st);
+    case TSM_EDR:
+    case TSM_LCSS: {
+        // FP32 match test with a guard band when the set's coordinate bound
+        // gives a useful one, else the FP64 test (both exact)
+        float lo, hi;
+        const bool band = !f64 && set->xy32 && count_band(thr, set->coord_max, lo, hi);
+        if (measure == TSM_EDR)
+            return band ? run_dp<kEDR, false>(w, L, *set, ps, npairs, thr, out, counts, st)
+                        : run_dp<kEDR, true>(w, L, *set, ps, npairs, thr, out, counts, st);
+        return band ? run_dp<kLCSS, false>(w, L, *set, ps, npairs, thr, out, counts, st)
+                    : run_dp<kLCSS, true>(w, L, *set, ps, npairs, thr, out, counts, st);
+    }
     }
     return fail(TSM_E_ARG, "unknown measure");
 }
@@ -277,7 +288,10 @@ int tsm_pairs_host(int measure, const double *xy64, const int64_t *offsets, int6
     TSM_HIP(hipMemcpy(c.h_pb.p, pb, npairs * 4, hipMemcpyHostToDevice));
     int rc = tsm_xy_to_fp32(c.h_xy64.as<double>(), c.h_xy32.as<float>(), npts, nullptr);
     if (rc) return rc;
-    tsm_trajset set{c.h_xy32.as<float>(), c.h_xy64.as<double>(), c.h_offs.as<int64_t>(), n, npts, max_len};
+    double cmax = 0.0;
+    for (int64_t i = 0; i < 2 * npts; ++i) cmax = std::fmax(cmax, std::fabs(xy64[i]));
+    if (!std::isfinite(cmax)) cmax = 0.0;  // unknown bound: EDR/LCSS keep the FP64 match test
+    tsm_trajset set{c.h_xy32.as<float>(), c.h_xy64.as<double>(), c.h_offs.as<int64_t>(), n, npts, max_len, cmax};
     rc = tsm_pairs(measure, &set, c.h_pa.as<int32_t>(), c.h_pb.as<int32_t>(), npairs, prm, c.h_out.as<double>(),
                    counts ? c.h_cnt.as<int32_t>() : nullptr, nullptr);
     if (rc) return rc;

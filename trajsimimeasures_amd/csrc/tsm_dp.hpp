@@ -79,6 +79,7 @@ struct DPArgs {
     void *bnd;               // strip boundary scratch (AT), kMaxSeg segments per wave
     int64_t bnd_stride;
     double thr;              // EDR/LCSS squared-distance threshold
+    float thr_lo, thr_hi;    // EDR/LCSS FP32 guard band: below lo a match, at or above hi none
     double *out;
     int32_t *counts;
 };
@@ -90,6 +91,7 @@ template <int M, bool F64> struct Rec;
 template <bool F64> struct RecDTW {
     using CT = typename std::conditional<F64, double, float>::type;
     using AT = CT;
+    static constexpr bool kBand = false;
     static constexpr bool kGap = false;
     __device__ static AT top(AT) { return (AT)__builtin_inf(); }
     __device__ static AT left0(AT) { return (AT)__builtin_inf(); }
@@ -104,6 +106,7 @@ template <bool F64> struct Rec<kDTW, F64> : RecDTW<F64> {};
 template <bool F64> struct RecFrechet {
     using CT = typename std::conditional<F64, double, float>::type;
     using AT = CT;
+    static constexpr bool kBand = false;
     static constexpr bool kGap = false;
     __device__ static AT top(AT) { return (AT)__builtin_inf(); }
     __device__ static AT left0(AT) { return (AT)__builtin_inf(); }
@@ -120,6 +123,7 @@ template <bool F64> struct Rec<kFrechet, F64> : RecFrechet<F64> {};
 template <bool F64> struct RecERP {
     using CT = typename std::conditional<F64, double, float>::type;
     using AT = CT;
+    static constexpr bool kBand = false;
     static constexpr bool kGap = true;
     __device__ static AT top(AT sb) { return sb; }
     __device__ static AT left0(AT sa) { return sa; }
@@ -131,43 +135,51 @@ template <bool F64> struct RecERP {
 };
 template <bool F64> struct Rec<kERP, F64> : RecERP<F64> {};
 
-// EDR / LCSS: integer counts carried in FP32 (exact below 2^24); the match
-// test uses FP64 coordinates and the FP64 threshold so it reproduces the
-// FP64 oracle's decision bit for bit.
-template <bool F64> struct Rec<kEDR, F64> {
-    using CT = double;
+// EDR / LCSS: integer counts carried in FP32 (exact below 2^24).  The match
+// decision must equal the FP64 oracle's, (dx*dx + dy*dy) < thr on FP64
+// coordinates, bit for bit.  Two forms:
+//   F64 = true:  the test itself runs on the FP64 coordinates.
+//   F64 = false: the test runs on the FP32 copy against a guard band
+//     [thr_lo, thr_hi) that provably contains every FP32 squared distance
+//     whose FP64 counterpart could fall on the other side of thr (the host
+//     derives it from a bound on |coordinate|); a lane that meets a value
+//     inside the band recomputes its whole row step with the FP64 test.
+template <int M> struct RecCount {
     using AT = float;
     static constexpr bool kGap = false;
     __device__ static AT top(AT) { return 0.0f; }
     __device__ static AT left0(AT) { return 0.0f; }
-    // min(diag + sub, up + 1, left + 1) with sub in {0, 1} equals
+    // EDR: min(diag + sub, up + 1, left + 1) with sub in {0, 1} equals
     // (match ? diag : min3(diag, up, left) + 1), since unit-cost edit
-    // distances satisfy diag <= min(up, left) + 1.
-    __device__ static AT cell(CT ax, CT ay, AT, CT bx, CT by, AT, AT diag, AT up, AT left, double thr) {
-        const AT v = min3f(diag, up, left) + 1.0f;
-        return dist2_f64(ax, ay, bx, by) < thr ? diag : v;
+    // distances satisfy diag <= min(up, left) + 1.  LCSS: match ? diag + 1 :
+    // max(up, left).
+    __device__ static AT pick(bool match, AT diag, AT up, AT left) {
+        if constexpr (M == kEDR) return match ? diag : min3f(diag, up, left) + 1.0f;
+        else return match ? diag + 1.0f : fmaxf(up, left);
     }
     __device__ static double finish(AT v, int64_t na, int64_t nb, int32_t *cnt) {
         int32_t c = (int32_t)v;
         if (cnt) *cnt = c;
-        return (double)c / (double)(na > nb ? na : nb);
+        if constexpr (M == kEDR) return (double)c / (double)(na > nb ? na : nb);
+        else return 1.0 - (double)c / (double)(na < nb ? na : nb);
     }
 };
-template <bool F64> struct Rec<kLCSS, F64> {
+template <int M> struct RecCount64 : RecCount<M> {
     using CT = double;
-    using AT = float;
-    static constexpr bool kGap = false;
-    __device__ static AT top(AT) { return 0.0f; }
-    __device__ static AT left0(AT) { return 0.0f; }
-    __device__ static AT cell(CT ax, CT ay, AT, CT bx, CT by, AT, AT diag, AT up, AT left, double thr) {
-        return dist2_f64(ax, ay, bx, by) < thr ? diag + 1.0f : fmaxf(up, left);
-    }
-    __device__ static double finish(AT v, int64_t na, int64_t nb, int32_t *cnt) {
-        int32_t c = (int32_t)v;
-        if (cnt) *cnt = c;
-        return 1.0 - (double)c / (double)(na < nb ? na : nb);
+    static constexpr bool kBand = false;
+    __device__ static float cell(CT ax, CT ay, float, CT bx, CT by, float, float diag, float up, float left,
+                                 double thr) {
+        return RecCount<M>::pick(dist2_f64(ax, ay, bx, by) < thr, diag, up, left);
     }
 };
+template <int M> struct RecCount32 : RecCount<M> {
+    using CT = float;
+    static constexpr bool kBand = true;
+};
+template <> struct Rec<kEDR, true> : RecCount64<kEDR> {};
+template <> struct Rec<kLCSS, true> : RecCount64<kLCSS> {};
+template <> struct Rec<kEDR, false> : RecCount32<kEDR> {};
+template <> struct Rec<kLCSS, false> : RecCount32<kLCSS> {};
 
 template <typename CT> struct Pt;
 template <> struct Pt<float> {
@@ -209,7 +221,7 @@ template <typename CT, typename AT> struct RowPt {
 // next row.  (Loading two steps ahead measured slower.)
 template <int M, bool F64, int W, bool kBnd>
 __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t nrow_s, int rlast, int64_t orow,
-                                        bool lane0, typename Rec<M, F64>::AT left_fixed,
+                                        int64_t ocol, int col0, int ncol, bool lane0, typename Rec<M, F64>::AT left_fixed,
                                         typename Rec<M, F64>::AT *bnd, bool write_bnd,
                                         const typename Rec<M, F64>::CT (&bx)[W],
                                         const typename Rec<M, F64>::CT (&by)[W],
@@ -239,12 +251,57 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
         AT diag = lp_in;  // D[r][col0]: the corner value before the first row
         lp_out = left_in;
         AT lf = left_in;
+        if constexpr (R::kBand) {
+            // FP32 match test; `amb` collects the lanes that met a squared
+            // distance inside the guard band (wave-uniform mask)
+            uint64_t amb = 0;
 #pragma unroll
-        for (int c = 0; c < W; ++c) {
-            const AT v = R::cell(cur.x, cur.y, cur.g, bx[c], by[c], gb[c], diag, src[c], lf, a.thr);
-            diag = src[c];
-            dst[c] = v;
-            lf = v;
+            for (int c = 0; c < W; ++c) {
+                const float d2 = dist2_f32(cur.x, cur.y, bx[c], by[c]);
+                const bool m = d2 < a.thr_lo;
+                amb |= __builtin_amdgcn_ballot_w64(d2 < a.thr_hi) & ~__builtin_amdgcn_ballot_w64(m);
+                const AT v = R::pick(m, diag, src[c], lf);
+                diag = src[c];
+                dst[c] = v;
+                lf = v;
+            }
+            if (amb != 0) {
+                if ((amb >> (threadIdx.x & (kWave - 1))) & 1ull) {
+                    // redo this lane's row step with the FP64 test; src (the
+                    // previous row), the corner and the left input are intact
+                    const double2 pr = reinterpret_cast<const double2 *>(a.xy64)[orow + r];
+                    // the column addresses are loop-invariant; the empty asm
+                    // keeps the compiler from hoisting them (and their
+                    // registers) out of this rarely taken branch.  Columns
+                    // past ncol are padding whose values nothing reads.
+                    const double2 *pc = reinterpret_cast<const double2 *>(a.xy64) + ocol + col0;
+                    asm volatile("" : "+v"(pc));
+                    const int nvalid = ncol - col0;
+                    AT dg = lp_in, lx = left_in;
+#pragma unroll
+                    for (int c = 0; c < W; ++c) {
+                        if (c < nvalid) {
+                            const double2 q = pc[c];
+                            const AT v = R::pick(dist2_f64(pr.x, pr.y, q.x, q.y) < a.thr, dg, src[c], lx);
+                            dst[c] = v;
+                            lx = v;
+                        }
+                        dg = src[c];
+                        // one column load in flight at a time: all W at once
+                        // would need 4W more registers in every path
+                        asm volatile("" ::: "memory");
+                    }
+                    lf = lx;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < W; ++c) {
+                const AT v = R::cell(cur.x, cur.y, cur.g, bx[c], by[c], gb[c], diag, src[c], lf, a.thr);
+                diag = src[c];
+                dst[c] = v;
+                lf = v;
+            }
         }
         if (write_bnd) bnd[r] = lf;
     }
@@ -321,19 +378,19 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
         AT bl0 = 0, bl1 = 0;
         if (strip == 0) {
             for (int s = 0; s < nsteps; s += 2) {
-                dp_step<M, F64, W, false>(a, s, j, nrow_s, rlast, orow, lane0, leftv, bnd, write_bnd, bx, by, gb, A,
+                dp_step<M, F64, W, false>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb, A,
                                           B, p0, p1, lpA, lpB, bl0, bl1);
                 if (s + 1 < nsteps)
-                    dp_step<M, F64, W, false>(a, s + 1, j, nrow_s, rlast, orow, lane0, leftv, bnd, write_bnd, bx, by,
+                    dp_step<M, F64, W, false>(a, s + 1, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by,
                                               gb, B, A, p1, p0, lpB, lpA, bl1, bl0);
             }
         } else {
             if (lane0) bl0 = bnd[r0];
             for (int s = 0; s < nsteps; s += 2) {
-                dp_step<M, F64, W, true>(a, s, j, nrow_s, rlast, orow, lane0, leftv, bnd, write_bnd, bx, by, gb, A,
+                dp_step<M, F64, W, true>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb, A,
                                          B, p0, p1, lpA, lpB, bl0, bl1);
                 if (s + 1 < nsteps)
-                    dp_step<M, F64, W, true>(a, s + 1, j, nrow_s, rlast, orow, lane0, leftv, bnd, write_bnd, bx, by,
+                    dp_step<M, F64, W, true>(a, s + 1, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by,
                                              gb, B, A, p1, p0, lpB, lpA, bl1, bl0);
             }
         }
@@ -370,8 +427,7 @@ __device__ __forceinline__ void dispatch_bucket(const DPArgs &a, int bk, int64_t
 }
 
 // Persistent kernel: each wavefront pulls tasks until the queue is drained.
-template <int M, bool F64>
-__global__ __launch_bounds__(kWave) void dp_kernel(DPArgs a) {
+template <int M, bool F64> __device__ __forceinline__ void dp_body(const DPArgs &a) {
     constexpr int WM = WMaxOf<typename Rec<M, F64>::CT>::value;
     const int64_t ntasks = a.meta[0];
     const int64_t wave_id = blockIdx.x;
@@ -394,5 +450,15 @@ __global__ __launch_bounds__(kWave) void dp_kernel(DPArgs a) {
         dispatch_bucket<M, F64, WM, 0>(a, bk, slot0, wave_id);
     }
 }
+
+// FP32 variants are held to 256 VGPRs so two waves share each SIMD (the
+// guard-band redo of EDR/LCSS and ERP's gap columns would otherwise push
+// them past it, to one wave per SIMD); the FP64 variants keep their full
+// register file rather than spill.
+template <int M, bool F64>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void dp_kernel(DPArgs a) {
+    dp_body<M, F64>(a);
+}
+template <int M, bool F64> __global__ __launch_bounds__(kWave) void dp_kernel_wide(DPArgs a) { dp_body<M, F64>(a); }
 
 }  // namespace tsm

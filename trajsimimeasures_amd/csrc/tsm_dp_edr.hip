@@ -1,4 +1,5 @@
-// tsm_dp_edr.hip -- EDR instances of the DP kernel and its launcher.
+// tsm_dp_edr.hip -- EDR instances of the DP kernel (FP32 match test with a
+// guard band) and its launcher.
 #include "tsm_launch.hpp"
 
 namespace tsm {

@@ -111,6 +111,42 @@ inline double eps_threshold(double eps, int mode) {
     return t;
 }
 
+// Guard band of the FP32 match test of EDR/LCSS.  With u = 2^-24, U a bound
+// on |coordinate| and r = sqrt(S), the FP32 squared distance d2' computed
+// from FP32-rounded coordinates (two subtractions, a product and an FMA)
+// differs from the FP64 oracle's d2 by at most
+//   6u S + 8u U r + 8u^2 (U + r)^2 + 4e-16 S
+// (coordinate rounding u|a|, subtraction rounding u|dx'|, squaring, two
+// roundings of the sum).  For S near thr, r >= sqrt(thr / 2), so the error
+// relative to thr is at most
+//   rel = 7u + 12u U / sqrt(thr) + 20u^2 (U + sqrt(thr))^2 / thr,
+// and for S far from thr the error cannot move d2' across the band.  With
+// the band [thr (1 - 3 rel), thr (1 + 3 rel)) rounded outwards to FP32, a
+// value below it is a match and a value at or above it is not, exactly as
+// in FP64.  Returns false when no useful band exists (unknown or huge
+// coordinates relative to eps): then the test runs in FP64.
+inline bool count_band(double thr, double coord_max, float &lo, float &hi) {
+    if (!(coord_max > 0.0) || !std::isfinite(coord_max) || coord_max > 1e15) return false;
+    if (!(thr > 0.0)) {  // nothing matches
+        lo = hi = 0.0f;
+        return true;
+    }
+    if (std::isinf(thr)) {  // everything matches (finite FP32 squared distances)
+        lo = hi = INFINITY;
+        return true;
+    }
+    const double u = 0x1p-24, e = std::sqrt(thr), U = coord_max;
+    const double rel = 7.0 * u + 12.0 * u * U / e + 20.0 * u * u * (U + e) * (U + e) / thr;
+    const double B = 3.0 * rel;
+    if (!(B <= 1e-2)) return false;
+    const double dlo = thr * (1.0 - B), dhi = thr * (1.0 + B);
+    lo = (float)dlo;
+    if ((double)lo > dlo) lo = std::nextafter(lo, 0.0f);
+    hi = (float)dhi;
+    if ((double)hi < dhi) hi = std::nextafter(hi, INFINITY);
+    return true;
+}
+
 inline int grid_for(int64_t n, int block) {
     int64_t g = (n + block - 1) / block;
     if (g < 1) g = 1;

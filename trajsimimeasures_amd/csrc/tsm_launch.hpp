@@ -52,7 +52,9 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
         (rc = c.slots.ensure(nslots * 4)) || (rc = c.meta.ensure((1 + 2 * NB) * 8)) || (rc = c.counter.ensure(4)))
         return rc;
 
-    auto kern = dp_kernel<M, F64>;
+    void (*kern)(DPArgs);
+    if constexpr (F64) kern = dp_kernel_wide<M, F64>;
+    else kern = dp_kernel<M, F64>;
     const int grid = persistent_grid((npairs + 3) / 4 + NB);
     int64_t bnd_stride = 0;
     if (set.max_len > tab.ncmax) {  // some pair may need column strips
@@ -94,6 +96,12 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     a.bnd = c.bnd.p;
     a.bnd_stride = bnd_stride;
     a.thr = thr;
+    a.thr_lo = 0.0f;
+    a.thr_hi = 0.0f;
+    if constexpr (Rec<M, F64>::kBand) {
+        if (!count_band(thr, set.coord_max, a.thr_lo, a.thr_hi))
+            return fail(TSM_E_ARG, "internal: FP32 match test chosen without a valid guard band");
+    }
     a.out = out;
     a.counts = counts;
     {

@@ -42,12 +42,14 @@ class DeviceTrajSet:
         self.lengths = torch.as_tensor(lens, dtype=torch.int64, device=device)
         self.xy64 = torch.as_tensor(ts.xy, dtype=torch.float64, device=device).contiguous()
         self.offsets = torch.as_tensor(ts.offsets, dtype=torch.int64, device=device).contiguous()
+        # bound on |coordinate| for the FP32 match test of EDR/LCSS (0 forces FP64)
+        self.coord_max = float(np.abs(ts.xy).max())
         self.xy32 = torch.empty((self.npts, 2), dtype=torch.float32, device=device)
         _lib.check(_lib.lib().tsm_xy_to_fp32(self.xy64.data_ptr(), self.xy32.data_ptr(), self.npts, _stream_ptr()))
 
     def cstruct(self) -> _lib.TrajSetC:
         return _lib.TrajSetC(self.xy32.data_ptr(), self.xy64.data_ptr(), self.offsets.data_ptr(), self.n,
-                             self.npts, self.max_len)
+                             self.npts, self.max_len, self.coord_max)
 
 
 def _outs(n: int, measure: int, device, want_counts: bool):
