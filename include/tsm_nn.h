@@ -29,6 +29,18 @@ extern "C" {
  * contiguous (K, lda, ldb multiples of 8, 16-byte aligned), C fp32. */
 int tsm_nn_gemm_nt(int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb, float *C, int ldc,
                    float alpha, float beta, const float *bias, void *stream);
+/* Embedding search without the distance matrix: v = alpha * A[M,K] . B[N,K]^T
+ * + bias[N]; every v < tau[row] is appended to row's candidate list (key[row,
+ * pos], idx[row, pos] for pos = atomic count < cap).  cnt[M] must be zeroed
+ * by the caller; it ends at the number of values below tau (possibly > cap). */
+int tsm_nn_gemm_nt_filter(int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb, float alpha,
+                          const float *bias, const float *tau, int cap, uint32_t *cnt, float *key, int32_t *idx,
+                          void *stream);
+/* The C smallest candidates of each row (sorted ascending; ties by column):
+ * out[rows, C].  flag[row] = 1 when the row had fewer than C or more than cap
+ * candidates (then out is not written for it), else 0.  cap <= 4096. */
+int tsm_nn_select_candidates(const uint32_t *cnt, const float *key, const int32_t *idx, int rows, int cap, int C,
+                             int32_t *out, int32_t *flag, void *stream);
 /* FP32 [rows, cols] -> BF16 (transpose != 0: dst[c][r]). */
 int tsm_nn_cast_bf16(const float *src, int rows, int cols, int ld_src, uint16_t *dst, int ld_dst, int transpose,
                      void *stream);

@@ -17,7 +17,8 @@ def _bf16_round(a: np.ndarray) -> np.ndarray:
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
-@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (37, 90, 40), (130, 17, 136), (1, 300, 384), (200, 129, 8)])
+@pytest.mark.parametrize("M,N,K", [(64, 64, 32), (37, 90, 40), (130, 17, 136), (1, 300, 384), (200, 129, 8),
+                                   (128, 128, 64), (300, 257, 136), (1000, 1100, 384), (129, 4000, 72)])
 def test_gemm_nt_matches_fp64_product(gpu, M, N, K):
     import torch
 
@@ -204,6 +205,32 @@ def test_split_bf16_search_distances(gpu):
     want = ((eq[:, None] - ed[None]) ** 2).sum(-1) - (eq ** 2).sum(1)[:, None]
     scale = (eq ** 2).sum(1)[:, None] + (ed ** 2).sum(1)[None]
     assert np.all(np.abs(dist.cpu().numpy() - want) <= 5e-5 * scale)
+
+
+def test_threshold_search_matches_full_search(gpu):
+    """The fused threshold search (sampled threshold, filtered GEMM, per-row
+    sort) returns the same C nearest database rows as the full distance rows
+    with a radix select, including queries forced onto the fallback (lists
+    too short with ks = 1, overflowing with ks = 4000).  By default the
+    threshold aims at 4 C candidates per query, so fallbacks are rare."""
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    from trajsimimeasures_amd.data import lengths_lognormal, random_walks
+    from trajsimimeasures_amd.engine import DeviceTrajSet
+    rng = np.random.default_rng(12)
+    nq, nd = 300, 70000
+    dset = DeviceTrajSet(random_walks(rng, lengths_lognormal(rng, nq + nd)))
+    enc = F.Encoder()
+    index = F.FilterIndex(enc, dset, torch.arange(nq, nq + nd, dtype=torch.int32, device="cuda"))
+    q = torch.arange(nq, dtype=torch.int32, device="cuda")
+    A = index._split(enc.embed(dset, q), query_side=True)
+    full = index._search_full(A, 200).cpu().numpy()
+    for ks, expect_redo in ((None, False), (1, True), (4000, True)):
+        got = index.search(q, 200, ks=ks).cpu().numpy()
+        same = np.mean([len(set(got[i]) & set(full[i])) == 200 for i in range(nq)])
+        assert same == 1.0, (ks, same)
+        assert (index.last_redo > 0) == expect_redo or (not expect_redo and index.last_redo < 3)
 
 
 def test_filtered_knn_small_recall(gpu):

@@ -98,6 +98,167 @@ __global__ __launch_bounds__(256) void gemm_nt_bf16_kernel(GemmArgs g) {
         }
 }
 
+// LDS-tiled variant for large products: a 256-thread block computes a
+// 128 x 128 tile of C, each wave a 64 x 64 quarter (4 x 4 MFMA tiles), over
+// K in steps of 64 staged through double-buffered LDS tiles (rows padded to
+// 72 BF16 so fragment reads spread over banks).  Every A and B fragment
+// read from LDS feeds four MFMAs.  With kFilter the product is not stored:
+// each value below its row's threshold tau[row] is appended to that row's
+// candidate list (cnt[row] counts, entries past `cap` are dropped and
+// counted), which is how the embedding search keeps the C nearest database
+// rows without writing the full distance matrix.
+constexpr int kBM = 128, kBN = 128, kBK = 64, kLdsStride = kBK + 8;
+
+struct FilterArgs {
+    const float *tau;  // [M] per-row threshold
+    int cap;           // candidate slots per row
+    uint32_t *cnt;     // [M] candidates found (may exceed cap)
+    float *key;        // [M, cap] candidate values
+    int32_t *idx;      // [M, cap] candidate columns
+};
+
+__device__ __forceinline__ void tile_load(const uint16_t *base, int ld, int row0, int rows, int k0, int K, uint4 (&r)[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int q = (int)threadIdx.x + 256 * i;
+        const int row = row0 + (q >> 3), k = k0 + (q & 7) * 8;
+        r[i] = (row < rows && k < K) ? *reinterpret_cast<const uint4 *>(base + (int64_t)row * ld + k)
+                                     : make_uint4(0, 0, 0, 0);
+    }
+}
+__device__ __forceinline__ void tile_store(uint16_t *s, const uint4 (&r)[4]) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int q = (int)threadIdx.x + 256 * i;
+        *reinterpret_cast<uint4 *>(s + (q >> 3) * kLdsStride + (q & 7) * 8) = r[i];
+    }
+}
+
+template <bool kFilter>
+__global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g, FilterArgs f) {
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][kBM * kLdsStride];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][kBN * kLdsStride];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int bm = blockIdx.y * kBM, bn = blockIdx.x * kBN;
+    const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int r16 = lane & 15, kb = (lane >> 4) * 8;
+    f32x4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    uint4 ra[4], rb[4];
+    tile_load(g.A, g.lda, bm, g.M, 0, g.K, ra);
+    tile_load(g.B, g.ldb, bn, g.N, 0, g.K, rb);
+    tile_store(As[0], ra);
+    tile_store(Bs[0], rb);
+    __syncthreads();
+    const int nk = (g.K + kBK - 1) / kBK;
+    for (int kt = 0; kt < nk; ++kt) {
+        const int cur = kt & 1;
+        const bool more = kt + 1 < nk;
+        if (more) {
+            tile_load(g.A, g.lda, bm, g.M, (kt + 1) * kBK, g.K, ra);
+            tile_load(g.B, g.ldb, bn, g.N, (kt + 1) * kBK, g.K, rb);
+        }
+#pragma unroll
+        for (int kk = 0; kk < kBK; kk += 32) {
+            bf16x8 af[4], bfr[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                af[i] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4 *>(&As[cur][(wm + 16 * i + r16) * kLdsStride + kk + kb]));
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                bfr[j] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4 *>(&Bs[cur][(wn + 16 * j + r16) * kLdsStride + kk + kb]));
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+        }
+        if (more) {
+            tile_store(As[cur ^ 1], ra);
+            tile_store(Bs[cur ^ 1], rb);
+        }
+        __syncthreads();
+    }
+    // D layout: lane holds column (lane & 15) of each 16 x 16 tile, rows 4 * (lane >> 4) + e
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int col = bn + wn + 16 * j + r16;
+        if (col >= g.N) continue;
+        const float bb = g.bias ? g.bias[col] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int row = bm + wm + 16 * i + 4 * (lane >> 4) + e;
+                if (row >= g.M) continue;
+                const float v = g.alpha * acc[i][j][e] + bb;
+                if constexpr (kFilter) {
+                    if (v < f.tau[row]) {
+                        const uint32_t pos = atomicAdd(&f.cnt[row], 1u);
+                        if (pos < (uint32_t)f.cap) {
+                            f.key[(int64_t)row * f.cap + pos] = v;
+                            f.idx[(int64_t)row * f.cap + pos] = col;
+                        }
+                    }
+                } else {
+                    float *c = g.C + (int64_t)row * g.ldc + col;
+                    *c = g.beta == 0.f ? v : v + g.beta * *c;
+                }
+            }
+    }
+}
+
+// The C smallest candidates of each row: keys (order-preserving bits of the
+// value, then the column) sorted by a bitonic network in LDS.  A row whose
+// list is short (fewer than C) or overflowed (more than cap) is flagged for
+// the caller's exact fallback.  One 256-thread block per row; cap <= 4096.
+__global__ __launch_bounds__(256) void select_candidates_kernel(const uint32_t *cnt, const float *key,
+                                                                const int32_t *idx, int cap, int C, int32_t *out,
+                                                                int32_t *flag) {
+    __shared__ uint64_t sk[4096];
+    const int64_t row = blockIdx.x;
+    const uint32_t n = cnt[row];
+    if (n < (uint32_t)C || n > (uint32_t)cap) {
+        if (threadIdx.x == 0) flag[row] = 1;
+        return;
+    }
+    if (threadIdx.x == 0) flag[row] = 0;
+    int P = 1;
+    while (P < (int)n) P <<= 1;
+    for (int t = threadIdx.x; t < P; t += blockDim.x) {
+        if (t < (int)n) {
+            const uint32_t u = __builtin_bit_cast(uint32_t, key[row * cap + t]);
+            const uint32_t k = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+            sk[t] = ((uint64_t)k << 32) | (uint32_t)idx[row * cap + t];
+        } else {
+            sk[t] = ~0ull;
+        }
+    }
+    __syncthreads();
+    for (int size = 2; size <= P; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int t = threadIdx.x; t < P; t += blockDim.x) {
+                const int o = t ^ stride;
+                if (o > t) {
+                    const bool up = (t & size) == 0;
+                    const uint64_t a = sk[t], b = sk[o];
+                    if ((a > b) == up) {
+                        sk[t] = b;
+                        sk[o] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int t = threadIdx.x; t < C; t += blockDim.x) out[row * C + t] = (int32_t)(uint32_t)sk[t];
+}
+
 // FP32 [rows, cols] (ld_src) -> BF16, optionally transposed: dst[c][r].
 __global__ void cast_bf16_kernel(const float *src, int rows, int cols, int ld_src, uint16_t *dst, int ld_dst,
                                  int transpose) {
@@ -542,8 +703,43 @@ int tsm_nn_gemm_nt(int M, int N, int K, const uint16_t *A, int lda, const uint16
         (reinterpret_cast<uintptr_t>(A) % 16) || (reinterpret_cast<uintptr_t>(B) % 16))
         return fail(TSM_E_ARG, "gemm: bad operands (16-byte aligned, leading dimensions multiples of 8)");
     GemmArgs g{M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, bias};
-    dim3 grid((N + 63) / 64, (M + 63) / 64);
-    hipLaunchKernelGGL(gemm_nt_bf16_kernel, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), g);
+    if (M >= kBM && N >= kBN && K >= kBK && (M + kBM - 1) / kBM <= 65535) {
+        dim3 grid((N + kBN - 1) / kBN, (M + kBM - 1) / kBM);
+        hipLaunchKernelGGL(gemm_nt_lds_kernel<false>, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), g,
+                           FilterArgs{});
+    } else {
+        if ((M + 63) / 64 > 65535) return fail(TSM_E_ARG, "gemm: M too large");
+        dim3 grid((N + 63) / 64, (M + 63) / 64);
+        hipLaunchKernelGGL(gemm_nt_bf16_kernel, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), g);
+    }
+    TSM_HIP(hipGetLastError());
+    return TSM_OK;
+}
+
+int tsm_nn_gemm_nt_filter(int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb, float alpha,
+                          const float *bias, const float *tau, int cap, uint32_t *cnt, float *key, int32_t *idx,
+                          void *stream) {
+    if (M < 0 || N < 0 || K < 0 || (K % 8) != 0) return fail(TSM_E_ARG, "gemm_filter: K must be a multiple of 8");
+    if (M == 0 || N == 0) return TSM_OK;
+    if (!A || !B || !tau || !cnt || !key || !idx || cap < 1 || lda < K || ldb < K || (lda % 8) || (ldb % 8) ||
+        (reinterpret_cast<uintptr_t>(A) % 16) || (reinterpret_cast<uintptr_t>(B) % 16))
+        return fail(TSM_E_ARG, "gemm_filter: bad operands (16-byte aligned, leading dimensions multiples of 8)");
+    if ((M + kBM - 1) / kBM > 65535) return fail(TSM_E_ARG, "gemm_filter: M too large");
+    GemmArgs g{M, N, K, A, lda, B, ldb, nullptr, 0, alpha, 0.f, bias};
+    dim3 grid((N + kBN - 1) / kBN, (M + kBM - 1) / kBM);
+    hipLaunchKernelGGL(gemm_nt_lds_kernel<true>, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), g,
+                       FilterArgs{tau, cap, cnt, key, idx});
+    TSM_HIP(hipGetLastError());
+    return TSM_OK;
+}
+
+int tsm_nn_select_candidates(const uint32_t *cnt, const float *key, const int32_t *idx, int rows, int cap, int C,
+                             int32_t *out, int32_t *flag, void *stream) {
+    if (rows < 0 || cap < 1 || cap > 4096 || C < 1 || C > cap || !cnt || !key || !idx || !out || !flag)
+        return fail(TSM_E_ARG, "select_candidates: need 1 <= C <= cap <= 4096");
+    if (rows == 0) return TSM_OK;
+    hipLaunchKernelGGL(select_candidates_kernel, dim3(rows), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), cnt,
+                       key, idx, cap, C, out, flag);
     TSM_HIP(hipGetLastError());
     return TSM_OK;
 }

@@ -71,9 +71,11 @@ def _nn():
         L.tsm_nn_topk_f64.argtypes = [P, I64, I64, I64, i, P, P, P]
         L.tsm_nn_linear_f32.argtypes = [P, I64, i, P, i, P, f, P]
         L.tsm_nn_split_bf16.argtypes = [P, I64, i, i, P, P, P]
+        L.tsm_nn_gemm_nt_filter.argtypes = [i, i, i, P, i, P, i, f, P, P, i, P, P, P, P]
+        L.tsm_nn_select_candidates.argtypes = [P, P, P, i, i, i, P, P, P]
         for n in ("gemm_nt", "cast_bf16", "resample", "conv_fwd", "conv_bwd", "gru_h0", "gru_h0_bwd",
                   "gru_step_fwd", "gru_step_bwd", "colsum", "rank_loss", "adam", "rownorm_bf16", "topk_f32",
-                  "topk_f64", "linear_f32", "split_bf16"):
+                  "topk_f64", "linear_f32", "split_bf16", "gemm_nt_filter", "select_candidates"):
             getattr(L, "tsm_nn_" + n).restype = ctypes.c_int
         L._nn_declared = True
     return L
@@ -296,6 +298,7 @@ class FilterIndex:
 
     def __init__(self, enc: Encoder, dset: DeviceTrajSet, db_ids):
         self.enc, self.dset = enc, dset
+        self.last_redo = 0
         self.db_ids = torch.as_tensor(db_ids, dtype=torch.int32, device=enc.device)
         E = enc.embed(dset, self.db_ids)
         self.norm = torch.empty(E.shape[0], dtype=torch.float32, device=E.device)
@@ -309,10 +312,52 @@ class FilterIndex:
         _lib.check(_nn().tsm_nn_split_bf16(_p(E), E.shape[0], D, int(query_side), _p(out), _p(norm), _s()))
         return out
 
-    def search(self, q_ids, C: int = 200, chunk: int = 1 << 20):
-        """Candidate database positions [nq, C] by embedding distance."""
+    def search(self, q_ids, C: int = 200, *, cap: int = 4096, sample: int = 16384, ks: int | None = None):
+        """Candidate database positions [nq, C] by embedding distance (the C
+        smallest |e_d|^2 - 2 e_q . e_d of each query, ascending).
+
+        Large databases skip the [nq, nd] distance matrix: a strided sample of
+        `sample` rows gives each query a threshold (its ks-th smallest sample
+        distance, about ks * nd / sample database rows below it; by default
+        ks aims at 4 C rows), one fused
+        GEMM keeps only the distances below the threshold, and a per-row sort
+        of those picks the C smallest.  A query whose list comes out shorter
+        than C or longer than `cap` (rare by construction) is redone over the
+        full distance row."""
         Eq = self.enc.embed(self.dset, q_ids)
         A = self._split(Eq, query_side=True)
+        nq, nd = A.shape[0], self.B.shape[0]
+        if nd < 4 * sample or C > cap // 2:
+            return self._search_full(A, C)
+        stride = nd // sample
+        if ks is None:
+            ks = int(round(4 * C * sample / nd))
+        ks = max(1, min(ks, sample))
+        Bs = self.B[::stride][:sample]
+        ns = torch.empty(sample, dtype=torch.float32, device=A.device)
+        ns.copy_(self.norm[::stride][:sample])
+        dist_s = torch.empty((nq, sample), dtype=torch.float32, device=A.device)
+        gemm_nt(A, Bs, dist_s, alpha=-2.0, bias=ns)
+        _, vals = topk_smallest(dist_s, ks)
+        tau = vals.max(dim=1).values.contiguous()
+        del dist_s
+        cnt = torch.zeros(nq, dtype=torch.int32, device=A.device)
+        key = torch.empty((nq, cap), dtype=torch.float32, device=A.device)
+        idx = torch.empty((nq, cap), dtype=torch.int32, device=A.device)
+        L = _nn()
+        _lib.check(L.tsm_nn_gemm_nt_filter(nq, nd, 3 * D, _p(A), A.stride(0), _p(self.B), self.B.stride(0), -2.0,
+                                           _p(self.norm), _p(tau), cap, _p(cnt), _p(key), _p(idx), _s()))
+        cand = torch.empty((nq, C), dtype=torch.int32, device=A.device)
+        flag = torch.empty(nq, dtype=torch.int32, device=A.device)
+        _lib.check(L.tsm_nn_select_candidates(_p(cnt), _p(key), _p(idx), nq, cap, C, _p(cand), _p(flag), _s()))
+        redo = torch.nonzero(flag).reshape(-1)
+        self.last_redo = int(redo.numel())
+        if self.last_redo:
+            cand[redo] = self._search_full(A[redo].contiguous(), C)
+        return cand
+
+    def _search_full(self, A: torch.Tensor, C: int, chunk: int = 1 << 20) -> torch.Tensor:
+        """The C smallest of every full distance row (dense GEMM + radix select)."""
         nq, nd = A.shape[0], self.B.shape[0]
         cand = torch.empty((nq, C), dtype=torch.int32, device=A.device)
         rows = max(1, min(nq, chunk // max(nd // 1024, 1)))
