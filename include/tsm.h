@@ -42,6 +42,8 @@ extern "C" {
 #define TSM_ERP 3       /* core/erp.py:10-120 */
 #define TSM_FRECHET 4   /* core/discret_frechet.py:9-74 */
 #define TSM_HAUSDORFF 5 /* core/hausdorff.py:12-123 */
+#define TSM_STEDR 6     /* core/stedr.py:14-86: EDR whose match also needs |t1 - t2| < delta */
+#define TSM_STLCSS 7    /* core/stlcss.py:10-80: LCSS whose match also needs |t1 - t2| < delta */
 
 /* Error codes. */
 #define TSM_OK 0
@@ -55,7 +57,7 @@ extern "C" {
 #define TSM_EPS_SQUARED 1 /* dx^2+dy^2 < eps*eps: reference GPU, core/edr.py:22,42 */
 
 /* Arithmetic of the DP. */
-#define TSM_PREC_FP32 0 /* default: FP32 DP (EDR/LCSS match tests always use FP64 coordinates) */
+#define TSM_PREC_FP32 0 /* default: FP32 DP (EDR/LCSS match decisions are exact FP64 decisions) */
 #define TSM_PREC_FP64 1 /* FP64 DP end to end (DTW, ERP, Frechet): the drift-check rerun */
 
 /* A packed trajectory set resident on the device. */
@@ -69,6 +71,7 @@ typedef struct tsm_trajset {
     double coord_max;       /* an upper bound on |x| and |y| over the set, or 0 if unknown.  EDR and
                                LCSS use it to run their match test on the FP32 copy with a proven
                                guard band (FP64 recheck inside the band); 0 keeps the test in FP64 */
+    const double *t64;      /* device [npts] per-point timestamps for STEDR/STLCSS, or NULL */
 } tsm_trajset;
 
 /* Per-call parameters; tsm_params_default() fills the defaults. */
@@ -78,6 +81,7 @@ typedef struct tsm_params {
     double gx, gy;  /* ERP gap point g (core/erp.py:94 `gpoint`, default (0,0) erp.py:103) */
     int lanes;      /* lanes per pair: 0 = autotuned per column count, else 4 or 8 (threads_per_traj hint) */
     int precision;  /* TSM_PREC_FP32 or TSM_PREC_FP64 */
+    double delta;   /* STEDR/STLCSS time threshold (core/stedr.py:62 `delta`): match needs |t1 - t2| < delta */
 } tsm_params;
 
 /* Library / device setup (reference: numba.cuda context, implicit). */
@@ -119,12 +123,13 @@ int tsm_pairs_triu(int measure, const tsm_trajset *set, int64_t n, const tsm_par
                    int32_t *counts, void *stream);
 
 /*
- * Host-memory convenience: uploads xy64 [npts][2] / offsets [n+1] and the
- * pair lists, runs tsm_pairs on the null stream, copies results back and
- * synchronises.  This is the body of the reference's per-batch functions.
+ * Host-memory convenience: uploads xy64 [npts][2] / offsets [n+1] (and the
+ * timestamps t64 [npts] for STEDR/STLCSS, else NULL) and the pair lists,
+ * runs tsm_pairs on the null stream, copies results back and synchronises.
+ * This is the body of the reference's per-batch functions.
  */
-int tsm_pairs_host(int measure, const double *xy64, const int64_t *offsets, int64_t n, int64_t npts,
-                   const int32_t *pa, const int32_t *pb, int64_t npairs, const tsm_params *prm,
+int tsm_pairs_host(int measure, const double *xy64, const double *t64, const int64_t *offsets, int64_t n,
+                   int64_t npts, const int32_t *pa, const int32_t *pb, int64_t npairs, const tsm_params *prm,
                    double *out, int32_t *counts);
 
 /*

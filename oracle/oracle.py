@@ -15,8 +15,9 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libtsm_oracle.so")
 
-DTW, EDR, LCSS, ERP, FRECHET, HAUSDORFF = 0, 1, 2, 3, 4, 5
-NAMES = {"dtw": DTW, "edr": EDR, "lcss": LCSS, "erp": ERP, "frechet": FRECHET, "hausdorff": HAUSDORFF}
+DTW, EDR, LCSS, ERP, FRECHET, HAUSDORFF, STEDR, STLCSS = 0, 1, 2, 3, 4, 5, 6, 7
+NAMES = {"dtw": DTW, "edr": EDR, "lcss": LCSS, "erp": ERP, "frechet": FRECHET, "hausdorff": HAUSDORFF,
+         "stedr": STEDR, "stlcss": STLCSS}
 
 _lib = None
 _P = ctypes.c_void_p
@@ -54,6 +55,12 @@ def lib():
         L.tsmo_erp.restype = d
         L.tsmo_pairs.argtypes = [ctypes.c_int, _P, _P, _P, _P, i64, d, _P, _P, _P, ctypes.c_int]
         L.tsmo_pairs.restype = ctypes.c_int
+        for name in ("tsmo_stedr_count", "tsmo_stlcss_count"):
+            f = getattr(L, name)
+            f.argtypes = [_P, _P, i64, _P, _P, i64, d, d]
+            f.restype = i64
+        L.tsmo_pairs_st.argtypes = [ctypes.c_int, _P, _P, _P, _P, _P, i64, d, d, _P, _P, ctypes.c_int]
+        L.tsmo_pairs_st.restype = ctypes.c_int
         L.tsmo_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -117,3 +124,31 @@ def pairs(measure: str, xy, offsets, pa, pb, eps: float = 0.0, eps_mode: str = "
 
 def max_threads() -> int:
     return lib().tsmo_max_threads()
+
+
+def st_count(measure: str, a, ta, b, tb, eps: float, delta: float, eps_mode: str = "sqrt") -> int:
+    """STEDR / STLCSS match count of one pair: a, b [n, 2], ta, tb [n] times."""
+    a, b = _arr(a), _arr(b)
+    ta = np.ascontiguousarray(np.asarray(ta, dtype=np.float64).reshape(-1))
+    tb = np.ascontiguousarray(np.asarray(tb, dtype=np.float64).reshape(-1))
+    f = lib().tsmo_stedr_count if measure == "stedr" else lib().tsmo_stlcss_count
+    return f(a.ctypes.data, ta.ctypes.data, len(a), b.ctypes.data, tb.ctypes.data, len(b),
+             eps_threshold(eps, eps_mode), float(delta))
+
+
+def pairs_st(measure: str, xy, t, offsets, pa, pb, eps: float, delta: float, eps_mode: str = "sqrt",
+             threads: int = 0):
+    """Batched STEDR / STLCSS: (values float64, counts int64)."""
+    xy = np.ascontiguousarray(np.asarray(xy, dtype=np.float64).reshape(-1, 2))
+    t = np.ascontiguousarray(np.asarray(t, dtype=np.float64).reshape(-1))
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    pa = np.ascontiguousarray(pa, dtype=np.int64)
+    pb = np.ascontiguousarray(pb, dtype=np.int64)
+    out = np.empty(pa.size, dtype=np.float64)
+    cnt = np.empty(pa.size, dtype=np.int64)
+    rc = lib().tsmo_pairs_st(NAMES[measure], xy.ctypes.data, t.ctypes.data, offsets.ctypes.data, pa.ctypes.data,
+                             pb.ctypes.data, pa.size, eps_threshold(eps, eps_mode), float(delta), out.ctypes.data,
+                             cnt.ctypes.data, int(threads))
+    if rc != 0:
+        raise ValueError(f"bad measure {measure!r}")
+    return out, cnt

@@ -17,6 +17,9 @@
  *   ERP       core/erp.py:29-91             constant-sum boundary D[i][0]=Sa, D[0][j]=Sb
  *   Frechet   core/discret_frechet.py:26-50 inf boundary, D = max(d, min3)
  *   Hausdorff core/hausdorff.py:12-69,121   point-to-segment, squared distances, sqrt at the end
+ *   STEDR     core/stedr.py:29-59           EDR whose match also needs |t1 - t2| < delta
+ *   STLCSS    core/stlcss.py:27-54          LCSS whose match also needs |t1 - t2| < delta
+ *             (CPU forms: core_cpu/stedr.pyx:15-42, core_cpu/stlcss.pyx:15-39)
  *
  * Distances are sqrt(dx*dx + dy*dy) evaluated without FMA contraction
  * (core/utils.py:8-17); build with -ffp-contract=off.
@@ -42,7 +45,8 @@
 #include <omp.h>
 #endif
 
-enum { TSMO_DTW = 0, TSMO_EDR = 1, TSMO_LCSS = 2, TSMO_ERP = 3, TSMO_FRECHET = 4, TSMO_HAUSDORFF = 5 };
+enum { TSMO_DTW = 0, TSMO_EDR = 1, TSMO_LCSS = 2, TSMO_ERP = 3, TSMO_FRECHET = 4, TSMO_HAUSDORFF = 5,
+       TSMO_STEDR = 6, TSMO_STLCSS = 7 };
 
 static inline double sqd(const double *p, const double *q) {
     double dx = p[0] - q[0];
@@ -129,6 +133,55 @@ int64_t tsmo_lcss_count(const double *a, int64_t n1, const double *b, int64_t n2
         for (int64_t j = 1; j <= n2; ++j) {
             int64_t v;
             if (sqd(a + 2 * (i - 1), b + 2 * (j - 1)) < thr) {
+                v = C[(i - 1) * w + j - 1] + 1;
+            } else {
+                int64_t up = C[(i - 1) * w + j], left = C[i * w + j - 1];
+                v = up > left ? up : left;
+            }
+            C[i * w + j] = v;
+        }
+    }
+    int64_t r = C[n1 * w + n2];
+    free(C);
+    return r;
+}
+
+/* Spatio-temporal match: (dx*dx + dy*dy) < thr and |ta - tb| < delta
+ * (core/stedr.py:44-46, core/stlcss.py:41-42; fabs as core_cpu/stedr.pyx:37). */
+static inline int st_match(const double *p, double tp, const double *q, double tq, double thr, double delta) {
+    return sqd(p, q) < thr && fabs(tp - tq) < delta;
+}
+
+/* core/stedr.py:29-59 -- EDR on spatio-temporal matches; returns the count. */
+int64_t tsmo_stedr_count(const double *a, const double *ta, int64_t n1, const double *b, const double *tb,
+                         int64_t n2, double thr, double delta) {
+    const int64_t w = n2 + 1;
+    int64_t *C = (int64_t *)calloc((size_t)(n1 + 1) * (size_t)w, sizeof(int64_t));
+    for (int64_t i = 1; i <= n1; ++i) {
+        for (int64_t j = 1; j <= n2; ++j) {
+            int64_t sub = st_match(a + 2 * (i - 1), ta[i - 1], b + 2 * (j - 1), tb[j - 1], thr, delta) ? 0 : 1;
+            int64_t v = C[(i - 1) * w + j - 1] + sub;
+            int64_t up = C[(i - 1) * w + j] + 1;
+            int64_t left = C[i * w + j - 1] + 1;
+            if (up < v) v = up;
+            if (left < v) v = left;
+            C[i * w + j] = v;
+        }
+    }
+    int64_t r = C[n1 * w + n2];
+    free(C);
+    return r;
+}
+
+/* core/stlcss.py:27-54 -- LCSS on spatio-temporal matches; returns the count. */
+int64_t tsmo_stlcss_count(const double *a, const double *ta, int64_t n1, const double *b, const double *tb,
+                          int64_t n2, double thr, double delta) {
+    const int64_t w = n2 + 1;
+    int64_t *C = (int64_t *)calloc((size_t)(n1 + 1) * (size_t)w, sizeof(int64_t));
+    for (int64_t i = 1; i <= n1; ++i) {
+        for (int64_t j = 1; j <= n2; ++j) {
+            int64_t v;
+            if (st_match(a + 2 * (i - 1), ta[i - 1], b + 2 * (j - 1), tb[j - 1], thr, delta)) {
                 v = C[(i - 1) * w + j - 1] + 1;
             } else {
                 int64_t up = C[(i - 1) * w + j], left = C[i * w + j - 1];
@@ -268,6 +321,36 @@ int tsmo_pairs(int measure, const double *xy, const int64_t *offsets, const int6
         const double *a = xy + 2 * offsets[ia];
         const double *b = xy + 2 * offsets[ib];
         out[k] = one_pair(measure, a, offsets[ia + 1] - offsets[ia], b, offsets[ib + 1] - offsets[ib], thr, g, &c);
+        if (counts) counts[k] = c;
+    }
+    return 0;
+}
+
+/* Batched STEDR (6) / STLCSS (7) over pairs of a set with per-point times
+ * t[npts]; values normalised as core/stedr.py:57 and core/stlcss.py:52. */
+int tsmo_pairs_st(int measure, const double *xy, const double *t, const int64_t *offsets, const int64_t *pa,
+                  const int64_t *pb, int64_t npairs, double thr, double delta, double *out, int64_t *counts,
+                  int nthreads) {
+    if (measure != TSMO_STEDR && measure != TSMO_STLCSS) return -1;
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    int64_t k;
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < npairs; ++k) {
+        const int64_t ia = pa[k], ib = pb[k];
+        const int64_t oa = offsets[ia], ob = offsets[ib];
+        const int64_t n1 = offsets[ia + 1] - oa, n2 = offsets[ib + 1] - ob;
+        int64_t c;
+        if (measure == TSMO_STEDR) {
+            c = tsmo_stedr_count(xy + 2 * oa, t + oa, n1, xy + 2 * ob, t + ob, n2, thr, delta);
+            out[k] = (double)c / (double)(n1 > n2 ? n1 : n2);
+        } else {
+            c = tsmo_stlcss_count(xy + 2 * oa, t + oa, n1, xy + 2 * ob, t + ob, n2, thr, delta);
+            out[k] = 1.0 - (double)c / (double)(n1 < n2 ? n1 : n2);
+        }
         if (counts) counts[k] = c;
     }
     return 0;

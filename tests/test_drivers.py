@@ -14,7 +14,10 @@ def test_dispatch_and_collate():
     with pytest.raises(NotImplementedError):
         drivers.heuristic_fn_wrapper("dtw_cpu")
     with pytest.raises(NotImplementedError):
-        drivers.heuristic_fn_wrapper("stedr")
+        drivers.heuristic_fn_wrapper("cdds")
+    g = drivers.heuristic_fn_wrapper("stlcss", eps=0.2)
+    assert g.keywords == {"eps": 0.2, "delta": drivers.DEFAULT_DELTA, "threads_per_traj": 64}
+    assert g.func is drivers.core.stlcss
     assert drivers.is_heuristic("frechet_cpu") and not drivers.is_heuristic("TrajCL")
     batch = [([[0.1, 0.2]], [[0.3, 0.4], [0.5, 0.6]]), ([[1 / 3, 0.0], [0.7, 0.8]], [[0.9, 1.0]])]
     t1, l1, t2, l2 = drivers.heuristic_colllate_fn(batch)

@@ -59,11 +59,11 @@ def test_argument_errors_without_device():
     offs = np.array([0, 2, 4], dtype=np.int64)  # does not end at npts
     pa = np.zeros(1, np.int32)
     out = np.zeros(1)
-    rc = L.tsm_pairs_host(0, xy.ctypes.data, offs.ctypes.data, 2, 3, pa.ctypes.data, pa.ctypes.data, 1,
+    rc = L.tsm_pairs_host(0, xy.ctypes.data, None, offs.ctypes.data, 2, 3, pa.ctypes.data, pa.ctypes.data, 1,
                           ctypes.byref(prm), out.ctypes.data, None)
     assert rc == -1 and b"offsets" in L.tsm_last_error()
     offs = np.array([0, 0, 3], dtype=np.int64)  # empty trajectory
-    rc = L.tsm_pairs_host(0, xy.ctypes.data, offs.ctypes.data, 2, 3, pa.ctypes.data, pa.ctypes.data, 1,
+    rc = L.tsm_pairs_host(0, xy.ctypes.data, None, offs.ctypes.data, 2, 3, pa.ctypes.data, pa.ctypes.data, 1,
                           ctypes.byref(prm), out.ctypes.data, None)
     assert rc == -1 and b"empty" in L.tsm_last_error()
     with pytest.raises(_lib.TsmError):
@@ -162,3 +162,33 @@ def test_pad_lists_to_array():
     arr = pad_lists_to_array([[[1, 2]], [[3, 4], [5, 6]]])
     assert arr.shape == (2, 2, 2) and arr.dtype == np.float64
     assert np.array_equal(arr[0], [[1, 2], [0, 0]])
+
+
+def test_timestamps_and_st_packing():
+    from trajsimimeasures_amd.data import with_timestamps
+    rng = np.random.default_rng(3)
+    ts = with_timestamps(rng, random_walks(rng, lengths_uniform(rng, 30, 1, 20)))
+    for i in range(ts.n):
+        t = ts.t[ts.offsets[i]:ts.offsets[i + 1]]
+        assert 0 <= t[0] < 3600 and np.all(np.diff(t) >= 10.0 - 1e-9) and np.all(np.diff(t) <= 20.0 + 1e-9)
+    sub = ts.subset([4, 2])
+    assert np.array_equal(sub.t[:sub.lengths[0]], ts.t[ts.offsets[4]:ts.offsets[5]])
+    cat = TrajSet.concat([ts, sub])
+    assert cat.t.size == cat.xy.shape[0]
+    # [B, L, 3] padded batches and [n, 3] lists carry the timestamps
+    lens = np.array([2, 3])
+    arr = np.zeros((2, 3, 3))
+    arr[0, :2] = [[0.1, 0.2, 5.0], [0.3, 0.4, 6.0]]
+    arr[1] = [[0.5, 0.6, 1.0], [0.7, 0.8, 2.0], [0.9, 1.0, 3.0]]
+    p = pack_padded(arr, lens)
+    assert np.array_equal(p.t, [5.0, 6.0, 1.0, 2.0, 3.0]) and p.xy.shape == (5, 2)
+    pl = pack_list([arr[0, :2], arr[1]])
+    assert np.array_equal(pl.t, p.t) and np.array_equal(pl.xy, p.xy)
+    with pytest.raises(ValueError):
+        TrajSet(p.xy, p.offsets, p.t[:-1])
+
+
+def test_st_dropin_rejects_2d_points():
+    from trajsimimeasures_amd import core
+    with pytest.raises(ValueError):
+        core.stedr(np.zeros((2, 3, 2)), [1, 1], np.zeros((2, 3, 2)), [1, 1], 0.1, 60)

@@ -13,10 +13,12 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("TSM_LIB", os.path.join(PKG, "libtsm.so"))
 
 # Measure ids (tsm.h)
-DTW, EDR, LCSS, ERP, FRECHET, HAUSDORFF = 0, 1, 2, 3, 4, 5
+DTW, EDR, LCSS, ERP, FRECHET, HAUSDORFF, STEDR, STLCSS = 0, 1, 2, 3, 4, 5, 6, 7
 MEASURES = {"dtw": DTW, "edr": EDR, "lcss": LCSS, "erp": ERP, "frechet": FRECHET, "dfrechet": FRECHET,
-            "discret_frechet": FRECHET, "hausdorff": HAUSDORFF}
-MEASURE_NAMES = {DTW: "dtw", EDR: "edr", LCSS: "lcss", ERP: "erp", FRECHET: "frechet", HAUSDORFF: "hausdorff"}
+            "discret_frechet": FRECHET, "hausdorff": HAUSDORFF, "stedr": STEDR, "stlcss": STLCSS}
+MEASURE_NAMES = {DTW: "dtw", EDR: "edr", LCSS: "lcss", ERP: "erp", FRECHET: "frechet", HAUSDORFF: "hausdorff",
+                 STEDR: "stedr", STLCSS: "stlcss"}
+COUNTED = (EDR, LCSS, STEDR, STLCSS)  # measures with integer match counts
 
 EPS_SQRT, EPS_SQUARED = 0, 1
 PREC_FP32, PREC_FP64 = 0, 1
@@ -44,12 +46,13 @@ class TsmError(RuntimeError):
 class TrajSetC(ctypes.Structure):
     _fields_ = [("xy32", ctypes.c_void_p), ("xy64", ctypes.c_void_p), ("offsets", ctypes.c_void_p),
                 ("n", ctypes.c_int64), ("npts", ctypes.c_int64), ("max_len", ctypes.c_int64),
-                ("coord_max", ctypes.c_double)]
+                ("coord_max", ctypes.c_double), ("t64", ctypes.c_void_p)]
 
 
 class ParamsC(ctypes.Structure):
     _fields_ = [("eps", ctypes.c_double), ("eps_mode", ctypes.c_int), ("gx", ctypes.c_double),
-                ("gy", ctypes.c_double), ("lanes", ctypes.c_int), ("precision", ctypes.c_int)]
+                ("gy", ctypes.c_double), ("lanes", ctypes.c_int), ("precision", ctypes.c_int),
+                ("delta", ctypes.c_double)]
 
 
 _lock = threading.Lock()
@@ -70,7 +73,7 @@ def _declare(L):
     L.tsm_pairs_grid.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _I64, _I64, _I64, _I64,
                                  ctypes.POINTER(ParamsC), _P, _P, _P]
     L.tsm_pairs_triu.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _I64, ctypes.POINTER(ParamsC), _P, _P, _P]
-    L.tsm_pairs_host.argtypes = [ctypes.c_int, _P, _P, _I64, _I64, _P, _P, _I64, ctypes.POINTER(ParamsC), _P, _P]
+    L.tsm_pairs_host.argtypes = [ctypes.c_int, _P, _P, _P, _I64, _I64, _P, _P, _I64, ctypes.POINTER(ParamsC), _P, _P]
     L.tsm_tune_set_lanes.argtypes = [ctypes.c_int, _P, ctypes.c_int]
     L.tsm_tune_set_waves_per_cu.argtypes = [ctypes.c_int]
     L.tsm_graph_capture_pairs.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _P, _P, _I64,
@@ -111,10 +114,12 @@ def check(rc: int):
         raise TsmError(rc, msg.decode() if msg else "")
 
 
-def params(eps: float = 0.0, eps_mode="sqrt", gpoint=None, lanes: int = 0, precision="fp32") -> ParamsC:
+def params(eps: float = 0.0, eps_mode="sqrt", gpoint=None, lanes: int = 0, precision="fp32",
+           delta: float = 0.0) -> ParamsC:
     p = ParamsC()
     lib().tsm_params_default(ctypes.byref(p))
     p.eps = float(eps)
+    p.delta = float(delta)
     if isinstance(eps_mode, str):
         if eps_mode not in ("sqrt", "squared"):
             raise ValueError("eps_mode must be 'sqrt' or 'squared'")

@@ -1,12 +1,15 @@
 """Drop-in for the reference's ``core`` package (core/__init__.py:1-10) for
-the six DP / Hausdorff measures: ``core.dtw(t1, len1, t2, len2, ...)`` etc.,
-same signatures and float64 [batch] results, computed by libtsm.so."""
+the six DP / Hausdorff measures and the spatio-temporal STEDR / STLCSS:
+``core.dtw(t1, len1, t2, len2, ...)`` etc., same signatures and float64
+[batch] results, computed by libtsm.so."""
 from .discret_frechet import *  # noqa: F401,F403
 from .dtw import *  # noqa: F401,F403
 from .edr import *  # noqa: F401,F403
 from .erp import *  # noqa: F401,F403
 from .hausdorff import *  # noqa: F401,F403
 from .lcss import *  # noqa: F401,F403
+from .stedr import *  # noqa: F401,F403
+from .stlcss import *  # noqa: F401,F403
 from .utils import pad_lists_to_array  # noqa: F401
 
-__all__ = ['dfrechet', 'dtw', 'edr', 'erp', 'hausdorff', 'lcss', 'pad_lists_to_array']
+__all__ = ['dfrechet', 'dtw', 'edr', 'erp', 'hausdorff', 'lcss', 'stedr', 'stlcss', 'pad_lists_to_array']

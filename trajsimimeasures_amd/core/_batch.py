@@ -25,11 +25,15 @@ def lanes_hint(threads_per_traj) -> int:
 
 
 def run_padded(measure: int, t1, len1, t2, len2, *, eps=0.0, eps_mode="sqrt", gpoint=None, threads_per_traj=64,
-               precision="fp32", return_counts=False):
+               precision="fp32", return_counts=False, delta=0.0):
     t1 = np.asarray(t1, dtype=np.float64)
     t2 = np.asarray(t2, dtype=np.float64)
-    if t1.ndim != 3 or t2.ndim != 3:
-        raise ValueError("t1 and t2 must be [batch, seq, 2] arrays")
+    st = measure in (_lib.STEDR, _lib.STLCSS)
+    width = 3 if st else 2
+    if t1.ndim != 3 or t2.ndim != 3 or (st and (t1.shape[2] != 3 or t2.shape[2] != 3)):
+        raise ValueError(f"t1 and t2 must be [batch, seq, {width}] arrays")
+    if not st:
+        t1, t2 = t1[..., :2], t2[..., :2]
     assert t1.shape[0] == t2.shape[0]
     B = t1.shape[0]
     if B == 0:
@@ -41,13 +45,17 @@ def run_padded(measure: int, t1, len1, t2, len2, *, eps=0.0, eps_mode="sqrt", gp
     if not np.all(np.isfinite(xy)):
         raise ValueError("coordinates must be finite")
     offs = np.concatenate([a.offsets, b.offsets[1:] + a.offsets[-1]]).astype(np.int64)
+    tt = np.ascontiguousarray(np.concatenate([a.t, b.t])) if st else None
+    if st and not np.all(np.isfinite(tt)):
+        raise ValueError("timestamps must be finite")
     pa = np.arange(B, dtype=np.int32)
     pb = pa + np.int32(B)
     out = np.empty(B, dtype=np.float64)
     cnt = np.empty(B, dtype=np.int32) if return_counts else None
-    prm = _lib.params(eps, eps_mode, gpoint, lanes_hint(threads_per_traj), precision)
+    prm = _lib.params(eps, eps_mode, gpoint, lanes_hint(threads_per_traj), precision, delta)
     L = _lib.lib()
-    _lib.check(L.tsm_pairs_host(measure, xy.ctypes.data, offs.ctypes.data, 2 * B, xy.shape[0], pa.ctypes.data,
+    _lib.check(L.tsm_pairs_host(measure, xy.ctypes.data, tt.ctypes.data if st else None, offs.ctypes.data, 2 * B,
+                                xy.shape[0], pa.ctypes.data,
                                 pb.ctypes.data, B, ctypes.byref(prm), out.ctypes.data,
                                 cnt.ctypes.data if cnt is not None else None))
     return (out, cnt) if return_counts else out

@@ -89,7 +89,7 @@ int erp_prologue(Workspace &c, const tsm_trajset &set, double gx, double gy, hip
 // overridden by prm_in.
 int validate(int measure, const tsm_trajset *set, int64_t npairs, const tsm_params *prm_in, const double *out,
              tsm_params &prm) {
-    if (measure < TSM_DTW || measure > TSM_HAUSDORFF) return fail(TSM_E_ARG, "unknown measure id " + std::to_string(measure));
+    if (measure < TSM_DTW || measure > TSM_STLCSS) return fail(TSM_E_ARG, "unknown measure id " + std::to_string(measure));
     if (!set) return fail(TSM_E_ARG, "null trajectory set");
     if (npairs < 0) return fail(TSM_E_ARG, "negative pair count");
     if (npairs > (int64_t)0xFFFFFFFEll) return fail(TSM_E_ARG, "at most 2^32-2 pairs per call");
@@ -105,9 +105,11 @@ int validate(int measure, const tsm_trajset *set, int64_t npairs, const tsm_para
         return fail(TSM_E_ARG, "precision must be TSM_PREC_FP32 or TSM_PREC_FP64");
     if (prm.eps_mode != TSM_EPS_SQRT && prm.eps_mode != TSM_EPS_SQUARED)
         return fail(TSM_E_ARG, "eps_mode must be TSM_EPS_SQRT or TSM_EPS_SQUARED");
-    const bool needs64 = f64 || measure == TSM_EDR || measure == TSM_LCSS || measure == TSM_ERP;
+    const bool st = measure == TSM_STEDR || measure == TSM_STLCSS;
+    if (st && !set->t64) return fail(TSM_E_ARG, "STEDR/STLCSS need per-point timestamps (t64)");
+    const bool needs64 = f64 || st || measure == TSM_EDR || measure == TSM_LCSS || measure == TSM_ERP;
     if (needs64 && !set->xy64) return fail(TSM_E_ARG, "this measure needs the FP64 coordinates (xy64)");
-    if (!f64 && measure != TSM_EDR && measure != TSM_LCSS && !set->xy32)
+    if (!f64 && !st && measure != TSM_EDR && measure != TSM_LCSS && !set->xy32)
         return fail(TSM_E_ARG, "FP32 measures need the FP32 coordinates (xy32)");
     if (set->max_len > (1 << 24)) return fail(TSM_E_ARG, "trajectories longer than 2^24 points are not supported");
     return TSM_OK;
@@ -169,6 +171,8 @@ int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &p
         return band ? run_dp<kLCSS, false>(w, L, *set, ps, npairs, thr, out, counts, st)
                     : run_dp<kLCSS, true>(w, L, *set, ps, npairs, thr, out, counts, st);
     }
+    case TSM_STEDR: return run_dp<kSTEDR, true>(w, L, *set, ps, npairs, thr, out, counts, st, prm.delta);
+    case TSM_STLCSS: return run_dp<kSTLCSS, true>(w, L, *set, ps, npairs, thr, out, counts, st, prm.delta);
     }
     return fail(TSM_E_ARG, "unknown measure");
 }
@@ -201,6 +205,7 @@ void tsm_params_default(tsm_params *p) {
     p->gy = 0.0;
     p->lanes = 0;
     p->precision = TSM_PREC_FP32;
+    p->delta = 0.0;
 }
 
 int tsm_init(int device) {
@@ -255,9 +260,9 @@ int tsm_pairs_triu(int measure, const tsm_trajset *set, int64_t n, const tsm_par
     return run(measure, set, ps, n < 2 ? 0 : n * (n - 1) / 2, prm, out, counts, stream);
 }
 
-int tsm_pairs_host(int measure, const double *xy64, const int64_t *offsets, int64_t n, int64_t npts,
-                   const int32_t *pa, const int32_t *pb, int64_t npairs, const tsm_params *prm, double *out,
-                   int32_t *counts) {
+int tsm_pairs_host(int measure, const double *xy64, const double *t64, const int64_t *offsets, int64_t n,
+                   int64_t npts, const int32_t *pa, const int32_t *pb, int64_t npairs, const tsm_params *prm,
+                   double *out, int32_t *counts) {
     if (!xy64 || !offsets || n < 1 || npts < 1) return fail(TSM_E_ARG, "empty trajectory set");
     if (npairs < 0) return fail(TSM_E_ARG, "negative pair count");
     if (npairs == 0) return TSM_OK;
@@ -279,10 +284,11 @@ int tsm_pairs_host(int measure, const double *xy64, const int64_t *offsets, int6
         if (rc) return rc;
         if ((rc = c.h_xy64.ensure(npts * 16)) || (rc = c.h_xy32.ensure(npts * 8)) || (rc = c.h_offs.ensure((n + 1) * 8)) ||
             (rc = c.h_pa.ensure(npairs * 4)) || (rc = c.h_pb.ensure(npairs * 4)) || (rc = c.h_out.ensure(npairs * 8)) ||
-            (rc = c.h_cnt.ensure(npairs * 4)))
+            (rc = c.h_cnt.ensure(npairs * 4)) || (t64 && (rc = c.h_t64.ensure(npts * 8))))
             return rc;
     }
     TSM_HIP(hipMemcpy(c.h_xy64.p, xy64, npts * 16, hipMemcpyHostToDevice));
+    if (t64) TSM_HIP(hipMemcpy(c.h_t64.p, t64, npts * 8, hipMemcpyHostToDevice));
     TSM_HIP(hipMemcpy(c.h_offs.p, offsets, (n + 1) * 8, hipMemcpyHostToDevice));
     TSM_HIP(hipMemcpy(c.h_pa.p, pa, npairs * 4, hipMemcpyHostToDevice));
     TSM_HIP(hipMemcpy(c.h_pb.p, pb, npairs * 4, hipMemcpyHostToDevice));
@@ -291,7 +297,8 @@ int tsm_pairs_host(int measure, const double *xy64, const int64_t *offsets, int6
     double cmax = 0.0;
     for (int64_t i = 0; i < 2 * npts; ++i) cmax = std::fmax(cmax, std::fabs(xy64[i]));
     if (!std::isfinite(cmax)) cmax = 0.0;  // unknown bound: EDR/LCSS keep the FP64 match test
-    tsm_trajset set{c.h_xy32.as<float>(), c.h_xy64.as<double>(), c.h_offs.as<int64_t>(), n, npts, max_len, cmax};
+    tsm_trajset set{c.h_xy32.as<float>(), c.h_xy64.as<double>(), c.h_offs.as<int64_t>(), n, npts, max_len, cmax,
+                    t64 ? c.h_t64.as<double>() : nullptr};
     rc = tsm_pairs(measure, &set, c.h_pa.as<int32_t>(), c.h_pb.as<int32_t>(), npairs, prm, c.h_out.as<double>(),
                    counts ? c.h_cnt.as<int32_t>() : nullptr, nullptr);
     if (rc) return rc;

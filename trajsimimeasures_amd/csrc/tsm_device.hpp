@@ -10,7 +10,7 @@ namespace tsm {
 constexpr int kWave = 64;
 constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;
 
-enum Measure : int { kDTW = 0, kEDR = 1, kLCSS = 2, kERP = 3, kFrechet = 4, kHausdorff = 5 };
+enum Measure : int { kDTW = 0, kEDR = 1, kLCSS = 2, kERP = 3, kFrechet = 4, kHausdorff = 5, kSTEDR = 6, kSTLCSS = 7 };
 
 // Pair addressing modes: explicit index arrays, a Q x D grid, or the upper
 // triangle of an all-pairs matrix.

@@ -80,6 +80,8 @@ struct DPArgs {
     int64_t bnd_stride;
     double thr;              // EDR/LCSS squared-distance threshold
     float thr_lo, thr_hi;    // EDR/LCSS FP32 guard band: below lo a match, at or above hi none
+    const double *t64;       // STEDR/STLCSS: per-point timestamps
+    double delta;            // STEDR/STLCSS: time threshold
     double *out;
     int32_t *counts;
 };
@@ -91,6 +93,8 @@ template <int M, bool F64> struct Rec;
 template <bool F64> struct RecDTW {
     using CT = typename std::conditional<F64, double, float>::type;
     using AT = CT;
+    static constexpr int kWMax = WMaxOf<CT>::value;
+    static constexpr bool kTime = false;
     static constexpr bool kBand = false;
     static constexpr bool kGap = false;
     __device__ static AT top(AT) { return (AT)__builtin_inf(); }
@@ -106,6 +110,8 @@ template <bool F64> struct Rec<kDTW, F64> : RecDTW<F64> {};
 template <bool F64> struct RecFrechet {
     using CT = typename std::conditional<F64, double, float>::type;
     using AT = CT;
+    static constexpr int kWMax = WMaxOf<CT>::value;
+    static constexpr bool kTime = false;
     static constexpr bool kBand = false;
     static constexpr bool kGap = false;
     __device__ static AT top(AT) { return (AT)__builtin_inf(); }
@@ -123,6 +129,8 @@ template <bool F64> struct Rec<kFrechet, F64> : RecFrechet<F64> {};
 template <bool F64> struct RecERP {
     using CT = typename std::conditional<F64, double, float>::type;
     using AT = CT;
+    static constexpr int kWMax = WMaxOf<CT>::value;
+    static constexpr bool kTime = false;
     static constexpr bool kBand = false;
     static constexpr bool kGap = true;
     __device__ static AT top(AT sb) { return sb; }
@@ -147,6 +155,7 @@ template <bool F64> struct Rec<kERP, F64> : RecERP<F64> {};
 template <int M> struct RecCount {
     using AT = float;
     static constexpr bool kGap = false;
+    static constexpr bool kEdit = M == kEDR || M == kSTEDR;  // EDR form, else LCSS form
     __device__ static AT top(AT) { return 0.0f; }
     __device__ static AT left0(AT) { return 0.0f; }
     // EDR: min(diag + sub, up + 1, left + 1) with sub in {0, 1} equals
@@ -154,18 +163,20 @@ template <int M> struct RecCount {
     // distances satisfy diag <= min(up, left) + 1.  LCSS: match ? diag + 1 :
     // max(up, left).
     __device__ static AT pick(bool match, AT diag, AT up, AT left) {
-        if constexpr (M == kEDR) return match ? diag : min3f(diag, up, left) + 1.0f;
+        if constexpr (kEdit) return match ? diag : min3f(diag, up, left) + 1.0f;
         else return match ? diag + 1.0f : fmaxf(up, left);
     }
     __device__ static double finish(AT v, int64_t na, int64_t nb, int32_t *cnt) {
         int32_t c = (int32_t)v;
         if (cnt) *cnt = c;
-        if constexpr (M == kEDR) return (double)c / (double)(na > nb ? na : nb);
+        if constexpr (kEdit) return (double)c / (double)(na > nb ? na : nb);
         else return 1.0 - (double)c / (double)(na < nb ? na : nb);
     }
 };
 template <int M> struct RecCount64 : RecCount<M> {
     using CT = double;
+    static constexpr int kWMax = WMaxOf<CT>::value;
+    static constexpr bool kTime = false;
     static constexpr bool kBand = false;
     __device__ static float cell(CT ax, CT ay, float, CT bx, CT by, float, float diag, float up, float left,
                                  double thr) {
@@ -174,8 +185,26 @@ template <int M> struct RecCount64 : RecCount<M> {
 };
 template <int M> struct RecCount32 : RecCount<M> {
     using CT = float;
+    static constexpr int kWMax = WMaxOf<CT>::value;
+    static constexpr bool kTime = false;
     static constexpr bool kBand = true;
 };
+// STEDR / STLCSS (core/stedr.py:44-46, core/stlcss.py:41-42): a match also
+// needs |t1 - t2| < delta, tested in FP64 like the spatial part.  Columns
+// carry a third FP64 value, so at most 16 per lane keep the registers.
+template <int M> struct RecST : RecCount<M> {
+    using CT = double;
+    static constexpr int kWMax = 16;
+    static constexpr bool kTime = true;
+    static constexpr bool kBand = false;
+    __device__ static float cell_t(CT ax, CT ay, CT at, CT bx, CT by, CT bt, float diag, float up, float left,
+                                   double thr, double delta) {
+        const bool m = (dist2_f64(ax, ay, bx, by) < thr) && (fabs(at - bt) < delta);
+        return RecCount<M>::pick(m, diag, up, left);
+    }
+};
+template <> struct Rec<kSTEDR, true> : RecST<kSTEDR> {};
+template <> struct Rec<kSTLCSS, true> : RecST<kSTLCSS> {};
 template <> struct Rec<kEDR, true> : RecCount64<kEDR> {};
 template <> struct Rec<kLCSS, true> : RecCount64<kLCSS> {};
 template <> struct Rec<kEDR, false> : RecCount32<kEDR> {};
@@ -213,6 +242,7 @@ __device__ __forceinline__ double shl_in(double v) { return shr1(v); }
 template <typename CT, typename AT> struct RowPt {
     CT x, y;
     AT g;
+    CT t;  // timestamp (STEDR/STLCSS)
 };
 
 // One step of the sweep: row r = s - j of this lane, reading the previous
@@ -226,6 +256,7 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
                                         const typename Rec<M, F64>::CT (&bx)[W],
                                         const typename Rec<M, F64>::CT (&by)[W],
                                         const typename Rec<M, F64>::AT (&gb)[W],
+                                        const typename Rec<M, F64>::CT (&bt)[W],
                                         const typename Rec<M, F64>::AT (&src)[W], typename Rec<M, F64>::AT (&dst)[W],
                                         const RowPt<typename Rec<M, F64>::CT, typename Rec<M, F64>::AT> &cur,
                                         RowPt<typename Rec<M, F64>::CT, typename Rec<M, F64>::AT> &nxt,
@@ -240,6 +271,7 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
     const uint32_t rn = min((uint32_t)(s + 1 - j), (uint32_t)rlast);
     Pt<CT>::load(a, orow + rn, nxt.x, nxt.y);
     if constexpr (R::kGap) nxt.g = gap_at<AT>(a, orow + rn);
+    if constexpr (R::kTime) nxt.t = a.t64[orow + rn];
     if constexpr (kBnd) {
         if (lane0) nxt_bl = bnd[rn];
     }
@@ -297,7 +329,11 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
         } else {
 #pragma unroll
             for (int c = 0; c < W; ++c) {
-                const AT v = R::cell(cur.x, cur.y, cur.g, bx[c], by[c], gb[c], diag, src[c], lf, a.thr);
+                AT v;
+                if constexpr (R::kTime)
+                    v = R::cell_t(cur.x, cur.y, cur.t, bx[c], by[c], bt[c], diag, src[c], lf, a.thr, a.delta);
+                else
+                    v = R::cell(cur.x, cur.y, cur.g, bx[c], by[c], gb[c], diag, src[c], lf, a.thr);
                 diag = src[c];
                 dst[c] = v;
                 lf = v;
@@ -353,7 +389,7 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
     for (int strip = 0; strip < maxstrips; ++strip) {
         const bool in_strip = strip < nstrips;
         const int col0 = strip * kSpan + j * W;  // first column (0-based) of this lane
-        CT bx[W], by[W];
+        CT bx[W], by[W], bt[W];
         AT gb[W], A[W], B[W];
 #pragma unroll
         for (int c = 0; c < W; ++c) {
@@ -362,6 +398,8 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
             Pt<CT>::load(a, ocol + col, bx[c], by[c]);
             if constexpr (R::kGap) gb[c] = gap_at<AT>(a, ocol + col);
             else gb[c] = 0;
+            if constexpr (R::kTime) bt[c] = a.t64[ocol + col];
+            else bt[c] = 0;
             A[c] = topv;
             B[c] = topv;
         }
@@ -374,24 +412,27 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
         Pt<CT>::load(a, orow + r0, p0.x, p0.y);
         p0.g = 0;
         p1.g = 0;
+        p0.t = 0;
+        p1.t = 0;
         if constexpr (R::kGap) p0.g = gap_at<AT>(a, orow + r0);
+        if constexpr (R::kTime) p0.t = a.t64[orow + r0];
         AT bl0 = 0, bl1 = 0;
         if (strip == 0) {
             for (int s = 0; s < nsteps; s += 2) {
-                dp_step<M, F64, W, false>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb, A,
+                dp_step<M, F64, W, false>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb, bt, A,
                                           B, p0, p1, lpA, lpB, bl0, bl1);
                 if (s + 1 < nsteps)
                     dp_step<M, F64, W, false>(a, s + 1, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by,
-                                              gb, B, A, p1, p0, lpB, lpA, bl1, bl0);
+                                              gb, bt, B, A, p1, p0, lpB, lpA, bl1, bl0);
             }
         } else {
             if (lane0) bl0 = bnd[r0];
             for (int s = 0; s < nsteps; s += 2) {
-                dp_step<M, F64, W, true>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb, A,
+                dp_step<M, F64, W, true>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb, bt, A,
                                          B, p0, p1, lpA, lpB, bl0, bl1);
                 if (s + 1 < nsteps)
                     dp_step<M, F64, W, true>(a, s + 1, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by,
-                                             gb, B, A, p1, p0, lpB, lpA, bl1, bl0);
+                                             gb, bt, B, A, p1, p0, lpB, lpA, bl1, bl0);
             }
         }
 
@@ -408,7 +449,7 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
                 int32_t cnt = 0;
                 double res = R::finish(v, na, nb, &cnt);
                 a.out[slot] = res;
-                if ((M == kEDR || M == kLCSS) && a.counts) a.counts[slot] = cnt;
+                if ((M == kEDR || M == kLCSS || M == kSTEDR || M == kSTLCSS) && a.counts) a.counts[slot] = cnt;
             }
         }
         if (maxstrips > 1) __threadfence();
@@ -428,7 +469,7 @@ __device__ __forceinline__ void dispatch_bucket(const DPArgs &a, int bk, int64_t
 
 // Persistent kernel: each wavefront pulls tasks until the queue is drained.
 template <int M, bool F64> __device__ __forceinline__ void dp_body(const DPArgs &a) {
-    constexpr int WM = WMaxOf<typename Rec<M, F64>::CT>::value;
+    constexpr int WM = Rec<M, F64>::kWMax;
     const int64_t ntasks = a.meta[0];
     const int64_t wave_id = blockIdx.x;
     const int lane = threadIdx.x;

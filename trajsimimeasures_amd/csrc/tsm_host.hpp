@@ -74,7 +74,7 @@ struct Context {
     int num_cu = 0;
     Workspace ws;
     // staging for tsm_pairs_host
-    DevBuf h_xy64, h_xy32, h_offs, h_pa, h_pb, h_out, h_cnt;
+    DevBuf h_xy64, h_xy32, h_offs, h_pa, h_pb, h_out, h_cnt, h_t64;
     // autotuned lanes per column count (index 0: FP32 coordinates, 1: FP64)
     uint8_t tuned_lanes[2][258] = {};
     bool tuned[2] = {false, false};
@@ -197,6 +197,6 @@ struct BucketTable;
 // Defined in tsm_dp_<measure>.hip.
 template <int M, bool F64>
 int run_dp(Workspace &w, int lanes_hint, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr,
-           double *out, int32_t *counts, hipStream_t st);
+           double *out, int32_t *counts, hipStream_t st, double delta = 0.0);
 
 }  // namespace tsm

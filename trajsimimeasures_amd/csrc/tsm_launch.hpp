@@ -39,10 +39,10 @@ inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint) {
 
 template <int M, bool F64>
 int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr,
-           double *out, int32_t *counts, hipStream_t st) {
+           double *out, int32_t *counts, hipStream_t st, double delta) {
     using CT = typename Rec<M, F64>::CT;
     using AT = typename Rec<M, F64>::AT;
-    constexpr int WM = WMaxOf<CT>::value;
+    constexpr int WM = Rec<M, F64>::kWMax;
     constexpr int NB = 2 * WM;
     static_assert(NB <= kMaxBuckets, "bucket count");
     const BucketTable tab = bucket_table(WM, sizeof(CT) * 8, lanes_hint);
@@ -96,6 +96,8 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     a.bnd = c.bnd.p;
     a.bnd_stride = bnd_stride;
     a.thr = thr;
+    a.t64 = set.t64;
+    a.delta = delta;
     a.thr_lo = 0.0f;
     a.thr_hi = 0.0f;
     if constexpr (Rec<M, F64>::kBand) {

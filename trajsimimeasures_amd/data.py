@@ -25,6 +25,7 @@ import numpy as np
 
 __all__ = [
     "TrajSet",
+    "with_timestamps",
     "pack_padded",
     "pack_list",
     "random_walks",
@@ -36,10 +37,13 @@ __all__ = [
 
 @dataclass
 class TrajSet:
-    """Packed trajectories: ``xy`` float64 [N, 2], ``offsets`` int64 [n + 1]."""
+    """Packed trajectories: ``xy`` float64 [N, 2], ``offsets`` int64 [n + 1],
+    and optionally per-point timestamps ``t`` float64 [N] (the third column
+    of the reference's spatio-temporal [batch, seq, 3] inputs)."""
 
     xy: np.ndarray
     offsets: np.ndarray
+    t: np.ndarray | None = None
 
     def __post_init__(self):
         self.xy = np.ascontiguousarray(self.xy, dtype=np.float64).reshape(-1, 2)
@@ -50,6 +54,10 @@ class TrajSet:
             raise ValueError("offsets must start at 0 and end at the number of points")
         if np.any(np.diff(self.offsets) < 0):
             raise ValueError("offsets must be non-decreasing")
+        if self.t is not None:
+            self.t = np.ascontiguousarray(self.t, dtype=np.float64).reshape(-1)
+            if self.t.size != self.xy.shape[0]:
+                raise ValueError("t must hold one timestamp per point")
 
     @property
     def n(self) -> int:
@@ -68,11 +76,12 @@ class TrajSet:
         offs = np.zeros(idx.size + 1, dtype=np.int64)
         np.cumsum(lens, out=offs[1:])
         if idx.size == 0:
-            return TrajSet(np.zeros((0, 2)), offs)
+            return TrajSet(np.zeros((0, 2)), offs, None if self.t is None else np.zeros(0))
         # gather point indices of every selected trajectory
         starts = self.offsets[idx]
         rel = np.arange(offs[-1], dtype=np.int64) - np.repeat(offs[:-1], lens)
-        return TrajSet(self.xy[np.repeat(starts, lens) + rel], offs)
+        pts = np.repeat(starts, lens) + rel
+        return TrajSet(self.xy[pts], offs, None if self.t is None else self.t[pts])
 
     @staticmethod
     def concat(sets) -> "TrajSet":
@@ -81,15 +90,19 @@ class TrajSet:
         lens = np.concatenate([s.lengths for s in sets]) if sets else np.zeros(0, np.int64)
         offs = np.zeros(lens.size + 1, dtype=np.int64)
         np.cumsum(lens, out=offs[1:])
-        return TrajSet(xy, offs)
+        t = None
+        if sets and all(s.t is not None for s in sets):
+            t = np.concatenate([s.t for s in sets])
+        return TrajSet(xy, offs, t)
 
 
 def pack_padded(t: np.ndarray, lens) -> TrajSet:
     """Pack a zero-padded batch [B, L, 2] with lengths [B] (reference layout,
-    core/dtw.py:58-59) into a TrajSet; padding rows are dropped."""
+    core/dtw.py:58-59) into a TrajSet; padding rows are dropped.  A
+    [B, L, 3] batch (x, y, timestamp; core/stedr.py:62-64) also fills t."""
     t = np.asarray(t)
-    if t.ndim != 3 or t.shape[2] != 2:
-        raise ValueError(f"expected a [B, L, 2] array, got shape {t.shape}")
+    if t.ndim != 3 or t.shape[2] not in (2, 3):
+        raise ValueError(f"expected a [B, L, 2] or [B, L, 3] array, got shape {t.shape}")
     lens = np.asarray(lens, dtype=np.int64).reshape(-1)
     if lens.size != t.shape[0]:
         raise ValueError(f"{lens.size} lengths for a batch of {t.shape[0]} trajectories")
@@ -98,17 +111,21 @@ def pack_padded(t: np.ndarray, lens) -> TrajSet:
     mask = np.arange(t.shape[1])[None, :] < lens[:, None]
     offs = np.zeros(lens.size + 1, dtype=np.int64)
     np.cumsum(lens, out=offs[1:])
-    return TrajSet(np.asarray(t, dtype=np.float64)[mask], offs)
+    pts = np.asarray(t, dtype=np.float64)[mask]
+    return TrajSet(pts[:, :2], offs, pts[:, 2] if t.shape[2] == 3 else None)
 
 
 def pack_list(trajs) -> TrajSet:
-    """Pack a list of [n_i, 2] arrays / lists."""
-    arrs = [np.asarray(a, dtype=np.float64).reshape(-1, 2) for a in trajs]
+    """Pack a list of [n_i, 2] (or, all of them, [n_i, 3] with timestamps)
+    arrays / lists."""
+    arrs = [np.asarray(a, dtype=np.float64) for a in trajs]
+    width = 3 if arrs and all(a.ndim == 2 and a.shape[1] == 3 for a in arrs) else 2
+    arrs = [a.reshape(-1, width) for a in arrs]
     lens = np.array([a.shape[0] for a in arrs], dtype=np.int64)
     offs = np.zeros(lens.size + 1, dtype=np.int64)
     np.cumsum(lens, out=offs[1:])
-    xy = np.concatenate(arrs, axis=0) if arrs else np.zeros((0, 2))
-    return TrajSet(xy, offs)
+    pts = np.concatenate(arrs, axis=0) if arrs else np.zeros((0, width))
+    return TrajSet(pts[:, :2], offs, pts[:, 2] if width == 3 else None)
 
 
 def lengths_uniform(rng: np.random.Generator, n: int, lo: int, hi: int) -> np.ndarray:
@@ -182,3 +199,20 @@ def random_walks(rng: np.random.Generator, lengths) -> TrajSet:
         xy[idx, 0] = px[:m]
         xy[idx, 1] = py[:m]
     return TrajSet(xy, offs)
+
+
+def with_timestamps(rng: np.random.Generator, ts: TrajSet, interval: float = 15.0, jitter: float = 5.0,
+                    span: float = 3600.0) -> TrajSet:
+    """The same trajectories with per-point timestamps in seconds: a start
+    uniform on [0, span), then sampling intervals uniform on
+    interval +- jitter (a GPS log's shape), drawn in point order."""
+    n = ts.n
+    start = rng.random(n) * span
+    gaps = interval + jitter * (2.0 * rng.random(ts.xy.shape[0]) - 1.0)
+    lens = ts.lengths
+    first = ts.offsets[:-1]
+    gaps[first[lens > 0]] = 0.0
+    csum = np.cumsum(gaps)
+    base = np.repeat(csum[first] if n else np.zeros(0), lens)
+    t = np.repeat(start, lens) + (csum - base)
+    return TrajSet(ts.xy, ts.offsets, t)

@@ -25,14 +25,16 @@ from . import _lib, core
 from .data import TrajSet, pack_list
 
 __all__ = ["heuristic_fn_wrapper", "heuristic_colllate_fn", "is_heuristic", "pairwise", "query_db", "all_pairs",
-           "symmetric_from_triu", "fp64_drift", "DEFAULT_EPS"]
+           "symmetric_from_triu", "fp64_drift", "DEFAULT_EPS", "DEFAULT_DELTA"]
 
 DEFAULT_EPS = 0.0025  # EDR/LCSS epsilon of the 100k-pair workload
+DEFAULT_DELTA = 60.0  # STEDR/STLCSS time threshold, seconds (config.py:53)
 
 _GPU_FNS = {"dtw": core.dtw, "edr": core.edr, "erp": core.erp, "frechet": core.dfrechet,
-            "hausdorff": core.hausdorff, "lcss": core.lcss}
+            "hausdorff": core.hausdorff, "lcss": core.lcss, "stedr": core.stedr, "stlcss": core.stlcss}
 _MEASURE = {"dtw": "dtw", "edr": "edr", "erp": "erp", "frechet": "frechet", "hausdorff": "hausdorff",
-            "lcss": "lcss"}
+            "lcss": "lcss", "stedr": "stedr", "stlcss": "stlcss"}
+_ST = ("stedr", "stlcss")
 
 
 def is_heuristic(fn_name: str) -> bool:
@@ -41,16 +43,19 @@ def is_heuristic(fn_name: str) -> bool:
     return any(h in fn_name for h in heurs)
 
 
-def heuristic_fn_wrapper(fn_name: str, eps: float = DEFAULT_EPS, threads_per_traj: int = 64):
+def heuristic_fn_wrapper(fn_name: str, eps: float = DEFAULT_EPS, threads_per_traj: int = 64,
+                         delta: float = DEFAULT_DELTA):
     """Name -> batch function with the reference's bound arguments
-    (method_helper.py:32-73): eps for edr/lcss, threads_per_traj for all.
-    The ``*_cpu`` names (traj_dist) and the spatio-temporal / symbolic
-    measures are not part of this engine."""
+    (method_helper.py:32-73): eps for edr/lcss, eps and delta for
+    stedr/stlcss, threads_per_traj for all.  The ``*_cpu`` names (traj_dist,
+    core_cpu) and CDDS / SAR are not part of this engine."""
     fn = _GPU_FNS.get(fn_name)
     if fn is None:
         raise NotImplementedError(f"{fn_name!r} is not served by this engine (GPU measures: {sorted(_GPU_FNS)})")
     if fn_name in ("edr", "lcss"):
         fn = partial(fn, eps=eps)
+    elif fn_name in _ST:
+        fn = partial(fn, eps=eps, delta=delta)
     return partial(fn, threads_per_traj=threads_per_traj)
 
 
@@ -70,11 +75,11 @@ def heuristic_colllate_fn(batch, is_pad: bool = True, quantize_fp32: bool = Fals
     return np.copy(padded[:len(len1)]), len1, np.copy(padded[len(len1):]), len2
 
 
-def _opts(method: str, eps, gpoint, lanes, precision):
+def _opts(method: str, eps, gpoint, lanes, precision, delta=DEFAULT_DELTA):
     if method not in _MEASURE:
         raise NotImplementedError(f"{method!r} is not served by this engine")
-    return dict(eps=eps if method in ("edr", "lcss") else 0.0, gpoint=gpoint if method == "erp" else None,
-                lanes=lanes, precision=precision)
+    return dict(eps=eps if method in ("edr", "lcss") + _ST else 0.0, gpoint=gpoint if method == "erp" else None,
+                lanes=lanes, precision=precision, delta=delta if method in _ST else 0.0)
 
 
 def _device_set(ts: TrajSet):
@@ -86,7 +91,8 @@ def _device_set(ts: TrajSet):
     return DeviceTrajSet(ts)
 
 
-def pairwise(method: str, pairs, *, eps: float = DEFAULT_EPS, gpoint=None, lanes: int = 0, precision="fp32"):
+def pairwise(method: str, pairs, *, eps: float = DEFAULT_EPS, gpoint=None, lanes: int = 0, precision="fp32",
+              delta: float = DEFAULT_DELTA):
     """Values of ``method`` for a list of (traj, traj2) pairs
     (test_trajsimi_time.py:222-276).  Returns (list of floats,
     [running, collate, compute] seconds)."""
@@ -104,14 +110,15 @@ def pairwise(method: str, pairs, *, eps: float = DEFAULT_EPS, gpoint=None, lanes
     collate = time.time() - t0
     t1 = time.time()
     pa = np.arange(n, dtype=np.int32)
-    out = engine.pairs(method, dset, pa, pa + n, validate=False, **_opts(method, eps, gpoint, lanes, precision))
+    out = engine.pairs(method, dset, pa, pa + n, validate=False, **_opts(method, eps, gpoint, lanes, precision, delta))
     torch.cuda.synchronize()
     dists = out.cpu().numpy().tolist()
     comp = time.time() - t1
     return dists, [time.time() - t0, collate, comp]
 
 
-def query_db(method: str, Q, D, *, eps: float = DEFAULT_EPS, gpoint=None, lanes: int = 0, precision="fp32"):
+def query_db(method: str, Q, D, *, eps: float = DEFAULT_EPS, gpoint=None, lanes: int = 0, precision="fp32",
+              delta: float = DEFAULT_DELTA):
     """[|Q|, |D|] matrix of ``method`` between every query and every database
     trajectory (test_trajsimi_time_DQ.py:136-181, whose flattened order is
     query-major).  Returns (float64 ndarray, [running, collate, compute])."""
@@ -124,7 +131,7 @@ def query_db(method: str, Q, D, *, eps: float = DEFAULT_EPS, gpoint=None, lanes:
     dset = _device_set(ts)
     collate = time.time() - t0
     t1 = time.time()
-    out = engine.grid(method, dset, 0, len(Q), len(Q), len(D), **_opts(method, eps, gpoint, lanes, precision))
+    out = engine.grid(method, dset, 0, len(Q), len(Q), len(D), **_opts(method, eps, gpoint, lanes, precision, delta))
     torch.cuda.synchronize()
     mat = out.cpu().numpy()
     comp = time.time() - t1
@@ -143,7 +150,8 @@ def symmetric_from_triu(vals: np.ndarray, n: int) -> np.ndarray:
     return m + m.T
 
 
-def all_pairs(method: str, trajs, *, eps: float = DEFAULT_EPS, gpoint=None, lanes: int = 0, precision="fp32"):
+def all_pairs(method: str, trajs, *, eps: float = DEFAULT_EPS, gpoint=None, lanes: int = 0, precision="fp32",
+              delta: float = DEFAULT_DELTA):
     """Symmetric all-pairs matrix over ``trajs`` computed on the upper
     triangle only (test_clustering_time.py:208-269).  Returns (float64
     [n, n], [running, collate, compute])."""
@@ -156,7 +164,7 @@ def all_pairs(method: str, trajs, *, eps: float = DEFAULT_EPS, gpoint=None, lane
     dset = _device_set(ts)
     collate = time.time() - t0
     t1 = time.time()
-    tri = engine.triu(method, dset, n, **_opts(method, eps, gpoint, lanes, precision))
+    tri = engine.triu(method, dset, n, **_opts(method, eps, gpoint, lanes, precision, delta))
     torch.cuda.synchronize()
     vals = tri.cpu().numpy()
     comp = time.time() - t1

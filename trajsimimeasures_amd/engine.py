@@ -46,22 +46,29 @@ class DeviceTrajSet:
         self.coord_max = float(np.abs(ts.xy).max())
         self.xy32 = torch.empty((self.npts, 2), dtype=torch.float32, device=device)
         _lib.check(_lib.lib().tsm_xy_to_fp32(self.xy64.data_ptr(), self.xy32.data_ptr(), self.npts, _stream_ptr()))
+        # per-point timestamps (STEDR / STLCSS), if the set has them
+        self.t64 = None
+        if ts.t is not None:
+            if not np.all(np.isfinite(ts.t)):
+                raise ValueError("timestamps must be finite")
+            self.t64 = torch.as_tensor(ts.t, dtype=torch.float64, device=device).contiguous()
 
     def cstruct(self) -> _lib.TrajSetC:
         return _lib.TrajSetC(self.xy32.data_ptr(), self.xy64.data_ptr(), self.offsets.data_ptr(), self.n,
-                             self.npts, self.max_len, self.coord_max)
+                             self.npts, self.max_len, self.coord_max,
+                             self.t64.data_ptr() if self.t64 is not None else None)
 
 
 def _outs(n: int, measure: int, device, want_counts: bool):
     out = torch.empty(n, dtype=torch.float64, device=device)
     cnt = None
-    if want_counts and measure in (_lib.EDR, _lib.LCSS):
+    if want_counts and measure in _lib.COUNTED:
         cnt = torch.empty(n, dtype=torch.int32, device=device)
     return out, cnt
 
 
 def pairs(measure, dset: DeviceTrajSet, pa, pb, *, eps=0.0, eps_mode="sqrt", gpoint=None, lanes=0,
-          precision="fp32", return_counts=False, validate=True, out=None):
+          precision="fp32", return_counts=False, validate=True, out=None, delta=0.0):
     """Measure over explicit pairs (pa[k], pb[k]) of ``dset`` (pa as t1).
 
     validate=False skips the (synchronising) index range check; ``out`` may
@@ -82,9 +89,9 @@ def pairs(measure, dset: DeviceTrajSet, pa, pb, *, eps=0.0, eps_mode="sqrt", gpo
         if out.dtype != torch.float64 or out.numel() != n or not out.is_contiguous() or out.device != dev:
             raise ValueError("out must be a contiguous float64 tensor of len(pa) on the set's device")
         cnt = None
-        if return_counts and m in (_lib.EDR, _lib.LCSS):
+        if return_counts and m in _lib.COUNTED:
             cnt = torch.empty(n, dtype=torch.int32, device=dev)
-    prm = _lib.params(eps, eps_mode, gpoint, lanes, precision)
+    prm = _lib.params(eps, eps_mode, gpoint, lanes, precision, delta)
     s = dset.cstruct()
     _lib.check(_lib.lib().tsm_pairs(m, ctypes.byref(s), pa.data_ptr(), pb.data_ptr(), n, ctypes.byref(prm),
                                     out.data_ptr(), cnt.data_ptr() if cnt is not None else None, _stream_ptr()))
@@ -92,12 +99,12 @@ def pairs(measure, dset: DeviceTrajSet, pa, pb, *, eps=0.0, eps_mode="sqrt", gpo
 
 
 def grid(measure, dset: DeviceTrajSet, q0: int, nq: int, d0: int, nd: int, *, eps=0.0, eps_mode="sqrt",
-         gpoint=None, lanes=0, precision="fp32", return_counts=False):
+         gpoint=None, lanes=0, precision="fp32", return_counts=False, delta=0.0):
     """Measure over the grid (q0 + q, d0 + d); returns [nq, nd] (row-major)."""
     m = _lib.measure_id(measure)
     dev = dset.xy64.device
     out, cnt = _outs(nq * nd, m, dev, return_counts)
-    prm = _lib.params(eps, eps_mode, gpoint, lanes, precision)
+    prm = _lib.params(eps, eps_mode, gpoint, lanes, precision, delta)
     s = dset.cstruct()
     _lib.check(_lib.lib().tsm_pairs_grid(m, ctypes.byref(s), q0, nq, d0, nd, ctypes.byref(prm), out.data_ptr(),
                                          cnt.data_ptr() if cnt is not None else None, _stream_ptr()))
@@ -108,7 +115,7 @@ def grid(measure, dset: DeviceTrajSet, q0: int, nq: int, d0: int, nd: int, *, ep
 
 
 def triu(measure, dset: DeviceTrajSet, n: int | None = None, *, eps=0.0, eps_mode="sqrt", gpoint=None,
-         lanes=0, precision="fp32", return_counts=False):
+         lanes=0, precision="fp32", return_counts=False, delta=0.0):
     """Upper triangle over trajectories [0, n): n*(n-1)/2 values in
     itertools.combinations order."""
     m = _lib.measure_id(measure)
@@ -116,7 +123,7 @@ def triu(measure, dset: DeviceTrajSet, n: int | None = None, *, eps=0.0, eps_mod
     dev = dset.xy64.device
     npairs = n * (n - 1) // 2
     out, cnt = _outs(npairs, m, dev, return_counts)
-    prm = _lib.params(eps, eps_mode, gpoint, lanes, precision)
+    prm = _lib.params(eps, eps_mode, gpoint, lanes, precision, delta)
     s = dset.cstruct()
     _lib.check(_lib.lib().tsm_pairs_triu(m, ctypes.byref(s), n, ctypes.byref(prm), out.data_ptr(),
                                          cnt.data_ptr() if cnt is not None else None, _stream_ptr()))
@@ -150,7 +157,7 @@ class PairsGraph:
     given."""
 
     def __init__(self, measure, dset: DeviceTrajSet, pa, pb, *, eps=0.0, eps_mode="sqrt", gpoint=None, lanes=0,
-                 precision="fp32", stream=None):
+                 precision="fp32", stream=None, delta=0.0):
         m = _lib.measure_id(measure)
         dev = dset.xy64.device
         self.dset = dset
@@ -160,9 +167,9 @@ class PairsGraph:
         if self.pb.numel() != n or n < 1:
             raise ValueError("pa and pb must be non-empty and of equal length")
         self.out = torch.empty(n, dtype=torch.float64, device=dev)
-        self.counts = torch.empty(n, dtype=torch.int32, device=dev) if m in (_lib.EDR, _lib.LCSS) else None
+        self.counts = torch.empty(n, dtype=torch.int32, device=dev) if m in _lib.COUNTED else None
         self.stream = stream if stream is not None else torch.cuda.Stream(device=dev)
-        self._prm = _lib.params(eps, eps_mode, gpoint, lanes, precision)
+        self._prm = _lib.params(eps, eps_mode, gpoint, lanes, precision, delta)
         self._set = dset.cstruct()
         self._h = ctypes.c_void_p()
         torch.cuda.current_stream().synchronize()

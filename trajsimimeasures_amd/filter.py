@@ -370,10 +370,14 @@ class FilterIndex:
         return cand
 
 
-def exact_knn(dset: DeviceTrajSet, q0: int, nq: int, d0: int, nd: int, k: int = 10, chunk_q: int = 64):
+def exact_knn(dset: DeviceTrajSet, q0: int, nq: int, d0: int, nd: int, k: int = 10, chunk_q: int | None = None):
     """Exact DTW k-nearest database positions [nq, k] (grid of the DP
-    kernel, then per-row selection), and the elapsed seconds."""
+    kernel, then per-row selection), and the elapsed seconds.  Queries go in
+    chunks of about 2 GB of FP64 distances (at least 256 rows, so the
+    per-row selection fills the GPU)."""
     from .engine import grid
+    if chunk_q is None:
+        chunk_q = max(256, (2 << 30) // (8 * max(nd, 1)))
     out = torch.empty((nq, k), dtype=torch.int32, device="cuda")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
