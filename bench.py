@@ -10,7 +10,10 @@ steps.  The other five measures are timed the same way and reported under
 
 roofline: the DP kernel is bound by VALU FP32 issue.  Algorithmic FLOP per DP
 cell (FMA = 2): DTW 9, Frechet 9, LCSS 9, EDR 10, ERP 11; Hausdorff 24 per
-(point, segment) evaluation.  achieved = FLOP per launch / mean kernel time
+(point, segment) evaluation; STEDR 13 and STLCSS 12 (the EDR / LCSS cell plus
+the time test: subtract, compare, and).  The spatio-temporal measures use
+timestamps 15 +- 5 s apart from a start uniform in [0, 3600) s, delta = 60 s.
+achieved = FLOP per launch / mean kernel time
 (HIP events on the launch stream); peak = 157.3 TFLOP/s (MI355X vector FP32).
 
 cpu_baseline: the FP64 full-matrix restatement of the DP (oracle/), one pair
@@ -42,8 +45,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3
-FLOP_PER_CELL = {"dtw": 9, "frechet": 9, "lcss": 9, "edr": 10, "erp": 11, "hausdorff": 24}
+FLOP_PER_CELL = {"dtw": 9, "frechet": 9, "lcss": 9, "edr": 10, "erp": 11, "hausdorff": 24, "stedr": 13, "stlcss": 12}
 EPS = 0.0025
+DELTA = 60.0
 
 
 def cells_of(measure, la, lb):
@@ -71,7 +75,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--pairs", type=int, default=100_000)
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--measures", default="dtw,edr,lcss,erp,frechet,hausdorff",
+    ap.add_argument("--measures", default="dtw,edr,lcss,erp,frechet,hausdorff,stedr,stlcss",
                     help="measures reported under 'measures' (dtw is always the headline)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work at most")
@@ -90,11 +94,12 @@ def main():
     import torch
 
     from trajsimimeasures_amd import engine
-    from trajsimimeasures_amd.data import lengths_uniform, random_walks
+    from trajsimimeasures_amd.data import lengths_uniform, random_walks, with_timestamps
 
     t0 = time.time()
     rng = np.random.default_rng(0)
     ts = random_walks(rng, lengths_uniform(rng, 2 * args.pairs, 20, 200))
+    ts = with_timestamps(np.random.default_rng(1), ts)  # for STEDR / STLCSS only
     pa = np.arange(0, 2 * args.pairs, 2)
     pb = pa + 1
     gpoint = ts.xy.mean(0)
@@ -116,7 +121,8 @@ def main():
     la, lb = ts.lengths[pa], ts.lengths[pb]
 
     def step(measure):
-        engine.pairs(measure, dset, pa_d, pb_d, eps=EPS, gpoint=gpoint, lanes=args.lanes, validate=False, out=out)
+        engine.pairs(measure, dset, pa_d, pb_d, eps=EPS, gpoint=gpoint, lanes=args.lanes, validate=False, out=out,
+                     delta=DELTA)
 
     def timed(measure, steps, warmup):
         for _ in range(warmup):
