@@ -36,6 +36,11 @@ int tsm_nn_gemm_nt(int M, int N, int K, const uint16_t *A, int lda, const uint16
 int tsm_nn_gemm_nt_filter(int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb, float alpha,
                           const float *bias, const float *tau, int cap, uint32_t *cnt, float *key, int32_t *idx,
                           void *stream);
+/* C[M,N] += alpha * A[M,K] . B[N,K]^T with K split across blocks (float
+ * atomic accumulation, so the summation order varies run to run): the
+ * weight-gradient products, whose outputs are small and K long. */
+int tsm_nn_gemm_nt_acc(int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb, float *C,
+                       int ldc, float alpha, void *stream);
 /* The C smallest candidates of each row (sorted ascending; ties by column):
  * out[rows, C].  flag[row] = 1 when the row had fewer than C or more than cap
  * candidates (then out is not written for it), else 0.  cap <= 4096. */

@@ -34,7 +34,8 @@ EXPORTS = ("tsm_init", "tsm_last_error", "tsm_abi_version", "tsm_params_default"
 NN_EXPORTS = ("tsm_nn_gemm_nt", "tsm_nn_cast_bf16", "tsm_nn_resample", "tsm_nn_conv_fwd", "tsm_nn_conv_bwd",
               "tsm_nn_gru_h0", "tsm_nn_gru_h0_bwd", "tsm_nn_gru_step_fwd", "tsm_nn_gru_step_bwd", "tsm_nn_colsum",
               "tsm_nn_rank_loss", "tsm_nn_adam", "tsm_nn_rownorm_bf16", "tsm_nn_topk_f32", "tsm_nn_topk_f64",
-              "tsm_nn_linear_f32", "tsm_nn_split_bf16", "tsm_nn_gemm_nt_filter", "tsm_nn_select_candidates")
+              "tsm_nn_linear_f32", "tsm_nn_split_bf16", "tsm_nn_gemm_nt_filter", "tsm_nn_select_candidates",
+              "tsm_nn_gemm_nt_acc")
 
 
 class TsmError(RuntimeError):

@@ -39,7 +39,12 @@
 namespace tsm {
 
 
-constexpr int kRowKeys = 128;   // row bins per bucket: min(nrow, 511) / 4
+// Row bins per bucket: bin = min(nrow, 511) * kRowKeys / 512.
+#ifndef TSM_ROW_KEYS
+#define TSM_ROW_KEYS 128
+#endif
+constexpr int kRowKeys = TSM_ROW_KEYS;
+static_assert(kRowKeys >= 16 && kRowKeys <= 128 && (kRowKeys & (kRowKeys - 1)) == 0, "row keys: power of two <= 128");
 constexpr int kMaxSeg = 16;     // segments per wave at L = 4
 constexpr int kTabMax = 257;    // bucket table covers column counts 1..256
 

@@ -49,7 +49,7 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     const int64_t nslots = npairs + (int64_t)NB * kMaxSeg;
     int rc;
     if ((rc = c.hist.ensure(kMaxBins * 4)) || (rc = c.cursor.ensure(kMaxBins * 4)) ||
-        (rc = c.slots.ensure(nslots * 4)) || (rc = c.meta.ensure((1 + 2 * NB) * 8)) || (rc = c.counter.ensure(4)))
+        (rc = c.slots.ensure(nslots * 4)) || (rc = c.meta.ensure((1 + 2 * NB) * 8)) || (rc = c.counter.ensure(8)))
         return rc;
 
     void (*kern)(DPArgs);
@@ -71,12 +71,12 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     pa.slots = c.slots.as<uint32_t>();
     pa.meta = c.meta.as<int64_t>();
     pa.counter = c.counter.as<unsigned int>();
+    pa.done = c.counter.as<unsigned int>() + 1;
     pa.nb = NB;
     pa.tab = tab;
 
     const int pblocks = (int)((npairs + kPairsPerBlock - 1) / kPairsPerBlock);
     hipLaunchKernelGGL(plan_hist_kernel, dim3(pblocks), dim3(kPlanBlock), 0, st, pa);
-    hipLaunchKernelGGL(plan_scan_kernel, dim3(1), dim3(kScanThreads), 0, st, pa);
     hipLaunchKernelGGL(plan_scatter_kernel, dim3(pblocks), dim3(kPlanBlock), 0, st, pa);
     TSM_HIP(hipGetLastError());
 

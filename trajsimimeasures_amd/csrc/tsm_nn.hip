@@ -134,8 +134,12 @@ __device__ __forceinline__ void tile_store(uint16_t *s, const uint4 (&r)[4]) {
     }
 }
 
-template <bool kFilter>
-__global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g, FilterArgs f) {
+// kMode: 0 store (alpha A B^T + beta C + bias), 1 filter (see above), 2
+// split-K accumulate: block z covers K range [z * kchunk, (z + 1) * kchunk)
+// and adds alpha * its partial product into C with float atomics.
+template <int kMode>
+__global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g, FilterArgs f, int kchunk) {
+    constexpr bool kFilter = kMode == 1;
     __shared__ __attribute__((aligned(16))) uint16_t As[2][kBM * kLdsStride];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[2][kBN * kLdsStride];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -148,18 +152,21 @@ __global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g, FilterArgs
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
     uint4 ra[4], rb[4];
-    tile_load(g.A, g.lda, bm, g.M, 0, g.K, ra);
-    tile_load(g.B, g.ldb, bn, g.N, 0, g.K, rb);
+    // K range of this block (the whole K unless split)
+    const int kbeg = kMode == 2 ? (int)blockIdx.z * kchunk : 0;
+    const int kend = kMode == 2 ? min(g.K, kbeg + kchunk) : g.K;
+    tile_load(g.A, g.lda, bm, g.M, kbeg, kend, ra);
+    tile_load(g.B, g.ldb, bn, g.N, kbeg, kend, rb);
     tile_store(As[0], ra);
     tile_store(Bs[0], rb);
     __syncthreads();
-    const int nk = (g.K + kBK - 1) / kBK;
+    const int nk = (kend - kbeg + kBK - 1) / kBK;
     for (int kt = 0; kt < nk; ++kt) {
         const int cur = kt & 1;
         const bool more = kt + 1 < nk;
         if (more) {
-            tile_load(g.A, g.lda, bm, g.M, (kt + 1) * kBK, g.K, ra);
-            tile_load(g.B, g.ldb, bn, g.N, (kt + 1) * kBK, g.K, rb);
+            tile_load(g.A, g.lda, bm, g.M, kbeg + (kt + 1) * kBK, kend, ra);
+            tile_load(g.B, g.ldb, bn, g.N, kbeg + (kt + 1) * kBK, kend, rb);
         }
 #pragma unroll
         for (int kk = 0; kk < kBK; kk += 32) {
@@ -205,6 +212,8 @@ __global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g, FilterArgs
                             f.idx[(int64_t)row * f.cap + pos] = col;
                         }
                     }
+                } else if constexpr (kMode == 2) {
+                    atomicAdd(g.C + (int64_t)row * g.ldc + col, g.alpha * acc[i][j][e]);
                 } else {
                     float *c = g.C + (int64_t)row * g.ldc + col;
                     *c = g.beta == 0.f ? v : v + g.beta * *c;
@@ -705,8 +714,8 @@ int tsm_nn_gemm_nt(int M, int N, int K, const uint16_t *A, int lda, const uint16
     GemmArgs g{M, N, K, A, lda, B, ldb, C, ldc, alpha, beta, bias};
     if (M >= kBM && N >= kBN && K >= kBK && (M + kBM - 1) / kBM <= 65535) {
         dim3 grid((N + kBN - 1) / kBN, (M + kBM - 1) / kBM);
-        hipLaunchKernelGGL(gemm_nt_lds_kernel<false>, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), g,
-                           FilterArgs{});
+        hipLaunchKernelGGL(gemm_nt_lds_kernel<0>, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), g,
+                           FilterArgs{}, 0);
     } else {
         if ((M + 63) / 64 > 65535) return fail(TSM_E_ARG, "gemm: M too large");
         dim3 grid((N + 63) / 64, (M + 63) / 64);
@@ -727,8 +736,32 @@ int tsm_nn_gemm_nt_filter(int M, int N, int K, const uint16_t *A, int lda, const
     if ((M + kBM - 1) / kBM > 65535) return fail(TSM_E_ARG, "gemm_filter: M too large");
     GemmArgs g{M, N, K, A, lda, B, ldb, nullptr, 0, alpha, 0.f, bias};
     dim3 grid((N + kBN - 1) / kBN, (M + kBM - 1) / kBM);
-    hipLaunchKernelGGL(gemm_nt_lds_kernel<true>, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), g,
-                       FilterArgs{tau, cap, cnt, key, idx});
+    hipLaunchKernelGGL(gemm_nt_lds_kernel<1>, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), g,
+                       FilterArgs{tau, cap, cnt, key, idx}, 0);
+    TSM_HIP(hipGetLastError());
+    return TSM_OK;
+}
+
+int tsm_nn_gemm_nt_acc(int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb, float *C,
+                       int ldc, float alpha, void *stream) {
+    if (M < 0 || N < 0 || K < 0 || (K % 8) != 0) return fail(TSM_E_ARG, "gemm_acc: K must be a multiple of 8");
+    if (M == 0 || N == 0 || K == 0) return TSM_OK;
+    if (!A || !B || !C || lda < K || ldb < K || ldc < N || (lda % 8) || (ldb % 8) ||
+        (reinterpret_cast<uintptr_t>(A) % 16) || (reinterpret_cast<uintptr_t>(B) % 16))
+        return fail(TSM_E_ARG, "gemm_acc: bad operands (16-byte aligned, leading dimensions multiples of 8)");
+    const int tm = (M + kBM - 1) / kBM, tn = (N + kBN - 1) / kBN;
+    if (tm > 65535) return fail(TSM_E_ARG, "gemm_acc: M too large");
+    // enough K slices for ~512 blocks, each at least 16 K steps long
+    const int ksteps = (K + kBK - 1) / kBK;
+    int splits = 512 / (tm * tn);
+    if (splits > ksteps / 16) splits = ksteps / 16;
+    if (splits < 1) splits = 1;
+    if (splits > 65535) splits = 65535;
+    const int kchunk = ((ksteps + splits - 1) / splits) * kBK;
+    splits = (K + kchunk - 1) / kchunk;
+    GemmArgs g{M, N, K, A, lda, B, ldb, C, ldc, alpha, 1.f, nullptr};
+    hipLaunchKernelGGL(gemm_nt_lds_kernel<2>, dim3(tn, tm, splits), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                       g, FilterArgs{}, kchunk);
     TSM_HIP(hipGetLastError());
     return TSM_OK;
 }

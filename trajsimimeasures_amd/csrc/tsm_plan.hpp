@@ -1,30 +1,27 @@
 // tsm_plan.hpp -- the task planner of the DP kernel: a counting sort of
 // pairs by (bucket, rows), packed 64/L slots to a task.
 //
-//   bin(pair) = bucket * kRowKeys + (kRowKeys - 1 - min(nrow, 511) / 4),
+//   bin(pair) = bucket * kRowKeys + (kRowKeys - 1 - min(nrow, 511) * kRowKeys / 512),
 //   bucket = table[ncol] (L, W) for ncol <= table range, else the strip bucket,
 // with nrow/ncol the longer/shorter length.  Buckets are numbered by W
 // descending, so ascending bins give descending W, then descending rows.
 // Each bucket is padded to whole tasks of 64/L slots so every task has one
 // (L, W); the persistent kernel takes tasks in bin order, largest work first.
 //
-// Three kernels: a histogram (LDS-aggregated per block), a one-block scan
-// that lays the buckets out and re-zeroes the histogram for the next call,
-// and a scatter that reserves each block's range per bin with one global
-// atomic and ranks pairs inside it through LDS.
+// Two kernels: a histogram (LDS-aggregated per block) whose last block to
+// finish lays the buckets out (scan, task counts, cursors) and re-zeroes the
+// histogram for the next call, and a scatter that reserves each block's range
+// per bin with one global atomic and ranks pairs inside it through LDS.
 #pragma once
 
 #include "tsm_dp.hpp"
 
 namespace tsm {
 
-constexpr int kScanThreads = 1024;
 constexpr int kMaxBuckets = 64;
 constexpr int kMaxBins = kMaxBuckets * kRowKeys;                   // 8192
-constexpr int kBinsPerThread = kMaxBins / kScanThreads;           // 8
 constexpr int kPlanBlock = 256;
 constexpr int kPairsPerBlock = 1024;
-static_assert(kRowKeys % kBinsPerThread == 0, "a thread's bins stay in one bucket");
 
 struct PlanArgs {
     PairSpace ps;
@@ -35,20 +32,151 @@ struct PlanArgs {
     uint32_t *slots;
     int64_t *meta;      // see DPArgs::meta
     unsigned int *counter;
+    unsigned int *done;  // histogram blocks finished; zero between calls
     int nb;
     BucketTable tab;
 };
 
-__device__ __forceinline__ int pair_bin(const PlanArgs &a, int64_t p) {
-    int64_t ia, ib;
-    decode_pair(a.ps, p, ia, ib);
-    const int64_t na = a.offs[ia + 1] - a.offs[ia];
-    const int64_t nb = a.offs[ib + 1] - a.offs[ib];
+__device__ __forceinline__ int bin_of(const PlanArgs &a, int64_t na, int64_t nb) {
     const int64_t nrow = na > nb ? na : nb;
     const int64_t ncol = na > nb ? nb : na;
     const int b = ncol <= a.tab.ncmax ? (int)a.tab.b[ncol] : a.tab.wide;
-    const int rk = (int)((nrow < 511 ? nrow : 511) >> 2);
+    const int rk = (int)((nrow < 511 ? nrow : 511) * kRowKeys / 512);
     return b * kRowKeys + (kRowKeys - 1 - rk);
+}
+
+// Bins of this thread's pairs p0 + threadIdx.x + k * kPlanBlock (-1 past
+// p1).  The pair lookups of all kPer pairs are issued before any is waited
+// for, then the offset lookups, so a thread pays two memory round trips
+// rather than two per pair.
+constexpr int kPer = kPairsPerBlock / kPlanBlock;  // pairs per thread
+
+__device__ __forceinline__ void pair_bins(const PlanArgs &a, int64_t p0, int64_t p1, int (&bins)[kPer]) {
+    int64_t ia[kPer], ib[kPer];
+    if (a.ps.mode == kExplicit) {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int64_t p = p0 + threadIdx.x + (int64_t)k * kPlanBlock;
+            const int64_t pc = p < p1 ? p : p0;
+            ia[k] = a.ps.pa[pc];
+            ib[k] = a.ps.pb[pc];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kPer; ++k) {
+            const int64_t p = p0 + threadIdx.x + (int64_t)k * kPlanBlock;
+            decode_pair(a.ps, p < p1 ? p : p0, ia[k], ib[k]);
+        }
+    }
+    int64_t na[kPer], nb[kPer];
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        na[k] = a.offs[ia[k] + 1] - a.offs[ia[k]];
+        nb[k] = a.offs[ib[k] + 1] - a.offs[ib[k]];
+    }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const int64_t p = p0 + threadIdx.x + (int64_t)k * kPlanBlock;
+        bins[k] = p < p1 ? bin_of(a, na[k], nb[k]) : -1;
+    }
+}
+
+// The bucket layout, run by the last histogram block once every block's
+// counts are in (a done-counter; no block waits for another).  Thread t owns
+// the kScanBins bins [kScanBins * (t % 4), + kScanBins) of bucket t / 4.  A
+// block scan of the per-thread sums gives each bin's position within its
+// bucket; buckets are laid out one after another, each padded to whole tasks
+// of 64/L slots whose unused slots are marked empty.  The histogram is
+// re-zeroed and the counters reset for the next call.
+constexpr int kScanTPB = kPlanBlock / kMaxBuckets;  // threads per bucket
+constexpr int kScanBins = kRowKeys / kScanTPB;      // bins per thread
+static_assert(kScanTPB * kMaxBuckets == kPlanBlock && kScanBins * kScanTPB == kRowKeys, "scan layout");
+
+__device__ __forceinline__ void plan_layout_block(const PlanArgs &a) {
+    __shared__ uint32_t wsum[kPlanBlock / kWave];
+    __shared__ uint32_t incl[kPlanBlock];
+    __shared__ int64_t slot_base[kMaxBuckets];
+    __shared__ int64_t slot_fill[kMaxBuckets];  // first padding slot of each bucket
+    __shared__ int64_t slot_end[kMaxBuckets];
+    const int t = threadIdx.x, lane = t & (kWave - 1), wave = t / kWave;
+    const int b = t / kScanTPB;
+    const int bin0 = b * kRowKeys + (t % kScanTPB) * kScanBins;
+    const bool live = b < a.nb;
+    uint32_t v[kScanBins];
+    uint32_t sum = 0;
+#pragma unroll
+    for (int i = 0; i < kScanBins; ++i) {
+        v[i] = live ? __hip_atomic_load(&a.hist[bin0 + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
+        sum += v[i];
+    }
+    if (live)
+#pragma unroll
+        for (int i = 0; i < kScanBins; ++i) a.hist[bin0 + i] = 0u;  // ready for the next call
+    // inclusive scan of `sum`: within each wavefront, then across the four
+    uint32_t x = sum;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, kWave);
+        if (lane >= o) x += y;
+    }
+    if (lane == kWave - 1) wsum[wave] = x;
+    __syncthreads();
+    for (int w = 0; w < wave; ++w) x += wsum[w];
+    incl[t] = x;
+    __syncthreads();
+    const int first = b * kScanTPB;
+    const uint32_t excl_t = x - sum;
+    const uint32_t excl_first = first > 0 ? incl[first - 1] : 0u;
+    if (wave == 0) {
+        // per bucket (one lane each): pairs, tasks of 64/L slots, then an
+        // inclusive scan of tasks and slots across the buckets
+        const int k = lane;
+        uint32_t cnt = 0, nt = 0, ns = 0;
+        if (k < a.nb) {
+            const uint32_t start = k > 0 ? incl[k * kScanTPB - 1] : 0u;
+            cnt = incl[(k + 1) * kScanTPB - 1] - start;
+            const uint32_t S = (uint32_t)(kWave / bucket_L(k));
+            nt = (cnt + S - 1) / S;
+            ns = nt * S;
+        }
+        uint32_t ti = nt, si = ns;
+#pragma unroll
+        for (int o = 1; o < kWave; o <<= 1) {
+            const uint32_t tu = __shfl_up(ti, o, kWave), su = __shfl_up(si, o, kWave);
+            if (lane >= o) {
+                ti += tu;
+                si += su;
+            }
+        }
+        if (k < a.nb) {
+            const int64_t base = (int64_t)(si - ns);
+            slot_base[k] = base;
+            slot_fill[k] = base + cnt;
+            slot_end[k] = (int64_t)si;
+            a.meta[1 + a.nb + k] = base;
+            a.meta[1 + k] = (int64_t)ti;
+            if (k == a.nb - 1) a.meta[0] = (int64_t)ti;
+        }
+        if (lane == 0) {
+            *a.counter = 0u;
+            *a.done = 0u;
+        }
+    }
+    __syncthreads();
+    if (live) {
+        uint32_t pos = (uint32_t)slot_base[b] + (excl_t - excl_first);
+#pragma unroll
+        for (int i = 0; i < kScanBins; ++i) {
+            a.cursor[bin0 + i] = pos;
+            pos += v[i];
+        }
+    }
+    // padding slots after each bucket's pairs: at most 64/L - 1 < 16 per bucket
+    for (int q = t; q < a.nb * 16; q += kPlanBlock) {
+        const int k = q / 16;
+        const int64_t s = slot_fill[k] + q % 16;
+        if (s < slot_end[k]) a.slots[s] = kEmptySlot;
+    }
 }
 
 static __global__ __launch_bounds__(kPlanBlock) void plan_hist_kernel(PlanArgs a) {
@@ -58,80 +186,23 @@ static __global__ __launch_bounds__(kPlanBlock) void plan_hist_kernel(PlanArgs a
     __syncthreads();
     const int64_t p0 = (int64_t)blockIdx.x * kPairsPerBlock;
     const int64_t p1 = p0 + kPairsPerBlock < a.npairs ? p0 + kPairsPerBlock : a.npairs;
-    for (int64_t p = p0 + threadIdx.x; p < p1; p += kPlanBlock) atomicAdd(&h[pair_bin(a, p)], 1u);
+    int bins[kPer];
+    pair_bins(a, p0, p1, bins);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k)
+        if (bins[k] >= 0) atomicAdd(&h[bins[k]], 1u);
     __syncthreads();
     for (int i = threadIdx.x; i < nbins; i += kPlanBlock)
         if (h[i]) atomicAdd(&a.hist[i], h[i]);
-}
-
-// One block: thread t owns bins [8t, 8t + 8) (32 threads per bucket).  A
-// block scan of the per-thread sums gives each bin's position within its
-// bucket; buckets are laid out one after another, each padded to whole tasks
-// of 64/L slots whose unused slots are marked empty.
-static __global__ __launch_bounds__(kScanThreads) void plan_scan_kernel(PlanArgs a) {
-    __shared__ uint32_t sh[kScanThreads];
-    __shared__ int64_t slot_base[kMaxBuckets];
-    __shared__ int64_t slot_fill[kMaxBuckets];  // first padding slot of each bucket
-    __shared__ int64_t slot_end[kMaxBuckets];
-    constexpr int kThreadsPerBucket = kRowKeys / kBinsPerThread;
-    const int t = threadIdx.x;
-    const int nbins = a.nb * kRowKeys;
-    uint32_t v[kBinsPerThread];
-    uint32_t sum = 0;
-#pragma unroll
-    for (int i = 0; i < kBinsPerThread; ++i) {
-        const int bin = t * kBinsPerThread + i;
-        v[i] = bin < nbins ? a.hist[bin] : 0u;
-        if (bin < nbins) a.hist[bin] = 0u;  // ready for the next call
-        sum += v[i];
-    }
-    sh[t] = sum;
+    // the last block to finish lays the buckets out
+    __shared__ bool last;
+    __threadfence();
     __syncthreads();
-    for (int o = 1; o < kScanThreads; o <<= 1) {  // Hillis-Steele inclusive scan
-        const uint32_t x = t >= o ? sh[t - o] : 0u;
-        __syncthreads();
-        sh[t] += x;
-        __syncthreads();
-    }
-    const int b = t / kThreadsPerBucket;
-    const int first = b * kThreadsPerBucket;
-    const uint32_t excl_t = sh[t] - sum;
-    const uint32_t excl_first = first > 0 ? sh[first - 1] : 0u;
-    if (t == 0) {
-        int64_t tasks = 0, slots = 0;
-        for (int k = 0; k < a.nb; ++k) {
-            const int last = (k + 1) * kThreadsPerBucket - 1;
-            const uint32_t start = k > 0 ? sh[k * kThreadsPerBucket - 1] : 0u;
-            const int64_t cnt = (int64_t)(sh[last] - start);
-            const int S = kWave / bucket_L(k);
-            const int64_t nt = (cnt + S - 1) / S;
-            slot_base[k] = slots;
-            slot_fill[k] = slots + cnt;
-            slot_end[k] = slots + nt * S;
-            a.meta[1 + a.nb + k] = slots;
-            tasks += nt;
-            slots += nt * S;
-            a.meta[1 + k] = tasks;
-        }
-        a.meta[0] = tasks;
-        *a.counter = 0u;
-    }
+    if (threadIdx.x == 0) last = atomicAdd(a.done, 1u) == gridDim.x - 1;
     __syncthreads();
-    if (b < a.nb) {
-        uint32_t pos = (uint32_t)slot_base[b] + (excl_t - excl_first);
-#pragma unroll
-        for (int i = 0; i < kBinsPerThread; ++i) {
-            const int bin = t * kBinsPerThread + i;
-            if (bin < nbins) a.cursor[bin] = pos;
-            pos += v[i];
-        }
-    }
-    // padding slots after each bucket's pairs: at most 64/L - 1 per bucket
-    if (t < a.nb * 16) {
-        const int k = t / 16;
-        const int64_t s = slot_fill[k] + t % 16;
-        if (s < slot_end[k]) a.slots[s] = kEmptySlot;
-    }
+    if (!last) return;
+    __threadfence();
+    plan_layout_block(a);
 }
 
 static __global__ __launch_bounds__(kPlanBlock) void plan_scatter_kernel(PlanArgs a) {
@@ -142,18 +213,27 @@ static __global__ __launch_bounds__(kPlanBlock) void plan_scatter_kernel(PlanArg
     __syncthreads();
     const int64_t p0 = (int64_t)blockIdx.x * kPairsPerBlock;
     const int64_t p1 = p0 + kPairsPerBlock < a.npairs ? p0 + kPairsPerBlock : a.npairs;
-    constexpr int kPer = kPairsPerBlock / kPlanBlock;  // 4 pairs per thread
     int bins[kPer];
     uint32_t rank[kPer];
+    pair_bins(a, p0, p1, bins);
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) {
-        const int64_t p = p0 + threadIdx.x + (int64_t)k * kPlanBlock;
-        bins[k] = p < p1 ? pair_bin(a, p) : -1;
-        rank[k] = bins[k] >= 0 ? atomicAdd(&cnt[bins[k]], 1u) : 0u;
-    }
+    for (int k = 0; k < kPer; ++k) rank[k] = bins[k] >= 0 ? atomicAdd(&cnt[bins[k]], 1u) : 0u;
     __syncthreads();
-    for (int i = threadIdx.x; i < nbins; i += kPlanBlock)
-        base[i] = cnt[i] ? atomicAdd(&a.cursor[i], cnt[i]) : 0u;
+    // reserve this block's range in every bin it uses: all of a thread's
+    // atomics are issued before any result is waited for
+    constexpr int kBinsPerThread = kMaxBins / kPlanBlock;
+    uint32_t got[kBinsPerThread];
+#pragma unroll
+    for (int k = 0; k < kBinsPerThread; ++k) {
+        const int i = threadIdx.x + k * kPlanBlock;
+        const uint32_t c = i < nbins ? cnt[i] : 0u;
+        got[k] = c ? atomicAdd(&a.cursor[i], c) : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kBinsPerThread; ++k) {
+        const int i = threadIdx.x + k * kPlanBlock;
+        if (i < nbins) base[i] = got[k];
+    }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {

@@ -73,9 +73,11 @@ def _nn():
         L.tsm_nn_split_bf16.argtypes = [P, I64, i, i, P, P, P]
         L.tsm_nn_gemm_nt_filter.argtypes = [i, i, i, P, i, P, i, f, P, P, i, P, P, P, P]
         L.tsm_nn_select_candidates.argtypes = [P, P, P, i, i, i, P, P, P]
+        L.tsm_nn_gemm_nt_acc.argtypes = [i, i, i, P, i, P, i, P, i, f, P]
         for n in ("gemm_nt", "cast_bf16", "resample", "conv_fwd", "conv_bwd", "gru_h0", "gru_h0_bwd",
                   "gru_step_fwd", "gru_step_bwd", "colsum", "rank_loss", "adam", "rownorm_bf16", "topk_f32",
-                  "topk_f64", "linear_f32", "split_bf16", "gemm_nt_filter", "select_candidates"):
+                  "topk_f64", "linear_f32", "split_bf16", "gemm_nt_filter", "select_candidates",
+                  "gemm_nt_acc"):
             getattr(L, "tsm_nn_" + n).restype = ctypes.c_int
         L._nn_declared = True
     return L
@@ -103,6 +105,15 @@ def gemm_nt(A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, alpha=1.0, beta=0
     N = B.shape[0]
     _lib.check(_nn().tsm_nn_gemm_nt(M, N, K, _p(A), A.stride(0), _p(B), B.stride(0), _p(C), C.stride(0),
                                     float(alpha), float(beta), _p(bias), _s()))
+    return C
+
+
+def gemm_acc(A: torch.Tensor, B: torch.Tensor, C: torch.Tensor, alpha=1.0):
+    """C += alpha * A . B^T with K split across blocks (weight gradients)."""
+    M, K = A.shape
+    N = B.shape[0]
+    _lib.check(_nn().tsm_nn_gemm_nt_acc(M, N, K, _p(A), A.stride(0), _p(B), B.stride(0), _p(C), C.stride(0),
+                                        float(alpha), _s()))
     return C
 
 
@@ -223,25 +234,27 @@ class Encoder:
         g = _views(grad)
         Hs = c["Hs"]
         # head: dW_o = dE^T hT, db_o = sum dE, dh = dE W_o
-        dET = _bf16(dE, True)
-        gemm_nt(dET, _bf16(Hs[T], True), g["W_o"])
+        gemm_acc(_bf16(dE, True), _bf16(Hs[T], True), g["W_o"])
         _lib.check(L.tsm_nn_colsum(_p(dE), n, D, D, _p(g["b_o"]), 0.0, s))
         dh = torch.empty((n, H), dtype=torch.float32, device=dev)
         gemm_nt(_bf16(dE), self.W_oT_b, dh)
         dGx = torch.empty((n * T, G3), dtype=torch.float32, device=dev)
-        dGh = torch.empty((n, G3), dtype=torch.float32, device=dev)
+        dGh = torch.empty((T, n, G3), dtype=torch.float32, device=dev)  # every step's, for one wgrad GEMM
         dGhb = torch.empty((n, G3), dtype=torch.int16, device=dev)
         for t in range(T - 1, -1, -1):
             dgx_t = dGx.view(n, T, G3)[:, t, :]
             _lib.check(L.tsm_nn_gru_step_bwd(_p(dh), _p(Hs[t]), _p(c["R"][t]), _p(c["Z"][t]), _p(c["NN"][t]),
-                                             _p(c["GHN"][t]), n, H, _p(dgx_t), T * G3, _p(dGh), _p(dGhb), _p(dh), s))
-            # dW_hh += dGh^T h_t ; db_hh += sum dGh ; dh = dh * z + dGh W_hh
-            gemm_nt(_bf16(dGh, True), _bf16(Hs[t], True), g["W_hh"], beta=1.0)
-            _lib.check(L.tsm_nn_colsum(_p(dGh), n, G3, G3, _p(g["b_hh"]), 1.0, s))
+                                             _p(c["GHN"][t]), n, H, _p(dgx_t), T * G3, _p(dGh[t]), _p(dGhb), _p(dh),
+                                             s))
+            # dh = dh * z + dGh W_hh  (the step wrote dh * z)
             gemm_nt(dGhb, self.W_hhT_b, dh, beta=1.0)
         _lib.check(L.tsm_nn_gru_h0_bwd(_p(dh), _p(Hs[0]), _p(c["lenf"]), n, H, _p(g["u"]), s))
+        # recurrent weights over all steps at once: dW_hh = sum_t dGh_t^T h_t, db_hh = sum dGh
+        dGh_all = dGh.view(T * n, G3)
+        gemm_acc(_bf16(dGh_all, True), _bf16(Hs[:T].reshape(T * n, H), True), g["W_hh"])
+        _lib.check(L.tsm_nn_colsum(_p(dGh_all), T * n, G3, G3, _p(g["b_hh"]), 0.0, s))
         # input projection: dW_ih = dGx^T h1, db_ih = sum dGx, dH1 = dGx W_ih
-        gemm_nt(_bf16(dGx, True), _bf16(c["H1"], True), g["W_ih"])
+        gemm_acc(_bf16(dGx, True), _bf16(c["H1"], True), g["W_ih"])
         _lib.check(L.tsm_nn_colsum(_p(dGx), n * T, G3, G3, _p(g["b_ih"]), 0.0, s))
         dH1 = torch.empty((n * T, C_CONV), dtype=torch.float32, device=dev)
         gemm_nt(_bf16(dGx), self.W_ihT_b, dH1)
