@@ -501,8 +501,11 @@ template <int M, bool F64> __device__ __forceinline__ void dp_body(const DPArgs 
 // guard-band redo of EDR/LCSS and ERP's gap columns would otherwise push
 // them past it, to one wave per SIMD); the FP64 variants keep their full
 // register file rather than spill.
+#ifndef TSM_DP_WAVES_PER_EU
+#define TSM_DP_WAVES_PER_EU 2
+#endif
 template <int M, bool F64>
-__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void dp_kernel(DPArgs a) {
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(TSM_DP_WAVES_PER_EU))) void dp_kernel(DPArgs a) {
     dp_body<M, F64>(a);
 }
 template <int M, bool F64> __global__ __launch_bounds__(kWave) void dp_kernel_wide(DPArgs a) { dp_body<M, F64>(a); }
