@@ -19,7 +19,11 @@
  *     "host" pointers are ordinary host memory.
  *   - Device-pointer entry points are stream-ordered on `stream`
  *     (a hipStream_t, NULL = the null stream) and do not synchronise, except
- *     when the library's workspace has to grow (a hipMalloc).
+ *     when the library's workspace has to grow (a hipMalloc).  Calls share
+ *     one device workspace; a call on a different stream than the previous
+ *     call first makes its stream wait for that call's kernels, so calls
+ *     from several streams or threads are safe and run one after another.
+ *     A captured graph (tsm_graph_capture_pairs) owns its own workspace.
  *   - Results are float64 like the reference's (core/dtw.py:70,76); EDR and
  *     LCSS also report the integer DP count.
  *   - Trajectory lengths must be >= 1.

@@ -67,3 +67,29 @@ def test_autotune_small(gpu):
     _lib.check(_lib.lib().tsm_tune_set_lanes(64, None, 0))
     _lib.check(_lib.lib().tsm_tune_set_waves_per_cu(0))
     autotune.load_shipped()
+
+
+def test_calls_on_two_streams_share_the_workspace_safely(gpu):
+    """Back-to-back calls on two streams (no host synchronisation between
+    them) reuse one workspace; the library orders them, so both results are
+    right."""
+    import torch
+
+    from trajsimimeasures_amd import engine
+    ts, pa, pb = uniform_pairs(4000, seed=9)
+    d = engine.DeviceTrajSet(ts)
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    outs = []
+    for k in range(6):
+        with torch.cuda.stream(s1 if k % 2 == 0 else s2):
+            m = ("dtw", "edr")[k % 2]
+            outs.append((m, engine.pairs(m, d, pa, pb, eps=EPS, validate=False)))
+    torch.cuda.synchronize()
+    for m, o in outs:
+        ref, _ = O.pairs(m, ts.xy, ts.offsets, pa, pb, eps=EPS)
+        got = o.cpu().numpy()
+        if m == "edr":
+            assert np.array_equal(got, ref)
+        else:
+            assert np.max(np.abs(got - ref) / np.maximum(ref, 1e-300)) <= 1e-5

@@ -83,8 +83,8 @@ def test_column_strips(gpu, measure):
     """Pairs wider than one strip of L*W register columns (swept in strips
     with a boundary column in scratch): up to 1200 x 1100."""
     print(f"[case] column strips measure={measure}", flush=True)
-    la = [300, 420, 600, 1200, 257, 385, 1000]
-    lb = [290, 500, 610, 1100, 300, 384, 999]
+    la = [300, 420, 600, 1200, 257, 385, 1000, 193, 192, 400, 2000]
+    lb = [290, 500, 610, 1100, 300, 384, 999, 195, 192, 768, 1999]
     ts, pa, pb = fixed_lengths(la, lb, seed=5)
     d = _dev(ts)
     g = (0.5, 0.5)

@@ -184,7 +184,17 @@ int run(int measure, const tsm_trajset *set, const PairSpace &ps, int64_t npairs
     if (rc || npairs == 0) return rc;
     std::lock_guard<std::mutex> lk(g_mu);
     if ((rc = ensure_ctx())) return rc;
-    return launch(g_ctx.ws, measure, set, ps, npairs, prm, out, counts, reinterpret_cast<hipStream_t>(stream));
+    hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+    // every call shares one workspace: a call on another stream than the
+    // previous one first waits for that call's kernels
+    if (!g_ctx.ws_event) TSM_HIP(hipEventCreateWithFlags(&g_ctx.ws_event, hipEventDisableTiming));
+    if (g_ctx.ws_used && g_ctx.ws_stream != st) TSM_HIP(hipStreamWaitEvent(st, g_ctx.ws_event, 0));
+    rc = launch(g_ctx.ws, measure, set, ps, npairs, prm, out, counts, st);
+    if (rc) return rc;
+    TSM_HIP(hipEventRecord(g_ctx.ws_event, st));
+    g_ctx.ws_stream = st;
+    g_ctx.ws_used = true;
+    return TSM_OK;
 }
 
 }  // namespace

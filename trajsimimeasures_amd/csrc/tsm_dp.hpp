@@ -12,7 +12,7 @@
 // longer trajectory on the rows (streamed) and the shorter on the columns
 // (held in registers).
 //
-// Layout of the work: a "segment" of L lanes (L = 4 or 8, so 16 or 8
+// Layout of the work: a "segment" of L lanes (L = 4, 8 or 16, so 16, 8 or 4
 // segments per wavefront) owns one pair.  Lane j of a segment holds W
 // consecutive columns -- their points in VGPRs and the previous row of D for
 // them -- and sweeps the rows in skewed order: at step s it computes row
@@ -55,16 +55,28 @@ constexpr int kTabMax = 257;    // bucket table covers column counts 1..256
 #endif
 template <typename CT> struct WMaxOf { static constexpr int value = sizeof(CT) == 4 ? TSM_WMAX32 : 24; };
 
-// Buckets: b = 2 * (WM - W) + (L == 4 ? 1 : 0), i.e. W descending, L = 8 first.
-__host__ __device__ constexpr int bucket_of(int WM, int L, int W) { return 2 * (WM - W) + (L == 4 ? 1 : 0); }
-__host__ __device__ constexpr int bucket_L(int b) { return (b & 1) ? 4 : 8; }
-__host__ __device__ constexpr int bucket_W(int WM, int b) { return WM - b / 2; }
+// Buckets, with H = WM / 2: b < H holds L = 16, W = WM - b (pairs wider
+// than 8 * WM columns; b = 0 also takes every pair wider than the table,
+// swept in strips of 16 * WM columns); b >= H holds b' = b - H:
+// L = (b' odd ? 4 : 8), W = WM - b' / 2.  Tasks run in bucket order, so the
+// widest pairs start first, and sixteen lanes per pair shorten their
+// critical path (the steps of one pair run one after another).
+__host__ __device__ constexpr int bucket_count(int WM) { return WM / 2 + 2 * WM; }
+__host__ __device__ constexpr int bucket_of(int WM, int L, int W) {
+    return L == 16 ? WM - W : WM / 2 + 2 * (WM - W) + (L == 4 ? 1 : 0);
+}
+__host__ __device__ constexpr int bucket_L(int WM, int b) {
+    return b < WM / 2 ? 16 : ((b - WM / 2) & 1) ? 4 : 8;
+}
+__host__ __device__ constexpr int bucket_W(int WM, int b) { return b < WM / 2 ? WM - b : WM - (b - WM / 2) / 2; }
 
 // Column count -> bucket table, plus the bucket of pairs wider than the
-// table (swept in strips of L = 8, W = WM).
+// table (L = 16, W = WM, swept in strips of 16 * WM columns), and the lanes
+// per pair of every bucket.
 struct BucketTable {
     uint8_t b[kTabMax];  // b[ncol] for 1 <= ncol < kTabMax
-    int ncmax;           // largest column count the table covers without strips
+    uint8_t lanes[64];   // L of bucket b
+    int ncmax;           // largest column count the table covers
     int wide;            // bucket for ncol > ncmax
 };
 
@@ -463,9 +475,9 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
 
 template <int M, bool F64, int WM, int b>
 __device__ __forceinline__ void dispatch_bucket(const DPArgs &a, int bk, int64_t slot0, int64_t wave_id) {
-    if constexpr (b < 2 * WM) {
+    if constexpr (b < bucket_count(WM)) {
         if (bk == b) {
-            run_task<M, F64, bucket_L(b), bucket_W(WM, b)>(a, slot0, wave_id);
+            run_task<M, F64, bucket_L(WM, b), bucket_W(WM, b)>(a, slot0, wave_id);
             return;
         }
         dispatch_bucket<M, F64, WM, b + 1>(a, bk, slot0, wave_id);
@@ -492,7 +504,7 @@ template <int M, bool F64> __device__ __forceinline__ void dp_body(const DPArgs 
         }
         const int bk = __builtin_amdgcn_readfirstlane(lo);
         const int64_t tstart = bk > 0 ? a.meta[bk] : 0;
-        const int64_t slot0 = a.meta[1 + a.nb + bk] + ((int64_t)t - tstart) * (kWave / bucket_L(bk));
+        const int64_t slot0 = a.meta[1 + a.nb + bk] + ((int64_t)t - tstart) * (kWave / bucket_L(WM, bk));
         dispatch_bucket<M, F64, WM, 0>(a, bk, slot0, wave_id);
     }
 }

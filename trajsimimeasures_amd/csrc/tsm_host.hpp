@@ -79,6 +79,11 @@ struct Context {
     uint8_t tuned_lanes[2][258] = {};
     bool tuned[2] = {false, false};
     int waves_per_cu = 8;  // resident waves per CU of the persistent DP kernel (autotuned)
+    // ordering of the shared workspace across streams: the last call's
+    // stream and an event recorded after its kernels
+    hipEvent_t ws_event = nullptr;
+    hipStream_t ws_stream = nullptr;
+    bool ws_used = false;
     bool capturing = false;  // inside a graph capture: no profiling events
     // profiling
     bool prof = false;

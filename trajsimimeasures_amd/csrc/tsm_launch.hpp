@@ -8,22 +8,26 @@
 
 namespace tsm {
 
-// Column count -> (L, W) bucket.  Priority: the caller's lanes hint, then
-// the autotuned lanes table (tsm_tune_set_lanes), then a cost model.  Per
+// Column count -> (L, W) bucket.  Sixteen lanes where eight would need more
+// than WM columns each, eight where four would; otherwise, by priority, the
+// caller's lanes hint, the autotuned lanes table (tsm_tune_set_lanes), then
+// a cost model.  Per
 // pair, a segment of L lanes takes (nrow + L - 1) steps of (c0 + c1 * W)
 // issue slots for 64/L pairs at once, so the per-pair cost is
 // L * (nrow + L - 1) * (c0 + c1 * W) / 64 with c0 ~ c1 (per-step bookkeeping
 // ~ one cell): L = 4 wins whenever its W fits in registers.
 inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint) {
     BucketTable t{};
-    t.ncmax = 8 * WM < kTabMax - 1 ? 8 * WM : kTabMax - 1;
-    t.wide = bucket_of(WM, 8, WM);
+    t.ncmax = 16 * WM < kTabMax - 1 ? 16 * WM : kTabMax - 1;
+    t.wide = bucket_of(WM, 16, WM);
+    for (int b = 0; b < bucket_count(WM); ++b) t.lanes[b] = (uint8_t)bucket_L(WM, b);
     const int k = coord_bits == 64 ? 1 : 0;
     const uint8_t *tuned = g_ctx.tuned[k] ? g_ctx.tuned_lanes[k] : nullptr;
     for (int nc = 1; nc <= t.ncmax; ++nc) {
         const int w4 = (nc + 3) / 4, w8 = (nc + 7) / 8;
         int L;
-        if (w4 > WM) L = 8;
+        if (w8 > WM) L = 16;
+        else if (w4 > WM) L = 8;
         else if (lanes_hint == 4 || lanes_hint == 8) L = lanes_hint;
         else if (tuned && (tuned[nc] == 4 || tuned[nc] == 8)) L = tuned[nc];
         else {
@@ -43,8 +47,8 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     using CT = typename Rec<M, F64>::CT;
     using AT = typename Rec<M, F64>::AT;
     constexpr int WM = Rec<M, F64>::kWMax;
-    constexpr int NB = 2 * WM;
-    static_assert(NB <= kMaxBuckets, "bucket count");
+    constexpr int NB = bucket_count(WM);
+    static_assert(NB <= kMaxBuckets && NB <= 64 && WM % 2 == 0, "bucket count");
     const BucketTable tab = bucket_table(WM, sizeof(CT) * 8, lanes_hint);
     const int64_t nslots = npairs + (int64_t)NB * kMaxSeg;
     int rc;

@@ -117,19 +117,23 @@ struct FilterArgs {
     int32_t *idx;      // [M, cap] candidate columns
 };
 
-__device__ __forceinline__ void tile_load(const uint16_t *base, int ld, int row0, int rows, int k0, int K, uint4 (&r)[4]) {
+// Global -> registers -> LDS staging of a ROWS x kBK BF16 tile by NT
+// threads, 16 bytes (8 values of one row) per chunk.
+template <int ROWS, int NT>
+__device__ __forceinline__ void tile_load(const uint16_t *base, int ld, int row0, int rows, int k0, int K,
+                                          uint4 (&r)[ROWS * 8 / NT]) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int q = (int)threadIdx.x + 256 * i;
+    for (int i = 0; i < ROWS * 8 / NT; ++i) {
+        const int q = (int)threadIdx.x + NT * i;
         const int row = row0 + (q >> 3), k = k0 + (q & 7) * 8;
         r[i] = (row < rows && k < K) ? *reinterpret_cast<const uint4 *>(base + (int64_t)row * ld + k)
                                      : make_uint4(0, 0, 0, 0);
     }
 }
-__device__ __forceinline__ void tile_store(uint16_t *s, const uint4 (&r)[4]) {
+template <int ROWS, int NT> __device__ __forceinline__ void tile_store(uint16_t *s, const uint4 (&r)[ROWS * 8 / NT]) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int q = (int)threadIdx.x + 256 * i;
+    for (int i = 0; i < ROWS * 8 / NT; ++i) {
+        const int q = (int)threadIdx.x + NT * i;
         *reinterpret_cast<uint4 *>(s + (q >> 3) * kLdsStride + (q & 7) * 8) = r[i];
     }
 }
@@ -137,13 +141,17 @@ __device__ __forceinline__ void tile_store(uint16_t *s, const uint4 (&r)[4]) {
 // kMode: 0 store (alpha A B^T + beta C + bias), 1 filter (see above), 2
 // split-K accumulate: block z covers K range [z * kchunk, (z + 1) * kchunk)
 // and adds alpha * its partial product into C with float atomics.
-template <int kMode>
-__global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g, FilterArgs f, int kchunk) {
+// BM (128 or 256) rows of C per block, 2 * BM threads: (BM / 64) x 2 waves
+// of 64 x 64.  The 256-row form halves how often the B operand (the
+// database, in the search) is streamed.
+template <int kMode, int BM = kBM>
+__global__ __launch_bounds__(2 * BM) void gemm_nt_lds_kernel(GemmArgs g, FilterArgs f, int kchunk) {
     constexpr bool kFilter = kMode == 1;
-    __shared__ __attribute__((aligned(16))) uint16_t As[2][kBM * kLdsStride];
+    constexpr int NT = 2 * BM;
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][BM * kLdsStride];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[2][kBN * kLdsStride];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int bm = blockIdx.y * kBM, bn = blockIdx.x * kBN;
+    const int bm = blockIdx.y * BM, bn = blockIdx.x * kBN;
     const int wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
     const int r16 = lane & 15, kb = (lane >> 4) * 8;
     f32x4 acc[4][4];
@@ -151,22 +159,22 @@ __global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g, FilterArgs
     for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    uint4 ra[4], rb[4];
+    uint4 ra[BM * 8 / NT], rb[kBN * 8 / NT];
     // K range of this block (the whole K unless split)
     const int kbeg = kMode == 2 ? (int)blockIdx.z * kchunk : 0;
     const int kend = kMode == 2 ? min(g.K, kbeg + kchunk) : g.K;
-    tile_load(g.A, g.lda, bm, g.M, kbeg, kend, ra);
-    tile_load(g.B, g.ldb, bn, g.N, kbeg, kend, rb);
-    tile_store(As[0], ra);
-    tile_store(Bs[0], rb);
+    tile_load<BM, NT>(g.A, g.lda, bm, g.M, kbeg, kend, ra);
+    tile_load<kBN, NT>(g.B, g.ldb, bn, g.N, kbeg, kend, rb);
+    tile_store<BM, NT>(As[0], ra);
+    tile_store<kBN, NT>(Bs[0], rb);
     __syncthreads();
     const int nk = (kend - kbeg + kBK - 1) / kBK;
     for (int kt = 0; kt < nk; ++kt) {
         const int cur = kt & 1;
         const bool more = kt + 1 < nk;
         if (more) {
-            tile_load(g.A, g.lda, bm, g.M, kbeg + (kt + 1) * kBK, kend, ra);
-            tile_load(g.B, g.ldb, bn, g.N, kbeg + (kt + 1) * kBK, kend, rb);
+            tile_load<BM, NT>(g.A, g.lda, bm, g.M, kbeg + (kt + 1) * kBK, kend, ra);
+            tile_load<kBN, NT>(g.B, g.ldb, bn, g.N, kbeg + (kt + 1) * kBK, kend, rb);
         }
 #pragma unroll
         for (int kk = 0; kk < kBK; kk += 32) {
@@ -186,8 +194,8 @@ __global__ __launch_bounds__(256) void gemm_nt_lds_kernel(GemmArgs g, FilterArgs
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
         }
         if (more) {
-            tile_store(As[cur ^ 1], ra);
-            tile_store(Bs[cur ^ 1], rb);
+            tile_store<BM, NT>(As[cur ^ 1], ra);
+            tile_store<kBN, NT>(Bs[cur ^ 1], rb);
         }
         __syncthreads();
     }
@@ -733,11 +741,18 @@ int tsm_nn_gemm_nt_filter(int M, int N, int K, const uint16_t *A, int lda, const
     if (!A || !B || !tau || !cnt || !key || !idx || cap < 1 || lda < K || ldb < K || (lda % 8) || (ldb % 8) ||
         (reinterpret_cast<uintptr_t>(A) % 16) || (reinterpret_cast<uintptr_t>(B) % 16))
         return fail(TSM_E_ARG, "gemm_filter: bad operands (16-byte aligned, leading dimensions multiples of 8)");
-    if ((M + kBM - 1) / kBM > 65535) return fail(TSM_E_ARG, "gemm_filter: M too large");
+    if ((M + 255) / 256 > 65535) return fail(TSM_E_ARG, "gemm_filter: M too large");
     GemmArgs g{M, N, K, A, lda, B, ldb, nullptr, 0, alpha, 0.f, bias};
-    dim3 grid((N + kBN - 1) / kBN, (M + kBM - 1) / kBM);
-    hipLaunchKernelGGL(gemm_nt_lds_kernel<1>, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), g,
-                       FilterArgs{tau, cap, cnt, key, idx}, 0);
+    const FilterArgs fa{tau, cap, cnt, key, idx};
+    if (M > kBM) {  // 256 query rows per block: the database is streamed half as often
+        dim3 grid((N + kBN - 1) / kBN, (M + 255) / 256);
+        hipLaunchKernelGGL((gemm_nt_lds_kernel<1, 256>), grid, dim3(512), 0, reinterpret_cast<hipStream_t>(stream), g,
+                           fa, 0);
+    } else {
+        dim3 grid((N + kBN - 1) / kBN, (M + kBM - 1) / kBM);
+        hipLaunchKernelGGL((gemm_nt_lds_kernel<1, kBM>), grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), g,
+                           fa, 0);
+    }
     TSM_HIP(hipGetLastError());
     return TSM_OK;
 }

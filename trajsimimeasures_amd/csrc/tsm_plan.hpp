@@ -135,7 +135,7 @@ __device__ __forceinline__ void plan_layout_block(const PlanArgs &a) {
         if (k < a.nb) {
             const uint32_t start = k > 0 ? incl[k * kScanTPB - 1] : 0u;
             cnt = incl[(k + 1) * kScanTPB - 1] - start;
-            const uint32_t S = (uint32_t)(kWave / bucket_L(k));
+            const uint32_t S = (uint32_t)(kWave / a.tab.lanes[k]);
             nt = (cnt + S - 1) / S;
             ns = nt * S;
         }
