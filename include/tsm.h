@@ -48,6 +48,7 @@ extern "C" {
 #define TSM_HAUSDORFF 5 /* core/hausdorff.py:12-123 */
 #define TSM_STEDR 6     /* core/stedr.py:14-86: EDR whose match also needs |t1 - t2| < delta */
 #define TSM_STLCSS 7    /* core/stlcss.py:10-80: LCSS whose match also needs |t1 - t2| < delta */
+#define TSM_CDDS 8      /* core/cdds.py:11-124: time the moving points stay within eps (needs t64) */
 
 /* Error codes. */
 #define TSM_OK 0
@@ -80,7 +81,7 @@ typedef struct tsm_trajset {
 
 /* Per-call parameters; tsm_params_default() fills the defaults. */
 typedef struct tsm_params {
-    double eps;     /* EDR/LCSS threshold (core/edr.py:55 `eps`) */
+    double eps;     /* EDR/LCSS/STEDR/STLCSS/CDDS distance threshold (core/edr.py:55 `eps`) */
     int eps_mode;   /* TSM_EPS_SQRT (default) or TSM_EPS_SQUARED */
     double gx, gy;  /* ERP gap point g (core/erp.py:94 `gpoint`, default (0,0) erp.py:103) */
     int lanes;      /* lanes per pair: 0 = autotuned per column count, else 4 or 8 (threads_per_traj hint) */

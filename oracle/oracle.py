@@ -61,6 +61,12 @@ def lib():
             f.restype = i64
         L.tsmo_pairs_st.argtypes = [ctypes.c_int, _P, _P, _P, _P, _P, i64, d, d, _P, _P, ctypes.c_int]
         L.tsmo_pairs_st.restype = ctypes.c_int
+        L.tsmo_cdd.argtypes = [_P, _P, _P, _P, d]
+        L.tsmo_cdd.restype = d
+        L.tsmo_cdds.argtypes = [_P, _P, i64, _P, _P, i64, d]
+        L.tsmo_cdds.restype = d
+        L.tsmo_pairs_cdds.argtypes = [_P, _P, _P, _P, _P, i64, d, _P, ctypes.c_int]
+        L.tsmo_pairs_cdds.restype = ctypes.c_int
         L.tsmo_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -152,3 +158,28 @@ def pairs_st(measure: str, xy, t, offsets, pa, pb, eps: float, delta: float, eps
     if rc != 0:
         raise ValueError(f"bad measure {measure!r}")
     return out, cnt
+
+
+def cdd(e1p1, e1p2, e2p1, e2p2, eps: float) -> float:
+    """Time two points moving along segments (x, y, t) -> (x, y, t) spend within eps."""
+    v = [np.ascontiguousarray(np.asarray(e, dtype=np.float64).reshape(3)) for e in (e1p1, e1p2, e2p1, e2p2)]
+    return lib().tsmo_cdd(v[0].ctypes.data, v[1].ctypes.data, v[2].ctypes.data, v[3].ctypes.data, float(eps))
+
+
+def cdds_single(a, ta, b, tb, eps: float) -> float:
+    a, b = _arr(a), _arr(b)
+    ta = np.ascontiguousarray(np.asarray(ta, dtype=np.float64).reshape(-1))
+    tb = np.ascontiguousarray(np.asarray(tb, dtype=np.float64).reshape(-1))
+    return lib().tsmo_cdds(a.ctypes.data, ta.ctypes.data, len(a), b.ctypes.data, tb.ctypes.data, len(b), float(eps))
+
+
+def pairs_cdds(xy, t, offsets, pa, pb, eps: float, threads: int = 0):
+    xy = np.ascontiguousarray(np.asarray(xy, dtype=np.float64).reshape(-1, 2))
+    t = np.ascontiguousarray(np.asarray(t, dtype=np.float64).reshape(-1))
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    pa = np.ascontiguousarray(pa, dtype=np.int64)
+    pb = np.ascontiguousarray(pb, dtype=np.int64)
+    out = np.empty(pa.size, dtype=np.float64)
+    lib().tsmo_pairs_cdds(xy.ctypes.data, t.ctypes.data, offsets.ctypes.data, pa.ctypes.data, pb.ctypes.data,
+                          pa.size, float(eps), out.ctypes.data, int(threads))
+    return out

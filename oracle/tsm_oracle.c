@@ -20,6 +20,8 @@
  *   STEDR     core/stedr.py:29-59           EDR whose match also needs |t1 - t2| < delta
  *   STLCSS    core/stlcss.py:27-54          LCSS whose match also needs |t1 - t2| < delta
  *             (CPU forms: core_cpu/stedr.pyx:15-42, core_cpu/stlcss.pyx:15-39)
+ *   CDDS      core/cdds.py:11-94            sum over all segment pairs of the time the two
+ *             moving points are within eps (CPU form: core_cpu/cdds.pyx)
  *
  * Distances are sqrt(dx*dx + dy*dy) evaluated without FMA contraction
  * (core/utils.py:8-17); build with -ffp-contract=off.
@@ -46,7 +48,7 @@
 #endif
 
 enum { TSMO_DTW = 0, TSMO_EDR = 1, TSMO_LCSS = 2, TSMO_ERP = 3, TSMO_FRECHET = 4, TSMO_HAUSDORFF = 5,
-       TSMO_STEDR = 6, TSMO_STLCSS = 7 };
+       TSMO_STEDR = 6, TSMO_STLCSS = 7, TSMO_CDDS = 8 };
 
 static inline double sqd(const double *p, const double *q) {
     double dx = p[0] - q[0];
@@ -352,6 +354,91 @@ int tsmo_pairs_st(int measure, const double *xy, const double *t, const int64_t 
             out[k] = 1.0 - (double)c / (double)(n1 < n2 ? n1 : n2);
         }
         if (counts) counts[k] = c;
+    }
+    return 0;
+}
+
+/* core/cdds.py:11-73 (cdd) -- time during which the points moving linearly
+ * along segment e1 = (p, q) and e2 = (r, s) (each point x, y, t) are within
+ * eps of each other.  Positions are x(t) = xk t + xb (core/cdds.py:11-18);
+ * the squared distance a t^2 + b t + c is solved against eps^2.  A segment
+ * with no duration contributes 0 (core_cpu/cdds.pyx:26; the reference GPU
+ * kernel would divide by zero there). */
+static void slope_intercept(const double *p, const double *q, double *xk, double *xb, double *yk, double *yb) {
+    const double dt = q[2] - p[2];
+    *yk = (q[1] - p[1]) / dt;
+    *yb = (p[1] * q[2] - q[1] * p[2]) / dt;
+    *xk = (q[0] - p[0]) / dt;
+    *xb = (p[0] * q[2] - q[0] * p[2]) / dt;
+}
+
+double tsmo_cdd(const double *e1p1, const double *e1p2, const double *e2p1, const double *e2p2, double eps) {
+    if (e1p2[2] <= e2p1[2] || e2p2[2] <= e1p1[2] || e1p1[2] == e1p2[2] || e2p1[2] == e2p2[2]) return 0.0;
+    const double se = eps * eps;
+    double axk, axb, ayk, ayb, bxk, bxb, byk, byb;
+    slope_intercept(e1p1, e1p2, &axk, &axb, &ayk, &ayb);
+    slope_intercept(e2p1, e2p2, &bxk, &bxb, &byk, &byb);
+    const double dxk = axk - bxk, dyk = ayk - byk, dxb = axb - bxb, dyb = ayb - byb;
+    const double a = dxk * dxk + dyk * dyk;
+    const double b = 2.0 * (dxk * dxb + dyk * dyb);
+    double c = dxb * dxb + dyb * dyb;
+    const double t_start = max2(e1p1[2], e2p1[2]);
+    const double t_end = min2(e1p2[2], e2p2[2]);
+    if (a == 0.0 && b == 0.0 && c == 0.0) return t_end - t_start;
+    if (a == 0.0 && b == 0.0) return c <= se ? t_end - t_start : 0.0;
+    if (a == 0.0) {
+        const double t1 = -1.0 * c / b, t2 = (se - c) / b;
+        if (t1 < t2) {
+            const double lo = max2(t1, t_start), hi = min2(t2, t_end);
+            return hi > lo ? hi - lo : 0.0;
+        }
+        if (t2 < t1) {
+            const double lo = max2(t2, t_start), hi = min2(t1, t_end);
+            return hi > lo ? hi - lo : 0.0;
+        }
+        return 0.0;
+    }
+    c -= se;
+    const double theta = b * b - 4.0 * a * c;
+    if (theta > 0.0) {
+        const double ts = sqrt(theta);
+        const double t1 = (-1.0 * ts - b) / 2.0 / a, t2 = (ts - b) / 2.0 / a;
+        const double lo = max2(t1, t_start), hi = min2(t2, t_end);
+        return hi > lo ? hi - lo : 0.0;
+    }
+    return 0.0;
+}
+
+/* core/cdds.py:76-94,124 -- for each segment i of a: the sum over the
+ * segments j of b in order; then the sum over i in order. */
+double tsmo_cdds(const double *a, const double *ta, int64_t n1, const double *b, const double *tb, int64_t n2,
+                 double eps) {
+    double total = 0.0;
+    for (int64_t i = 0; i + 1 < n1; ++i) {
+        const double p[3] = {a[2 * i], a[2 * i + 1], ta[i]}, q[3] = {a[2 * i + 2], a[2 * i + 3], ta[i + 1]};
+        double d = 0.0;
+        for (int64_t j = 0; j + 1 < n2; ++j) {
+            const double r[3] = {b[2 * j], b[2 * j + 1], tb[j]}, s[3] = {b[2 * j + 2], b[2 * j + 3], tb[j + 1]};
+            d += tsmo_cdd(p, q, r, s, eps);
+        }
+        total += d;
+    }
+    return total;
+}
+
+int tsmo_pairs_cdds(const double *xy, const double *t, const int64_t *offsets, const int64_t *pa, const int64_t *pb,
+                    int64_t npairs, double eps, double *out, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    int64_t k;
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < npairs; ++k) {
+        const int64_t oa = offsets[pa[k]], ob = offsets[pb[k]];
+        out[k] = tsmo_cdds(xy + 2 * oa, t + oa, offsets[pa[k] + 1] - oa, xy + 2 * ob, t + ob,
+                           offsets[pb[k] + 1] - ob, eps);
     }
     return 0;
 }

@@ -14,7 +14,7 @@ def test_dispatch_and_collate():
     with pytest.raises(NotImplementedError):
         drivers.heuristic_fn_wrapper("dtw_cpu")
     with pytest.raises(NotImplementedError):
-        drivers.heuristic_fn_wrapper("cdds")
+        drivers.heuristic_fn_wrapper("sar")
     g = drivers.heuristic_fn_wrapper("stlcss", eps=0.2)
     assert g.keywords == {"eps": 0.2, "delta": drivers.DEFAULT_DELTA, "threads_per_traj": 64}
     assert g.func is drivers.core.stlcss

@@ -89,3 +89,35 @@ def test_st_grid_triu_and_dropin(gpu):
         got = f(t1, l1, t2, l2, 0.01, 45.0)
         ref, _ = O.pairs_st(m, ts.xy, ts.t, ts.offsets, np.arange(30), np.arange(30, 60), 0.01, 45.0)
         assert np.array_equal(got, ref)
+
+
+@pytest.mark.parametrize("eps", [0.0025, 0.02, 0.2])
+def test_cdds_matches_oracle(gpu, eps):
+    """CDDS (core/cdds.py) on the GPU adds the same FP64 terms in the same
+    order as the oracle: equal values; a pair of identical trajectories gives
+    their duration."""
+    from trajsimimeasures_amd import engine
+    print(f"[case] cdds eps={eps}", flush=True)
+    ts, pa, pb = _st_set(1500, 2, 150, seed=51)
+    d = engine.DeviceTrajSet(ts)
+    got = engine.pairs("cdds", d, pa, pb, eps=eps).cpu().numpy()
+    ref = O.pairs_cdds(ts.xy, ts.t, ts.offsets, pa, pb, eps)
+    assert np.array_equal(got, ref), float(np.max(np.abs(got - ref)))
+    same = engine.pairs("cdds", d, pa[:50], pa[:50], eps=eps).cpu().numpy()
+    dur = np.array([ts.t[ts.offsets[i + 1] - 1] - ts.t[ts.offsets[i]] for i in pa[:50]])
+    assert np.allclose(same, dur, rtol=1e-12, atol=0)
+
+
+def test_cdds_dropin_and_long(gpu):
+    from trajsimimeasures_amd import core
+    from trajsimimeasures_amd.core import pad_lists_to_array
+    from trajsimimeasures_amd.data import lengths_uniform, random_walks, with_timestamps
+    rng = np.random.default_rng(53)
+    lens = np.concatenate([[1, 2, 65, 64, 130, 700], lengths_uniform(rng, 34, 1, 90)])
+    ts = with_timestamps(rng, random_walks(rng, lens), interval=5.0, jitter=4.0, span=60.0)
+    trajs = [np.concatenate([ts.traj(i), ts.t[ts.offsets[i]:ts.offsets[i + 1], None]], 1) for i in range(ts.n)]
+    t1, t2 = pad_lists_to_array(trajs[:20]), pad_lists_to_array(trajs[20:])
+    l1, l2 = [len(t) for t in trajs[:20]], [len(t) for t in trajs[20:]]
+    got = core.cdds(t1, l1, t2, l2, 0.02)
+    ref = O.pairs_cdds(ts.xy, ts.t, ts.offsets, np.arange(20), np.arange(20, 40), 0.02)
+    assert np.array_equal(got, ref)

@@ -78,7 +78,7 @@ def test_params_validation():
     with pytest.raises(ValueError):
         _lib.params(gpoint=[1, 2, 3])
     with pytest.raises(ValueError):
-        _lib.measure_id("cdds")
+        _lib.measure_id("sar")
     assert _lib.measure_id("dfrechet") == _lib.FRECHET
 
 

@@ -242,3 +242,67 @@ def test_st_pairs_matches_single(rng):
             assert cnts[k] == c
             want = c / max(len(a), len(b)) if m == "stedr" else 1.0 - c / min(len(a), len(b))
             assert vals[k] == want
+
+
+def _within_time(e1p1, e1p2, e2p1, e2p2, eps, n=200001):
+    """Measure of {t : |x1(t) - x2(t)| <= eps} over the common time span, by
+    dense sampling of the two linear motions."""
+    t0, t1 = max(e1p1[2], e2p1[2]), min(e1p2[2], e2p2[2])
+    if t1 <= t0:
+        return 0.0
+    t = np.linspace(t0, t1, n)
+
+    def pos(p, q):
+        w = (t - p[2]) / (q[2] - p[2])
+        return p[0] + w * (q[0] - p[0]), p[1] + w * (q[1] - p[1])
+    x1, y1 = pos(e1p1, e1p2)
+    x2, y2 = pos(e2p1, e2p2)
+    inside = (x1 - x2) ** 2 + (y1 - y2) ** 2 <= eps * eps
+    return inside.mean() * (t1 - t0)
+
+
+def test_cdd_matches_sampled_duration(rng):
+    """core/cdds.py:21-73: the closed-form duration within eps equals a
+    dense sampling of the two motions (moving and parallel cases)."""
+    for _ in range(40):
+        ts = np.sort(rng.random(4) * 10)
+        e1p1, e1p2 = [*rng.random(2), ts[0]], [*rng.random(2), ts[2]]
+        e2p1, e2p2 = [*rng.random(2), ts[1]], [*rng.random(2), ts[3]]
+        for eps in (0.2, 0.5, 1.0):
+            want = _within_time(e1p1, e1p2, e2p1, e2p2, eps)
+            span = max(0.0, min(e1p2[2], e2p2[2]) - max(e1p1[2], e2p1[2]))
+            assert abs(O.cdd(e1p1, e1p2, e2p1, e2p2, eps) - want) <= 1e-4 * span + 1e-12
+    # parallel motion at constant distance 0.5: all or nothing
+    p, q = [0.0, 0.0, 0.0], [1.0, 0.0, 2.0]
+    r, s = [0.0, 0.5, 1.0], [1.0, 0.5, 3.0]
+    assert O.cdd(p, q, [0.0, 0.5, 0.0], [1.0, 0.5, 2.0], 0.6) == 2.0
+    assert O.cdd(p, q, [0.0, 0.5, 0.0], [1.0, 0.5, 2.0], 0.4) == 0.0
+    # disjoint or touching time spans, and zero-duration segments, give 0
+    assert O.cdd(p, q, [0, 0, 2.0], [1, 0, 3.0], 1.0) == 0.0
+    assert O.cdd(p, [1.0, 0.0, 0.0], r, s, 1.0) == 0.0
+
+
+def test_cdds_closed_forms_and_sweep(rng):
+    """A trajectory against itself is within any eps > 0 for its whole
+    duration; and with increasing timestamps the all-segment-pairs sum of
+    core/cdds.py equals the merge sweep of core_cpu/cdds.pyx:80-104."""
+    for _ in range(10):
+        n1, n2 = rng.integers(2, 25, 2)
+        a, b = rnd(rng, n1), rnd(rng, n2)
+        ta = np.cumsum(rng.random(n1) + 0.1)
+        tb = np.cumsum(rng.random(n2) + 0.1) + rng.random()
+        assert O.cdds_single(a, ta, a, ta, 1e-9) == pytest.approx(ta[-1] - ta[0], rel=1e-12)
+        for eps in (0.1, 0.4):
+            i = j = 0
+            sweep = 0.0
+            while i < n1 - 1 and j < n2 - 1:
+                sweep += O.cdd([*a[i], ta[i]], [*a[i + 1], ta[i + 1]], [*b[j], tb[j]], [*b[j + 1], tb[j + 1]], eps)
+                if ta[i + 1] < tb[j + 1]:
+                    i += 1
+                elif tb[j + 1] < ta[i + 1]:
+                    j += 1
+                else:
+                    i += 1
+                    j += 1
+            assert O.cdds_single(a, ta, b, tb, eps) == pytest.approx(sweep, rel=1e-12, abs=1e-15)
+            assert O.cdds_single(b, tb, a, ta, eps) == pytest.approx(sweep, rel=1e-9, abs=1e-12)

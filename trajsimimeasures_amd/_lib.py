@@ -13,12 +13,13 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("TSM_LIB", os.path.join(PKG, "libtsm.so"))
 
 # Measure ids (tsm.h)
-DTW, EDR, LCSS, ERP, FRECHET, HAUSDORFF, STEDR, STLCSS = 0, 1, 2, 3, 4, 5, 6, 7
+DTW, EDR, LCSS, ERP, FRECHET, HAUSDORFF, STEDR, STLCSS, CDDS = 0, 1, 2, 3, 4, 5, 6, 7, 8
 MEASURES = {"dtw": DTW, "edr": EDR, "lcss": LCSS, "erp": ERP, "frechet": FRECHET, "dfrechet": FRECHET,
-            "discret_frechet": FRECHET, "hausdorff": HAUSDORFF, "stedr": STEDR, "stlcss": STLCSS}
+            "discret_frechet": FRECHET, "hausdorff": HAUSDORFF, "stedr": STEDR, "stlcss": STLCSS, "cdds": CDDS}
 MEASURE_NAMES = {DTW: "dtw", EDR: "edr", LCSS: "lcss", ERP: "erp", FRECHET: "frechet", HAUSDORFF: "hausdorff",
-                 STEDR: "stedr", STLCSS: "stlcss"}
+                 STEDR: "stedr", STLCSS: "stlcss", CDDS: "cdds"}
 COUNTED = (EDR, LCSS, STEDR, STLCSS)  # measures with integer match counts
+TIMED = (STEDR, STLCSS, CDDS)  # measures that need per-point timestamps
 
 EPS_SQRT, EPS_SQUARED = 0, 1
 PREC_FP32, PREC_FP64 = 0, 1

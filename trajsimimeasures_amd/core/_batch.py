@@ -28,7 +28,7 @@ def run_padded(measure: int, t1, len1, t2, len2, *, eps=0.0, eps_mode="sqrt", gp
                precision="fp32", return_counts=False, delta=0.0):
     t1 = np.asarray(t1, dtype=np.float64)
     t2 = np.asarray(t2, dtype=np.float64)
-    st = measure in (_lib.STEDR, _lib.STLCSS)
+    st = measure in _lib.TIMED
     width = 3 if st else 2
     if t1.ndim != 3 or t2.ndim != 3 or (st and (t1.shape[2] != 3 or t2.shape[2] != 3)):
         raise ValueError(f"t1 and t2 must be [batch, seq, {width}] arrays")

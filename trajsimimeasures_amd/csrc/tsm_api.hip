@@ -12,6 +12,7 @@
 #include <cmath>
 #include <cstring>
 
+#include "tsm_cdds.hpp"
 #include "tsm_hausdorff.hpp"
 #include "tsm_host.hpp"
 
@@ -89,7 +90,7 @@ int erp_prologue(Workspace &c, const tsm_trajset &set, double gx, double gy, hip
 // overridden by prm_in.
 int validate(int measure, const tsm_trajset *set, int64_t npairs, const tsm_params *prm_in, const double *out,
              tsm_params &prm) {
-    if (measure < TSM_DTW || measure > TSM_STLCSS) return fail(TSM_E_ARG, "unknown measure id " + std::to_string(measure));
+    if (measure < TSM_DTW || measure > TSM_CDDS) return fail(TSM_E_ARG, "unknown measure id " + std::to_string(measure));
     if (!set) return fail(TSM_E_ARG, "null trajectory set");
     if (npairs < 0) return fail(TSM_E_ARG, "negative pair count");
     if (npairs > (int64_t)0xFFFFFFFEll) return fail(TSM_E_ARG, "at most 2^32-2 pairs per call");
@@ -105,8 +106,8 @@ int validate(int measure, const tsm_trajset *set, int64_t npairs, const tsm_para
         return fail(TSM_E_ARG, "precision must be TSM_PREC_FP32 or TSM_PREC_FP64");
     if (prm.eps_mode != TSM_EPS_SQRT && prm.eps_mode != TSM_EPS_SQUARED)
         return fail(TSM_E_ARG, "eps_mode must be TSM_EPS_SQRT or TSM_EPS_SQUARED");
-    const bool st = measure == TSM_STEDR || measure == TSM_STLCSS;
-    if (st && !set->t64) return fail(TSM_E_ARG, "STEDR/STLCSS need per-point timestamps (t64)");
+    const bool st = measure == TSM_STEDR || measure == TSM_STLCSS || measure == TSM_CDDS;
+    if (st && !set->t64) return fail(TSM_E_ARG, "STEDR/STLCSS/CDDS need per-point timestamps (t64)");
     const bool needs64 = f64 || st || measure == TSM_EDR || measure == TSM_LCSS || measure == TSM_ERP;
     if (needs64 && !set->xy64) return fail(TSM_E_ARG, "this measure needs the FP64 coordinates (xy64)");
     if (!f64 && !st && measure != TSM_EDR && measure != TSM_LCSS && !set->xy32)
@@ -142,6 +143,25 @@ int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &p
         ProfScope p_(st);
         if (f64) hipLaunchKernelGGL(hausdorff_kernel<double>, dim3(grid), dim3(kWave), 0, st, h);
         else hipLaunchKernelGGL(hausdorff_kernel<float>, dim3(grid), dim3(kWave), 0, st, h);
+        TSM_HIP(hipGetLastError());
+        return TSM_OK;
+    }
+    if (measure == TSM_CDDS) {
+        if ((rc = w.line.ensure(set->npts * sizeof(double4)))) return rc;
+        hipLaunchKernelGGL(cdds_prep_kernel, dim3(grid_for(set->npts, 256)), dim3(256), 0, st, set->xy64, set->t64,
+                           set->npts, w.line.as<double4>());
+        CArgs c;
+        c.xy = set->xy64;
+        c.t = set->t64;
+        c.line = w.line.as<double4>();
+        c.offs = set->offsets;
+        c.ps = ps;
+        c.npairs = npairs;
+        c.eps = prm.eps;
+        c.out = out;
+        int grid = (int)(npairs < 1048576 ? npairs : 1048576);
+        ProfScope p_(st);
+        hipLaunchKernelGGL(cdds_kernel, dim3(grid), dim3(kWave), 0, st, c);
         TSM_HIP(hipGetLastError());
         return TSM_OK;
     }

@@ -62,9 +62,10 @@ struct DevBuf {
 // each captured graph owns its own, so replays never see reallocations.
 struct Workspace {
     DevBuf hist, cursor, slots, meta, counter, bnd;
-    DevBuf gap64, gap32, gsum64, gsum32, seg, rseg;
+    DevBuf gap64, gap32, gsum64, gsum32, seg, rseg, line;
     void release() {
-        for (DevBuf *b : {&hist, &cursor, &slots, &meta, &counter, &bnd, &gap64, &gap32, &gsum64, &gsum32, &seg, &rseg})
+        for (DevBuf *b : {&hist, &cursor, &slots, &meta, &counter, &bnd, &gap64, &gap32, &gsum64, &gsum32, &seg, &rseg,
+                          &line})
             b->release();
     }
 };

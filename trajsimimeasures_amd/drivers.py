@@ -31,9 +31,10 @@ DEFAULT_EPS = 0.0025  # EDR/LCSS epsilon of the 100k-pair workload
 DEFAULT_DELTA = 60.0  # STEDR/STLCSS time threshold, seconds (config.py:53)
 
 _GPU_FNS = {"dtw": core.dtw, "edr": core.edr, "erp": core.erp, "frechet": core.dfrechet,
-            "hausdorff": core.hausdorff, "lcss": core.lcss, "stedr": core.stedr, "stlcss": core.stlcss}
+            "hausdorff": core.hausdorff, "lcss": core.lcss, "stedr": core.stedr, "stlcss": core.stlcss,
+            "cdds": core.cdds}
 _MEASURE = {"dtw": "dtw", "edr": "edr", "erp": "erp", "frechet": "frechet", "hausdorff": "hausdorff",
-            "lcss": "lcss", "stedr": "stedr", "stlcss": "stlcss"}
+            "lcss": "lcss", "stedr": "stedr", "stlcss": "stlcss", "cdds": "cdds"}
 _ST = ("stedr", "stlcss")
 
 
@@ -47,12 +48,12 @@ def heuristic_fn_wrapper(fn_name: str, eps: float = DEFAULT_EPS, threads_per_tra
                          delta: float = DEFAULT_DELTA):
     """Name -> batch function with the reference's bound arguments
     (method_helper.py:32-73): eps for edr/lcss, eps and delta for
-    stedr/stlcss, threads_per_traj for all.  The ``*_cpu`` names (traj_dist,
-    core_cpu) and CDDS / SAR are not part of this engine."""
+    stedr/stlcss, eps for cdds, threads_per_traj for all.  The ``*_cpu`` names
+    (traj_dist, core_cpu) and SAR are not part of this engine."""
     fn = _GPU_FNS.get(fn_name)
     if fn is None:
         raise NotImplementedError(f"{fn_name!r} is not served by this engine (GPU measures: {sorted(_GPU_FNS)})")
-    if fn_name in ("edr", "lcss"):
+    if fn_name in ("edr", "lcss", "cdds"):
         fn = partial(fn, eps=eps)
     elif fn_name in _ST:
         fn = partial(fn, eps=eps, delta=delta)
@@ -78,7 +79,7 @@ def heuristic_colllate_fn(batch, is_pad: bool = True, quantize_fp32: bool = Fals
 def _opts(method: str, eps, gpoint, lanes, precision, delta=DEFAULT_DELTA):
     if method not in _MEASURE:
         raise NotImplementedError(f"{method!r} is not served by this engine")
-    return dict(eps=eps if method in ("edr", "lcss") + _ST else 0.0, gpoint=gpoint if method == "erp" else None,
+    return dict(eps=eps if method in ("edr", "lcss", "cdds") + _ST else 0.0, gpoint=gpoint if method == "erp" else None,
                 lanes=lanes, precision=precision, delta=delta if method in _ST else 0.0)
 
 
