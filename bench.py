@@ -12,7 +12,9 @@ roofline: the DP kernel is bound by VALU FP32 issue.  Algorithmic FLOP per DP
 cell (FMA = 2): DTW 9, Frechet 9, LCSS 9, EDR 10, ERP 11; Hausdorff 24 per
 (point, segment) evaluation; STEDR 13 and STLCSS 12 (the EDR / LCSS cell plus
 the time test: subtract, compare, and).  The spatio-temporal measures use
-timestamps 15 +- 5 s apart from a start uniform in [0, 3600) s, delta = 60 s.
+timestamps 15 +- 5 s apart from a start uniform in [0, 3600) s, delta = 60 s;
+CDDS (eps 0.0025) reports segment pairs per second and no FLOP roofline (it is
+FP64 and nearly all segment pairs stop at the time-overlap test).
 achieved = FLOP per launch / mean kernel time
 (HIP events on the launch stream); peak = 157.3 TFLOP/s (MI355X vector FP32).
 
@@ -55,6 +57,8 @@ def cells_of(measure, la, lb):
     lb = lb.astype(np.float64)
     if measure == "hausdorff":  # point-segment evaluations, both directions
         return float(np.sum(la * np.maximum(lb - 1, 0) + lb * np.maximum(la - 1, 0)))
+    if measure == "cdds":  # segment pairs
+        return float(np.sum(np.maximum(la - 1, 0) * np.maximum(lb - 1, 0)))
     return float(np.sum(la * lb))
 
 
@@ -75,7 +79,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--pairs", type=int, default=100_000)
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--measures", default="dtw,edr,lcss,erp,frechet,hausdorff,stedr,stlcss",
+    ap.add_argument("--measures", default="dtw,edr,lcss,erp,frechet,hausdorff,stedr,stlcss,cdds",
                     help="measures reported under 'measures' (dtw is always the headline)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work at most")
@@ -142,14 +146,16 @@ def main():
         steps = args.steps if m == "dtw" else max(3, args.steps // 4)
         el, kms, nl = timed(m, steps, args.warmup)
         cells = cells_of(m, la, lb)
-        flops = cells * FLOP_PER_CELL[m]
         report[m] = {
             "pairs_per_s": args.pairs * steps / el,
             "ms_per_step": 1e3 * el / steps,
             "kernel_ms": kms,
             "cells_per_s": cells / (kms * 1e-3),
-            "roofline_frac": flops / (kms * 1e-3) / (PEAK_FP32_TFLOPS * 1e12),
         }
+        if m in FLOP_PER_CELL:
+            report[m]["roofline_frac"] = cells * FLOP_PER_CELL[m] / (kms * 1e-3) / (PEAK_FP32_TFLOPS * 1e12)
+        else:  # CDDS: FP64, mostly a time-overlap test per segment pair; no FLOP roofline
+            report[m]["unit"] = "segment pairs/s"
         print(f"[bench] {m}: {report[m]}", file=sys.stderr, flush=True)
 
     el, kms, nl = timed("dtw", args.steps, args.warmup)
