@@ -27,6 +27,8 @@
  *   - Results are float64 like the reference's (core/dtw.py:70,76); EDR and
  *     LCSS also report the integer DP count.
  *   - Trajectory lengths must be >= 1.
+ *   - A pair whose trajectory index lies outside [0, n) is not computed: its
+ *     result is NaN (count -1); no memory outside the set is read.
  */
 #ifndef TSM_H
 #define TSM_H

@@ -243,3 +243,35 @@ def test_full_size_six_measures(gpu):
         got, cnt = _gpu(m, d, pa, pb, eps=EPS, gpoint=g)
         _check(m, ts, pa, pb, got, cnt, gpoint=g)
         print(f"  {m} ok", flush=True)
+
+
+def test_out_of_range_pairs_give_nan(gpu):
+    """A pair index outside [0, n) (caller error that skipped validation)
+    yields NaN (count -1) for that pair only; the kernels check the indices
+    before any access."""
+    from trajsimimeasures_amd.data import with_timestamps
+    ts, pa, pb = uniform_pairs(300, lo=5, hi=80, seed=61)
+    ts = with_timestamps(np.random.default_rng(62), ts)
+    n = ts.n
+    pa = pa.astype(np.int32).copy()
+    pb = pb.astype(np.int32).copy()
+    bad = np.array([0, 7, 150, 299])
+    pa[bad[:2]] = [n, -1]
+    pb[bad[2:]] = [n + 1000, -5]
+    good = np.setdiff1d(np.arange(pa.size), bad)
+    d = _dev(ts)
+    for m in ALL + ["stedr", "cdds"]:
+        print(f"[case] out-of-range pairs measure={m}", flush=True)
+        out = _engine().pairs(m, d, pa, pb, eps=EPS, delta=60.0, validate=False,
+                              return_counts=m in ("edr", "lcss", "stedr"))
+        got, cnt = (out[0].cpu().numpy(), out[1].cpu().numpy()) if isinstance(out, tuple) else (out.cpu().numpy(), None)
+        assert np.all(np.isnan(got[bad])), m
+        assert not np.any(np.isnan(got[good])), m
+        if cnt is not None:
+            assert np.all(cnt[bad] == -1), m
+        if m in ALL:
+            ref, _ = O.pairs(m, ts.xy, ts.offsets, pa[good], pb[good], eps=EPS)
+            if m in ("edr", "lcss"):
+                assert np.array_equal(got[good], ref)
+            else:
+                assert np.max(np.abs(got[good] - ref) / np.maximum(ref, 1e-300)) <= FP32_RTOL

@@ -128,6 +128,7 @@ int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &p
         h.offs = set->offsets;
         h.ps = ps;
         h.npairs = npairs;
+        h.nset = set->n;
         h.out = out;
         h.seg = nullptr;
         h.rseg = nullptr;
@@ -157,6 +158,7 @@ int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &p
         c.offs = set->offsets;
         c.ps = ps;
         c.npairs = npairs;
+        c.nset = set->n;
         c.eps = prm.eps;
         c.out = out;
         int grid = (int)(npairs < 1048576 ? npairs : 1048576);

@@ -30,6 +30,7 @@ struct HArgs {
     const int64_t *offs;
     PairSpace ps;
     int64_t npairs;
+    int64_t nset;  // trajectories in the set
     double *out;
 };
 
@@ -156,6 +157,10 @@ __global__ __launch_bounds__(kWave) void hausdorff_kernel(HArgs a) {
     for (int64_t p = blockIdx.x; p < a.npairs; p += gridDim.x) {
         int64_t ia, ib;
         decode_pair(a.ps, p, ia, ib);
+        if (ia < 0 || ia >= a.nset || ib < 0 || ib >= a.nset) {  // index outside the set: NaN, no access
+            if (lane == 0) a.out[p] = __builtin_bit_cast(double, 0x7ff8000000000000ull);
+            continue;
+        }
         const int64_t oa = a.offs[ia], ob = a.offs[ib];
         const int na = (int)(a.offs[ia + 1] - oa), nb = (int)(a.offs[ib + 1] - ob);
         T m = 0;

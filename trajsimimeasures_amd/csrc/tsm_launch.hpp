@@ -76,6 +76,9 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     pa.meta = c.meta.as<int64_t>();
     pa.counter = c.counter.as<unsigned int>();
     pa.done = c.counter.as<unsigned int>() + 1;
+    pa.nset = set.n;
+    pa.out = out;
+    pa.counts = counts;
     pa.nb = NB;
     pa.tab = tab;
 

@@ -33,6 +33,9 @@ struct PlanArgs {
     int64_t *meta;      // see DPArgs::meta
     unsigned int *counter;
     unsigned int *done;  // histogram blocks finished; zero between calls
+    int64_t nset;        // trajectories in the set: explicit pair indices must lie in [0, nset)
+    double *out;         // results: NaN for a pair with an index outside the set
+    int32_t *counts;     // EDR/LCSS counts (or null): -1 for such a pair
     int nb;
     BucketTable tab;
 };
@@ -51,8 +54,13 @@ __device__ __forceinline__ int bin_of(const PlanArgs &a, int64_t na, int64_t nb)
 // rather than two per pair.
 constexpr int kPer = kPairsPerBlock / kPlanBlock;  // pairs per thread
 
+// bins[k] = -1 past p1, kBadPair for an explicit pair with an index outside
+// the set (it is never scheduled; the scatter writes its NaN result).
+constexpr int kBadPair = -2;
+
 __device__ __forceinline__ void pair_bins(const PlanArgs &a, int64_t p0, int64_t p1, int (&bins)[kPer]) {
     int64_t ia[kPer], ib[kPer];
+    bool bad[kPer];
     if (a.ps.mode == kExplicit) {
 #pragma unroll
         for (int k = 0; k < kPer; ++k) {
@@ -68,6 +76,11 @@ __device__ __forceinline__ void pair_bins(const PlanArgs &a, int64_t p0, int64_t
             decode_pair(a.ps, p < p1 ? p : p0, ia[k], ib[k]);
         }
     }
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        bad[k] = ia[k] < 0 || ia[k] >= a.nset || ib[k] < 0 || ib[k] >= a.nset;
+        if (bad[k]) ia[k] = ib[k] = 0;
+    }
     int64_t na[kPer], nb[kPer];
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
@@ -77,7 +90,7 @@ __device__ __forceinline__ void pair_bins(const PlanArgs &a, int64_t p0, int64_t
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         const int64_t p = p0 + threadIdx.x + (int64_t)k * kPlanBlock;
-        bins[k] = p < p1 ? bin_of(a, na[k], nb[k]) : -1;
+        bins[k] = p >= p1 ? -1 : bad[k] ? kBadPair : bin_of(a, na[k], nb[k]);
     }
 }
 
@@ -217,7 +230,14 @@ static __global__ __launch_bounds__(kPlanBlock) void plan_scatter_kernel(PlanArg
     uint32_t rank[kPer];
     pair_bins(a, p0, p1, bins);
 #pragma unroll
-    for (int k = 0; k < kPer; ++k) rank[k] = bins[k] >= 0 ? atomicAdd(&cnt[bins[k]], 1u) : 0u;
+    for (int k = 0; k < kPer; ++k) {
+        rank[k] = bins[k] >= 0 ? atomicAdd(&cnt[bins[k]], 1u) : 0u;
+        if (bins[k] == kBadPair) {
+            const int64_t p = p0 + threadIdx.x + (int64_t)k * kPlanBlock;
+            a.out[p] = __builtin_bit_cast(double, 0x7ff8000000000000ull);  // quiet NaN
+            if (a.counts) a.counts[p] = -1;
+        }
+    }
     __syncthreads();
     // reserve this block's range in every bin it uses: all of a thread's
     // atomics are issued before any result is waited for
