@@ -29,6 +29,9 @@
  *   - Trajectory lengths must be >= 1.
  *   - A pair whose trajectory index lies outside [0, n) is not computed: its
  *     result is NaN (count -1); no memory outside the set is read.
+ *   - SAR uses eps as distance_eps (a sample pair matches when its distance
+ *     is <= eps) and time_step / target_length / replicas; timestamps must
+ *     not be NaN.
  */
 #ifndef TSM_H
 #define TSM_H
@@ -39,7 +42,7 @@
 extern "C" {
 #endif
 
-#define TSM_ABI_VERSION 2
+#define TSM_ABI_VERSION 3
 
 /* Measure ids. */
 #define TSM_DTW 0       /* core/dtw.py:10-77 */
@@ -51,6 +54,7 @@ extern "C" {
 #define TSM_STEDR 6     /* core/stedr.py:14-86: EDR whose match also needs |t1 - t2| < delta */
 #define TSM_STLCSS 7    /* core/stlcss.py:10-80: LCSS whose match also needs |t1 - t2| < delta */
 #define TSM_CDDS 8      /* core/cdds.py:11-124: time the moving points stay within eps (needs t64) */
+#define TSM_SAR 9       /* core/sar.py:10-286: time within eps on a common time grid, binned (needs t64) */
 
 /* Error codes. */
 #define TSM_OK 0
@@ -89,6 +93,12 @@ typedef struct tsm_params {
     int lanes;      /* lanes per pair: 0 = autotuned per column count, else 4 or 8 (threads_per_traj hint) */
     int precision;  /* TSM_PREC_FP32 or TSM_PREC_FP64 */
     double delta;   /* STEDR/STLCSS time threshold (core/stedr.py:62 `delta`): match needs |t1 - t2| < delta */
+    int64_t time_step;     /* SAR time slice (core/sar.py:247 `time_eps`), a positive integer; default 10 */
+    int32_t target_length; /* SAR bins (core/sar.py:248 `target_length`), >= 1; default 10 */
+    int32_t replicas;      /* SAR: the reference's threads_per_traj (core/sar.py:248, default 64).  Each of
+                              its threads stores the same per-sample count when no binning happens
+                              (core/sar.py:177-181, 236, 285), so that result is replicas * count *
+                              time_step; 1 gives the single count.  Binned results do not depend on it. */
 } tsm_params;
 
 /* Library / device setup (reference: numba.cuda context, implicit). */

@@ -67,6 +67,10 @@ def lib():
         L.tsmo_cdds.restype = d
         L.tsmo_pairs_cdds.argtypes = [_P, _P, _P, _P, _P, i64, d, _P, ctypes.c_int]
         L.tsmo_pairs_cdds.restype = ctypes.c_int
+        L.tsmo_sar.argtypes = [_P, _P, i64, _P, _P, i64, d, i64, i64, i64]
+        L.tsmo_sar.restype = d
+        L.tsmo_pairs_sar.argtypes = [_P, _P, _P, _P, _P, i64, d, i64, i64, i64, _P, ctypes.c_int]
+        L.tsmo_pairs_sar.restype = ctypes.c_int
         L.tsmo_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -182,4 +186,28 @@ def pairs_cdds(xy, t, offsets, pa, pb, eps: float, threads: int = 0):
     out = np.empty(pa.size, dtype=np.float64)
     lib().tsmo_pairs_cdds(xy.ctypes.data, t.ctypes.data, offsets.ctypes.data, pa.ctypes.data, pb.ctypes.data,
                           pa.size, float(eps), out.ctypes.data, int(threads))
+    return out
+
+
+def sar_single(a, ta, b, tb, distance_eps: float, time_eps: int, target_length: int,
+               threads_per_traj: int = 64) -> float:
+    """SAR of one pair, the reference GPU form (core/sar.py:62-212)."""
+    a, b = _arr(a), _arr(b)
+    ta = np.ascontiguousarray(np.asarray(ta, dtype=np.float64).reshape(-1))
+    tb = np.ascontiguousarray(np.asarray(tb, dtype=np.float64).reshape(-1))
+    return lib().tsmo_sar(a.ctypes.data, ta.ctypes.data, len(a), b.ctypes.data, tb.ctypes.data, len(b),
+                          float(distance_eps), int(time_eps), int(target_length), int(threads_per_traj))
+
+
+def pairs_sar(xy, t, offsets, pa, pb, distance_eps: float, time_eps: int, target_length: int,
+              threads_per_traj: int = 64, threads: int = 0):
+    xy = np.ascontiguousarray(np.asarray(xy, dtype=np.float64).reshape(-1, 2))
+    t = np.ascontiguousarray(np.asarray(t, dtype=np.float64).reshape(-1))
+    offsets = np.ascontiguousarray(offsets, dtype=np.int64)
+    pa = np.ascontiguousarray(pa, dtype=np.int64)
+    pb = np.ascontiguousarray(pb, dtype=np.int64)
+    out = np.empty(pa.size, dtype=np.float64)
+    lib().tsmo_pairs_sar(xy.ctypes.data, t.ctypes.data, offsets.ctypes.data, pa.ctypes.data, pb.ctypes.data,
+                         pa.size, float(distance_eps), int(time_eps), int(target_length), int(threads_per_traj),
+                         out.ctypes.data, int(threads))
     return out

@@ -22,6 +22,8 @@
  *             (CPU forms: core_cpu/stedr.pyx:15-42, core_cpu/stlcss.pyx:15-39)
  *   CDDS      core/cdds.py:11-94            sum over all segment pairs of the time the two
  *             moving points are within eps (CPU form: core_cpu/cdds.pyx)
+ *   SAR       core/sar.py:10-212            time-aligned, binned comparison of the overlapping
+ *             stretch (CPU form: core_cpu/sar.pyx)
  *
  * Distances are sqrt(dx*dx + dy*dy) evaluated without FMA contraction
  * (core/utils.py:8-17); build with -ffp-contract=off.
@@ -48,7 +50,7 @@
 #endif
 
 enum { TSMO_DTW = 0, TSMO_EDR = 1, TSMO_LCSS = 2, TSMO_ERP = 3, TSMO_FRECHET = 4, TSMO_HAUSDORFF = 5,
-       TSMO_STEDR = 6, TSMO_STLCSS = 7, TSMO_CDDS = 8 };
+       TSMO_STEDR = 6, TSMO_STLCSS = 7, TSMO_CDDS = 8, TSMO_SAR = 9 };
 
 static inline double sqd(const double *p, const double *q) {
     double dx = p[0] - q[0];
@@ -439,6 +441,212 @@ int tsmo_pairs_cdds(const double *xy, const double *t, const int64_t *offsets, c
         const int64_t oa = offsets[pa[k]], ob = offsets[pb[k]];
         out[k] = tsmo_cdds(xy + 2 * oa, t + oa, offsets[pa[k] + 1] - oa, xy + 2 * ob, t + ob,
                            offsets[pb[k] + 1] - ob, eps);
+    }
+    return 0;
+}
+
+/* ---- SAR (core/sar.py) ----------------------------------------------------
+ * The reference GPU form, step by step:
+ *  1. sar_preprocess (core/sar.py:62-103): a merge over both trajectories'
+ *     segments in order of end time finds the first and last segments that
+ *     overlap the other trajectory in time (open intervals, zero-duration
+ *     segments excluded); no overlap gives 0.
+ *  2. align_to_timestamps (core/sar.py:31-59, 10-28): the overlapping stretch
+ *     of each trajectory is sampled at every multiple of time_eps, segment by
+ *     segment (grid indices ceil(t_k/time_eps) .. ceil(t_k+1/time_eps) - 1,
+ *     positions interpolated as x = xk*ts + xb), plus the stretch's last point
+ *     when its time is a multiple of time_eps.  The samples are stored in
+ *     FP32 (interpolated_points_g is float32, core/sar.py:271).
+ *  3. Trimming (core/sar.py:118-150): the later end loses
+ *     (difference // time_eps) samples, then the earlier start does; the time
+ *     differences are FP32 (the samples' type) and // is Python's floor
+ *     division.
+ *  4. sar_compute (core/sar.py:163-212): with sz samples left and
+ *     target_length M >= sz, count the sample pairs within distance_eps (FP32
+ *     differences, sqrt <= distance_eps) times time_eps; every one of the
+ *     reference's threads_per_traj threads computes and stores that same
+ *     count and the host sums them (core/sar.py:236,285), so the result is
+ *     threads_per_traj * count * time_eps.  With M < sz: M bins, bin b
+ *     covering samples [b*div + min(b, mod), (b+1)*div + min(b+1, mod) + 1)
+ *     (div, mod = sz // M, sz % M; note the "+ 1", core/sar.py:195), FP64
+ *     means in sample order; a bin whose means are within distance_eps adds
+ *     its sample count times time_eps.
+ * Two reads of the reference fall outside the samples it holds, and are
+ * defined here: the last bin's "+ 1" sample lies past the trimmed stretch
+ * (uninitialised device memory for at least one side), so the last bin ends
+ * at sz; and when the two trimmed stretches differ in length (which needs
+ * timestamps past 2^24 or decreasing ones) the shorter length is used.
+ * A pair with no overlap returns 0 (the reference leaves its slot
+ * unwritten).  A stretch of more than 2^24 samples gives NaN. */
+typedef struct { float x, y, t; } sar_pt;
+
+#define SAR_MAX_SAMPLES (1ll << 24)
+
+/* float -> int64 as the conversion of the reference's math.ceil / int():
+ * truncation, saturating, NaN -> 0 */
+static int64_t sar_to_i64(double v) {
+    if (v != v) return 0;
+    if (v >= 9223372036854775807.0) return INT64_MAX;
+    if (v <= -9223372036854775808.0) return INT64_MIN;
+    return (int64_t)v;
+}
+
+/* Python's float floor division a // b (the algorithm of CPython's
+ * float_floor_div, which Numba follows): fmod first, so that an exact
+ * multiple never rounds down. */
+static double sar_floordiv(double a, double b) {
+    double mod = fmod(a, b);
+    double div = (a - mod) / b;
+    if (mod != 0.0 && ((b < 0.0) != (mod < 0.0))) div -= 1.0;
+    double fl;
+    if (div != 0.0) {
+        fl = floor(div);
+        if (div - fl > 0.5) fl += 1.0;
+    } else {
+        fl = copysign(0.0, a / b);
+    }
+    return fl;
+}
+
+/* align_to_timestamps over points [0, n) of (xy, t); writes up to cap
+ * samples to out (NULL: count only); returns the sample count. */
+static int64_t sar_align(const double *xy, const double *t, int64_t n, int64_t ts, sar_pt *out, int64_t cap) {
+    const double tsd = (double)ts;
+    int64_t c = 0;
+    for (int64_t i = 0; i + 1 < n; ++i) {
+        const double t1 = t[i], t2 = t[i + 1];
+        const int64_t s = sar_to_i64(ceil(t1 / tsd)), e = sar_to_i64(ceil(t2 / tsd));
+        if (s >= e) continue;
+        if (e - s > cap - c) return cap + 1;
+        const double x1 = xy[2 * i], y1 = xy[2 * i + 1], x2 = xy[2 * i + 2], y2 = xy[2 * i + 3];
+        const double yk = (y2 - y1) / (t2 - t1), yb = (y1 * t2 - y2 * t1) / (t2 - t1);
+        const double xk = (x2 - x1) / (t2 - t1), xb = (x1 * t2 - x2 * t1) / (t2 - t1);
+        for (int64_t g = s; g < e; ++g, ++c) {
+            if (!out) continue;
+            const int64_t tt = g * ts;
+            out[c].x = (float)(xk * (double)tt + xb);
+            out[c].y = (float)(yk * (double)tt + yb);
+            out[c].t = (float)tt;
+        }
+    }
+    if (fmod(t[n - 1], tsd) == 0.0) {
+        if (c + 1 > cap) return cap + 1;
+        if (out) {
+            out[c].x = (float)xy[2 * (n - 1)];
+            out[c].y = (float)xy[2 * (n - 1) + 1];
+            out[c].t = (float)t[n - 1];
+        }
+        ++c;
+    }
+    return c;
+}
+
+static int64_t min_i64(int64_t a, int64_t b) { return a < b ? a : b; }
+
+double tsmo_sar(const double *a, const double *ta, int64_t n1, const double *b, const double *tb, int64_t n2,
+                double deps, int64_t ts, int64_t M, int64_t R) {
+    /* 1. overlap merge (core/sar.py:79-100) */
+    int64_t i = 0, j = 0, is = -1, ie = -1, js = -1, je = -1;
+    while (i < n1 - 1 && j < n2 - 1) {
+        const double a0 = ta[i], a1 = ta[i + 1], b0 = tb[j], b1 = tb[j + 1];
+        if (!(a1 <= b0 || b1 <= a0 || a0 == a1 || b0 == b1)) {
+            if (is < 0) is = i;
+            if (js < 0) js = j;
+            ie = i + 1;
+            je = j + 1;
+        }
+        if (a1 < b1) ++i;
+        else if (b1 < a1) ++j;
+        else { ++i; ++j; }
+    }
+    if (is < 0) return 0.0;
+    /* 2. samples of both stretches */
+    const int64_t c1 = sar_align(a + 2 * is, ta + is, ie - is + 1, ts, NULL, SAR_MAX_SAMPLES);
+    const int64_t c2 = sar_align(b + 2 * js, tb + js, je - js + 1, ts, NULL, SAR_MAX_SAMPLES);
+    if (c1 > SAR_MAX_SAMPLES || c2 > SAR_MAX_SAMPLES) return NAN;
+    if (c1 == 0 || c2 == 0) return 0.0;
+    sar_pt *A = (sar_pt *)malloc(sizeof(sar_pt) * (size_t)(c1 + c2));
+    if (!A) return NAN;
+    sar_pt *B = A + c1;
+    sar_align(a + 2 * is, ta + is, ie - is + 1, ts, A, c1);
+    sar_align(b + 2 * js, tb + js, je - js + 1, ts, B, c2);
+    /* 3. trimming (core/sar.py:118-150) */
+    const float as = A[0].t, ae = A[c1 - 1].t, bs = B[0].t, be = B[c2 - 1].t;
+    const double tsd = (double)ts;
+    int64_t s1 = 0, e1 = c1, s2 = 0, e2 = c2, z1 = c1, z2 = c2;
+    if (ae == be) {
+    } else if (ae < be) {
+        const int64_t k = sar_to_i64(sar_floordiv((double)(float)(be - ae), tsd));
+        e2 -= min_i64(k, z2);
+        z2 = e2 - s2;
+    } else if (ae > be) {
+        const int64_t k = sar_to_i64(sar_floordiv((double)(float)(ae - be), tsd));
+        e1 -= min_i64(k, z1);
+        z1 = e1 - s1;
+    }
+    double result = 0.0;
+    if (z1 != 0 && z2 != 0) {
+        if (as == bs) {
+        } else if (as < bs) {
+            const int64_t k = sar_to_i64(sar_floordiv((double)(float)(bs - as), tsd));
+            s1 += min_i64(k, c1);
+            z1 = e1 - s1;
+        } else if (as > bs) {
+            const int64_t k = sar_to_i64(sar_floordiv((double)(float)(as - bs), tsd));
+            s2 += min_i64(k, c2);
+            z2 = e2 - s2;
+        }
+        const int64_t sz = min_i64(z1, z2);
+        /* 4. comparison (core/sar.py:163-212) */
+        if (sz > 0 && M >= sz) {
+            int64_t cnt = 0;
+            for (int64_t m = 0; m < sz; ++m) {
+                const float dx = B[s2 + m].x - A[s1 + m].x, dy = B[s2 + m].y - A[s1 + m].y;
+                const float d2 = dx * dx + dy * dy;
+                if ((double)(float)sqrt((double)d2) <= deps) ++cnt;
+            }
+            result = (double)(R * cnt * ts);
+        } else if (sz > 0) {
+            const int64_t dv = sz / M, md = sz % M;
+            int64_t total = 0;
+            for (int64_t bi = 0; bi < M; ++bi) {
+                const int64_t st = bi * dv + min_i64(bi, md);
+                int64_t en = (bi + 1) * dv + min_i64(bi + 1, md) + 1;
+                if (en > sz) en = sz;
+                double ax = 0.0, ay = 0.0, bx = 0.0, by = 0.0;
+                for (int64_t k = st; k < en; ++k) {
+                    ax += A[s1 + k].x;
+                    ay += A[s1 + k].y;
+                    bx += B[s2 + k].x;
+                    by += B[s2 + k].y;
+                }
+                const double cn = (double)(en - st);
+                ax /= cn;
+                ay /= cn;
+                bx /= cn;
+                by /= cn;
+                if (sqrt((bx - ax) * (bx - ax) + (by - ay) * (by - ay)) <= deps) total += (en - st) * ts;
+            }
+            result = (double)total;
+        }
+    }
+    free(A);
+    return result;
+}
+
+int tsmo_pairs_sar(const double *xy, const double *t, const int64_t *offsets, const int64_t *pa, const int64_t *pb,
+                   int64_t npairs, double deps, int64_t ts, int64_t M, int64_t R, double *out, int nthreads) {
+#ifdef _OPENMP
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+    (void)nthreads;
+#endif
+    int64_t k;
+#pragma omp parallel for schedule(static)
+    for (k = 0; k < npairs; ++k) {
+        const int64_t oa = offsets[pa[k]], ob = offsets[pb[k]];
+        out[k] = tsmo_sar(xy + 2 * oa, t + oa, offsets[pa[k] + 1] - oa, xy + 2 * ob, t + ob,
+                          offsets[pb[k] + 1] - ob, deps, ts, M, R);
     }
     return 0;
 }

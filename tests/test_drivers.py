@@ -14,7 +14,11 @@ def test_dispatch_and_collate():
     with pytest.raises(NotImplementedError):
         drivers.heuristic_fn_wrapper("dtw_cpu")
     with pytest.raises(NotImplementedError):
-        drivers.heuristic_fn_wrapper("sar")
+        drivers.heuristic_fn_wrapper("sar_cpu")
+    # SAR binds config.py:55-57 and leaves threads_per_traj at its default
+    # (method_helper.py:64-71)
+    h = drivers.heuristic_fn_wrapper("sar", threads_per_traj=16)
+    assert h.keywords == {"distance_eps": 200.0, "time_eps": 10, "target_length": 10} and h.func is drivers.core.sar
     g = drivers.heuristic_fn_wrapper("stlcss", eps=0.2)
     assert g.keywords == {"eps": 0.2, "delta": drivers.DEFAULT_DELTA, "threads_per_traj": 64}
     assert g.func is drivers.core.stlcss

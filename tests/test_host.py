@@ -28,7 +28,7 @@ def test_library_exports_every_declared_function():
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(_lib.EXPORTS)
-    assert L.tsm_abi_version() == 2
+    assert L.tsm_abi_version() == 3
     nn = header_functions("tsm_nn.h")
     for n in nn:
         assert hasattr(L, n), n
@@ -39,6 +39,9 @@ def test_params_default():
     p = _lib.ParamsC()
     _lib.lib().tsm_params_default(ctypes.byref(p))
     assert (p.eps, p.eps_mode, p.gx, p.gy, p.lanes, p.precision) == (0.0, 0, 0.0, 0.0, 0, 0)
+    # SAR: config.py:56-57 and core/sar.py:248
+    assert (p.delta, p.time_step, p.target_length, p.replicas) == (0.0, 10, 10, 64)
+    assert ctypes.sizeof(_lib.ParamsC) == 64
 
 
 def test_argument_errors_without_device():
@@ -70,6 +73,29 @@ def test_argument_errors_without_device():
         _lib.check(-1)
 
 
+def test_sar_argument_errors_without_device():
+    """SAR needs timestamps and positive time_step / target_length /
+    replicas; the checks run before any device work (the pointers here are
+    never dereferenced)."""
+    L = _lib.lib()
+    fake = ctypes.c_void_p(16)
+    s = _lib.TrajSetC(fake, fake, fake, 5, 5, 1, 0.0, None)
+    prm = _lib.params(0.1)
+    rc = L.tsm_pairs(_lib.SAR, ctypes.byref(s), fake, fake, 1, ctypes.byref(prm), fake, None, None)
+    assert rc == -1 and b"timestamps" in L.tsm_last_error()
+    s.t64 = fake
+    for field in ("time_step", "target_length", "replicas"):
+        bad = _lib.params(0.1)
+        setattr(bad, field, 0)
+        rc = L.tsm_pairs(_lib.SAR, ctypes.byref(s), fake, fake, 1, ctypes.byref(bad), fake, None, None)
+        assert rc == -1 and b"SAR" in L.tsm_last_error(), field
+    with pytest.raises(ValueError):
+        _lib.params(0.1, time_step=2.5)
+    with pytest.raises(ValueError):
+        _lib.params(0.1, target_length=0)
+    assert _lib.params(0.1, time_step=30.0, target_length=4, replicas=1).time_step == 30
+
+
 def test_params_validation():
     with pytest.raises(ValueError):
         _lib.params(eps_mode="cube")
@@ -78,7 +104,7 @@ def test_params_validation():
     with pytest.raises(ValueError):
         _lib.params(gpoint=[1, 2, 3])
     with pytest.raises(ValueError):
-        _lib.measure_id("sar")
+        _lib.measure_id("RSTS")
     assert _lib.measure_id("dfrechet") == _lib.FRECHET
 
 

@@ -13,13 +13,14 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.environ.get("TSM_LIB", os.path.join(PKG, "libtsm.so"))
 
 # Measure ids (tsm.h)
-DTW, EDR, LCSS, ERP, FRECHET, HAUSDORFF, STEDR, STLCSS, CDDS = 0, 1, 2, 3, 4, 5, 6, 7, 8
+DTW, EDR, LCSS, ERP, FRECHET, HAUSDORFF, STEDR, STLCSS, CDDS, SAR = 0, 1, 2, 3, 4, 5, 6, 7, 8, 9
 MEASURES = {"dtw": DTW, "edr": EDR, "lcss": LCSS, "erp": ERP, "frechet": FRECHET, "dfrechet": FRECHET,
-            "discret_frechet": FRECHET, "hausdorff": HAUSDORFF, "stedr": STEDR, "stlcss": STLCSS, "cdds": CDDS}
+            "discret_frechet": FRECHET, "hausdorff": HAUSDORFF, "stedr": STEDR, "stlcss": STLCSS, "cdds": CDDS,
+            "sar": SAR}
 MEASURE_NAMES = {DTW: "dtw", EDR: "edr", LCSS: "lcss", ERP: "erp", FRECHET: "frechet", HAUSDORFF: "hausdorff",
-                 STEDR: "stedr", STLCSS: "stlcss", CDDS: "cdds"}
+                 STEDR: "stedr", STLCSS: "stlcss", CDDS: "cdds", SAR: "sar"}
 COUNTED = (EDR, LCSS, STEDR, STLCSS)  # measures with integer match counts
-TIMED = (STEDR, STLCSS, CDDS)  # measures that need per-point timestamps
+TIMED = (STEDR, STLCSS, CDDS, SAR)  # measures that need per-point timestamps
 
 EPS_SQRT, EPS_SQUARED = 0, 1
 PREC_FP32, PREC_FP64 = 0, 1
@@ -54,7 +55,8 @@ class TrajSetC(ctypes.Structure):
 class ParamsC(ctypes.Structure):
     _fields_ = [("eps", ctypes.c_double), ("eps_mode", ctypes.c_int), ("gx", ctypes.c_double),
                 ("gy", ctypes.c_double), ("lanes", ctypes.c_int), ("precision", ctypes.c_int),
-                ("delta", ctypes.c_double)]
+                ("delta", ctypes.c_double), ("time_step", ctypes.c_int64), ("target_length", ctypes.c_int32),
+                ("replicas", ctypes.c_int32)]
 
 
 _lock = threading.Lock()
@@ -117,11 +119,25 @@ def check(rc: int):
 
 
 def params(eps: float = 0.0, eps_mode="sqrt", gpoint=None, lanes: int = 0, precision="fp32",
-           delta: float = 0.0) -> ParamsC:
+           delta: float = 0.0, time_step=None, target_length=None, replicas=None) -> ParamsC:
     p = ParamsC()
     lib().tsm_params_default(ctypes.byref(p))
     p.eps = float(eps)
     p.delta = float(delta)
+    # SAR (core/sar.py:247-248): time_eps must be a whole number of seconds
+    # (the reference steps a Python range by it)
+    if time_step is not None:
+        if float(time_step) != int(time_step) or int(time_step) < 1:
+            raise ValueError("time_step (time_eps) must be a positive integer")
+        p.time_step = int(time_step)
+    if target_length is not None:
+        if int(target_length) != target_length or not 1 <= int(target_length) < 2 ** 31:
+            raise ValueError("target_length must be an integer in [1, 2^31)")
+        p.target_length = int(target_length)
+    if replicas is not None:
+        if int(replicas) != replicas or not 1 <= int(replicas) < 2 ** 31:
+            raise ValueError("threads_per_traj must be an integer in [1, 2^31)")
+        p.replicas = int(replicas)
     if isinstance(eps_mode, str):
         if eps_mode not in ("sqrt", "squared"):
             raise ValueError("eps_mode must be 'sqrt' or 'squared'")

@@ -25,7 +25,7 @@ def lanes_hint(threads_per_traj) -> int:
 
 
 def run_padded(measure: int, t1, len1, t2, len2, *, eps=0.0, eps_mode="sqrt", gpoint=None, threads_per_traj=64,
-               precision="fp32", return_counts=False, delta=0.0):
+               precision="fp32", return_counts=False, delta=0.0, sar=None):
     t1 = np.asarray(t1, dtype=np.float64)
     t2 = np.asarray(t2, dtype=np.float64)
     st = measure in _lib.TIMED
@@ -52,7 +52,10 @@ def run_padded(measure: int, t1, len1, t2, len2, *, eps=0.0, eps_mode="sqrt", gp
     pb = pa + np.int32(B)
     out = np.empty(B, dtype=np.float64)
     cnt = np.empty(B, dtype=np.int32) if return_counts else None
-    prm = _lib.params(eps, eps_mode, gpoint, lanes_hint(threads_per_traj), precision, delta)
+    if sar is not None:  # (time_eps, target_length, threads_per_traj) of core/sar.py:247-248
+        prm = _lib.params(eps, time_step=sar[0], target_length=sar[1], replicas=sar[2])
+    else:
+        prm = _lib.params(eps, eps_mode, gpoint, lanes_hint(threads_per_traj), precision, delta)
     L = _lib.lib()
     _lib.check(L.tsm_pairs_host(measure, xy.ctypes.data, tt.ctypes.data if st else None, offs.ctypes.data, 2 * B,
                                 xy.shape[0], pa.ctypes.data,

@@ -13,6 +13,7 @@
 #include <cstring>
 
 #include "tsm_cdds.hpp"
+#include "tsm_sar.hpp"
 #include "tsm_hausdorff.hpp"
 #include "tsm_host.hpp"
 
@@ -90,7 +91,7 @@ int erp_prologue(Workspace &c, const tsm_trajset &set, double gx, double gy, hip
 // overridden by prm_in.
 int validate(int measure, const tsm_trajset *set, int64_t npairs, const tsm_params *prm_in, const double *out,
              tsm_params &prm) {
-    if (measure < TSM_DTW || measure > TSM_CDDS) return fail(TSM_E_ARG, "unknown measure id " + std::to_string(measure));
+    if (measure < TSM_DTW || measure > TSM_SAR) return fail(TSM_E_ARG, "unknown measure id " + std::to_string(measure));
     if (!set) return fail(TSM_E_ARG, "null trajectory set");
     if (npairs < 0) return fail(TSM_E_ARG, "negative pair count");
     if (npairs > (int64_t)0xFFFFFFFEll) return fail(TSM_E_ARG, "at most 2^32-2 pairs per call");
@@ -106,8 +107,13 @@ int validate(int measure, const tsm_trajset *set, int64_t npairs, const tsm_para
         return fail(TSM_E_ARG, "precision must be TSM_PREC_FP32 or TSM_PREC_FP64");
     if (prm.eps_mode != TSM_EPS_SQRT && prm.eps_mode != TSM_EPS_SQUARED)
         return fail(TSM_E_ARG, "eps_mode must be TSM_EPS_SQRT or TSM_EPS_SQUARED");
-    const bool st = measure == TSM_STEDR || measure == TSM_STLCSS || measure == TSM_CDDS;
-    if (st && !set->t64) return fail(TSM_E_ARG, "STEDR/STLCSS/CDDS need per-point timestamps (t64)");
+    const bool st = measure == TSM_STEDR || measure == TSM_STLCSS || measure == TSM_CDDS || measure == TSM_SAR;
+    if (st && !set->t64) return fail(TSM_E_ARG, "STEDR/STLCSS/CDDS/SAR need per-point timestamps (t64)");
+    if (measure == TSM_SAR) {
+        if (prm.time_step < 1) return fail(TSM_E_ARG, "SAR time_step must be a positive integer");
+        if (prm.target_length < 1) return fail(TSM_E_ARG, "SAR target_length must be >= 1");
+        if (prm.replicas < 1) return fail(TSM_E_ARG, "SAR replicas (threads_per_traj) must be >= 1");
+    }
     const bool needs64 = f64 || st || measure == TSM_EDR || measure == TSM_LCSS || measure == TSM_ERP;
     if (needs64 && !set->xy64) return fail(TSM_E_ARG, "this measure needs the FP64 coordinates (xy64)");
     if (!f64 && !st && measure != TSM_EDR && measure != TSM_LCSS && !set->xy32)
@@ -164,6 +170,25 @@ int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &p
         int grid = (int)(npairs < 1048576 ? npairs : 1048576);
         ProfScope p_(st);
         hipLaunchKernelGGL(cdds_kernel, dim3(grid), dim3(kWave), 0, st, c);
+        TSM_HIP(hipGetLastError());
+        return TSM_OK;
+    }
+    if (measure == TSM_SAR) {
+        SArgs s;
+        s.xy = set->xy64;
+        s.t = set->t64;
+        s.offs = set->offsets;
+        s.ps = ps;
+        s.npairs = npairs;
+        s.nset = set->n;
+        s.deps = prm.eps;
+        s.ts = prm.time_step;
+        s.M = prm.target_length;
+        s.R = prm.replicas;
+        s.out = out;
+        int grid = (int)(npairs < 1048576 ? npairs : 1048576);
+        ProfScope p_(st);
+        hipLaunchKernelGGL(sar_kernel, dim3(grid), dim3(kWave), 0, st, s);
         TSM_HIP(hipGetLastError());
         return TSM_OK;
     }
@@ -238,6 +263,9 @@ void tsm_params_default(tsm_params *p) {
     p->lanes = 0;
     p->precision = TSM_PREC_FP32;
     p->delta = 0.0;
+    p->time_step = 10;     // config.py:56 trajsimi_sar_time_eps
+    p->target_length = 10; // config.py:57 trajsimi_sar_target_length
+    p->replicas = 64;      // core/sar.py:248 threads_per_traj default
 }
 
 int tsm_init(int device) {

@@ -25,16 +25,19 @@ from . import _lib, core
 from .data import TrajSet, pack_list
 
 __all__ = ["heuristic_fn_wrapper", "heuristic_colllate_fn", "is_heuristic", "pairwise", "query_db", "all_pairs",
-           "symmetric_from_triu", "fp64_drift", "DEFAULT_EPS", "DEFAULT_DELTA"]
+           "symmetric_from_triu", "fp64_drift", "DEFAULT_EPS", "DEFAULT_DELTA", "DEFAULT_SAR"]
 
 DEFAULT_EPS = 0.0025  # EDR/LCSS epsilon of the 100k-pair workload
 DEFAULT_DELTA = 60.0  # STEDR/STLCSS time threshold, seconds (config.py:53)
+# SAR (distance_eps, time_eps, target_length, threads_per_traj): config.py:55-57
+# and core/sar.py:248 (the reference's wrapper leaves threads_per_traj at 64)
+DEFAULT_SAR = (200.0, 10, 10, 64)
 
 _GPU_FNS = {"dtw": core.dtw, "edr": core.edr, "erp": core.erp, "frechet": core.dfrechet,
             "hausdorff": core.hausdorff, "lcss": core.lcss, "stedr": core.stedr, "stlcss": core.stlcss,
-            "cdds": core.cdds}
+            "cdds": core.cdds, "sar": core.sar}
 _MEASURE = {"dtw": "dtw", "edr": "edr", "erp": "erp", "frechet": "frechet", "hausdorff": "hausdorff",
-            "lcss": "lcss", "stedr": "stedr", "stlcss": "stlcss", "cdds": "cdds"}
+            "lcss": "lcss", "stedr": "stedr", "stlcss": "stlcss", "cdds": "cdds", "sar": "sar"}
 _ST = ("stedr", "stlcss")
 
 
@@ -45,14 +48,17 @@ def is_heuristic(fn_name: str) -> bool:
 
 
 def heuristic_fn_wrapper(fn_name: str, eps: float = DEFAULT_EPS, threads_per_traj: int = 64,
-                         delta: float = DEFAULT_DELTA):
+                         delta: float = DEFAULT_DELTA, sar=DEFAULT_SAR):
     """Name -> batch function with the reference's bound arguments
     (method_helper.py:32-73): eps for edr/lcss, eps and delta for
-    stedr/stlcss, eps for cdds, threads_per_traj for all.  The ``*_cpu`` names
-    (traj_dist, core_cpu) and SAR are not part of this engine."""
+    stedr/stlcss, eps for cdds, threads_per_traj for all but sar, and
+    distance_eps / time_eps / target_length for sar (from ``sar``).  The
+    ``*_cpu`` names (traj_dist, core_cpu) are not part of this engine."""
     fn = _GPU_FNS.get(fn_name)
     if fn is None:
         raise NotImplementedError(f"{fn_name!r} is not served by this engine (GPU measures: {sorted(_GPU_FNS)})")
+    if fn_name == "sar":
+        return partial(fn, distance_eps=sar[0], time_eps=sar[1], target_length=sar[2])
     if fn_name in ("edr", "lcss", "cdds"):
         fn = partial(fn, eps=eps)
     elif fn_name in _ST:
@@ -76,9 +82,11 @@ def heuristic_colllate_fn(batch, is_pad: bool = True, quantize_fp32: bool = Fals
     return np.copy(padded[:len(len1)]), len1, np.copy(padded[len(len1):]), len2
 
 
-def _opts(method: str, eps, gpoint, lanes, precision, delta=DEFAULT_DELTA):
+def _opts(method: str, eps, gpoint, lanes, precision, delta=DEFAULT_DELTA, sar=DEFAULT_SAR):
     if method not in _MEASURE:
         raise NotImplementedError(f"{method!r} is not served by this engine")
+    if method == "sar":
+        return dict(eps=float(sar[0]), time_step=sar[1], target_length=sar[2], replicas=sar[3])
     return dict(eps=eps if method in ("edr", "lcss", "cdds") + _ST else 0.0, gpoint=gpoint if method == "erp" else None,
                 lanes=lanes, precision=precision, delta=delta if method in _ST else 0.0)
 
@@ -93,7 +101,7 @@ def _device_set(ts: TrajSet):
 
 
 def pairwise(method: str, pairs, *, eps: float = DEFAULT_EPS, gpoint=None, lanes: int = 0, precision="fp32",
-              delta: float = DEFAULT_DELTA):
+              delta: float = DEFAULT_DELTA, sar=DEFAULT_SAR):
     """Values of ``method`` for a list of (traj, traj2) pairs
     (test_trajsimi_time.py:222-276).  Returns (list of floats,
     [running, collate, compute] seconds)."""
@@ -111,7 +119,7 @@ def pairwise(method: str, pairs, *, eps: float = DEFAULT_EPS, gpoint=None, lanes
     collate = time.time() - t0
     t1 = time.time()
     pa = np.arange(n, dtype=np.int32)
-    out = engine.pairs(method, dset, pa, pa + n, validate=False, **_opts(method, eps, gpoint, lanes, precision, delta))
+    out = engine.pairs(method, dset, pa, pa + n, validate=False, **_opts(method, eps, gpoint, lanes, precision, delta, sar))
     torch.cuda.synchronize()
     dists = out.cpu().numpy().tolist()
     comp = time.time() - t1
@@ -119,7 +127,7 @@ def pairwise(method: str, pairs, *, eps: float = DEFAULT_EPS, gpoint=None, lanes
 
 
 def query_db(method: str, Q, D, *, eps: float = DEFAULT_EPS, gpoint=None, lanes: int = 0, precision="fp32",
-              delta: float = DEFAULT_DELTA):
+              delta: float = DEFAULT_DELTA, sar=DEFAULT_SAR):
     """[|Q|, |D|] matrix of ``method`` between every query and every database
     trajectory (test_trajsimi_time_DQ.py:136-181, whose flattened order is
     query-major).  Returns (float64 ndarray, [running, collate, compute])."""
@@ -132,7 +140,7 @@ def query_db(method: str, Q, D, *, eps: float = DEFAULT_EPS, gpoint=None, lanes:
     dset = _device_set(ts)
     collate = time.time() - t0
     t1 = time.time()
-    out = engine.grid(method, dset, 0, len(Q), len(Q), len(D), **_opts(method, eps, gpoint, lanes, precision, delta))
+    out = engine.grid(method, dset, 0, len(Q), len(Q), len(D), **_opts(method, eps, gpoint, lanes, precision, delta, sar))
     torch.cuda.synchronize()
     mat = out.cpu().numpy()
     comp = time.time() - t1
@@ -152,7 +160,7 @@ def symmetric_from_triu(vals: np.ndarray, n: int) -> np.ndarray:
 
 
 def all_pairs(method: str, trajs, *, eps: float = DEFAULT_EPS, gpoint=None, lanes: int = 0, precision="fp32",
-              delta: float = DEFAULT_DELTA):
+              delta: float = DEFAULT_DELTA, sar=DEFAULT_SAR):
     """Symmetric all-pairs matrix over ``trajs`` computed on the upper
     triangle only (test_clustering_time.py:208-269).  Returns (float64
     [n, n], [running, collate, compute])."""
@@ -165,7 +173,7 @@ def all_pairs(method: str, trajs, *, eps: float = DEFAULT_EPS, gpoint=None, lane
     dset = _device_set(ts)
     collate = time.time() - t0
     t1 = time.time()
-    tri = engine.triu(method, dset, n, **_opts(method, eps, gpoint, lanes, precision, delta))
+    tri = engine.triu(method, dset, n, **_opts(method, eps, gpoint, lanes, precision, delta, sar))
     torch.cuda.synchronize()
     vals = tri.cpu().numpy()
     comp = time.time() - t1

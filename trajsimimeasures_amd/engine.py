@@ -68,11 +68,14 @@ def _outs(n: int, measure: int, device, want_counts: bool):
 
 
 def pairs(measure, dset: DeviceTrajSet, pa, pb, *, eps=0.0, eps_mode="sqrt", gpoint=None, lanes=0,
-          precision="fp32", return_counts=False, validate=True, out=None, delta=0.0):
+          precision="fp32", return_counts=False, validate=True, out=None, delta=0.0, time_step=None,
+          target_length=None, replicas=None):
     """Measure over explicit pairs (pa[k], pb[k]) of ``dset`` (pa as t1).
 
     validate=False skips the (synchronising) index range check; ``out`` may
-    supply a float64 device tensor of len(pa) to write into."""
+    supply a float64 device tensor of len(pa) to write into.  SAR reads
+    ``eps`` (distance_eps), ``time_step`` (time_eps), ``target_length`` and
+    ``replicas`` (the reference's threads_per_traj)."""
     m = _lib.measure_id(measure)
     dev = dset.xy64.device
     pa = torch.as_tensor(pa, dtype=torch.int32, device=dev).contiguous()
@@ -91,7 +94,7 @@ def pairs(measure, dset: DeviceTrajSet, pa, pb, *, eps=0.0, eps_mode="sqrt", gpo
         cnt = None
         if return_counts and m in _lib.COUNTED:
             cnt = torch.empty(n, dtype=torch.int32, device=dev)
-    prm = _lib.params(eps, eps_mode, gpoint, lanes, precision, delta)
+    prm = _lib.params(eps, eps_mode, gpoint, lanes, precision, delta, time_step, target_length, replicas)
     s = dset.cstruct()
     _lib.check(_lib.lib().tsm_pairs(m, ctypes.byref(s), pa.data_ptr(), pb.data_ptr(), n, ctypes.byref(prm),
                                     out.data_ptr(), cnt.data_ptr() if cnt is not None else None, _stream_ptr()))
@@ -99,12 +102,13 @@ def pairs(measure, dset: DeviceTrajSet, pa, pb, *, eps=0.0, eps_mode="sqrt", gpo
 
 
 def grid(measure, dset: DeviceTrajSet, q0: int, nq: int, d0: int, nd: int, *, eps=0.0, eps_mode="sqrt",
-         gpoint=None, lanes=0, precision="fp32", return_counts=False, delta=0.0):
+         gpoint=None, lanes=0, precision="fp32", return_counts=False, delta=0.0, time_step=None,
+         target_length=None, replicas=None):
     """Measure over the grid (q0 + q, d0 + d); returns [nq, nd] (row-major)."""
     m = _lib.measure_id(measure)
     dev = dset.xy64.device
     out, cnt = _outs(nq * nd, m, dev, return_counts)
-    prm = _lib.params(eps, eps_mode, gpoint, lanes, precision, delta)
+    prm = _lib.params(eps, eps_mode, gpoint, lanes, precision, delta, time_step, target_length, replicas)
     s = dset.cstruct()
     _lib.check(_lib.lib().tsm_pairs_grid(m, ctypes.byref(s), q0, nq, d0, nd, ctypes.byref(prm), out.data_ptr(),
                                          cnt.data_ptr() if cnt is not None else None, _stream_ptr()))
@@ -115,7 +119,8 @@ def grid(measure, dset: DeviceTrajSet, q0: int, nq: int, d0: int, nd: int, *, ep
 
 
 def triu(measure, dset: DeviceTrajSet, n: int | None = None, *, eps=0.0, eps_mode="sqrt", gpoint=None,
-         lanes=0, precision="fp32", return_counts=False, delta=0.0):
+         lanes=0, precision="fp32", return_counts=False, delta=0.0, time_step=None, target_length=None,
+         replicas=None):
     """Upper triangle over trajectories [0, n): n*(n-1)/2 values in
     itertools.combinations order."""
     m = _lib.measure_id(measure)
@@ -123,7 +128,7 @@ def triu(measure, dset: DeviceTrajSet, n: int | None = None, *, eps=0.0, eps_mod
     dev = dset.xy64.device
     npairs = n * (n - 1) // 2
     out, cnt = _outs(npairs, m, dev, return_counts)
-    prm = _lib.params(eps, eps_mode, gpoint, lanes, precision, delta)
+    prm = _lib.params(eps, eps_mode, gpoint, lanes, precision, delta, time_step, target_length, replicas)
     s = dset.cstruct()
     _lib.check(_lib.lib().tsm_pairs_triu(m, ctypes.byref(s), n, ctypes.byref(prm), out.data_ptr(),
                                          cnt.data_ptr() if cnt is not None else None, _stream_ptr()))
@@ -157,7 +162,7 @@ class PairsGraph:
     given."""
 
     def __init__(self, measure, dset: DeviceTrajSet, pa, pb, *, eps=0.0, eps_mode="sqrt", gpoint=None, lanes=0,
-                 precision="fp32", stream=None, delta=0.0):
+                 precision="fp32", stream=None, delta=0.0, time_step=None, target_length=None, replicas=None):
         m = _lib.measure_id(measure)
         dev = dset.xy64.device
         self.dset = dset
@@ -169,7 +174,8 @@ class PairsGraph:
         self.out = torch.empty(n, dtype=torch.float64, device=dev)
         self.counts = torch.empty(n, dtype=torch.int32, device=dev) if m in _lib.COUNTED else None
         self.stream = stream if stream is not None else torch.cuda.Stream(device=dev)
-        self._prm = _lib.params(eps, eps_mode, gpoint, lanes, precision, delta)
+        self._prm = _lib.params(eps, eps_mode, gpoint, lanes, precision, delta, time_step, target_length,
+                                replicas)
         self._set = dset.cstruct()
         self._h = ctypes.c_void_p()
         torch.cuda.current_stream().synchronize()
