@@ -14,7 +14,11 @@ cell (FMA = 2): DTW 9, Frechet 9, LCSS 9, EDR 10, ERP 11; Hausdorff 24 per
 the time test: subtract, compare, and).  The spatio-temporal measures use
 timestamps 15 +- 5 s apart from a start uniform in [0, 3600) s, delta = 60 s;
 CDDS (eps 0.0025) reports segment pairs per second and no FLOP roofline (it is
-FP64 and nearly all segment pairs stop at the time-overlap test).
+FP64 and nearly all segment pairs stop at the time-overlap test).  SAR
+(distance_eps 0.0025, time_eps 10 s, target_length 10, threads_per_traj 64:
+the reference's defaults but for distance_eps, config.py:55-57) reports
+points per second and the HBM fraction of its input bytes (timestamp and
+FP64 x, y: 24 B per point) over the kernel time.
 achieved = FLOP per launch / mean kernel time
 (HIP events on the launch stream); peak = 157.3 TFLOP/s (MI355X vector FP32).
 
@@ -50,6 +54,8 @@ PEAK_FP32_TFLOPS = 157.3
 FLOP_PER_CELL = {"dtw": 9, "frechet": 9, "lcss": 9, "edr": 10, "erp": 11, "hausdorff": 24, "stedr": 13, "stlcss": 12}
 EPS = 0.0025
 DELTA = 60.0
+SAR = dict(time_step=10, target_length=10, replicas=64)
+PEAK_HBM_GBS = 8000.0
 
 
 def cells_of(measure, la, lb):
@@ -59,6 +65,8 @@ def cells_of(measure, la, lb):
         return float(np.sum(la * np.maximum(lb - 1, 0) + lb * np.maximum(la - 1, 0)))
     if measure == "cdds":  # segment pairs
         return float(np.sum(np.maximum(la - 1, 0) * np.maximum(lb - 1, 0)))
+    if measure == "sar":  # points (the work is linear in them)
+        return float(np.sum(la + lb))
     return float(np.sum(la * lb))
 
 
@@ -79,7 +87,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--pairs", type=int, default=100_000)
     ap.add_argument("--lanes", type=int, default=0)
-    ap.add_argument("--measures", default="dtw,edr,lcss,erp,frechet,hausdorff,stedr,stlcss,cdds",
+    ap.add_argument("--measures", default="dtw,edr,lcss,erp,frechet,hausdorff,stedr,stlcss,cdds,sar",
                     help="measures reported under 'measures' (dtw is always the headline)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work at most")
@@ -103,7 +111,7 @@ def main():
     t0 = time.time()
     rng = np.random.default_rng(0)
     ts = random_walks(rng, lengths_uniform(rng, 2 * args.pairs, 20, 200))
-    ts = with_timestamps(np.random.default_rng(1), ts)  # for STEDR / STLCSS only
+    ts = with_timestamps(np.random.default_rng(1), ts)  # for STEDR / STLCSS / CDDS / SAR only
     pa = np.arange(0, 2 * args.pairs, 2)
     pb = pa + 1
     gpoint = ts.xy.mean(0)
@@ -125,6 +133,9 @@ def main():
     la, lb = ts.lengths[pa], ts.lengths[pb]
 
     def step(measure):
+        if measure == "sar":
+            engine.pairs(measure, dset, pa_d, pb_d, eps=EPS, validate=False, out=out, **SAR)
+            return
         engine.pairs(measure, dset, pa_d, pb_d, eps=EPS, gpoint=gpoint, lanes=args.lanes, validate=False, out=out,
                      delta=DELTA)
 
@@ -154,6 +165,9 @@ def main():
         }
         if m in FLOP_PER_CELL:
             report[m]["roofline_frac"] = cells * FLOP_PER_CELL[m] / (kms * 1e-3) / (PEAK_FP32_TFLOPS * 1e12)
+        elif m == "sar":  # linear in the points: HBM fraction of the input bytes
+            report[m]["unit"] = "points/s"
+            report[m]["hbm_frac"] = 24.0 * cells / (kms * 1e-3) / (PEAK_HBM_GBS * 1e9)
         else:  # CDDS: FP64, mostly a time-overlap test per segment pair; no FLOP roofline
             report[m]["unit"] = "segment pairs/s"
         print(f"[bench] {m}: {report[m]}", file=sys.stderr, flush=True)
