@@ -71,6 +71,10 @@ def lib():
         L.tsmo_sar.restype = d
         L.tsmo_pairs_sar.argtypes = [_P, _P, _P, _P, _P, i64, d, i64, i64, i64, _P, ctypes.c_int]
         L.tsmo_pairs_sar.restype = ctypes.c_int
+        L.tsmo_fasterpam.argtypes = [_P, i64, ctypes.c_int, _P, ctypes.c_int, _P, _P, _P, _P]
+        L.tsmo_fasterpam.restype = ctypes.c_int
+        L.tsmo_fasterpam_delta.argtypes = [_P, i64, ctypes.c_int, _P, i64, _P]
+        L.tsmo_fasterpam_delta.restype = d
         L.tsmo_max_threads.restype = ctypes.c_int
         _lib = L
     return _lib
@@ -211,3 +215,30 @@ def pairs_sar(xy, t, offsets, pa, pb, distance_eps: float, time_eps: int, target
                          pa.size, float(distance_eps), int(time_eps), int(target_length), int(threads_per_traj),
                          out.ctypes.data, int(threads))
     return out
+
+
+def fasterpam(diss, medoids, max_iter: int = 100):
+    """FasterPAM from the given initial medoids (see tsm_oracle.c); returns
+    (loss, labels, medoids, n_iter, n_swap) like kmedoids.fasterpam."""
+    D = np.ascontiguousarray(np.asarray(diss, dtype=np.float64))
+    n = D.shape[0]
+    med = np.ascontiguousarray(np.asarray(medoids, dtype=np.int64).copy())
+    labels = np.empty(n, dtype=np.int32)
+    loss = ctypes.c_double()
+    it = ctypes.c_int()
+    sw = ctypes.c_int()
+    rc = lib().tsmo_fasterpam(D.ctypes.data, n, med.size, med.ctypes.data, int(max_iter), labels.ctypes.data,
+                              ctypes.byref(loss), ctypes.byref(it), ctypes.byref(sw))
+    if rc != 0:
+        raise ValueError("bad k-medoids arguments")
+    return loss.value, labels, med, it.value, sw.value
+
+
+def fasterpam_delta(diss, medoids, j: int):
+    """(loss change, medoid position swapped out) of the best swap bringing
+    point j in, from the assignment to the given medoids."""
+    D = np.ascontiguousarray(np.asarray(diss, dtype=np.float64))
+    med = np.ascontiguousarray(np.asarray(medoids, dtype=np.int64))
+    b = ctypes.c_int()
+    v = lib().tsmo_fasterpam_delta(D.ctypes.data, D.shape[0], med.size, med.ctypes.data, int(j), ctypes.byref(b))
+    return v, b.value

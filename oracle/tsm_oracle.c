@@ -24,6 +24,7 @@
  *             moving points are within eps (CPU form: core_cpu/cdds.pyx)
  *   SAR       core/sar.py:10-212            time-aligned, binned comparison of the overlapping
  *             stretch (CPU form: core_cpu/sar.pyx)
+ *   k-medoids test_clustering_time.py:353-367   FasterPAM over the all-pairs matrix
  *
  * Distances are sqrt(dx*dx + dy*dy) evaluated without FMA contraction
  * (core/utils.py:8-17); build with -ffp-contract=off.
@@ -649,6 +650,260 @@ int tsmo_pairs_sar(const double *xy, const double *t, const int64_t *offsets, co
                           offsets[pb[k] + 1] - ob, deps, ts, M, R);
     }
     return 0;
+}
+
+/* ---- k-medoids: FasterPAM -------------------------------------------------
+ * The consumer of the all-pairs matrix, test_clustering_time.py:353-367:
+ * kmedoids.fasterpam(dists, Config.clustering_num_centroids) from the
+ * kmedoids package (kmedoids==0.4.3, requirements377.txt:6; a Rust
+ * extension, absent here).  Restated from the published algorithm (E.
+ * Schubert, P. J. Rousseeuw, "Fast and eager k-medoids clustering: O(k)
+ * runtime improvement of the PAM, CLARA, and CLARANS algorithms",
+ * Information Systems 101 (2021), FasterPAM):
+ *  - every point keeps its nearest medoid (an index into the medoid list and
+ *    the distance) and its second nearest; a point that is itself a medoid
+ *    is assigned to itself;
+ *  - removal loss of medoid m: the sum over its points of (second - nearest);
+ *  - candidates x = 0 .. n-1 in order (medoids skipped): one pass over the
+ *    points gives, for every medoid m, the loss change of swapping m out and
+ *    x in: points for which x is nearer than their nearest add the shared
+ *    d(x,o) - dn and take back their removal term (dn - ds) from m = n(o);
+ *    points for which x lies between nearest and second change the removal
+ *    term of n(o) to d(x,o) - ds.  The best m (first minimum) is swapped at
+ *    once when the change is negative (eager);
+ *  - a pass ends at the candidate of the last swap; passes repeat until one
+ *    makes no swap or max_iter passes ran.  k = 1 picks the point with the
+ *    least distance sum (the initial medoid unless another is strictly less).
+ * Every sum over the points runs in one fixed order, the one the GPU kernels
+ * use: 64 partial sums, partial l over the points o = l, l+64, l+128, ... in
+ * ascending o, then combined pairwise (l + 32 into l, then l + 16 into l, ...
+ * down to l + 1).  The Rust package adds in plain sequence; the two orders
+ * differ by rounding only.  Distances are read as D[row * n + col] with the
+ * rows the package uses (mat.get(j, o) for the candidate's pass). */
+#define KM_LANES 64
+#define KM_NONE 0xFFFFFFFFu
+
+typedef struct {
+    uint32_t ni, si;
+    double nd, sd;
+} km_rec;
+
+static double km_tree(double *p) {
+    for (int s = KM_LANES / 2; s >= 1; s >>= 1)
+        for (int l = 0; l < s; ++l) p[l] = p[l] + p[l + s];
+    return p[0];
+}
+
+static void km_assign(const double *D, int64_t n, int k, const int64_t *med, km_rec *r) {
+    for (int64_t i = 0; i < n; ++i) {
+        km_rec c = {0, KM_NONE, D[i * n + med[0]], 0.0};
+        for (int m = 1; m < k; ++m) {
+            const double d = D[i * n + med[m]];
+            if (d < c.nd || i == med[m]) {
+                c.si = c.ni;
+                c.sd = c.nd;
+                c.ni = (uint32_t)m;
+                c.nd = d;
+            } else if (c.si == KM_NONE || d < c.sd) {
+                c.si = (uint32_t)m;
+                c.sd = d;
+            }
+        }
+        r[i] = c;
+    }
+}
+
+/* loss = sum of nearest distances; rl[m] = removal loss of medoid m */
+static double km_totals(const km_rec *r, int64_t n, int k, double *rl, double *Q) {
+    double P[KM_LANES];
+    memset(Q, 0, sizeof(double) * (size_t)k * KM_LANES);
+    for (int l = 0; l < KM_LANES; ++l) {
+        P[l] = 0.0;
+        for (int64_t o = l; o < n; o += KM_LANES) {
+            P[l] += r[o].nd;
+            Q[(size_t)r[o].ni * KM_LANES + l] += r[o].sd - r[o].nd;
+        }
+    }
+    for (int m = 0; m < k; ++m) rl[m] = km_tree(Q + (size_t)m * KM_LANES);
+    return km_tree(P);
+}
+
+/* loss change of the best swap that brings candidate j in; *best = the medoid out */
+static double km_best_swap(const double *D, int64_t n, int k, const km_rec *r, const double *rl, int64_t j,
+                           int *best, double *Q) {
+    double A[KM_LANES];
+    memset(Q, 0, sizeof(double) * (size_t)k * KM_LANES);
+    for (int l = 0; l < KM_LANES; ++l) {
+        A[l] = 0.0;
+        for (int64_t o = l; o < n; o += KM_LANES) {
+            const double djo = D[j * n + o];
+            const km_rec c = r[o];
+            if (djo < c.nd) {
+                A[l] += djo - c.nd;
+                Q[(size_t)c.ni * KM_LANES + l] += c.nd - c.sd;
+            } else if (djo < c.sd) {
+                Q[(size_t)c.ni * KM_LANES + l] += djo - c.sd;
+            }
+        }
+    }
+    const double acc = km_tree(A);
+    int b = 0;
+    double bl = 0.0;
+    for (int m = 0; m < k; ++m) {
+        const double v = rl[m] + km_tree(Q + (size_t)m * KM_LANES);
+        if (m == 0 || v < bl) {
+            bl = v;
+            b = m;
+        }
+    }
+    *best = b;
+    return bl + acc;
+}
+
+static void km_second(const double *D, int64_t n, int k, const int64_t *med, uint32_t nn, uint32_t b, int64_t o,
+                      double djo, uint32_t *si, double *sd) {
+    uint32_t s = b;
+    double d = djo;
+    for (int i = 0; i < k; ++i) {
+        if ((uint32_t)i == nn || (uint32_t)i == b) continue;
+        const double e = D[o * n + med[i]];
+        if (e < d) {
+            s = (uint32_t)i;
+            d = e;
+        }
+    }
+    *si = s;
+    *sd = d;
+}
+
+static void km_swap(const double *D, int64_t n, int k, int64_t *med, km_rec *r, uint32_t b, int64_t j) {
+    med[b] = j;
+    for (int64_t o = 0; o < n; ++o) {
+        km_rec c = r[o];
+        if (o == j) {
+            if (c.ni != b) {
+                c.si = c.ni;
+                c.sd = c.nd;
+            }
+            c.ni = b;
+            c.nd = 0.0;
+            r[o] = c;
+            continue;
+        }
+        const double djo = D[j * n + o];
+        if (c.ni == b) {
+            if (djo < c.sd) {
+                c.nd = djo;
+            } else {
+                c.ni = c.si;
+                c.nd = c.sd;
+                km_second(D, n, k, med, c.ni, b, o, djo, &c.si, &c.sd);
+            }
+        } else if (djo < c.nd) {
+            c.si = c.ni;
+            c.sd = c.nd;
+            c.ni = b;
+            c.nd = djo;
+        } else if (djo < c.sd) {
+            c.si = b;
+            c.sd = djo;
+        } else if (c.si == b) {
+            km_second(D, n, k, med, c.ni, b, o, djo, &c.si, &c.sd);
+        }
+        r[o] = c;
+    }
+}
+
+/* sum of row j of D over o != j in the fixed order */
+static double km_rowsum(const double *D, int64_t n, int64_t j) {
+    double P[KM_LANES];
+    for (int l = 0; l < KM_LANES; ++l) {
+        P[l] = 0.0;
+        for (int64_t o = l; o < n; o += KM_LANES)
+            if (o != j) P[l] += D[j * n + o];
+    }
+    return km_tree(P);
+}
+
+/* medoids: in = initial (distinct indices), out = final.  Returns 0, or -1
+ * on bad arguments / allocation failure. */
+int tsmo_fasterpam(const double *D, int64_t n, int k, int64_t *med, int max_iter, int32_t *labels, double *loss_out,
+                   int *iter_out, int *swaps_out) {
+    if (n < 1 || k < 1 || k > n) return -1;
+    if (k == 1) {
+        const int64_t first = med[0];
+        int64_t best = first;
+        double sb = km_rowsum(D, n, first);
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == first) continue;
+            const double sj = km_rowsum(D, n, j);
+            if (sj < sb) {
+                best = j;
+                sb = sj;
+            }
+        }
+        med[0] = best;
+        for (int64_t o = 0; o < n; ++o) labels[o] = 0;
+        *loss_out = sb;
+        *iter_out = 1;
+        *swaps_out = best != first;
+        return 0;
+    }
+    km_rec *r = (km_rec *)malloc(sizeof(km_rec) * (size_t)n);
+    double *rl = (double *)malloc(sizeof(double) * (size_t)k);
+    double *Q = (double *)malloc(sizeof(double) * (size_t)k * KM_LANES);
+    if (!r || !rl || !Q) {
+        free(r);
+        free(rl);
+        free(Q);
+        return -1;
+    }
+    km_assign(D, n, k, med, r);
+    double loss = km_totals(r, n, k, rl, Q);
+    int64_t lastswap = n;
+    int swaps = 0, iter = 0;
+    while (iter < max_iter) {
+        ++iter;
+        const int before = swaps;
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == lastswap) break;
+            if (j == med[r[j].ni]) continue; /* already a medoid */
+            int b;
+            const double change = km_best_swap(D, n, k, r, rl, j, &b, Q);
+            if (change >= 0.0) continue;
+            ++swaps;
+            lastswap = j;
+            km_swap(D, n, k, med, r, (uint32_t)b, j);
+            loss = km_totals(r, n, k, rl, Q);
+        }
+        if (swaps == before) break;
+    }
+    for (int64_t o = 0; o < n; ++o) labels[o] = (int32_t)r[o].ni;
+    *loss_out = loss;
+    *iter_out = iter;
+    *swaps_out = swaps;
+    free(r);
+    free(rl);
+    free(Q);
+    return 0;
+}
+
+/* the loss change km_best_swap reports for candidate j in the state reached
+ * from the given medoids (test hook for the swap-delta identity) */
+double tsmo_fasterpam_delta(const double *D, int64_t n, int k, const int64_t *med, int64_t j, int *best) {
+    km_rec *r = (km_rec *)malloc(sizeof(km_rec) * (size_t)n);
+    double *rl = (double *)malloc(sizeof(double) * (size_t)k);
+    double *Q = (double *)malloc(sizeof(double) * (size_t)k * KM_LANES);
+    double v = NAN;
+    if (r && rl && Q && k >= 2) {
+        km_assign(D, n, k, med, r);
+        km_totals(r, n, k, rl, Q);
+        v = km_best_swap(D, n, k, r, rl, j, best, Q);
+    }
+    free(r);
+    free(rl);
+    free(Q);
+    return v;
 }
 
 int tsmo_max_threads(void) {

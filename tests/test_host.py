@@ -33,6 +33,10 @@ def test_library_exports_every_declared_function():
     for n in nn:
         assert hasattr(L, n), n
     assert set(nn) == set(_lib.NN_EXPORTS)
+    cl = header_functions("tsm_cluster.h")
+    for n in cl:
+        assert hasattr(L, n), n
+    assert set(cl) == set(_lib.CLUSTER_EXPORTS)
 
 
 def test_params_default():
@@ -94,6 +98,25 @@ def test_sar_argument_errors_without_device():
     with pytest.raises(ValueError):
         _lib.params(0.1, target_length=0)
     assert _lib.params(0.1, time_step=30.0, target_length=4, replicas=1).time_step == 30
+
+
+def test_kmedoids_argument_errors_without_device():
+    """Argument checks of the k-medoids entry point run before any device
+    work (the matrix pointer is never dereferenced)."""
+    L = _lib.lib()
+    fake = ctypes.c_void_p(16)
+    for n, k, med in ((10, 0, [0]), (10, 65, list(range(65))), (3, 4, [0, 1, 2, 3]), (10, 2, [3, 3]),
+                      (10, 2, [1, 10])):
+        m = np.array(med, dtype=np.int64)
+        rc = L.tsm_kmedoids_fasterpam(fake, n, k, m.ctypes.data, 100, None, None, None, None, None)
+        assert rc == -1 and b"kmedoids" in L.tsm_last_error(), (n, k, med)
+    from trajsimimeasures_amd import cluster
+    assert list(cluster.initial_medoids(10, 3, "first")) == [0, 1, 2]
+    assert list(cluster.initial_medoids(100, 4, "random", 5)) == list(np.random.RandomState(5).choice(100, 4, False))
+    with pytest.raises(ValueError):
+        cluster.initial_medoids(10, [1, 1])
+    with pytest.raises(ValueError):
+        cluster.initial_medoids(10, 3, "build")
 
 
 def test_params_validation():

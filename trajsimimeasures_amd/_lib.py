@@ -38,6 +38,8 @@ NN_EXPORTS = ("tsm_nn_gemm_nt", "tsm_nn_cast_bf16", "tsm_nn_resample", "tsm_nn_c
               "tsm_nn_rank_loss", "tsm_nn_adam", "tsm_nn_rownorm_bf16", "tsm_nn_topk_f32", "tsm_nn_topk_f64",
               "tsm_nn_linear_f32", "tsm_nn_split_bf16", "tsm_nn_gemm_nt_filter", "tsm_nn_select_candidates",
               "tsm_nn_gemm_nt_acc")
+# include/tsm_cluster.h
+CLUSTER_EXPORTS = ("tsm_square_from_triu", "tsm_kmedoids_fasterpam")
 
 
 class TsmError(RuntimeError):
@@ -79,6 +81,8 @@ def _declare(L):
     L.tsm_pairs_triu.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _I64, ctypes.POINTER(ParamsC), _P, _P, _P]
     L.tsm_pairs_host.argtypes = [ctypes.c_int, _P, _P, _P, _I64, _I64, _P, _P, _I64, ctypes.POINTER(ParamsC), _P, _P]
     L.tsm_tune_set_lanes.argtypes = [ctypes.c_int, _P, ctypes.c_int]
+    L.tsm_square_from_triu.argtypes = [_P, _I64, _P, _P]
+    L.tsm_kmedoids_fasterpam.argtypes = [_P, _I64, ctypes.c_int32, _P, ctypes.c_int32, _P, _P, _P, _P, _P]
     L.tsm_tune_set_waves_per_cu.argtypes = [ctypes.c_int]
     L.tsm_graph_capture_pairs.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _P, _P, _I64,
                                           ctypes.POINTER(ParamsC), _P, _P, _P, ctypes.POINTER(ctypes.c_void_p)]
@@ -101,7 +105,7 @@ def lib():
             if not os.path.exists(LIB_PATH):
                 raise ImportError(f"libtsm.so not built at {LIB_PATH}; run python -m trajsimimeasures_amd.build")
             L = ctypes.CDLL(LIB_PATH)
-            missing = [n for n in EXPORTS + NN_EXPORTS if not hasattr(L, n)]
+            missing = [n for n in EXPORTS + NN_EXPORTS + CLUSTER_EXPORTS if not hasattr(L, n)]
             if missing:
                 raise ImportError(f"{LIB_PATH} lacks {missing}")
             _declare(L)

@@ -1,0 +1,394 @@
+// tsm_kmedoids.hip -- FasterPAM k-medoids over a device dissimilarity
+// matrix, the clustering step of test_clustering_time.py:353-367
+// (kmedoids.fasterpam; E. Schubert, P. J. Rousseeuw, "Fast and eager
+// k-medoids clustering: O(k) runtime improvement of the PAM, CLARA, and
+// CLARANS algorithms", Information Systems 101, 2021), and the expansion of
+// the all-pairs upper triangle into the symmetric matrix it clusters
+// (test_clustering_time.py:256-269).
+//
+// State per point (structure of arrays): nearest medoid (position in the
+// medoid list, distance) and second nearest.  FasterPAM scans candidates in
+// index order and swaps eagerly, so candidate j's evaluation depends on every
+// swap before it.  The GPU evaluates a batch of consecutive candidates at
+// once against the current state (one wavefront per candidate: lanes stride
+// over the points reading the candidate's matrix row, with per-lane partial
+// sums of the shared gain and of the per-medoid removal-term changes), then
+// takes the first candidate whose best swap lowers the loss.  Candidates up
+// to it saw exactly the state the sequential algorithm gives them; after the
+// swap the scan resumes behind it.  The batch doubles while no swap is found
+// and shrinks after one.
+//
+// Sums over the points use one fixed order: partial l adds the points
+// l, l + 64, ... in ascending order, then partials combine pairwise
+// (l + 32 into l, ..., l + 1 into l: a butterfly whose lane 0 holds this
+// tree).  The matrix is HBM-bound input: each candidate streams its row once.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/tsm.h"
+#include "../../include/tsm_cluster.h"
+#include "tsm_host.hpp"
+
+namespace tsm {
+namespace {
+
+constexpr int kKmMax = 64;
+constexpr uint32_t kKmNone = 0xFFFFFFFFu;
+
+struct KmState {
+    uint32_t *ni, *si;  // nearest / second nearest medoid (position in the medoid list)
+    double *nd, *sd;    // their distances
+};
+
+// lane 0 gets the fixed-order tree of the 64 lanes' partials
+__device__ __forceinline__ double km_tree(double v) {
+#pragma unroll
+    for (int o = kWave / 2; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+__global__ void km_assign_kernel(const double *D, int64_t n, int k, const int64_t *med, KmState s) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t ni = 0, si = kKmNone;
+        double nd = D[i * n + med[0]], sd = 0.0;
+        for (int m = 1; m < k; ++m) {
+            const double d = D[i * n + med[m]];
+            if (d < nd || i == med[m]) {  // a medoid is its own nearest
+                si = ni;
+                sd = nd;
+                ni = (uint32_t)m;
+                nd = d;
+            } else if (si == kKmNone || d < sd) {
+                si = (uint32_t)m;
+                sd = d;
+            }
+        }
+        s.ni[i] = ni;
+        s.si[i] = si;
+        s.nd[i] = nd;
+        s.sd[i] = sd;
+    }
+}
+
+// loss = sum of nearest distances; rl[m] = removal loss of medoid m (sum of
+// second - nearest over its points).  One wavefront.
+// (dynamic LDS: k * 64 doubles, lane-private columns)
+__global__ __launch_bounds__(kWave) void km_totals_kernel(KmState s, int64_t n, int k, double *rl, double *loss) {
+    extern __shared__ double q[];
+    const int l = threadIdx.x;
+    for (int m = 0; m < k; ++m) q[m * kWave + l] = 0.0;
+    double p = 0.0;
+    for (int64_t o = l; o < n; o += kWave) {
+        const double nd = s.nd[o];
+        p += nd;
+        q[s.ni[o] * kWave + l] += s.sd[o] - nd;
+    }
+    p = km_tree(p);
+    if (l == 0) *loss = p;
+    for (int m = 0; m < k; ++m) {
+        const double v = km_tree(q[m * kWave + l]);
+        if (l == 0) rl[m] = v;
+    }
+}
+
+// Candidates j0 + blockIdx.x: loss change of the best swap bringing j in and
+// the medoid it replaces (+inf for a current medoid).
+__global__ __launch_bounds__(kWave) void km_eval_kernel(const double *D, int64_t n, int k, KmState s,
+                                                        const double *rl, const int64_t *med, int64_t j0,
+                                                        double *change, int32_t *best) {
+    extern __shared__ double q[];  // k * 64 doubles, lane-private columns
+    const int l = threadIdx.x;
+    const int64_t j = j0 + blockIdx.x;
+    if (j == med[s.ni[j]]) {
+        if (l == 0) {
+            change[blockIdx.x] = __builtin_huge_val();
+            best[blockIdx.x] = 0;
+        }
+        return;
+    }
+    for (int m = 0; m < k; ++m) q[m * kWave + l] = 0.0;
+    const double *row = D + j * n;
+    double acc = 0.0;
+    for (int64_t o = l; o < n; o += kWave) {
+        const double djo = row[o], nd = s.nd[o], sd = s.sd[o];
+        const uint32_t ni = s.ni[o];
+        if (djo < nd) {  // j would be o's nearest: shared gain, and n(o)'s removal term is void
+            acc += djo - nd;
+            q[ni * kWave + l] += nd - sd;
+        } else if (djo < sd) {  // j would be o's second: n(o)'s removal term shrinks
+            q[ni * kWave + l] += djo - sd;
+        }
+    }
+    acc = km_tree(acc);
+    double bl = 0.0;
+    int b = 0;
+    for (int m = 0; m < k; ++m) {
+        const double v = rl[m] + km_tree(q[m * kWave + l]);
+        if (m == 0 || v < bl) {
+            bl = v;
+            b = m;
+        }
+    }
+    if (l == 0) {
+        change[blockIdx.x] = bl + acc;
+        best[blockIdx.x] = b;
+    }
+}
+
+// First candidate of the batch whose change is negative: res = {index or -1, medoid}.
+__global__ __launch_bounds__(kWave) void km_pick_kernel(const double *change, const int32_t *best, int64_t count,
+                                                        int64_t *res) {
+    const int l = threadIdx.x;
+    for (int64_t c0 = 0; c0 < count; c0 += kWave) {
+        const int64_t c = c0 + l;
+        const bool neg = c < count && change[c] < 0.0;
+        const uint64_t mask = __builtin_amdgcn_ballot_w64(neg);
+        if (mask) {
+            const int64_t f = c0 + __builtin_ctzll(mask);
+            if (l == 0) {
+                res[0] = f;
+                res[1] = best[f];
+            }
+            return;
+        }
+    }
+    if (l == 0) {
+        res[0] = -1;
+        res[1] = 0;
+    }
+}
+
+__device__ __forceinline__ void km_second(const double *D, int64_t n, int k, const int64_t *med, uint32_t nn,
+                                          uint32_t b, int64_t o, double djo, uint32_t &si, double &sd) {
+    uint32_t s = b;
+    double d = djo;
+    for (int i = 0; i < k; ++i) {
+        if ((uint32_t)i == nn || (uint32_t)i == b) continue;
+        const double e = D[o * n + med[i]];
+        if (e < d) {
+            s = (uint32_t)i;
+            d = e;
+        }
+    }
+    si = s;
+    sd = d;
+}
+
+// Medoid b is now point j (med already updated): every point's nearest and
+// second nearest.
+__global__ void km_swap_kernel(const double *D, int64_t n, int k, const int64_t *med, KmState s, uint32_t b,
+                               int64_t j) {
+    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < n; o += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t ni = s.ni[o], si = s.si[o];
+        double nd = s.nd[o], sd = s.sd[o];
+        if (o == j) {
+            if (ni != b) {
+                si = ni;
+                sd = nd;
+            }
+            ni = b;
+            nd = 0.0;
+        } else {
+            const double djo = D[j * n + o];
+            if (ni == b) {  // its nearest medoid left
+                if (djo < sd) {
+                    nd = djo;
+                } else {
+                    ni = si;
+                    nd = sd;
+                    km_second(D, n, k, med, ni, b, o, djo, si, sd);
+                }
+            } else if (djo < nd) {
+                si = ni;
+                sd = nd;
+                ni = b;
+                nd = djo;
+            } else if (djo < sd) {
+                si = b;
+                sd = djo;
+            } else if (si == b) {  // its second nearest left
+                km_second(D, n, k, med, ni, b, o, djo, si, sd);
+            }
+        }
+        s.ni[o] = ni;
+        s.si[o] = si;
+        s.nd[o] = nd;
+        s.sd[o] = sd;
+    }
+}
+
+// Row sums without the diagonal, fixed order (k = 1).
+__global__ __launch_bounds__(kWave) void km_rowsum_kernel(const double *D, int64_t n, double *sums) {
+    const int l = threadIdx.x;
+    const int64_t j = blockIdx.x;
+    double p = 0.0;
+    for (int64_t o = l; o < n; o += kWave)
+        if (o != j) p += D[j * n + o];
+    p = km_tree(p);
+    if (l == 0) sums[j] = p;
+}
+
+__global__ void km_labels_kernel(const uint32_t *ni, int64_t n, int32_t *labels) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        labels[i] = (int32_t)ni[i];
+}
+
+__global__ void square_from_triu_kernel(const double *tri, int64_t n, double *D) {
+    const int64_t total = n * n;
+    for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / n, j = e - i * n;
+        double v = 0.0;
+        if (i < j) v = tri[triu_row_start(i, n) + (j - i - 1)];
+        else if (j < i) v = tri[triu_row_start(j, n) + (i - j - 1)];
+        D[e] = v;
+    }
+}
+
+struct KmBuffers {
+    DevBuf ni, si, nd, sd, rl, loss, med, change, best, res, sums;
+    int64_t *h_pin = nullptr;  // pinned: [0, 2) batch outcome, [2] loss, [8, 8 + 64) medoids
+};
+std::mutex g_km_mu;
+KmBuffers g_km;
+
+int km_grid(int64_t n) {
+    const int64_t g = (n + 255) / 256;
+    return (int)(g < 65536 ? (g > 0 ? g : 1) : 65536);
+}
+
+}  // namespace
+}  // namespace tsm
+
+using namespace tsm;
+
+extern "C" int tsm_square_from_triu(const double *tri, int64_t n, double *diss, void *stream) {
+    if (n < 1) return fail(TSM_E_ARG, "square_from_triu: n must be >= 1");
+    if (!diss || (n > 1 && !tri)) return fail(TSM_E_ARG, "square_from_triu: null buffer");
+    if (n > (int64_t)3037000499ll) return fail(TSM_E_ARG, "square_from_triu: n too large");
+    const hipStream_t st = (hipStream_t)stream;
+    const int64_t total = n * n;
+    const int64_t g = (total + 255) / 256;
+    hipLaunchKernelGGL(square_from_triu_kernel, dim3((unsigned)(g < 1048576 ? g : 1048576)), dim3(256), 0, st, tri, n,
+                       diss);
+    TSM_HIP(hipGetLastError());
+    return TSM_OK;
+}
+
+extern "C" int tsm_kmedoids_fasterpam(const double *diss, int64_t n, int32_t k, int64_t *medoids, int32_t max_iter,
+                                      int32_t *labels, double *loss, int32_t *n_iter, int32_t *n_swap, void *stream) {
+    if (!diss || !medoids) return fail(TSM_E_ARG, "kmedoids: null matrix or medoid array");
+    if (n < 1 || n > 0x7FFFFFFFll) return fail(TSM_E_ARG, "kmedoids: n must be in [1, 2^31)");
+    if (k < 1 || k > kKmMax || k > n) return fail(TSM_E_ARG, "kmedoids: k must be in [1, min(n, 64)]");
+    if (max_iter < 0) return fail(TSM_E_ARG, "kmedoids: max_iter must be >= 0");
+    {
+        std::vector<int64_t> seen(medoids, medoids + k);
+        for (int i = 0; i < k; ++i) {
+            if (medoids[i] < 0 || medoids[i] >= n) return fail(TSM_E_ARG, "kmedoids: initial medoid outside [0, n)");
+            for (int j = 0; j < i; ++j)
+                if (seen[j] == medoids[i]) return fail(TSM_E_ARG, "kmedoids: initial medoids must be distinct");
+        }
+    }
+    const hipStream_t st = (hipStream_t)stream;
+    std::lock_guard<std::mutex> lk(g_km_mu);
+    KmBuffers &w = g_km;
+    int rc;
+    if ((rc = w.med.ensure(sizeof(int64_t) * k)) || (rc = w.res.ensure(2 * sizeof(int64_t)))) return rc;
+    if (!w.h_pin) TSM_HIP(hipHostMalloc((void **)&w.h_pin, 80 * sizeof(int64_t), hipHostMallocDefault));
+    int64_t *h_res = w.h_pin, *h_med = w.h_pin + 8;
+    double *h_lossp = reinterpret_cast<double *>(w.h_pin + 2);
+    std::vector<int64_t> med(medoids, medoids + k);
+    double h_loss = 0.0;
+    int iter = 0, swaps = 0;
+
+    if (k == 1) {  // the point with the least distance sum (the initial one unless another is strictly less)
+        if ((rc = w.sums.ensure(sizeof(double) * n))) return rc;
+        hipLaunchKernelGGL(km_rowsum_kernel, dim3((unsigned)n), dim3(kWave), 0, st, diss, n, w.sums.as<double>());
+        TSM_HIP(hipGetLastError());
+        std::vector<double> sums(n);
+        TSM_HIP(hipStreamSynchronize(st));
+        TSM_HIP(hipMemcpy(sums.data(), w.sums.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+        const int64_t first = med[0];
+        int64_t bestj = first;
+        double sb = sums[first];
+        for (int64_t j = 0; j < n; ++j) {
+            if (j == first) continue;
+            if (sums[j] < sb) {
+                bestj = j;
+                sb = sums[j];
+            }
+        }
+        med[0] = bestj;
+        h_loss = sb;
+        iter = 1;
+        swaps = bestj != first;
+        if (labels) TSM_HIP(hipMemsetAsync(labels, 0, sizeof(int32_t) * n, st));
+    } else {
+        if ((rc = w.ni.ensure(4 * n)) || (rc = w.si.ensure(4 * n)) || (rc = w.nd.ensure(8 * n)) ||
+            (rc = w.sd.ensure(8 * n)) || (rc = w.rl.ensure(8 * k)) || (rc = w.loss.ensure(8)) ||
+            (rc = w.change.ensure(8 * n)) || (rc = w.best.ensure(4 * n)))
+            return rc;
+        KmState s{w.ni.as<uint32_t>(), w.si.as<uint32_t>(), w.nd.as<double>(), w.sd.as<double>()};
+        int64_t *dmed = w.med.as<int64_t>();
+        const size_t lds = sizeof(double) * kWave * (size_t)k;
+        std::memcpy(h_med, med.data(), sizeof(int64_t) * k);
+        TSM_HIP(hipMemcpyAsync(dmed, h_med, sizeof(int64_t) * k, hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(km_assign_kernel, dim3(km_grid(n)), dim3(256), 0, st, diss, n, (int)k, dmed, s);
+        hipLaunchKernelGGL(km_totals_kernel, dim3(1), dim3(kWave), lds, st, s, n, (int)k, w.rl.as<double>(),
+                           w.loss.as<double>());
+        TSM_HIP(hipGetLastError());
+        const int64_t kMinBatch = 512;
+        int64_t batch = kMinBatch < n ? kMinBatch : n;
+        int64_t lastswap = n;
+        while (iter < max_iter) {
+            ++iter;
+            const int before = swaps;
+            int64_t end = lastswap;  // a pass stops at the candidate of the last swap ...
+            int64_t j = 0;
+            while (j < end) {
+                const int64_t cnt = batch < end - j ? batch : end - j;
+                hipLaunchKernelGGL(km_eval_kernel, dim3((unsigned)cnt), dim3(kWave), lds, st, diss, n, (int)k, s,
+                                   w.rl.as<double>(), dmed, j, w.change.as<double>(), w.best.as<int32_t>());
+                hipLaunchKernelGGL(km_pick_kernel, dim3(1), dim3(kWave), 0, st, w.change.as<double>(),
+                                   w.best.as<int32_t>(), cnt, w.res.as<int64_t>());
+                TSM_HIP(hipGetLastError());
+                TSM_HIP(hipMemcpyAsync(h_res, w.res.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+                TSM_HIP(hipStreamSynchronize(st));
+                if (h_res[0] < 0) {
+                    j += cnt;
+                    batch = 2 * batch < n ? 2 * batch : n;
+                    continue;
+                }
+                const int64_t c = j + h_res[0];
+                const uint32_t b = (uint32_t)h_res[1];
+                ++swaps;
+                lastswap = c;
+                end = n;  // ... unless it made a swap itself
+                med[b] = c;
+                h_med[b] = c;  // the previous copies finished at the last synchronisation
+                TSM_HIP(hipMemcpyAsync(dmed + b, h_med + b, sizeof(int64_t), hipMemcpyHostToDevice, st));
+                hipLaunchKernelGGL(km_swap_kernel, dim3(km_grid(n)), dim3(256), 0, st, diss, n, (int)k, dmed, s, b, c);
+                hipLaunchKernelGGL(km_totals_kernel, dim3(1), dim3(kWave), lds, st, s, n, (int)k, w.rl.as<double>(),
+                                   w.loss.as<double>());
+                TSM_HIP(hipGetLastError());
+                j = c + 1;
+                batch = batch / 2 > kMinBatch ? batch / 2 : (kMinBatch < n ? kMinBatch : n);
+            }
+            if (swaps == before) break;
+        }
+        TSM_HIP(hipMemcpyAsync(h_lossp, w.loss.p, sizeof(double), hipMemcpyDeviceToHost, st));
+        if (labels)
+            hipLaunchKernelGGL(km_labels_kernel, dim3(km_grid(n)), dim3(256), 0, st, w.ni.as<uint32_t>(), n, labels);
+        TSM_HIP(hipGetLastError());
+        TSM_HIP(hipStreamSynchronize(st));
+        h_loss = *h_lossp;
+    }
+    for (int i = 0; i < k; ++i) medoids[i] = med[i];
+    if (loss) *loss = h_loss;
+    if (n_iter) *n_iter = iter;
+    if (n_swap) *n_swap = swaps;
+    return TSM_OK;
+}

@@ -6,6 +6,8 @@ helpers and its three GPU drivers, on the engine.
   pairwise               test_trajsimi_time.py:222-276   (pairs of trajectories)
   query_db               test_trajsimi_time_DQ.py:136-181 (|Q| x |D| grid)
   all_pairs              test_clustering_time.py:208-269  (upper triangle, symmetrised)
+  clustering_kmedoids    test_clustering_time.py:353-367  (FasterPAM on the matrix, timed)
+  cluster_kmedoids       all_pairs + clustering_kmedoids with the matrix kept on the device
 
 The reference drivers pad every 512-pair batch on the host, gather it with
 fancy indexing and launch one kernel per batch.  Here a trajectory list is
@@ -25,13 +27,15 @@ from . import _lib, core
 from .data import TrajSet, pack_list
 
 __all__ = ["heuristic_fn_wrapper", "heuristic_colllate_fn", "is_heuristic", "pairwise", "query_db", "all_pairs",
-           "symmetric_from_triu", "fp64_drift", "DEFAULT_EPS", "DEFAULT_DELTA", "DEFAULT_SAR"]
+           "symmetric_from_triu", "fp64_drift", "clustering_kmedoids", "cluster_kmedoids", "DEFAULT_EPS",
+           "DEFAULT_DELTA", "DEFAULT_SAR", "DEFAULT_CENTROIDS"]
 
 DEFAULT_EPS = 0.0025  # EDR/LCSS epsilon of the 100k-pair workload
 DEFAULT_DELTA = 60.0  # STEDR/STLCSS time threshold, seconds (config.py:53)
 # SAR (distance_eps, time_eps, target_length, threads_per_traj): config.py:55-57
 # and core/sar.py:248 (the reference's wrapper leaves threads_per_traj at 64)
 DEFAULT_SAR = (200.0, 10, 10, 64)
+DEFAULT_CENTROIDS = 10  # config.py:72 clustering_num_centroids
 
 _GPU_FNS = {"dtw": core.dtw, "edr": core.edr, "erp": core.erp, "frechet": core.dfrechet,
             "hausdorff": core.hausdorff, "lcss": core.lcss, "stedr": core.stedr, "stlcss": core.stlcss,
@@ -178,6 +182,49 @@ def all_pairs(method: str, trajs, *, eps: float = DEFAULT_EPS, gpoint=None, lane
     vals = tri.cpu().numpy()
     comp = time.time() - t1
     return symmetric_from_triu(vals, n), [time.time() - t0, collate, comp]
+
+
+def clustering_kmedoids(dists, k: int = DEFAULT_CENTROIDS, random_state=None):
+    """test_clustering_time.py:353-367: FasterPAM k-medoids of the distance
+    matrix, timed; returns [clustering_time] in seconds, or [-1] when the
+    clustering fails (the reference logs the error)."""
+    import torch
+
+    from . import cluster
+    t0 = time.time()
+    try:
+        cluster.fasterpam(dists, k, random_state=random_state)
+        torch.cuda.synchronize()
+    except (ValueError, RuntimeError):
+        return [-1]
+    return [time.time() - t0]
+
+
+def cluster_kmedoids(method: str, trajs, k: int = DEFAULT_CENTROIDS, *, eps: float = DEFAULT_EPS, gpoint=None,
+                     lanes: int = 0, precision="fp32", delta: float = DEFAULT_DELTA, sar=DEFAULT_SAR,
+                     max_iter: int = 100, init: str = "random", random_state=None):
+    """The clustering workload end to end on the device: ``method`` over the
+    upper triangle of ``trajs``, the symmetric matrix expanded on the device,
+    FasterPAM with ``k`` medoids.  Returns (KMedoidsResult, [running,
+    collate, matrix, clustering] seconds)."""
+    import torch
+
+    from . import cluster, engine
+    t0 = time.time()
+    ts = trajs if isinstance(trajs, TrajSet) else pack_list(list(trajs))
+    n = ts.n
+    dset = _device_set(ts)
+    collate = time.time() - t0
+    t1 = time.time()
+    tri = engine.triu(method, dset, n, **_opts(method, eps, gpoint, lanes, precision, delta, sar))
+    D = cluster.square_from_triu(tri, n)
+    del tri
+    torch.cuda.synchronize()
+    t_mat = time.time() - t1
+    t2 = time.time()
+    res = cluster.fasterpam(D, k, max_iter=max_iter, init=init, random_state=random_state)
+    t_clu = time.time() - t2
+    return res, [time.time() - t0, collate, t_mat, t_clu]
 
 
 def method_names():
