@@ -35,6 +35,12 @@ encoder's HIP forward/backward kernels); filtered kNN = embed the queries,
 BF16-MFMA distance GEMM against the embedded database, 200 candidates per
 query, exact DTW re-rank, top 10.  Recall@10 is judged against the exact
 kNN; q/s excludes the one-off database embedding (reported as index_build_s).
+
+cluster: the clustering leg.  10,000 random walks (lengths 98 % uniform
+10..500, 2 % uniform 1000..2000), DTW over the upper triangle (49,995,000
+pairs), the symmetric matrix expanded on the device, FasterPAM k-medoids
+with k = 10; the oracle's FasterPAM runs on one host core beside it and must
+give the same result.
 """
 from __future__ import annotations
 
@@ -94,6 +100,8 @@ def main():
     ap.add_argument("--no-knn", action="store_true", help="skip the learned kNN filter leg")
     ap.add_argument("--knn-db", type=int, default=1_000_000)
     ap.add_argument("--knn-queries", type=int, default=1000)
+    ap.add_argument("--no-cluster", action="store_true", help="skip the all-pairs + k-medoids leg")
+    ap.add_argument("--cluster-n", type=int, default=10_000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain 256 MiB device copy (calibrates HBM byte counters under rocprofv3)")
     args = ap.parse_args()
@@ -205,6 +213,7 @@ def main():
         traffic = None
 
     knn = None if args.no_knn else knn_leg(args.knn_queries, args.knn_db)
+    clu = None if args.no_cluster else cluster_leg(args.cluster_n, cpu_too=not args.no_cpu)
 
     line = {
         "metric": "DTW pairs/s & cells/s (% VALU roofline); kNN q/s at recall@10\u22650.95 vs exact",
@@ -229,8 +238,61 @@ def main():
         "cpu_baseline": cpu,
         "measures": report,
         "knn": knn,
+        "cluster": clu,
     }
     print(json.dumps(line))
+
+
+def cluster_leg(n: int, k: int = 10, cpu_too: bool = True) -> dict:
+    """The clustering workload: DTW over the upper triangle of n random walks
+    (lengths 98 % uniform 10..500, 2 % uniform 1000..2000, seed 0), the
+    symmetric matrix expanded on the device, FasterPAM with k medoids from
+    RandomState(0) initial medoids.  The oracle's FasterPAM (one core, same
+    matrix and initial medoids) is timed beside it and must give the same
+    loss, medoids and labels."""
+    import torch
+
+    from trajsimimeasures_amd import cluster, engine
+    from trajsimimeasures_amd.data import lengths_uniform_with_tail, random_walks
+
+    def log(msg):
+        print(f"[bench] cluster: {msg}", file=sys.stderr, flush=True)
+
+    rng = np.random.default_rng(0)
+    ts = random_walks(rng, lengths_uniform_with_tail(rng, n))
+    dset = engine.DeviceTrajSet(ts)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    tri = engine.triu("dtw", dset, n)
+    D = cluster.square_from_triu(tri, n)
+    del tri
+    torch.cuda.synchronize()
+    t_mat = time.perf_counter() - t
+    log(f"all-pairs DTW of {n} trajectories + symmetric matrix: {t_mat:.3f}s")
+    med0 = np.random.RandomState(0).choice(n, k, False)
+    cluster.fasterpam(D, med0)  # warm-up (module loads, workspace)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    res = cluster.fasterpam(D, med0)
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t
+    log(f"FasterPAM k={k}: {t_gpu * 1e3:.2f} ms, {res.n_iter} passes, {res.n_swap} swaps, loss {res.loss:.6f}")
+    out = {"workload": f"{n} random walks (lengths 98% U(10..500), 2% U(1000..2000), seed 0), DTW upper "
+                       f"triangle ({n * (n - 1) // 2} pairs), symmetric matrix on the device, FasterPAM k={k} "
+                       f"from RandomState(0) initial medoids",
+           "matrix_s": t_mat, "fasterpam_s": t_gpu, "n_iter": res.n_iter, "n_swap": res.n_swap, "loss": res.loss}
+    if cpu_too:
+        from oracle import oracle as O
+        Dh = D.cpu().numpy()
+        del D
+        t = time.perf_counter()
+        ref = O.fasterpam(Dh, med0)
+        t_cpu = time.perf_counter() - t
+        same = ref[0] == res.loss and np.array_equal(ref[2], res.medoids) and np.array_equal(ref[1], res.labels)
+        log(f"oracle FasterPAM (1 core): {t_cpu:.2f}s, same result: {same}")
+        out.update({"cpu_fasterpam_s": t_cpu, "cpu_cores": 1, "same_as_oracle": bool(same),
+                    "speedup": t_cpu / t_gpu})
+    return out
 
 
 def knn_leg(nq: int, nd: int) -> dict:

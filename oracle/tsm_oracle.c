@@ -674,11 +674,12 @@ int tsmo_pairs_sar(const double *xy, const double *t, const int64_t *offsets, co
  *  - a pass ends at the candidate of the last swap; passes repeat until one
  *    makes no swap or max_iter passes ran.  k = 1 picks the point with the
  *    least distance sum (the initial medoid unless another is strictly less).
- * Every sum over the points runs in one fixed order, the one the GPU kernels
- * use: 64 partial sums, partial l over the points o = l, l+64, l+128, ... in
- * ascending o, then combined pairwise (l + 32 into l, then l + 16 into l, ...
- * down to l + 1).  The Rust package adds in plain sequence; the two orders
- * differ by rounding only.  Distances are read as D[row * n + col] with the
+ * Every sum over the points runs in a fixed order, the one the GPU kernels
+ * use.  A candidate's sums: 64 partial sums, partial l over the points
+ * o = l, l+64, l+128, ... in ascending o, then combined pairwise (l + 32 into
+ * l, then l + 16 into l, ... down to l + 1).  The loss and removal losses:
+ * see km_totals.  The Rust package adds in plain sequence; the orders differ
+ * by rounding only.  Distances are read as D[row * n + col] with the
  * rows the package uses (mat.get(j, o) for the candidate's pass). */
 #define KM_LANES 64
 #define KM_NONE 0xFFFFFFFFu
@@ -713,19 +714,31 @@ static void km_assign(const double *D, int64_t n, int k, const int64_t *med, km_
     }
 }
 
-/* loss = sum of nearest distances; rl[m] = removal loss of medoid m */
+/* loss = sum of nearest distances; rl[m] = removal loss of medoid m.  These
+ * sums run block by block: blocks of 256 points in order, each the pairwise
+ * tree of its four 64-point groups' trees, (g0 + g2) + (g1 + g3), added in
+ * block order to a running sum from 0. */
+#define KM_BLOCK 256
 static double km_totals(const km_rec *r, int64_t n, int k, double *rl, double *Q) {
-    double P[KM_LANES];
-    memset(Q, 0, sizeof(double) * (size_t)k * KM_LANES);
-    for (int l = 0; l < KM_LANES; ++l) {
-        P[l] = 0.0;
-        for (int64_t o = l; o < n; o += KM_LANES) {
-            P[l] += r[o].nd;
-            Q[(size_t)r[o].ni * KM_LANES + l] += r[o].sd - r[o].nd;
+    double p[KM_LANES], g[4];
+    double *tot = Q; /* k + 1 running sums: removal losses, then the loss */
+    for (int v = 0; v <= k; ++v) tot[v] = 0.0;
+    for (int64_t b0 = 0; b0 < n; b0 += KM_BLOCK) {
+        for (int v = 0; v <= k; ++v) {
+            for (int w = 0; w < 4; ++w) {
+                for (int t = 0; t < KM_LANES; ++t) {
+                    const int64_t o = b0 + w * KM_LANES + t;
+                    double x = 0.0;
+                    if (o < n) x = v == k ? r[o].nd : (r[o].ni == (uint32_t)v ? r[o].sd - r[o].nd : 0.0);
+                    p[t] = x;
+                }
+                g[w] = km_tree(p);
+            }
+            tot[v] += (g[0] + g[2]) + (g[1] + g[3]);
         }
     }
-    for (int m = 0; m < k; ++m) rl[m] = km_tree(Q + (size_t)m * KM_LANES);
-    return km_tree(P);
+    for (int m = 0; m < k; ++m) rl[m] = tot[m];
+    return tot[k];
 }
 
 /* loss change of the best swap that brings candidate j in; *best = the medoid out */
@@ -851,7 +864,7 @@ int tsmo_fasterpam(const double *D, int64_t n, int k, int64_t *med, int max_iter
     }
     km_rec *r = (km_rec *)malloc(sizeof(km_rec) * (size_t)n);
     double *rl = (double *)malloc(sizeof(double) * (size_t)k);
-    double *Q = (double *)malloc(sizeof(double) * (size_t)k * KM_LANES);
+    double *Q = (double *)malloc(sizeof(double) * ((size_t)k * KM_LANES + 1));
     if (!r || !rl || !Q) {
         free(r);
         free(rl);
@@ -893,7 +906,7 @@ int tsmo_fasterpam(const double *D, int64_t n, int k, int64_t *med, int max_iter
 double tsmo_fasterpam_delta(const double *D, int64_t n, int k, const int64_t *med, int64_t j, int *best) {
     km_rec *r = (km_rec *)malloc(sizeof(km_rec) * (size_t)n);
     double *rl = (double *)malloc(sizeof(double) * (size_t)k);
-    double *Q = (double *)malloc(sizeof(double) * (size_t)k * KM_LANES);
+    double *Q = (double *)malloc(sizeof(double) * ((size_t)k * KM_LANES + 1));
     double v = NAN;
     if (r && rl && Q && k >= 2) {
         km_assign(D, n, k, med, r);

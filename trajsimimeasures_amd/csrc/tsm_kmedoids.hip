@@ -18,10 +18,12 @@
 // swap the scan resumes behind it.  The batch doubles while no swap is found
 // and shrinks after one.
 //
-// Sums over the points use one fixed order: partial l adds the points
-// l, l + 64, ... in ascending order, then partials combine pairwise
+// Sums over the points use fixed orders.  A candidate's: partial l adds the
+// points l, l + 64, ... in ascending order, then partials combine pairwise
 // (l + 32 into l, ..., l + 1 into l: a butterfly whose lane 0 holds this
-// tree).  The matrix is HBM-bound input: each candidate streams its row once.
+// tree).  The loss and removal losses: per block of 256 points, then blocks in
+// order (km_block_partials).  The matrix is HBM-bound input: each candidate
+// streams its row once.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -51,10 +53,55 @@ __device__ __forceinline__ double km_tree(double v) {
     return v;
 }
 
-__global__ void km_assign_kernel(const double *D, int64_t n, int k, const int64_t *med, KmState s) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t ni = 0, si = kKmNone;
-        double nd = D[i * n + med[0]], sd = 0.0;
+// Loss and removal losses (sum of nearest distances; per medoid, sum of
+// second - nearest over its points) in their fixed order: the point kernels
+// run blocks of 256 points, one point per thread; each block reduces its
+// k + 1 values as the trees of its four 64-point groups (wave butterflies)
+// combined as (g0 + g2) + (g1 + g3) into part[block][v] (v = k: the loss);
+// km_blocksum_kernel then adds the blocks in order.  Terms of other medoids
+// are +0, which never changes a sum here (no partial is ever -0).
+constexpr int kKmBlock = 256;
+
+__device__ __forceinline__ void km_block_partials(int k, bool ok, uint32_t ni, double nd, double sd, double *part) {
+    __shared__ double g[4][kKmMax + 1];
+    const int t = threadIdx.x, w = t >> 6, l = t & (kWave - 1);
+    for (int v = 0; v <= k; ++v) {
+        double x = 0.0;
+        if (ok) x = v == k ? nd : (ni == (uint32_t)v ? sd - nd : 0.0);
+        x = km_tree(x);
+        if (l == 0) g[w][v] = x;
+    }
+    __syncthreads();
+    for (int v = t; v <= k; v += kKmBlock) part[(int64_t)blockIdx.x * (k + 1) + v] = (g[0][v] + g[2][v]) + (g[1][v] + g[3][v]);
+}
+
+// One wavefront: rl[m] = sum over blocks of part[b][m] in block order, loss likewise.
+__global__ __launch_bounds__(kWave) void km_blocksum_kernel(const double *part, int64_t nblk, int k, double *rl,
+                                                            double *loss) {
+    for (int v = threadIdx.x; v <= k; v += kWave) {
+        double s = 0.0;
+        constexpr int U = 16;
+        for (int64_t b0 = 0; b0 < nblk; b0 += U) {
+            double x[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) x[u] = part[(b0 + u < nblk ? b0 + u : nblk - 1) * (k + 1) + v];
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (b0 + u < nblk) s += x[u];
+        }
+        if (v < k) rl[v] = s;
+        else *loss = s;
+    }
+}
+
+__global__ __launch_bounds__(kKmBlock) void km_assign_kernel(const double *D, int64_t n, int k, const int64_t *med,
+                                                              KmState s, double *part) {
+    const int64_t i = blockIdx.x * (int64_t)kKmBlock + threadIdx.x;
+    const bool ok = i < n;
+    uint32_t ni = 0, si = kKmNone;
+    double nd = 0.0, sd = 0.0;
+    if (ok) {
+        nd = D[i * n + med[0]];
         for (int m = 1; m < k; ++m) {
             const double d = D[i * n + med[m]];
             if (d < nd || i == med[m]) {  // a medoid is its own nearest
@@ -72,27 +119,7 @@ __global__ void km_assign_kernel(const double *D, int64_t n, int k, const int64_
         s.nd[i] = nd;
         s.sd[i] = sd;
     }
-}
-
-// loss = sum of nearest distances; rl[m] = removal loss of medoid m (sum of
-// second - nearest over its points).  One wavefront.
-// (dynamic LDS: k * 64 doubles, lane-private columns)
-__global__ __launch_bounds__(kWave) void km_totals_kernel(KmState s, int64_t n, int k, double *rl, double *loss) {
-    extern __shared__ double q[];
-    const int l = threadIdx.x;
-    for (int m = 0; m < k; ++m) q[m * kWave + l] = 0.0;
-    double p = 0.0;
-    for (int64_t o = l; o < n; o += kWave) {
-        const double nd = s.nd[o];
-        p += nd;
-        q[s.ni[o] * kWave + l] += s.sd[o] - nd;
-    }
-    p = km_tree(p);
-    if (l == 0) *loss = p;
-    for (int m = 0; m < k; ++m) {
-        const double v = km_tree(q[m * kWave + l]);
-        if (l == 0) rl[m] = v;
-    }
+    km_block_partials(k, ok, ni, nd, sd, part);
 }
 
 // Candidates j0 + blockIdx.x: loss change of the best swap bringing j in and
@@ -113,14 +140,27 @@ __global__ __launch_bounds__(kWave) void km_eval_kernel(const double *D, int64_t
     for (int m = 0; m < k; ++m) q[m * kWave + l] = 0.0;
     const double *row = D + j * n;
     double acc = 0.0;
-    for (int64_t o = l; o < n; o += kWave) {
-        const double djo = row[o], nd = s.nd[o], sd = s.sd[o];
-        const uint32_t ni = s.ni[o];
-        if (djo < nd) {  // j would be o's nearest: shared gain, and n(o)'s removal term is void
-            acc += djo - nd;
-            q[ni * kWave + l] += nd - sd;
-        } else if (djo < sd) {  // j would be o's second: n(o)'s removal term shrinks
-            q[ni * kWave + l] += djo - sd;
+    constexpr int U = 16;  // points per lane whose loads are in flight together
+    for (int64_t base = l; base < n; base += U * kWave) {
+        double dj[U], nd[U], sd[U];
+        uint32_t ni[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {  // unconditional loads (index clamped in bounds; unused past n)
+            const int64_t o = base + u * kWave < n ? base + u * kWave : n - 1;
+            dj[u] = row[o];
+            nd[u] = s.nd[o];
+            sd[u] = s.sd[o];
+            ni[u] = s.ni[o];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            // j would be o's nearest: shared gain, and n(o)'s removal term is void; j would be o's
+            // second: n(o)'s removal term shrinks.  Branch-free: otherwise (and past n) both terms
+            // are +0, which leaves the partials as they are (they are never -0: every term is <= 0).
+            const bool ok = base + u * kWave < n;
+            const bool nearer = ok && dj[u] < nd[u], second = ok && !nearer && dj[u] < sd[u];
+            acc += nearer ? dj[u] - nd[u] : 0.0;
+            q[ni[u] * kWave + l] += nearer ? nd[u] - sd[u] : (second ? dj[u] - sd[u] : 0.0);
         }
     }
     acc = km_tree(acc);
@@ -180,11 +220,17 @@ __device__ __forceinline__ void km_second(const double *D, int64_t n, int k, con
 
 // Medoid b is now point j (med already updated): every point's nearest and
 // second nearest.
-__global__ void km_swap_kernel(const double *D, int64_t n, int k, const int64_t *med, KmState s, uint32_t b,
-                               int64_t j) {
-    for (int64_t o = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; o < n; o += (int64_t)gridDim.x * blockDim.x) {
-        uint32_t ni = s.ni[o], si = s.si[o];
-        double nd = s.nd[o], sd = s.sd[o];
+__global__ __launch_bounds__(kKmBlock) void km_swap_kernel(const double *D, int64_t n, int k, const int64_t *med,
+                                                            KmState s, uint32_t b, int64_t j, double *part) {
+    const int64_t o = blockIdx.x * (int64_t)kKmBlock + threadIdx.x;
+    const bool ok = o < n;
+    uint32_t ni = 0, si = 0;
+    double nd = 0.0, sd = 0.0;
+    if (ok) {
+        ni = s.ni[o];
+        si = s.si[o];
+        nd = s.nd[o];
+        sd = s.sd[o];
         if (o == j) {
             if (ni != b) {
                 si = ni;
@@ -219,6 +265,7 @@ __global__ void km_swap_kernel(const double *D, int64_t n, int k, const int64_t 
         s.nd[o] = nd;
         s.sd[o] = sd;
     }
+    km_block_partials(k, ok, ni, nd, sd, part);
 }
 
 // Row sums without the diagonal, fixed order (k = 1).
@@ -249,7 +296,7 @@ __global__ void square_from_triu_kernel(const double *tri, int64_t n, double *D)
 }
 
 struct KmBuffers {
-    DevBuf ni, si, nd, sd, rl, loss, med, change, best, res, sums;
+    DevBuf ni, si, nd, sd, rl, loss, med, change, best, res, sums, part;
     int64_t *h_pin = nullptr;  // pinned: [0, 2) batch outcome, [2] loss, [8, 8 + 64) medoids
 };
 std::mutex g_km_mu;
@@ -331,13 +378,17 @@ extern "C" int tsm_kmedoids_fasterpam(const double *diss, int64_t n, int32_t k, 
             (rc = w.sd.ensure(8 * n)) || (rc = w.rl.ensure(8 * k)) || (rc = w.loss.ensure(8)) ||
             (rc = w.change.ensure(8 * n)) || (rc = w.best.ensure(4 * n)))
             return rc;
+        const int64_t nblk = (n + kKmBlock - 1) / kKmBlock;
+        if ((rc = w.part.ensure(sizeof(double) * nblk * (k + 1)))) return rc;
+        double *part = w.part.as<double>();
         KmState s{w.ni.as<uint32_t>(), w.si.as<uint32_t>(), w.nd.as<double>(), w.sd.as<double>()};
         int64_t *dmed = w.med.as<int64_t>();
         const size_t lds = sizeof(double) * kWave * (size_t)k;
         std::memcpy(h_med, med.data(), sizeof(int64_t) * k);
         TSM_HIP(hipMemcpyAsync(dmed, h_med, sizeof(int64_t) * k, hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(km_assign_kernel, dim3(km_grid(n)), dim3(256), 0, st, diss, n, (int)k, dmed, s);
-        hipLaunchKernelGGL(km_totals_kernel, dim3(1), dim3(kWave), lds, st, s, n, (int)k, w.rl.as<double>(),
+        hipLaunchKernelGGL(km_assign_kernel, dim3((unsigned)nblk), dim3(kKmBlock), 0, st, diss, n, (int)k, dmed, s,
+                           part);
+        hipLaunchKernelGGL(km_blocksum_kernel, dim3(1), dim3(kWave), 0, st, part, nblk, (int)k, w.rl.as<double>(),
                            w.loss.as<double>());
         TSM_HIP(hipGetLastError());
         const int64_t kMinBatch = 512;
@@ -370,9 +421,10 @@ extern "C" int tsm_kmedoids_fasterpam(const double *diss, int64_t n, int32_t k, 
                 med[b] = c;
                 h_med[b] = c;  // the previous copies finished at the last synchronisation
                 TSM_HIP(hipMemcpyAsync(dmed + b, h_med + b, sizeof(int64_t), hipMemcpyHostToDevice, st));
-                hipLaunchKernelGGL(km_swap_kernel, dim3(km_grid(n)), dim3(256), 0, st, diss, n, (int)k, dmed, s, b, c);
-                hipLaunchKernelGGL(km_totals_kernel, dim3(1), dim3(kWave), lds, st, s, n, (int)k, w.rl.as<double>(),
-                                   w.loss.as<double>());
+                hipLaunchKernelGGL(km_swap_kernel, dim3((unsigned)nblk), dim3(kKmBlock), 0, st, diss, n, (int)k, dmed,
+                                   s, b, c, part);
+                hipLaunchKernelGGL(km_blocksum_kernel, dim3(1), dim3(kWave), 0, st, part, nblk, (int)k,
+                                   w.rl.as<double>(), w.loss.as<double>());
                 TSM_HIP(hipGetLastError());
                 j = c + 1;
                 batch = batch / 2 > kMinBatch ? batch / 2 : (kMinBatch < n ? kMinBatch : n);
