@@ -207,6 +207,44 @@ def test_split_bf16_search_distances(gpu):
     assert np.all(np.abs(dist.cpu().numpy() - want) <= 5e-5 * scale)
 
 
+@pytest.mark.parametrize("M,N", [(1, 1000), (100, 3000), (128, 70001), (300, 20000), (1000, 70001), (1300, 9000)])
+def test_search_filter_gemm_matches_dense(gpu, M, N):
+    """The fused filter GEMM (K = 384: the persistent search kernel) appends
+    exactly the (column, value) pairs of the dense GEMM that lie below each
+    row's threshold, with bit-identical values (same MFMA accumulation
+    order)."""
+    import torch
+
+    from trajsimimeasures_amd import _lib
+    from trajsimimeasures_amd import filter as F
+    print(f"[case] search filter gemm M={M} N={N}", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(M + N)
+    A = torch.randn((M, 384), device="cuda", generator=g).to(torch.bfloat16)
+    B = torch.randn((N, 384), device="cuda", generator=g).to(torch.bfloat16)
+    bias = torch.randn(N, device="cuda", generator=g)
+    dense = torch.empty((M, N), dtype=torch.float32, device="cuda")
+    F.gemm_nt(A.view(torch.int16), B.view(torch.int16), dense, alpha=-2.0, bias=bias)
+    k = max(1, min(N // 50, 300))
+    tau = torch.kthvalue(dense, k, dim=1).values.contiguous()  # about k values per row below tau
+    cap = 4096
+    cnt = torch.zeros(M, dtype=torch.int32, device="cuda")
+    key = torch.empty((M, cap), dtype=torch.float32, device="cuda")
+    idx = torch.empty((M, cap), dtype=torch.int32, device="cuda")
+    L = F._nn()
+    P = lambda t: t.data_ptr()
+    _lib.check(L.tsm_nn_gemm_nt_filter(M, N, 384, P(A), 384, P(B), 384, -2.0, P(bias), P(tau), cap, P(cnt), P(key),
+                                       P(idx), F._s()))
+    torch.cuda.synchronize()
+    d, t, c = dense.cpu().numpy(), tau.cpu().numpy(), cnt.cpu().numpy()
+    kk, ii = key.cpu().numpy(), idx.cpu().numpy()
+    for r in range(M):
+        want = np.nonzero(d[r] < t[r])[0]
+        assert c[r] == want.size, (r, c[r], want.size)
+        order = np.argsort(ii[r, :c[r]])
+        assert np.array_equal(ii[r, :c[r]][order], want)
+        assert np.array_equal(kk[r, :c[r]][order], d[r, want])
+
+
 def test_threshold_search_matches_full_search(gpu):
     """The fused threshold search (sampled threshold, filtered GEMM, per-row
     sort) returns the same C nearest database rows as the full distance rows

@@ -119,6 +119,10 @@ struct FilterArgs {
 
 // Global -> registers -> LDS staging of a ROWS x kBK BF16 tile by NT
 // threads, 16 bytes (8 values of one row) per chunk.
+// Branch-free: every lane loads from an index clamped into the operand and
+// zeroes what lies outside it, so the loads stay straight-line code and the
+// compiler can wait for exactly the ones it needs (a load under a branch
+// makes it wait for every load in flight).
 template <int ROWS, int NT>
 __device__ __forceinline__ void tile_load(const uint16_t *base, int ld, int row0, int rows, int k0, int K,
                                           uint4 (&r)[ROWS * 8 / NT]) {
@@ -126,8 +130,31 @@ __device__ __forceinline__ void tile_load(const uint16_t *base, int ld, int row0
     for (int i = 0; i < ROWS * 8 / NT; ++i) {
         const int q = (int)threadIdx.x + NT * i;
         const int row = row0 + (q >> 3), k = k0 + (q & 7) * 8;
-        r[i] = (row < rows && k < K) ? *reinterpret_cast<const uint4 *>(base + (int64_t)row * ld + k)
-                                     : make_uint4(0, 0, 0, 0);
+        const bool ok = row < rows && k < K;
+        const uint4 v = *reinterpret_cast<const uint4 *>(base + (int64_t)(row < rows ? row : rows - 1) * ld +
+                                                         (k < K ? k : 0));
+        r[i] = ok ? v : make_uint4(0, 0, 0, 0);
+    }
+}
+// Unconditional form for operands whose K range is whole steps: rows past
+// `rows` read row rows - 1 (their products land in columns nobody keeps).
+// Native vectors (not the uint4 struct), so the values stay in registers.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+template <int ROWS, int NT>
+__device__ __forceinline__ void tile_load_rows(const uint16_t *base, int ld, int row0, int rows, int k0,
+                                               u32x4 (&r)[ROWS * 8 / NT]) {
+#pragma unroll
+    for (int i = 0; i < ROWS * 8 / NT; ++i) {
+        const int q = (int)threadIdx.x + NT * i;
+        const int row = min(row0 + (q >> 3), rows - 1), k = k0 + (q & 7) * 8;
+        r[i] = *reinterpret_cast<const u32x4 *>(base + (int64_t)row * ld + k);
+    }
+}
+template <int ROWS, int NT> __device__ __forceinline__ void tile_store_v(uint16_t *s, const u32x4 (&r)[ROWS * 8 / NT]) {
+#pragma unroll
+    for (int i = 0; i < ROWS * 8 / NT; ++i) {
+        const int q = (int)threadIdx.x + NT * i;
+        *reinterpret_cast<u32x4 *>(s + (q >> 3) * kLdsStride + (q & 7) * 8) = r[i];
     }
 }
 template <int ROWS, int NT> __device__ __forceinline__ void tile_store(uint16_t *s, const uint4 (&r)[ROWS * 8 / NT]) {
@@ -227,6 +254,149 @@ __global__ __launch_bounds__(2 * BM) void gemm_nt_lds_kernel(GemmArgs g, FilterA
                     *c = g.beta == 0.f ? v : v + g.beta * *c;
                 }
             }
+    }
+}
+
+// Persistent search GEMM (the embedding search: M queries x N database rows,
+// K = 32 * KC <= 384).  A 256-thread block owns 128 query rows, whose whole A
+// operand it stages in LDS once, and streams 128-column database tiles
+// through double-buffered LDS in K steps of 64, tile after tile, with no
+// per-tile prologue.  The database loads run a full tile ahead: step s of
+// tile t stores the registers loaded during step s of tile t - 1 and reloads
+// them with step s of tile t + 1, so each load has the six steps of a tile to
+// arrive (a load issued one step ahead would stall every step on HBM / L2
+// latency at this occupancy).  The filter epilogue runs after each tile's
+// last step.  The grid has one block per CU-sized share: block b runs on XCD
+// b % 8 (round-robin dispatch), and the blocks of one XCD take consecutive
+// database chunks with all query blocks of a chunk on that XCD, so a database
+// tile comes from HBM once per XCD and from its L2 for the other query
+// blocks.  Accumulation order per output equals gemm_nt_lds_kernel's (K
+// chunks of 32 in order), so values are bit-identical to the stored GEMM's.
+constexpr int kSearchRows = 128;
+
+// The one-tile-ahead register ring of the search kernel: six named tiles
+// picked at compile time (an array here ends up in scratch memory).
+struct SearchRing {
+    u32x4 r0[4], r1[4], r2[4], r3[4], r4[4], r5[4];
+    template <int S> __device__ __forceinline__ u32x4 (&at())[4] {
+        if constexpr (S == 0) return r0;
+        else if constexpr (S == 1) return r1;
+        else if constexpr (S == 2) return r2;
+        else if constexpr (S == 3) return r3;
+        else if constexpr (S == 4) return r4;
+        else return r5;
+    }
+};
+
+// One K step s of tile t: store the ring slot (this tile's step s) into LDS,
+// reload it with tile tn's step s, then multiply from LDS.
+template <int s>
+__device__ __forceinline__ void search_step(const GemmArgs &g, SearchRing &ring, uint16_t *As, int AS,
+                                           uint16_t (*Bs)[kBN * kLdsStride], int &cur, int tn, int wm, int wn,
+                                           int r16, int kb, f32x4 (&acc)[4][4]) {
+    tile_store_v<kBN, 256>(Bs[cur], ring.at<s>());
+    tile_load_rows<kBN, 256>(g.B, g.ldb, tn * kBN, g.N, s * kBK, ring.at<s>());
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk) {
+        bf16x8 af[4], bfr[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+            af[i] = __builtin_bit_cast(
+                bf16x8, *reinterpret_cast<const uint4 *>(&As[(wm + 16 * i + r16) * AS + 64 * s + 32 * kk + kb]));
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            bfr[j] = __builtin_bit_cast(
+                bf16x8, *reinterpret_cast<const uint4 *>(&Bs[cur][(wn + 16 * j + r16) * kLdsStride + 32 * kk + kb]));
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
+    }
+    cur ^= 1;
+}
+
+template <int KC>
+__global__ __launch_bounds__(256) void search_filter_kernel(GemmArgs g, FilterArgs f, int qblocks, int chunks,
+                                                            int tiles_per_chunk) {
+    static_assert(KC == 12, "the register ring holds six K steps of 64");
+    constexpr int AS = 32 * KC + 8;      // A row stride in LDS (BF16), padded
+    __shared__ __attribute__((aligned(16))) uint16_t As[kSearchRows * AS];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][kBN * kLdsStride];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r16 = lane & 15, kb = (lane >> 4) * 8;
+    const int xcd = blockIdx.x & 7, jx = blockIdx.x >> 3;
+    const int q = jx % qblocks, c = xcd * (chunks / 8) + jx / qblocks;
+    const int ntiles = (g.N + kBN - 1) / kBN;
+    const int t0 = c * tiles_per_chunk, t1 = min(ntiles, t0 + tiles_per_chunk);
+    if (t0 >= t1) return;  // block-uniform
+    const int bm = q * kSearchRows, wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    // the block's A rows, all of K, into LDS (16-byte chunks; zero past M / K)
+    for (int e = threadIdx.x; e < kSearchRows * 4 * KC; e += 256) {
+        const int row = e / (4 * KC), k = (e % (4 * KC)) * 8;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (bm + row < g.M && k < g.K) v = *reinterpret_cast<const uint4 *>(g.A + (int64_t)(bm + row) * g.lda + k);
+        *reinterpret_cast<uint4 *>(&As[row * AS + k]) = v;
+    }
+    // the thresholds of this lane's 16 output rows, once per block (-inf past M: nothing passes)
+    float taur[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int row = bm + wm + 16 * i + 4 * (lane >> 4) + e;
+            const float tv = f.tau[row < g.M ? row : g.M - 1];
+            taur[i][e] = row < g.M ? tv : -__builtin_huge_valf();
+        }
+    SearchRing ring;  // step s of the next tile
+    tile_load_rows<kBN, 256>(g.B, g.ldb, t0 * kBN, g.N, 0 * kBK, ring.at<0>());
+    tile_load_rows<kBN, 256>(g.B, g.ldb, t0 * kBN, g.N, 1 * kBK, ring.at<1>());
+    tile_load_rows<kBN, 256>(g.B, g.ldb, t0 * kBN, g.N, 2 * kBK, ring.at<2>());
+    tile_load_rows<kBN, 256>(g.B, g.ldb, t0 * kBN, g.N, 3 * kBK, ring.at<3>());
+    tile_load_rows<kBN, 256>(g.B, g.ldb, t0 * kBN, g.N, 4 * kBK, ring.at<4>());
+    tile_load_rows<kBN, 256>(g.B, g.ldb, t0 * kBN, g.N, 5 * kBK, ring.at<5>());
+    int cur = 0;
+    for (int t = t0; t < t1; ++t) {
+        f32x4 acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const int tn = t + 1 < t1 ? t + 1 : t;  // the last tile reloads itself (unused)
+        search_step<0>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
+        search_step<1>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
+        search_step<2>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
+        search_step<3>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
+        search_step<4>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
+        search_step<5>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
+        // filter epilogue of tile t: biases of the lane's 4 columns in one batch, then
+        // compare (hits are rare: about 4 C per query over the whole database)
+        float bcol[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = t * kBN + wn + 16 * j + r16;
+            const float bv = g.bias ? g.bias[col < g.N ? col : g.N - 1] : 0.f;
+            bcol[j] = bv;
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int col = t * kBN + wn + 16 * j + r16;
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const float v = g.alpha * acc[i][j][e] + bcol[j];
+                    if (col < g.N && v < taur[i][e]) {
+                        const int row = bm + wm + 16 * i + 4 * (lane >> 4) + e;
+                        const uint32_t pos = atomicAdd(&f.cnt[row], 1u);
+                        if (pos < (uint32_t)f.cap) {
+                            f.key[(int64_t)row * f.cap + pos] = v;
+                            f.idx[(int64_t)row * f.cap + pos] = col;
+                        }
+                    }
+                }
+        }
     }
 }
 
@@ -744,7 +914,17 @@ int tsm_nn_gemm_nt_filter(int M, int N, int K, const uint16_t *A, int lda, const
     if ((M + 255) / 256 > 65535) return fail(TSM_E_ARG, "gemm_filter: M too large");
     GemmArgs g{M, N, K, A, lda, B, ldb, nullptr, 0, alpha, 0.f, bias};
     const FilterArgs fa{tau, cap, cnt, key, idx};
-    if (M > kBM) {  // 256 query rows per block: the database is streamed half as often
+    if (K == 384) {  // the embedding search: persistent blocks (K in whole steps of 64)
+        int dev = 0, ncu = 256;
+        TSM_HIP(hipGetDevice(&dev));
+        TSM_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+        const int qblocks = (M + kSearchRows - 1) / kSearchRows, ntiles = (N + kBN - 1) / kBN;
+        int chunks = (ncu / qblocks) / 8 * 8;
+        if (chunks < 8) chunks = 8;
+        const int tpc = (ntiles + chunks - 1) / chunks;
+        hipLaunchKernelGGL(search_filter_kernel<12>, dim3(qblocks * chunks), dim3(256), 0,
+                           reinterpret_cast<hipStream_t>(stream), g, fa, qblocks, chunks, tpc);
+    } else if (M > kBM) {  // 256 query rows per block: the database is streamed half as often
         dim3 grid((N + kBN - 1) / kBN, (M + 255) / 256);
         hipLaunchKernelGGL((gemm_nt_lds_kernel<1, 256>), grid, dim3(512), 0, reinterpret_cast<hipStream_t>(stream), g,
                            fa, 0);
