@@ -81,6 +81,12 @@ int tsm_nn_linear_f32(const float *X, int64_t n, int K, const float *W, int Dout
                       void *stream);
 /* BF16 hi/lo split of FP32 rows into [hi, lo, hi] (query_side = 0) or
  * [hi, hi, lo] (query_side = 1), out [n, 3D]; norm[n] (or NULL) = |E_r|^2. */
+/* The whole GRU forward over T steps (Gx: [n, T, 3H] input projections; Hs:
+ * [T + 1, n, H] with Hs[0] the initial state; Hb: BF16 copy of the current h;
+ * Gh: [n, 3H] scratch; r/z/nn/ghn: [T, n, H] when keep, else [1, n, H]):
+ * per step the recurrent GEMM and tsm_nn_gru_step_fwd, launched from C. */
+int tsm_nn_gru_forward(const float *Gx, int T, int n, int Hd, const uint16_t *Whh, const float *bhh, float *Hs,
+                       uint16_t *Hb, float *Gh, float *r, float *z, float *nn, float *ghn, int keep, void *stream);
 int tsm_nn_split_bf16(const float *E, int64_t n, int D, int query_side, uint16_t *out, float *norm, void *stream);
 /* Squared row norms of a bf16 [n, D] matrix. */
 int tsm_nn_rownorm_bf16(const uint16_t *E, int64_t n, int D, float *out, void *stream);

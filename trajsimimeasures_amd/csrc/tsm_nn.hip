@@ -771,12 +771,18 @@ __global__ void linear_f32_kernel(const float *X, int64_t n, int K, const float 
 // Split FP32 rows E [n, D] into BF16 hi and lo = bf16(e - hi) parts laid
 // out as [hi, lo, hi] (database side) or [hi, hi, lo] (query side) in
 // out [n, 3D], and the FP32 squared row norms of E.
-__global__ void split_bf16_kernel(const float *E, int64_t n, int D, int query_side, uint16_t *out, float *norm) {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+// One wavefront per row: lanes over the D values (coalesced), the row's
+// squared norm as a fixed-order wave sum (lane partials over q = lane,
+// lane + 64, ..., then a butterfly).
+__global__ __launch_bounds__(256) void split_bf16_kernel(const float *E, int64_t n, int D, int query_side,
+                                                         uint16_t *out, float *norm) {
+    const int lane = threadIdx.x & 63;
+    for (int64_t r = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; r < n;
+         r += ((int64_t)gridDim.x * blockDim.x) >> 6) {
         const float *e = E + r * D;
         uint16_t *o = out + r * 3 * D;
         float s = 0.f;
-        for (int q = 0; q < D; ++q) {
+        for (int q = lane; q < D; q += 64) {
             const float v = e[q];
             s = fmaf(v, v, s);
             const uint16_t hi = f2bf(v);
@@ -785,7 +791,9 @@ __global__ void split_bf16_kernel(const float *E, int64_t n, int D, int query_si
             o[D + q] = query_side ? hi : lo;
             o[2 * D + q] = query_side ? lo : hi;
         }
-        if (norm) norm[r] = s;
+#pragma unroll
+        for (int w = 32; w >= 1; w >>= 1) s += __shfl_xor(s, w, 64);
+        if (norm && lane == 0) norm[r] = s;
     }
 }
 
@@ -1032,6 +1040,22 @@ int tsm_nn_gru_step_fwd(const float *Gx, int64_t ldgx, const float *Gh, const fl
     TSM_LAUNCHED();
 }
 
+int tsm_nn_gru_forward(const float *Gx, int T, int n, int Hd, const uint16_t *Whh, const float *bhh, float *Hs,
+                       uint16_t *Hb, float *Gh, float *r, float *z, float *nn, float *ghn, int keep, void *stream) {
+    if (T < 1 || n < 0 || Hd < 1 || (Hd % 8) != 0) return fail(TSM_E_ARG, "gru_forward: bad arguments");
+    if (n == 0) return TSM_OK;
+    const int64_t nh = (int64_t)n * Hd;
+    for (int t = 0; t < T; ++t) {
+        int rc = tsm_nn_gemm_nt(n, 3 * Hd, Hd, Hb, Hd, Whh, Hd, Gh, 3 * Hd, 1.f, 0.f, bhh, stream);
+        if (rc) return rc;
+        const int64_t k = keep ? t : 0;
+        rc = tsm_nn_gru_step_fwd(Gx + (int64_t)t * 3 * Hd, (int64_t)T * 3 * Hd, Gh, Hs + t * nh, n, Hd, r + k * nh,
+                                 z + k * nh, nn + k * nh, ghn + k * nh, Hs + (t + 1) * nh, Hb, stream);
+        if (rc) return rc;
+    }
+    return TSM_OK;
+}
+
 int tsm_nn_gru_step_bwd(const float *dh, const float *h, const float *r, const float *z, const float *nn,
                         const float *ghn, int n, int Hd, float *dGx, int64_t lddgx, float *dGh, uint16_t *dGhb,
                         float *dhp, void *stream) {
@@ -1108,7 +1132,7 @@ int tsm_nn_linear_f32(const float *X, int64_t n, int K, const float *W, int Dout
 int tsm_nn_split_bf16(const float *E, int64_t n, int D, int query_side, uint16_t *out, float *norm, void *stream) {
     if (n < 0 || D < 1 || !E || !out) return fail(TSM_E_ARG, "split: bad arguments");
     if (n == 0) return TSM_OK;
-    hipLaunchKernelGGL(split_bf16_kernel, dim3(grid1(n)), dim3(256), 0, TSM_ST, E, n, D, query_side, out, norm);
+    hipLaunchKernelGGL(split_bf16_kernel, dim3(grid1(n * 64)), dim3(256), 0, TSM_ST, E, n, D, query_side, out, norm);
     TSM_LAUNCHED();
 }
 

@@ -74,10 +74,11 @@ def _nn():
         L.tsm_nn_gemm_nt_filter.argtypes = [i, i, i, P, i, P, i, f, P, P, i, P, P, P, P]
         L.tsm_nn_select_candidates.argtypes = [P, P, P, i, i, i, P, P, P]
         L.tsm_nn_gemm_nt_acc.argtypes = [i, i, i, P, i, P, i, P, i, f, P]
+        L.tsm_nn_gru_forward.argtypes = [P, i, i, i, P, P, P, P, P, P, P, P, P, i, P]
         for n in ("gemm_nt", "cast_bf16", "resample", "conv_fwd", "conv_bwd", "gru_h0", "gru_h0_bwd",
                   "gru_step_fwd", "gru_step_bwd", "colsum", "rank_loss", "adam", "rownorm_bf16", "topk_f32",
                   "topk_f64", "linear_f32", "split_bf16", "gemm_nt_filter", "select_candidates",
-                  "gemm_nt_acc"):
+                  "gemm_nt_acc", "gru_forward"):
             getattr(L, "tsm_nn_" + n).restype = ctypes.c_int
         L._nn_declared = True
     return L
@@ -210,12 +211,9 @@ class Encoder:
             R, Z, NN, GHN = (torch.empty((T, n, H), dtype=torch.float32, device=dev) for _ in range(4))
         else:
             R, Z, NN, GHN = (torch.empty((1, n, H), dtype=torch.float32, device=dev) for _ in range(4))
-        for t in range(T):
-            gemm_nt(Hb, self.W_hh_b, Gh, bias=w["b_hh"])
-            k = t if keep else 0
-            gx_t = Gx.view(n, T, G3)[:, t, :]
-            _lib.check(L.tsm_nn_gru_step_fwd(_p(gx_t), T * G3, _p(Gh), _p(Hs[t]), n, H, _p(R[k]), _p(Z[k]), _p(NN[k]),
-                                             _p(GHN[k]), _p(Hs[t + 1]), _p(Hb), s))
+        # T steps of recurrent GEMM + gate kernel, launched from C
+        _lib.check(L.tsm_nn_gru_forward(_p(Gx), T, n, H, _p(self.W_hh_b), _p(w["b_hh"]), _p(Hs), _p(Hb), _p(Gh),
+                                        _p(R), _p(Z), _p(NN), _p(GHN), int(keep), s))
         E = torch.empty((n, D), dtype=torch.float32, device=dev)
         gemm_nt(Hb, self.W_o_b, E, bias=w["b_o"])
         _lib.check(L.tsm_nn_linear_f32(_p(X), n, 2 * T, _p(self.S), D, _p(E), 1.0, s))  # fixed skip map, FP32
