@@ -258,7 +258,7 @@ __global__ __launch_bounds__(2 * BM) void gemm_nt_lds_kernel(GemmArgs g, FilterA
 }
 
 // Persistent search GEMM (the embedding search: M queries x N database rows,
-// K = 32 * KC <= 384).  A 256-thread block owns 128 query rows, whose whole A
+// K = 32 * KC <= 384).  A 512-thread block owns 128 query rows, whose whole A
 // operand it stages in LDS once, and streams 128-column database tiles
 // through double-buffered LDS in K steps of 64, tile after tile, with no
 // per-tile prologue.  The database loads run a full tile ahead: step s of
@@ -276,9 +276,11 @@ constexpr int kSearchRows = 128;
 
 // The one-tile-ahead register ring of the search kernel: six named tiles
 // picked at compile time (an array here ends up in scratch memory).
+constexpr int kSearchThreads = 512;
+constexpr int kRingVec = kBN * 8 / kSearchThreads;  // 16-byte loads per thread per step
 struct SearchRing {
-    u32x4 r0[4], r1[4], r2[4], r3[4], r4[4], r5[4];
-    template <int S> __device__ __forceinline__ u32x4 (&at())[4] {
+    u32x4 r0[kRingVec], r1[kRingVec], r2[kRingVec], r3[kRingVec], r4[kRingVec], r5[kRingVec];
+    template <int S> __device__ __forceinline__ u32x4 (&at())[kRingVec] {
         if constexpr (S == 0) return r0;
         else if constexpr (S == 1) return r1;
         else if constexpr (S == 2) return r2;
@@ -289,19 +291,20 @@ struct SearchRing {
 };
 
 // One K step s of tile t: store the ring slot (this tile's step s) into LDS,
-// reload it with tile tn's step s, then multiply from LDS.
+// reload it with tile tn's step s, then multiply from LDS.  A wave owns 32
+// rows x 64 columns (2 x 4 MFMA tiles).
 template <int s>
 __device__ __forceinline__ void search_step(const GemmArgs &g, SearchRing &ring, uint16_t *As, int AS,
                                            uint16_t (*Bs)[kBN * kLdsStride], int &cur, int tn, int wm, int wn,
-                                           int r16, int kb, f32x4 (&acc)[4][4]) {
-    tile_store_v<kBN, 256>(Bs[cur], ring.at<s>());
-    tile_load_rows<kBN, 256>(g.B, g.ldb, tn * kBN, g.N, s * kBK, ring.at<s>());
+                                           int r16, int kb, f32x4 (&acc)[2][4]) {
+    tile_store_v<kBN, kSearchThreads>(Bs[cur], ring.at<s>());
+    tile_load_rows<kBN, kSearchThreads>(g.B, g.ldb, tn * kBN, g.N, s * kBK, ring.at<s>());
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk) {
-        bf16x8 af[4], bfr[4];
+        bf16x8 af[2], bfr[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 2; ++i)
             af[i] = __builtin_bit_cast(
                 bf16x8, *reinterpret_cast<const uint4 *>(&As[(wm + 16 * i + r16) * AS + 64 * s + 32 * kk + kb]));
 #pragma unroll
@@ -309,7 +312,7 @@ __device__ __forceinline__ void search_step(const GemmArgs &g, SearchRing &ring,
             bfr[j] = __builtin_bit_cast(
                 bf16x8, *reinterpret_cast<const uint4 *>(&Bs[cur][(wn + 16 * j + r16) * kLdsStride + 32 * kk + kb]));
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
@@ -318,8 +321,8 @@ __device__ __forceinline__ void search_step(const GemmArgs &g, SearchRing &ring,
 }
 
 template <int KC>
-__global__ __launch_bounds__(256) void search_filter_kernel(GemmArgs g, FilterArgs f, int qblocks, int chunks,
-                                                            int tiles_per_chunk) {
+__global__ __launch_bounds__(kSearchThreads, 2) void search_filter_kernel(GemmArgs g, FilterArgs f, int qblocks,
+                                                                          int chunks, int tiles_per_chunk) {
     static_assert(KC == 12, "the register ring holds six K steps of 64");
     constexpr int AS = 32 * KC + 8;      // A row stride in LDS (BF16), padded
     __shared__ __attribute__((aligned(16))) uint16_t As[kSearchRows * AS];
@@ -331,18 +334,18 @@ __global__ __launch_bounds__(256) void search_filter_kernel(GemmArgs g, FilterAr
     const int ntiles = (g.N + kBN - 1) / kBN;
     const int t0 = c * tiles_per_chunk, t1 = min(ntiles, t0 + tiles_per_chunk);
     if (t0 >= t1) return;  // block-uniform
-    const int bm = q * kSearchRows, wm = (wave >> 1) * 64, wn = (wave & 1) * 64;
+    const int bm = q * kSearchRows, wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
     // the block's A rows, all of K, into LDS (16-byte chunks; zero past M / K)
-    for (int e = threadIdx.x; e < kSearchRows * 4 * KC; e += 256) {
+    for (int e = threadIdx.x; e < kSearchRows * 4 * KC; e += kSearchThreads) {
         const int row = e / (4 * KC), k = (e % (4 * KC)) * 8;
         uint4 v = make_uint4(0, 0, 0, 0);
         if (bm + row < g.M && k < g.K) v = *reinterpret_cast<const uint4 *>(g.A + (int64_t)(bm + row) * g.lda + k);
         *reinterpret_cast<uint4 *>(&As[row * AS + k]) = v;
     }
-    // the thresholds of this lane's 16 output rows, once per block (-inf past M: nothing passes)
-    float taur[4][4];
+    // the thresholds of this lane's 8 output rows, once per block (-inf past M: nothing passes)
+    float taur[2][4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
             const int row = bm + wm + 16 * i + 4 * (lane >> 4) + e;
@@ -350,17 +353,17 @@ __global__ __launch_bounds__(256) void search_filter_kernel(GemmArgs g, FilterAr
             taur[i][e] = row < g.M ? tv : -__builtin_huge_valf();
         }
     SearchRing ring;  // step s of the next tile
-    tile_load_rows<kBN, 256>(g.B, g.ldb, t0 * kBN, g.N, 0 * kBK, ring.at<0>());
-    tile_load_rows<kBN, 256>(g.B, g.ldb, t0 * kBN, g.N, 1 * kBK, ring.at<1>());
-    tile_load_rows<kBN, 256>(g.B, g.ldb, t0 * kBN, g.N, 2 * kBK, ring.at<2>());
-    tile_load_rows<kBN, 256>(g.B, g.ldb, t0 * kBN, g.N, 3 * kBK, ring.at<3>());
-    tile_load_rows<kBN, 256>(g.B, g.ldb, t0 * kBN, g.N, 4 * kBK, ring.at<4>());
-    tile_load_rows<kBN, 256>(g.B, g.ldb, t0 * kBN, g.N, 5 * kBK, ring.at<5>());
+    tile_load_rows<kBN, kSearchThreads>(g.B, g.ldb, t0 * kBN, g.N, 0 * kBK, ring.at<0>());
+    tile_load_rows<kBN, kSearchThreads>(g.B, g.ldb, t0 * kBN, g.N, 1 * kBK, ring.at<1>());
+    tile_load_rows<kBN, kSearchThreads>(g.B, g.ldb, t0 * kBN, g.N, 2 * kBK, ring.at<2>());
+    tile_load_rows<kBN, kSearchThreads>(g.B, g.ldb, t0 * kBN, g.N, 3 * kBK, ring.at<3>());
+    tile_load_rows<kBN, kSearchThreads>(g.B, g.ldb, t0 * kBN, g.N, 4 * kBK, ring.at<4>());
+    tile_load_rows<kBN, kSearchThreads>(g.B, g.ldb, t0 * kBN, g.N, 5 * kBK, ring.at<5>());
     int cur = 0;
     for (int t = t0; t < t1; ++t) {
-        f32x4 acc[4][4];
+        f32x4 acc[2][4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i)
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
         const int tn = t + 1 < t1 ? t + 1 : t;  // the last tile reloads itself (unused)
@@ -383,7 +386,7 @@ __global__ __launch_bounds__(256) void search_filter_kernel(GemmArgs g, FilterAr
         for (int j = 0; j < 4; ++j) {
             const int col = t * kBN + wn + 16 * j + r16;
 #pragma unroll
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const float v = g.alpha * acc[i][j][e] + bcol[j];
@@ -930,7 +933,7 @@ int tsm_nn_gemm_nt_filter(int M, int N, int K, const uint16_t *A, int lda, const
         int chunks = (ncu / qblocks) / 8 * 8;
         if (chunks < 8) chunks = 8;
         const int tpc = (ntiles + chunks - 1) / chunks;
-        hipLaunchKernelGGL(search_filter_kernel<12>, dim3(qblocks * chunks), dim3(256), 0,
+        hipLaunchKernelGGL(search_filter_kernel<12>, dim3(qblocks * chunks), dim3(kSearchThreads), 0,
                            reinterpret_cast<hipStream_t>(stream), g, fa, qblocks, chunks, tpc);
     } else if (M > kBM) {  // 256 query rows per block: the database is streamed half as often
         dim3 grid((N + kBN - 1) / kBN, (M + 255) / 256);
