@@ -60,6 +60,7 @@ struct SArgs {
 };
 
 constexpr int64_t kSarMaxSamples = int64_t(1) << 24;
+constexpr int kSarStage = 1024;  // segments of a stretch whose grid indices are staged in LDS
 
 __device__ __forceinline__ double sar_nan() { return __builtin_bit_cast(double, 0x7ff8000000000000ull); }
 
@@ -174,11 +175,13 @@ __device__ __forceinline__ int64_t wave_max_i64(int64_t v) {
 // ---- data-parallel path (non-decreasing timestamps) -----------------------
 
 // One stretch: points [is, ie] of a trajectory, samples on grid indices
-// [g0, g1) plus the last point when fin.
+// [g0, g1) plus the last point when fin.  gl, when set, holds
+// ceil(t_k / ts) - g0 for the segments k in [is, ie) (staged in LDS).
 struct SarSpan {
     const double *xy, *t;
     int64_t is, ie, g0, g1;
     bool fin;
+    const int32_t *gl;
     __device__ int64_t count() const { return g1 - g0 + (fin ? 1 : 0); }
     __device__ float first_time(int64_t ts) const { return g1 > g0 ? (float)(g0 * ts) : (float)t[ie]; }
     __device__ float last_time(int64_t ts) const { return fin ? (float)t[ie] : (float)((g1 - 1) * ts); }
@@ -188,10 +191,19 @@ struct SarSpan {
         const int64_t g = g0 + m;
         if (g >= g1) return make_float2((float)xy[2 * ie], (float)xy[2 * ie + 1]);
         int64_t lo = is, hi = ie - 1;
-        while (lo < hi) {
-            const int64_t mid = lo + (hi - lo + 1) / 2;
-            if (sar_grid(t[mid], tsd) <= g) lo = mid;
-            else hi = mid - 1;
+        if (gl) {
+            const int64_t gr = g - g0;
+            while (lo < hi) {
+                const int64_t mid = lo + (hi - lo + 1) / 2;
+                if (gl[mid - is] <= gr) lo = mid;
+                else hi = mid - 1;
+            }
+        } else {
+            while (lo < hi) {
+                const int64_t mid = lo + (hi - lo + 1) / 2;
+                if (sar_grid(t[mid], tsd) <= g) lo = mid;
+                else hi = mid - 1;
+            }
         }
         return sar_interp(xy, t, lo, g * ts);
     }
@@ -343,6 +355,7 @@ __device__ double sar_serial(const SArgs &a, const double *xa, const double *ta,
 
 __global__ __launch_bounds__(kWave) void sar_kernel(SArgs a) {
     __shared__ float4 chunk[kWave];  // (A.x, A.y, B.x, B.y) of 64 consecutive samples
+    __shared__ int32_t gla[kSarStage], glb[kSarStage];
     const int lane = threadIdx.x;
     const double tsd = (double)a.ts;
     for (int64_t p = blockIdx.x; p < a.npairs; p += gridDim.x) {
@@ -388,8 +401,24 @@ __global__ __launch_bounds__(kWave) void sar_kernel(SArgs a) {
             je = wave_max_i64(je);
             res = 0.0;
             if (ie >= 0 && je >= 0) {
-                SarSpan SA{xa, ta, is, ie, sar_grid(ta[is], tsd), sar_grid(ta[ie], tsd), fmod(ta[ie], tsd) == 0.0};
-                SarSpan SB{xb, tb, js, je, sar_grid(tb[js], tsd), sar_grid(tb[je], tsd), fmod(tb[je], tsd) == 0.0};
+                SarSpan SA{xa, ta, is, ie, sar_grid(ta[is], tsd), sar_grid(ta[ie], tsd), fmod(ta[ie], tsd) == 0.0,
+                           nullptr};
+                SarSpan SB{xb, tb, js, je, sar_grid(tb[js], tsd), sar_grid(tb[je], tsd), fmod(tb[je], tsd) == 0.0,
+                           nullptr};
+                // grid indices of both stretches' segments into LDS (coalesced, one division
+                // each) for the per-sample binary searches; longer stretches search in memory
+                __syncthreads();  // the previous pair's readers are done
+                if (ie - is <= kSarStage) {
+                    for (int64_t k = lane; k < ie - is; k += kWave)
+                        gla[k] = (int32_t)(sar_grid(ta[is + k], tsd) - SA.g0);
+                    SA.gl = gla;
+                }
+                if (je - js <= kSarStage) {
+                    for (int64_t k = lane; k < je - js; k += kWave)
+                        glb[k] = (int32_t)(sar_grid(tb[js + k], tsd) - SB.g0);
+                    SB.gl = glb;
+                }
+                __syncthreads();
                 const int64_t c1 = SA.count(), c2 = SB.count();
                 int64_t s1 = 0, s2 = 0, sz = 0;
                 if (c1 > kSarMaxSamples || c2 > kSarMaxSamples) res = sar_nan();
