@@ -108,6 +108,31 @@ def test_cdds_matches_oracle(gpu, eps):
     assert np.allclose(same, dur, rtol=1e-12, atol=0)
 
 
+def test_cdds_scrambled_and_repeated_timestamps(gpu):
+    """Second trajectories with decreasing timestamps take the full walk,
+    non-decreasing ones (repeated timestamps included) the time window; both
+    equal the oracle."""
+    from trajsimimeasures_amd import engine
+    from trajsimimeasures_amd.data import TrajSet
+    print("[case] cdds scrambled / repeated timestamps", flush=True)
+    ts, pa, pb = _st_set(1200, 2, 120, seed=55)
+    rng = np.random.default_rng(56)
+    t = ts.t.copy()
+    for i in rng.choice(ts.n, ts.n // 3, replace=False):
+        o, n = ts.offsets[i], ts.offsets[i + 1] - ts.offsets[i]
+        a, b = o + rng.integers(0, n, 2)
+        if rng.random() < 0.5:
+            t[a], t[b] = t[b], t[a]       # decreasing somewhere
+        elif n > 1:
+            t[o + 1] = t[o]               # a zero-duration segment
+    ts2 = TrajSet(ts.xy, ts.offsets, t)
+    d = engine.DeviceTrajSet(ts2)
+    for eps in (0.005, 0.05):
+        got = engine.pairs("cdds", d, pa, pb, eps=eps).cpu().numpy()
+        ref = O.pairs_cdds(ts2.xy, ts2.t, ts2.offsets, pa, pb, eps)
+        assert np.array_equal(got, ref), float(np.max(np.abs(got - ref)))
+
+
 def test_cdds_dropin_and_long(gpu):
     from trajsimimeasures_amd import core
     from trajsimimeasures_amd.core import pad_lists_to_array

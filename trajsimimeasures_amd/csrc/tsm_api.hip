@@ -154,13 +154,9 @@ int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &p
         return TSM_OK;
     }
     if (measure == TSM_CDDS) {
-        if ((rc = w.line.ensure(set->npts * sizeof(double4)))) return rc;
-        hipLaunchKernelGGL(cdds_prep_kernel, dim3(grid_for(set->npts, 256)), dim3(256), 0, st, set->xy64, set->t64,
-                           set->npts, w.line.as<double4>());
         CArgs c;
         c.xy = set->xy64;
         c.t = set->t64;
-        c.line = w.line.as<double4>();
         c.offs = set->offsets;
         c.ps = ps;
         c.npairs = npairs;

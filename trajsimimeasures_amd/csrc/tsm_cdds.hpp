@@ -2,15 +2,21 @@
 // timestamped points, the total time their linearly moving positions spend
 // within eps of each other, summed over every pair of segments.
 //
-// Per segment, x(t) = xk t + xb and y(t) = yk t + yb (core/cdds.py:11-18)
-// are computed once by cdds_prep_kernel.  One wavefront per pair: lanes take
-// 64 segments of the first trajectory at a time; the second trajectory's
-// segments are wave-uniform (scalar loads) and every lane adds the time its
-// segment overlaps each of them within eps (core/cdds.py:21-73), in segment
-// order.  The per-segment sums are then added in segment order by one lane,
-// the oracle's order, so results match it bit for bit.  A segment without
-// duration contributes 0, as in core_cpu/cdds.pyx:26 (the reference GPU
-// kernel would divide by zero).  Everything is FP64.
+// Per segment, x(t) = xk t + xb and y(t) = yk t + yb (core/cdds.py:11-18),
+// computed where a segment pair is evaluated (cdds_line).  One wavefront per
+// pair: lanes take 64 segments of the first trajectory at a time and add the
+// time their segment stays within eps of each segment of the second
+// (core/cdds.py:21-73), in segment order.  When the second trajectory's
+// timestamps are non-decreasing (checked per pair), a lane visits only the
+// segments that overlap its own in time: a binary search finds the first one
+// ending after its start, and it stops at the first one starting at or after
+// its end.  Every skipped segment would add exactly +0 (no time overlap), so
+// the sums are the same numbers.  Otherwise the second trajectory's segments
+// are walked wave-uniformly, all of them.  The per-segment sums are then
+// added in segment order by one lane, the oracle's order, so results match it
+// bit for bit.  A segment without duration contributes 0, as in
+// core_cpu/cdds.pyx:26 (the reference GPU kernel would divide by zero).
+// Everything is FP64.
 #pragma once
 
 #include "tsm_device.hpp"
@@ -20,7 +26,6 @@ namespace tsm {
 struct CArgs {
     const double *xy;
     const double *t;
-    const double4 *line;  // per point i: segment (i, i+1) as (xk, xb, yk, yb)
     const int64_t *offs;
     PairSpace ps;
     int64_t npairs;
@@ -29,26 +34,27 @@ struct CArgs {
     double *out;
 };
 
-// Segment (i, i + 1) of every trajectory as slope / intercept in time; the
-// last point of a trajectory (or a zero-duration segment) gets zeros, which
-// the time test never reads (it rejects by the timestamps first).
-__global__ void cdds_prep_kernel(const double *xy, const double *t, int64_t npts, double4 *line) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npts; i += (int64_t)gridDim.x * blockDim.x) {
-        double4 l = make_double4(0.0, 0.0, 0.0, 0.0);
-        if (i + 1 < npts && t[i + 1] != t[i]) {
-            const double x1 = xy[2 * i], y1 = xy[2 * i + 1], t1 = t[i];
-            const double x2 = xy[2 * i + 2], y2 = xy[2 * i + 3], t2 = t[i + 1];
-            const double dt = t2 - t1;
-            l.z = (y2 - y1) / dt;
-            l.w = (y1 * t2 - y2 * t1) / dt;
-            l.x = (x2 - x1) / dt;
-            l.y = (x1 * t2 - x2 * t1) / dt;
-        }
-        line[i] = l;
+// Segment (i, i + 1) as slope / intercept in time, x(t) = xk t + xb and
+// y(t) = yk t + yb (core/cdds.py:11-18), returned as (xk, xb, yk, yb); a
+// zero-duration segment gets zeros, which the time test never reads (it
+// rejects by the timestamps first).  Computed where a segment pair is
+// evaluated: a pass writing these for every point would cost more HBM
+// traffic than the measure itself.
+__device__ __forceinline__ double4 cdds_line(const double *xy, const double *t, int64_t i) {
+    double4 l = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (t[i + 1] != t[i]) {
+        const double x1 = xy[2 * i], y1 = xy[2 * i + 1], t1 = t[i];
+        const double x2 = xy[2 * i + 2], y2 = xy[2 * i + 3], t2 = t[i + 1];
+        const double dt = t2 - t1;
+        l.z = (y2 - y1) / dt;
+        l.w = (y1 * t2 - y2 * t1) / dt;
+        l.x = (x2 - x1) / dt;
+        l.y = (x1 * t2 - x2 * t1) / dt;
     }
+    return l;
 }
 
-// The oracle's cdd (core/cdds.py:21-73) on prepared segments.
+// The oracle's cdd (core/cdds.py:21-73) on segments as cdds_line gives them.
 __device__ __forceinline__ double cdd_line(double4 A, double as, double ae, double4 B, double bs, double be,
                                            double se) {
     if (ae <= bs || be <= as || as == ae || bs == be) return 0.0;
@@ -108,19 +114,35 @@ __global__ __launch_bounds__(kWave) void cdds_kernel(CArgs a) {
         }
         const int64_t oa = a.offs[ia], ob = a.offs[ib];
         const int na = (int)(a.offs[ia + 1] - oa), nb = (int)(a.offs[ib + 1] - ob);
+        const double *tb = a.t + ob;
+        bool mono = true;
+        for (int j = lane; j + 1 < nb; j += kWave) mono = mono && tb[j + 1] >= tb[j];
+        const bool window = __builtin_amdgcn_ballot_w64(!mono) == 0;
         double total = 0.0;  // wave-uniform
         for (int base = 0; base + 1 < na; base += kWave) {
             const int i = base + lane;
             const bool ok = i + 1 < na;
             const int ic = ok ? i : 0;
-            const double4 A = a.line[oa + ic];
+            const double4 A = cdds_line(a.xy + 2 * oa, a.t + oa, ic);
             const double as = a.t[oa + ic], ae = a.t[oa + ic + (ok ? 1 : 0)];
             double d = 0.0;
-            for (int j = 0; j + 1 < nb; ++j) {
-                const double4 Bv = a.line[ob + j];
-                const double4 B = make_double4(uni_d(Bv.x), uni_d(Bv.y), uni_d(Bv.z), uni_d(Bv.w));
-                const double bs = uni_d(a.t[ob + j]), be = uni_d(a.t[ob + j + 1]);
-                d += cdd_line(A, as, ae, B, bs, be, se);
+            if (window) {
+                // first segment of the second trajectory that ends after as (nb - 1: none)
+                int lo = 0, hi = nb - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (tb[mid + 1] > as) hi = mid;
+                    else lo = mid + 1;
+                }
+                for (int j = lo; j + 1 < nb && tb[j] < ae; ++j)
+                    d += cdd_line(A, as, ae, cdds_line(a.xy + 2 * ob, tb, j), tb[j], tb[j + 1], se);
+            } else {
+                for (int j = 0; j + 1 < nb; ++j) {
+                    const double4 Bv = cdds_line(a.xy + 2 * ob, tb, j);
+                    const double4 B = make_double4(uni_d(Bv.x), uni_d(Bv.y), uni_d(Bv.z), uni_d(Bv.w));
+                    const double bs = uni_d(a.t[ob + j]), be = uni_d(a.t[ob + j + 1]);
+                    d += cdd_line(A, as, ae, B, bs, be, se);
+                }
             }
             // per-segment sums in segment order, as the oracle adds them
             const int cnt = na - 1 - base < kWave ? na - 1 - base : kWave;

@@ -62,10 +62,9 @@ struct DevBuf {
 // each captured graph owns its own, so replays never see reallocations.
 struct Workspace {
     DevBuf hist, cursor, slots, meta, counter, bnd;
-    DevBuf gap64, gap32, gsum64, gsum32, seg, rseg, line;
+    DevBuf gap64, gap32, gsum64, gsum32, seg, rseg;
     void release() {
-        for (DevBuf *b : {&hist, &cursor, &slots, &meta, &counter, &bnd, &gap64, &gap32, &gsum64, &gsum32, &seg, &rseg,
-                          &line})
+        for (DevBuf *b : {&hist, &cursor, &slots, &meta, &counter, &bnd, &gap64, &gap32, &gsum64, &gsum32, &seg, &rseg})
             b->release();
     }
 };
