@@ -93,6 +93,10 @@ int tsm_nn_rownorm_bf16(const uint16_t *E, int64_t n, int D, float *out, void *s
 /* The k smallest entries of each row (unordered): idx [rows, k], val [rows, k] or NULL. */
 int tsm_nn_topk_f32(const float *V, int64_t rows, int64_t cols, int64_t ld, int k, int32_t *idx, float *val,
                     void *stream);
+/* The k-th smallest value of each row of V [rows, cols <= 16384] (row stride
+ * ld) into out [rows]: the largest of the k smallest, which the threshold
+ * search takes as its per-query threshold. */
+int tsm_nn_kth_smallest_f32(const float *V, int64_t rows, int64_t cols, int64_t ld, int k, float *out, void *stream);
 int tsm_nn_topk_f64(const double *V, int64_t rows, int64_t cols, int64_t ld, int k, int32_t *idx, double *val,
                     void *stream);
 

@@ -75,6 +75,23 @@ def test_topk_smallest_matches_sort(gpu, dtype, rows, cols, k):
         assert np.array_equal(np.sort(val[r]), np.sort(v[r])[:k])
 
 
+@pytest.mark.parametrize("rows,cols,k", [(7, 16384, 13), (300, 16384, 1), (5, 1000, 1000), (64, 4097, 200)])
+def test_kth_smallest_matches_kthvalue(gpu, rows, cols, k):
+    """The LDS radix select gives torch.kthvalue's value, with negative
+    values, exact ties and duplicated rows in the data."""
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    g = torch.Generator(device="cuda").manual_seed(rows * cols + k)
+    V = torch.randn((rows, cols), device="cuda", generator=g)
+    V[:, ::7] = torch.round(V[:, ::7] * 4) / 4      # many exact ties
+    V[0] = V[1]                                     # duplicate rows
+    V[-1, :] = 2.5                                  # a constant row
+    got = F.kth_smallest(V, k)
+    want = torch.kthvalue(V, k, dim=1).values
+    assert torch.equal(got, want)
+
+
 def test_resample_matches_numpy(gpu):
     import torch
 

@@ -877,6 +877,75 @@ __global__ __launch_bounds__(256) void topk_kernel(const T *V, int64_t rows, int
     }
 }
 
+// The k-th smallest value of each row (what the threshold search needs: the
+// largest of the k smallest), for rows of at most kKthCols values: the row's
+// order-preserving keys sit in LDS, and four radix passes of 8 bits build
+// per-wave histograms there (so a pass never re-reads global memory and
+// same-bin atomics collide only within a wave), merged by 256 threads and
+// scanned by one wave.
+constexpr int kKthCols = 16384, kKthThreads = 1024;
+
+__global__ __launch_bounds__(kKthThreads) void kth_smallest_kernel(const float *V, int64_t cols, int64_t ld, int k,
+                                                                   float *out) {
+    __shared__ uint32_t keys[kKthCols];
+    __shared__ uint32_t hist[kKthThreads / 64][256];
+    __shared__ uint32_t tot[256];
+    __shared__ uint32_t prefix_s, remain_s;
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const float *v = V + (int64_t)blockIdx.x * ld;
+    for (int64_t c = t; c < cols; c += kKthThreads) keys[c] = fkey(v[c]);
+    uint32_t prefix = 0, mask = 0, remain = (uint32_t)k;
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int i = t; i < (kKthThreads / 64) * 256; i += kKthThreads) (&hist[0][0])[i] = 0;
+        __syncthreads();
+        for (int64_t c = t; c < cols; c += kKthThreads) {
+            const uint32_t key = keys[c];
+            if ((key & mask) == prefix) atomicAdd(&hist[wave][(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (t < 256) {
+            uint32_t sum = 0;
+#pragma unroll
+            for (int w = 0; w < kKthThreads / 64; ++w) sum += hist[w][t];
+            tot[t] = sum;
+        }
+        __syncthreads();
+        if (wave == 0) {  // digit d with cum(d - 1) < remain <= cum(d): lane l scans bins 4l .. 4l + 3
+            const uint32_t h0 = tot[4 * lane], h1 = tot[4 * lane + 1], h2 = tot[4 * lane + 2], h3 = tot[4 * lane + 3];
+            uint32_t incl = h0 + h1 + h2 + h3;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            const uint32_t before = incl - (h0 + h1 + h2 + h3);
+            const bool here = before < remain && remain <= incl;
+            const uint64_t m = __builtin_amdgcn_ballot_w64(here);
+            const int owner = m ? __builtin_ctzll(m) : 63;
+            if (lane == owner) {
+                uint32_t cum = before, d = 4 * lane;
+                const uint32_t hs[4] = {h0, h1, h2, h3};
+                int j = 0;
+                while (j < 3 && cum + hs[j] < remain) {
+                    cum += hs[j];
+                    ++j;
+                }
+                d += j;
+                prefix_s = prefix | (d << shift);
+                remain_s = remain - cum;
+            }
+        }
+        __syncthreads();
+        prefix = prefix_s;
+        remain = remain_s;
+        mask |= 255u << shift;
+    }
+    if (t == 0) {
+        const uint32_t u = (prefix & 0x80000000u) ? (prefix & 0x7fffffffu) : ~prefix;
+        out[blockIdx.x] = __builtin_bit_cast(float, u);
+    }
+}
+
 }  // namespace tsm
 
 // ---------------------------------------------------------------- C ABI (part)
@@ -1111,6 +1180,15 @@ int tsm_nn_topk_f32(const float *V, int64_t rows, int64_t cols, int64_t ld, int 
     if (rows == 0) return TSM_OK;
     if (rows > 0x7fffffff) return fail(TSM_E_ARG, "topk: too many rows");
     hipLaunchKernelGGL(topk_kernel<float>, dim3((unsigned)rows), dim3(256), 0, TSM_ST, V, rows, cols, ld, k, idx, val);
+    TSM_LAUNCHED();
+}
+
+int tsm_nn_kth_smallest_f32(const float *V, int64_t rows, int64_t cols, int64_t ld, int k, float *out, void *stream) {
+    if (rows < 0 || k < 1 || k > cols || ld < cols || !V || !out) return fail(TSM_E_ARG, "kth_smallest: need 1 <= k <= cols");
+    if (cols > kKthCols) return fail(TSM_E_ARG, "kth_smallest: at most 16384 columns");
+    if (rows == 0) return TSM_OK;
+    if (rows > 0x7fffffff) return fail(TSM_E_ARG, "kth_smallest: too many rows");
+    hipLaunchKernelGGL(kth_smallest_kernel, dim3((unsigned)rows), dim3(kKthThreads), 0, TSM_ST, V, cols, ld, k, out);
     TSM_LAUNCHED();
 }
 

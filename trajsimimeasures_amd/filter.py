@@ -75,10 +75,11 @@ def _nn():
         L.tsm_nn_select_candidates.argtypes = [P, P, P, i, i, i, P, P, P]
         L.tsm_nn_gemm_nt_acc.argtypes = [i, i, i, P, i, P, i, P, i, f, P]
         L.tsm_nn_gru_forward.argtypes = [P, i, i, i, P, P, P, P, P, P, P, P, P, i, P]
+        L.tsm_nn_kth_smallest_f32.argtypes = [P, I64, I64, I64, i, P, P]
         for n in ("gemm_nt", "cast_bf16", "resample", "conv_fwd", "conv_bwd", "gru_h0", "gru_h0_bwd",
                   "gru_step_fwd", "gru_step_bwd", "colsum", "rank_loss", "adam", "rownorm_bf16", "topk_f32",
                   "topk_f64", "linear_f32", "split_bf16", "gemm_nt_filter", "select_candidates",
-                  "gemm_nt_acc", "gru_forward"):
+                  "gemm_nt_acc", "gru_forward", "kth_smallest_f32"):
             getattr(L, "tsm_nn_" + n).restype = ctypes.c_int
         L._nn_declared = True
     return L
@@ -302,6 +303,19 @@ def topk_smallest(V: torch.Tensor, k: int):
     return idx, val
 
 
+def kth_smallest(V: torch.Tensor, k: int) -> torch.Tensor:
+    """The k-th smallest value of each row (FP32), the largest of the k
+    smallest: an LDS radix select for rows of up to 16,384 values, else the
+    top-k's maximum."""
+    rows, cols = V.shape
+    if cols <= 16384:
+        out = torch.empty(rows, dtype=torch.float32, device=V.device)
+        _lib.check(_nn().tsm_nn_kth_smallest_f32(_p(V), rows, cols, V.stride(0), int(k), _p(out), _s()))
+        return out
+    _, vals = topk_smallest(V, k)
+    return vals.max(dim=1).values.contiguous()
+
+
 class FilterIndex:
     """A database embedded once (the index build), searched per query batch:
     BF16 hi/lo split embeddings so one K = 384 MFMA GEMM gives
@@ -349,8 +363,7 @@ class FilterIndex:
         ns.copy_(self.norm[::stride][:sample])
         dist_s = torch.empty((nq, sample), dtype=torch.float32, device=A.device)
         gemm_nt(A, Bs, dist_s, alpha=-2.0, bias=ns)
-        _, vals = topk_smallest(dist_s, ks)
-        tau = vals.max(dim=1).values.contiguous()
+        tau = kth_smallest(dist_s, ks)
         del dist_s
         cnt = torch.zeros(nq, dtype=torch.int32, device=A.device)
         key = torch.empty((nq, cap), dtype=torch.float32, device=A.device)
