@@ -73,13 +73,16 @@ extern "C" {
 
 /* A packed trajectory set resident on the device. */
 typedef struct tsm_trajset {
-    const float *xy32;      /* device [npts][2] FP32 copy of xy64 (tsm_xy_to_fp32) */
+    const float *xy32;      /* device [npts][2] FP32 copy of xy64, or of xy64 minus an origin
+                               (tsm_xy_to_fp32 / tsm_xy_to_fp32_centred): the FP32 kernels use
+                               coordinate differences only, so any origin gives the same results */
     const double *xy64;     /* device [npts][2] FP64 coordinates */
     const int64_t *offsets; /* device [n + 1], offsets[0] = 0 */
     int64_t n;              /* number of trajectories */
     int64_t npts;           /* number of points, offsets[n] */
     int64_t max_len;        /* longest trajectory (sizes scratch; must be >= every length) */
-    double coord_max;       /* an upper bound on |x| and |y| over the set, or 0 if unknown.  EDR and
+    double coord_max;       /* an upper bound on |x| and |y| of the FP32 copy (after its origin is
+                               subtracted), or 0 if unknown.  EDR and
                                LCSS use it to run their match test on the FP32 copy with a proven
                                guard band (FP64 recheck inside the band); 0 keeps the test in FP64 */
     const double *t64;      /* device [npts] per-point timestamps for STEDR/STLCSS, or NULL */
@@ -109,6 +112,16 @@ void tsm_params_default(tsm_params *p);
 
 /* FP64 -> FP32 copy of a packed point buffer (2*npts values), stream-ordered. */
 int tsm_xy_to_fp32(const double *xy64, float *xy32, int64_t npts, void *stream);
+
+/* The same copy relative to an origin: xy32 = (float)(x - ox), (float)(y - oy), the
+ * subtraction in FP64.  Mercator coordinates (~1e7 m, the dataset files of
+ * preprocessing/pkl2h5py.py:16-27) lose ~0.5 m to a plain FP32 cast; relative to the
+ * set's centre they keep FP32's precision of the set's extent.  The reference computes
+ * in FP64 on whatever it is given (core/dtw.py:56-77), after its pairwise collate has
+ * rounded coordinates to FP32 (utilities/method_helper.py:142-153); with the origin a
+ * multiple of the coordinates' FP32 spacing such inputs stay exact.  coord_max of the
+ * set must then bound |x - ox| and |y - oy|. */
+int tsm_xy_to_fp32_centred(const double *xy64, float *xy32, int64_t npts, double ox, double oy, void *stream);
 
 /*
  * Explicit pairs: pair k is (pa[k], pb[k]) of `set`, pa as t1 and pb as t2

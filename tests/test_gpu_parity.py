@@ -127,9 +127,9 @@ def test_eps_boundary_points(gpu):
 @pytest.mark.parametrize("shift", [0.0, 5.0, 1000.0])
 def test_count_match_paths(gpu, shift):
     """EDR and LCSS test matches on the FP32 copy against a guard band when
-    the set's coordinate bound allows it (shift 0: narrow band; shift 5: a
-    wide band with many FP64 rechecks) and in FP64 otherwise (shift 1000, or
-    coord_max = 0).  Every path is bit-exact against the oracle."""
+    the set's coordinate bound allows it and in FP64 otherwise (coord_max =
+    0).  The FP32 copy is centred, so shifting the set (5, 1000) leaves the
+    band as narrow as at 0.  Every path is bit-exact against the oracle."""
     from trajsimimeasures_amd.data import TrajSet
     print(f"[case] count match paths shift={shift}", flush=True)
     ts, pa, pb = uniform_pairs(2000, lo=5, hi=120, seed=31)
@@ -139,9 +139,9 @@ def test_count_match_paths(gpu, shift):
         for eps in (0.0025, 0.01):
             got, cnt = _gpu(m, d, pa, pb, eps=eps)
             _check(m, ts, pa, pb, got, cnt, eps=eps)
-            d.coord_max = 0.0
+            cm, d.coord_max = d.coord_max, 0.0
             got64, cnt64 = _gpu(m, d, pa, pb, eps=eps)
-            d.coord_max = float(np.abs(ts.xy).max())
+            d.coord_max = cm
             assert np.array_equal(cnt, cnt64) and np.array_equal(got, got64)
 
 
@@ -160,12 +160,9 @@ def test_identity_exact_zero(gpu):
     d = _dev(ts)
     for m in ALL:
         got, cnt = _gpu(m, d, pa, pa, eps=EPS)
-        if m == "hausdorff":
-            # the FP32 projection onto a segment's own end point can miss it by
-            # one rounding (u = u1 * rcp(|s|^2) need not be exactly 1)
-            assert np.all(got <= 1e-6), m
-        else:
-            assert np.all(got == 0.0), m
+        # Hausdorff too: a trajectory's last point projects onto its own last
+        # segment at u >= 1 exactly (1/|s|^2 is rounded up), residual 0
+        assert np.all(got == 0.0), m
 
 
 def test_swap_symmetry(gpu):

@@ -19,9 +19,10 @@
 
 namespace tsm {
 
-__global__ void xy_to_f32_kernel(const double *in, float *out, int64_t n) {
+// Interleaved (x, y) pairs: even elements subtract ox, odd ones oy.
+__global__ void xy_to_f32_kernel(const double *in, float *out, int64_t n, double ox, double oy) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = (float)in[i];
+        out[i] = (float)(in[i] - ((i & 1) ? oy : ox));
 }
 
 // ERP gap distances |p - g| per point (core/erp.py:32-40) in FP64 and FP32,
@@ -273,13 +274,19 @@ int tsm_init(int device) {
     return ensure_ctx();
 }
 
-int tsm_xy_to_fp32(const double *xy64, float *xy32, int64_t npts, void *stream) {
+int tsm_xy_to_fp32_centred(const double *xy64, float *xy32, int64_t npts, double ox, double oy, void *stream) {
     if (npts < 0 || (npts > 0 && (!xy64 || !xy32))) return fail(TSM_E_ARG, "bad point buffers");
+    if (!std::isfinite(ox) || !std::isfinite(oy)) return fail(TSM_E_ARG, "the origin must be finite");
     if (npts == 0) return TSM_OK;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-    hipLaunchKernelGGL(xy_to_f32_kernel, dim3(grid_for(2 * npts, 256)), dim3(256), 0, st, xy64, xy32, 2 * npts);
+    hipLaunchKernelGGL(xy_to_f32_kernel, dim3(grid_for(2 * npts, 256)), dim3(256), 0, st, xy64, xy32, 2 * npts, ox,
+                       oy);
     TSM_HIP(hipGetLastError());
     return TSM_OK;
+}
+
+int tsm_xy_to_fp32(const double *xy64, float *xy32, int64_t npts, void *stream) {
+    return tsm_xy_to_fp32_centred(xy64, xy32, npts, 0.0, 0.0, stream);
 }
 
 int tsm_pairs(int measure, const tsm_trajset *set, const int32_t *pa, const int32_t *pb, int64_t npairs,
@@ -348,11 +355,14 @@ int tsm_pairs_host(int measure, const double *xy64, const double *t64, const int
     TSM_HIP(hipMemcpy(c.h_offs.p, offsets, (n + 1) * 8, hipMemcpyHostToDevice));
     TSM_HIP(hipMemcpy(c.h_pa.p, pa, npairs * 4, hipMemcpyHostToDevice));
     TSM_HIP(hipMemcpy(c.h_pb.p, pb, npairs * 4, hipMemcpyHostToDevice));
-    int rc = tsm_xy_to_fp32(c.h_xy64.as<double>(), c.h_xy32.as<float>(), npts, nullptr);
+    // the FP32 copy relative to the centre of the set's bounding box
+    double ox = 0.0, oy = 0.0, cmax = 0.0;
+    if (fp32_origin(xy64, npts, ox, oy, cmax)) {
+        ox = oy = 0.0;
+        cmax = 0.0;  // non-finite coordinates: no bound, EDR/LCSS keep the FP64 match test
+    }
+    int rc = tsm_xy_to_fp32_centred(c.h_xy64.as<double>(), c.h_xy32.as<float>(), npts, ox, oy, nullptr);
     if (rc) return rc;
-    double cmax = 0.0;
-    for (int64_t i = 0; i < 2 * npts; ++i) cmax = std::fmax(cmax, std::fabs(xy64[i]));
-    if (!std::isfinite(cmax)) cmax = 0.0;  // unknown bound: EDR/LCSS keep the FP64 match test
     tsm_trajset set{c.h_xy32.as<float>(), c.h_xy64.as<double>(), c.h_offs.as<int64_t>(), n, npts, max_len, cmax,
                     t64 ? c.h_t64.as<double>() : nullptr};
     rc = tsm_pairs(measure, &set, c.h_pa.as<int32_t>(), c.h_pb.as<int32_t>(), npairs, prm, c.h_out.as<double>(),

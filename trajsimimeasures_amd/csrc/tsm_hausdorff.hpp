@@ -11,7 +11,8 @@
 // excluded from the max.
 //
 // Mapping: one wavefront per pair.  Lanes take 64 points of one side at a
-// time; the other side's segments are wave-uniform, so their endpoints and
+// time (128, two per lane, while more than 64 remain); the other side's
+// segments are wave-uniform, so their endpoints and
 // per-segment constants (dx, dy, 1/|s|^2, computed once per call by
 // seg_prep_kernel) come through scalar loads and every lane keeps a running
 // minimum in a VGPR.  A wave max-reduction and one square root finish the
@@ -89,26 +90,27 @@ __device__ __forceinline__ T p2t_sq(T px, T py, const T *__restrict__ Y, int ny)
 
 // FP32 form with precomputed segment constants.  G[k] = (x, y) of the
 // segment's end point s2 and (dx, dy) = s2 - s1; R[k] = 1/|s2 - s1|^2, or 0
-// for a degenerate segment, so that u = 0 < 1e-5 selects
-// min(dps1, dps2) = dps1, the reference's value for it.  The projection
-// residual is formed as (p - s1) - u (s2 - s1) with FMAs.  Both arrays are
-// read with wave-uniform addresses (scalar loads).
+// for a degenerate segment.  One clamped projection gives the reference's
+// value in every case: with c = 0 if u < 1e-5, else min(u, 1), the residual
+// |(p - s1) - c (s2 - s1)|^2 is
+//   - |p - s1|^2 exactly when u < 1e-5 (c = 0; the reference's
+//     min(dps1, dps2) is dps1 there, since u < 1/2 means s1 is nearer), and
+//     for a degenerate segment (R = 0 gives u = 0), as the reference;
+//   - |p - s2|^2 up to rounding when u > 1 (c = 1; u > 1 means s2 is
+//     nearer, so min(dps1, dps2) = dps2);
+//   - the distance to the foot of the perpendicular otherwise.
+// Both arrays are read with wave-uniform addresses (scalar loads).
 struct SegState {
-    float ex, ey, dps1, best;
+    float ex, ey, best;  // p - s1 of the current segment, running minimum
 };
 
 __device__ __forceinline__ void p2s_step(float px, float py, float4 g, float r, SegState &st) {
-    const float fx = px - g.x, fy = py - g.y;
-    const float dps2 = fmaf(fy, fy, fx * fx);
     const float u = fmaf(st.ey, g.w, st.ex * g.z) * r;
-    const bool far = __builtin_amdgcn_fmed3f(u, 0.00001f, 1.0f) != u;  // u < 1e-5 or u > 1
-    const float gx = fmaf(-u, g.z, st.ex), gy = fmaf(-u, g.w, st.ey);
-    const float dp = fmaf(gy, gy, gx * gx);
-    const float dist = far ? fminf(st.dps1, dps2) : dp;
-    st.best = fminf(st.best, dist);
-    st.ex = fx;
-    st.ey = fy;
-    st.dps1 = dps2;
+    const float c = u < 0.00001f ? 0.0f : fminf(u, 1.0f);
+    const float gx = fmaf(-c, g.z, st.ex), gy = fmaf(-c, g.w, st.ey);
+    st.best = fminf(st.best, fmaf(gy, gy, gx * gx));
+    st.ex = px - g.x;  // p - s2 is the next segment's p - s1
+    st.ey = py - g.y;
 }
 
 __device__ __forceinline__ float p2t_sq_f32(float px, float py, const float *__restrict__ Y,
@@ -116,14 +118,13 @@ __device__ __forceinline__ float p2t_sq_f32(float px, float py, const float *__r
     SegState st;
     st.ex = px - Y[0];
     st.ey = py - Y[1];
-    st.dps1 = fmaf(st.ey, st.ey, st.ex * st.ex);
     st.best = __builtin_inff();
     const int nseg = ny - 1;
-    int k = 0;
     // G and R are indexed by the wave-uniform k: read them as scalars
     const int *Gi = reinterpret_cast<const int *>(G);
     auto uni = [](int v) { return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(v)); };
     auto seg_at = [&](int q) { return make_float4(uni(Gi[4 * q]), uni(Gi[4 * q + 1]), uni(Gi[4 * q + 2]), uni(Gi[4 * q + 3])); };
+    int k = 0;
     for (; k + 2 <= nseg; k += 2) {
         p2s_step(px, py, seg_at(k), R[k], st);
         p2s_step(px, py, seg_at(k + 1), R[k + 1], st);
@@ -132,9 +133,43 @@ __device__ __forceinline__ float p2t_sq_f32(float px, float py, const float *__r
     return st.best;
 }
 
+// The same for two points per lane: each segment's scalar loads feed twice
+// the arithmetic (used when more than a wavefront of points remains).
+__device__ __forceinline__ void p2t_sq_f32_x2(float px0, float py0, float px1, float py1, const float *__restrict__ Y,
+                                              const float4 *__restrict__ G, const float *__restrict__ R, int ny,
+                                              float &d0, float &d1) {
+    SegState s0, s1;
+    s0.ex = px0 - Y[0];
+    s0.ey = py0 - Y[1];
+    s1.ex = px1 - Y[0];
+    s1.ey = py1 - Y[1];
+    s0.best = s1.best = __builtin_inff();
+    const int nseg = ny - 1;
+    const int *Gi = reinterpret_cast<const int *>(G);
+    auto uni = [](int v) { return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(v)); };
+    auto seg_at = [&](int q) { return make_float4(uni(Gi[4 * q]), uni(Gi[4 * q + 1]), uni(Gi[4 * q + 2]), uni(Gi[4 * q + 3])); };
+    int k = 0;
+    for (; k + 2 <= nseg; k += 2) {
+        const float4 g0 = seg_at(k), g1 = seg_at(k + 1);
+        const float r0 = R[k], r1 = R[k + 1];
+        p2s_step(px0, py0, g0, r0, s0);
+        p2s_step(px1, py1, g0, r0, s1);
+        p2s_step(px0, py0, g1, r1, s0);
+        p2s_step(px1, py1, g1, r1, s1);
+    }
+    if (k < nseg) {
+        const float4 g0 = seg_at(k);
+        const float r0 = R[k];
+        p2s_step(px0, py0, g0, r0, s0);
+        p2s_step(px1, py1, g0, r0, s1);
+    }
+    d0 = s0.best;
+    d1 = s1.best;
+}
+
 // Per point i of every trajectory: segment (p_i, p_i+1) as G[i] = (x_i+1,
-// y_i+1, dx, dy) and R[i] = 1/|s|^2 (0 if degenerate); the last point of a
-// trajectory gets zeros (never read).
+// y_i+1, dx, dy) and R[i] = 1/|s|^2 rounded up (0 if degenerate); the last
+// point of a trajectory gets zeros (never read).
 __global__ void seg_prep_kernel(const float *xy, int64_t npts, float4 *G, float *R) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < npts; i += (int64_t)gridDim.x * blockDim.x) {
         float4 g = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -143,7 +178,16 @@ __global__ void seg_prep_kernel(const float *xy, int64_t npts, float4 *G, float 
             const float x1 = xy[2 * i], y1 = xy[2 * i + 1], x2 = xy[2 * i + 2], y2 = xy[2 * i + 3];
             const float xd = x2 - x1, yd = y2 - y1;
             g = make_float4(x2, y2, xd, yd);
-            r = (x1 == x2 && y1 == y2) ? 0.f : 1.0f / fmaf(yd, yd, xd * xd);
+            if (!(x1 == x2 && y1 == y2)) {
+                // 1/|s|^2 rounded up, so that r |s|^2 >= 1 exactly: a point
+                // equal to s2 has u1 = |s|^2 (the same expression on the same
+                // values), hence u >= 1, c = 1 and a residual of exactly 0,
+                // as the reference's FP64 u = 1 gives
+                const float segl = fmaf(yd, yd, xd * xd);
+                r = 1.0f / segl;
+                if (fmaf(r, segl, -1.0f) < 0.0f)  // r > 0 finite: the next float up
+                    r = __builtin_bit_cast(float, __builtin_bit_cast(unsigned int, r) + 1u);
+            }
         }
         G[i] = g;
         R[i] = r;
@@ -171,7 +215,23 @@ __global__ __launch_bounds__(kWave) void hausdorff_kernel(HArgs a) {
             const int nx = dir == 0 ? na : nb;
             const int ny = dir == 0 ? nb : na;
             if (ny < 2) continue;
-            for (int base = 0; base < nx; base += kWave) {
+            for (int base = 0; base < nx;) {
+                if constexpr (sizeof(T) == 4) {
+                    const float4 *G = a.seg + (dir == 0 ? ob : oa);
+                    const float *R = a.rseg + (dir == 0 ? ob : oa);
+                    if (nx - base > kWave) {  // wave-uniform: points base + lane and base + 64 + lane
+                        const int i0 = base + lane;
+                        int i1 = base + kWave + lane;
+                        const bool ok1 = i1 < nx;
+                        i1 = ok1 ? i1 : nx - 1;
+                        float d0, d1;
+                        p2t_sq_f32_x2(X[2 * i0], X[2 * i0 + 1], X[2 * i1], X[2 * i1 + 1], Y, G, R, ny, d0, d1);
+                        m = d0 > m ? d0 : m;
+                        if (ok1) m = d1 > m ? d1 : m;
+                        base += 2 * kWave;
+                        continue;
+                    }
+                }
                 int i = base + lane;
                 const bool ok = i < nx;
                 i = ok ? i : nx - 1;
@@ -182,6 +242,7 @@ __global__ __launch_bounds__(kWave) void hausdorff_kernel(HArgs a) {
                 else
                     d = p2t_sq<T>(X[2 * i], X[2 * i + 1], Y, ny);
                 if (ok) m = d > m ? d : m;
+                base += kWave;
             }
         }
         double r;

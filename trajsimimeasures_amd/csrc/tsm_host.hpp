@@ -130,6 +130,10 @@ inline double eps_threshold(double eps, int mode) {
 // value below it is a match and a value at or above it is not, exactly as
 // in FP64.  Returns false when no useful band exists (unknown or huge
 // coordinates relative to eps): then the test runs in FP64.
+// coord_max bounds the FP32 copy's coordinates, which may be taken relative
+// to an origin: fl32(fl64(x - o)).  The FP64 subtraction adds at most
+// 2^-53 |x - o| to the u |x - o| of the FP32 rounding; U is widened by
+// 2^-20 to cover it (u 2^-20 U = 2^-44 U > 2^-52 U).
 inline bool count_band(double thr, double coord_max, float &lo, float &hi) {
     if (!(coord_max > 0.0) || !std::isfinite(coord_max) || coord_max > 1e15) return false;
     if (!(thr > 0.0)) {  // nothing matches
@@ -140,7 +144,7 @@ inline bool count_band(double thr, double coord_max, float &lo, float &hi) {
         lo = hi = INFINITY;
         return true;
     }
-    const double u = 0x1p-24, e = std::sqrt(thr), U = coord_max;
+    const double u = 0x1p-24, e = std::sqrt(thr), U = coord_max * (1.0 + 0x1p-20);
     const double rel = 7.0 * u + 12.0 * u * U / e + 20.0 * u * u * (U + e) * (U + e) / thr;
     const double B = 3.0 * rel;
     if (!(B <= 1e-2)) return false;
@@ -150,6 +154,37 @@ inline bool count_band(double thr, double coord_max, float &lo, float &hi) {
     hi = (float)dhi;
     if ((double)hi < dhi) hi = std::nextafter(hi, INFINITY);
     return true;
+}
+
+// Origin of a set's FP32 copy: per axis, the middle of the bounding box
+// rounded to a multiple of the FP32 spacing of the axis' largest |value|.
+// Every FP32 value on the axis is a multiple of its own spacing, which
+// divides that one, so x - o is exact in FP32 whenever |x - o| stays below
+// 2^24 spacings of x (FP32 inputs of one binade per axis, as Mercator
+// coordinates are).  cmax bounds |x - ox| and |y - oy|.  Returns nonzero
+// when a coordinate is not finite.
+inline int fp32_origin(const double *xy, int64_t npts, double &ox, double &oy, double &cmax) {
+    double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+    for (int64_t i = 0; i < npts; ++i)
+        for (int k = 0; k < 2; ++k) {
+            const double v = xy[2 * i + k];
+            if (!std::isfinite(v)) return 1;
+            lo[k] = std::fmin(lo[k], v);
+            hi[k] = std::fmax(hi[k], v);
+        }
+    double o[2] = {0.0, 0.0};
+    cmax = 0.0;
+    for (int k = 0; k < 2 && npts > 0; ++k) {
+        const double big = std::fmax(std::fabs(lo[k]), std::fabs(hi[k]));
+        if (big > 0.0) {
+            const double q = std::ldexp(1.0, std::ilogb(big) - 23);  // FP32 spacing at |big|
+            o[k] = std::nearbyint((0.5 * lo[k] + 0.5 * hi[k]) / q) * q;
+        }
+        cmax = std::fmax(cmax, std::fmax(std::fabs(lo[k] - o[k]), std::fabs(hi[k] - o[k])));
+    }
+    ox = o[0];
+    oy = o[1];
+    return 0;
 }
 
 inline int grid_for(int64_t n, int block) {

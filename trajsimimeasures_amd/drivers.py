@@ -107,23 +107,31 @@ def _device_set(ts: TrajSet):
 def pairwise(method: str, pairs, *, eps: float = DEFAULT_EPS, gpoint=None, lanes: int = 0, precision="fp32",
               delta: float = DEFAULT_DELTA, sar=DEFAULT_SAR):
     """Values of ``method`` for a list of (traj, traj2) pairs
-    (test_trajsimi_time.py:222-276).  Returns (list of floats,
-    [running, collate, compute] seconds)."""
+    (test_trajsimi_time.py:222-276), or for a dataset with a ``packed()``
+    form (``datasets.DatasetST_h5``: its pairs go to the device without
+    building per-pair lists).  Returns (list of floats, [running, collate,
+    compute] seconds)."""
     import torch
 
     from . import engine
     t0 = time.time()
-    pairs = list(pairs)
-    if not pairs:
+    if hasattr(pairs, "packed"):
+        ts, pa, pb = pairs.packed()
+        n = int(pa.size)
+    else:
+        pairs = list(pairs)
+        n = len(pairs)
+        if n:
+            a, b = zip(*pairs)
+            ts = pack_list(list(a) + list(b))
+            pa = np.arange(n, dtype=np.int32)
+            pb = pa + n
+    if not n:
         return [], [0.0, 0.0, 0.0]
-    a, b = zip(*pairs)
-    ts = pack_list(list(a) + list(b))
-    n = len(pairs)
     dset = _device_set(ts)
     collate = time.time() - t0
     t1 = time.time()
-    pa = np.arange(n, dtype=np.int32)
-    out = engine.pairs(method, dset, pa, pa + n, validate=False, **_opts(method, eps, gpoint, lanes, precision, delta, sar))
+    out = engine.pairs(method, dset, pa, pb, validate=False, **_opts(method, eps, gpoint, lanes, precision, delta, sar))
     torch.cuda.synchronize()
     dists = out.cpu().numpy().tolist()
     comp = time.time() - t1

@@ -10,6 +10,7 @@ compute is the HIP kernels of libtsm.so.
 from __future__ import annotations
 
 import ctypes
+import math
 
 import numpy as np
 import torch
@@ -17,7 +18,28 @@ import torch
 from . import _lib
 from .data import TrajSet
 
-__all__ = ["DeviceTrajSet", "pairs", "grid", "triu", "KernelTimer", "PairsGraph"]
+__all__ = ["DeviceTrajSet", "pairs", "grid", "triu", "KernelTimer", "PairsGraph", "fp32_origin"]
+
+
+def fp32_origin(xy: np.ndarray):
+    """Origin of a set's FP32 copy and the bound on |coordinate - origin|:
+    per axis the middle of the bounding box, rounded to a multiple of the
+    FP32 spacing of the axis' largest |value| (FP32-valued coordinates of
+    one binade per axis then stay exact after the subtraction).  The same
+    rule as the library's own host entry point uses."""
+    xy = np.asarray(xy, np.float64).reshape(-1, 2)
+    if xy.size == 0:
+        return (0.0, 0.0), 0.0
+    lo, hi = xy.min(axis=0), xy.max(axis=0)
+    o = [0.0, 0.0]
+    cmax = 0.0
+    for k in range(2):
+        big = max(abs(float(lo[k])), abs(float(hi[k])))
+        if big > 0.0:
+            q = math.ldexp(1.0, math.frexp(big)[1] - 1 - 23)
+            o[k] = float(np.rint((0.5 * float(lo[k]) + 0.5 * float(hi[k])) / q) * q)
+        cmax = max(cmax, abs(float(lo[k]) - o[k]), abs(float(hi[k]) - o[k]))
+    return (o[0], o[1]), cmax
 
 
 def _stream_ptr():
@@ -42,10 +64,13 @@ class DeviceTrajSet:
         self.lengths = torch.as_tensor(lens, dtype=torch.int64, device=device)
         self.xy64 = torch.as_tensor(ts.xy, dtype=torch.float64, device=device).contiguous()
         self.offsets = torch.as_tensor(ts.offsets, dtype=torch.int64, device=device).contiguous()
-        # bound on |coordinate| for the FP32 match test of EDR/LCSS (0 forces FP64)
-        self.coord_max = float(np.abs(ts.xy).max())
+        # the FP32 copy is taken relative to the set's centre (Mercator
+        # coordinates are ~1e7 m); coord_max bounds its |coordinates| for the
+        # FP32 match test of EDR/LCSS
+        self.origin, self.coord_max = fp32_origin(ts.xy)
         self.xy32 = torch.empty((self.npts, 2), dtype=torch.float32, device=device)
-        _lib.check(_lib.lib().tsm_xy_to_fp32(self.xy64.data_ptr(), self.xy32.data_ptr(), self.npts, _stream_ptr()))
+        _lib.check(_lib.lib().tsm_xy_to_fp32_centred(self.xy64.data_ptr(), self.xy32.data_ptr(), self.npts,
+                                                     self.origin[0], self.origin[1], _stream_ptr()))
         # per-point timestamps (STEDR / STLCSS), if the set has them
         self.t64 = None
         if ts.t is not None:
