@@ -125,11 +125,25 @@ __device__ __forceinline__ float p2t_sq_f32(float px, float py, const float *__r
     auto uni = [](int v) { return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(v)); };
     auto seg_at = [&](int q) { return make_float4(uni(Gi[4 * q]), uni(Gi[4 * q + 1]), uni(Gi[4 * q + 2]), uni(Gi[4 * q + 3])); };
     int k = 0;
-    for (; k + 2 <= nseg; k += 2) {
-        p2s_step(px, py, seg_at(k), R[k], st);
-        p2s_step(px, py, seg_at(k + 1), R[k + 1], st);
+    for (; k + 8 <= nseg; k += 8) {  // eight segments per scalar-load wait
+        float4 g[8];
+        float r[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            g[j] = seg_at(k + j);
+            r[j] = R[k + j];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) p2s_step(px, py, g[j], r[j], st);
     }
-    if (k < nseg) p2s_step(px, py, seg_at(k), R[k], st);
+    for (; k + 4 <= nseg; k += 4) {
+        const float4 g0 = seg_at(k), g1 = seg_at(k + 1), g2 = seg_at(k + 2), g3 = seg_at(k + 3);
+        p2s_step(px, py, g0, R[k], st);
+        p2s_step(px, py, g1, R[k + 1], st);
+        p2s_step(px, py, g2, R[k + 2], st);
+        p2s_step(px, py, g3, R[k + 3], st);
+    }
+    for (; k < nseg; ++k) p2s_step(px, py, seg_at(k), R[k], st);
     return st.best;
 }
 
@@ -149,15 +163,19 @@ __device__ __forceinline__ void p2t_sq_f32_x2(float px0, float py0, float px1, f
     auto uni = [](int v) { return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(v)); };
     auto seg_at = [&](int q) { return make_float4(uni(Gi[4 * q]), uni(Gi[4 * q + 1]), uni(Gi[4 * q + 2]), uni(Gi[4 * q + 3])); };
     int k = 0;
-    for (; k + 2 <= nseg; k += 2) {
-        const float4 g0 = seg_at(k), g1 = seg_at(k + 1);
-        const float r0 = R[k], r1 = R[k + 1];
+    for (; k + 4 <= nseg; k += 4) {  // four segments per scalar-load wait
+        const float4 g0 = seg_at(k), g1 = seg_at(k + 1), g2 = seg_at(k + 2), g3 = seg_at(k + 3);
+        const float r0 = R[k], r1 = R[k + 1], r2 = R[k + 2], r3 = R[k + 3];
         p2s_step(px0, py0, g0, r0, s0);
         p2s_step(px1, py1, g0, r0, s1);
         p2s_step(px0, py0, g1, r1, s0);
         p2s_step(px1, py1, g1, r1, s1);
+        p2s_step(px0, py0, g2, r2, s0);
+        p2s_step(px1, py1, g2, r2, s1);
+        p2s_step(px0, py0, g3, r3, s0);
+        p2s_step(px1, py1, g3, r3, s1);
     }
-    if (k < nseg) {
+    for (; k < nseg; ++k) {
         const float4 g0 = seg_at(k);
         const float r0 = R[k];
         p2s_step(px0, py0, g0, r0, s0);
