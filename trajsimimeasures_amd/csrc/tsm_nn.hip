@@ -107,7 +107,11 @@ __global__ __launch_bounds__(256) void gemm_nt_bf16_kernel(GemmArgs g) {
 // candidate list (cnt[row] counts, entries past `cap` are dropped and
 // counted), which is how the embedding search keeps the C nearest database
 // rows without writing the full distance matrix.
-constexpr int kBM = 128, kBN = 128, kBK = 64, kLdsStride = kBK + 8;
+// LDS row stride of a 64-wide K step: 80 BF16 (40 dwords).  Row strides of
+// 8 or 24 dwords mod 32 measured fastest for these MFMA operand reads (16
+// rows x 16 bytes per read); 36 dwords (a 72-element row) ran the search
+// GEMM 9 % slower.
+constexpr int kBM = 128, kBN = 128, kBK = 64, kLdsStride = kBK + 16;
 
 struct FilterArgs {
     const float *tau;  // [M] per-row threshold
@@ -150,11 +154,12 @@ __device__ __forceinline__ void tile_load_rows(const uint16_t *base, int ld, int
         r[i] = *reinterpret_cast<const u32x4 *>(base + (int64_t)row * ld + k);
     }
 }
-template <int ROWS, int NT> __device__ __forceinline__ void tile_store_v(uint16_t *s, const u32x4 (&r)[ROWS * 8 / NT]) {
+template <int ROWS, int NT, int ST = kLdsStride>
+__device__ __forceinline__ void tile_store_v(uint16_t *s, const u32x4 (&r)[ROWS * 8 / NT]) {
 #pragma unroll
     for (int i = 0; i < ROWS * 8 / NT; ++i) {
         const int q = (int)threadIdx.x + NT * i;
-        *reinterpret_cast<u32x4 *>(s + (q >> 3) * kLdsStride + (q & 7) * 8) = r[i];
+        *reinterpret_cast<u32x4 *>(s + (q >> 3) * ST + (q & 7) * 8) = r[i];
     }
 }
 template <int ROWS, int NT> __device__ __forceinline__ void tile_store(uint16_t *s, const uint4 (&r)[ROWS * 8 / NT]) {
@@ -273,6 +278,11 @@ __global__ __launch_bounds__(2 * BM) void gemm_nt_lds_kernel(GemmArgs g, FilterA
 // blocks.  Accumulation order per output equals gemm_nt_lds_kernel's (K
 // chunks of 32 in order), so values are bit-identical to the stored GEMM's.
 constexpr int kSearchRows = 128;
+// LDS row strides of the search kernel (BF16 elements): A rows 384 + 48
+// (216 dwords), database rows kLdsStride.  Measured on the 1,000 x 1M
+// search: strides of 8 or 24 dwords mod 32 ran it 9 % faster than 4 or 16
+// (at 4, SQ_LDS_BANK_CONFLICT was 43 % of the LDS-active cycles).
+constexpr int kSearchAPad = 48, kSearchBStride = kLdsStride;
 
 // The one-tile-ahead register ring of the search kernel: six named tiles
 // picked at compile time (an array here ends up in scratch memory).
@@ -295,9 +305,9 @@ struct SearchRing {
 // rows x 64 columns (2 x 4 MFMA tiles).
 template <int s>
 __device__ __forceinline__ void search_step(const GemmArgs &g, SearchRing &ring, uint16_t *As, int AS,
-                                           uint16_t (*Bs)[kBN * kLdsStride], int &cur, int tn, int wm, int wn,
+                                           uint16_t (*Bs)[kBN * kSearchBStride], int &cur, int tn, int wm, int wn,
                                            int r16, int kb, f32x4 (&acc)[2][4]) {
-    tile_store_v<kBN, kSearchThreads>(Bs[cur], ring.at<s>());
+    tile_store_v<kBN, kSearchThreads, kSearchBStride>(Bs[cur], ring.at<s>());
     tile_load_rows<kBN, kSearchThreads>(g.B, g.ldb, tn * kBN, g.N, s * kBK, ring.at<s>());
     __syncthreads();
 #pragma unroll
@@ -310,7 +320,7 @@ __device__ __forceinline__ void search_step(const GemmArgs &g, SearchRing &ring,
 #pragma unroll
         for (int j = 0; j < 4; ++j)
             bfr[j] = __builtin_bit_cast(
-                bf16x8, *reinterpret_cast<const uint4 *>(&Bs[cur][(wn + 16 * j + r16) * kLdsStride + 32 * kk + kb]));
+                bf16x8, *reinterpret_cast<const uint4 *>(&Bs[cur][(wn + 16 * j + r16) * kSearchBStride + 32 * kk + kb]));
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -324,9 +334,9 @@ template <int KC>
 __global__ __launch_bounds__(kSearchThreads, 2) void search_filter_kernel(GemmArgs g, FilterArgs f, int qblocks,
                                                                           int chunks, int tiles_per_chunk) {
     static_assert(KC == 12, "the register ring holds six K steps of 64");
-    constexpr int AS = 32 * KC + 8;      // A row stride in LDS (BF16), padded
+    constexpr int AS = 32 * KC + kSearchAPad;  // A row stride in LDS (BF16), padded
     __shared__ __attribute__((aligned(16))) uint16_t As[kSearchRows * AS];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][kBN * kLdsStride];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][kBN * kSearchBStride];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, kb = (lane >> 4) * 8;
     const int xcd = blockIdx.x & 7, jx = blockIdx.x >> 3;
