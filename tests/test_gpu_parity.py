@@ -78,19 +78,28 @@ def test_edge_lengths(gpu, measure):
         _check(measure, ts, pa, pb, got, cnt, eps=0.02)
 
 
+@pytest.mark.parametrize("wide", [0, 16, 64])
 @pytest.mark.parametrize("measure", ["dtw", "edr", "lcss", "erp", "frechet"])
-def test_column_strips(gpu, measure):
+def test_column_strips(gpu, measure, wide):
     """Pairs wider than one strip of L*W register columns (swept in strips
-    with a boundary column in scratch): up to 1200 x 1100."""
-    print(f"[case] column strips measure={measure}", flush=True)
-    la = [300, 420, 600, 1200, 257, 385, 1000, 193, 192, 400, 2000]
-    lb = [290, 500, 610, 1100, 300, 384, 999, 195, 192, 768, 1999]
+    with a boundary column in scratch): up to 2000 x 1999, with the pairs
+    wider than 384 columns on 16-lane strips (wide = 16) or whole
+    wavefronts (64; 0 picks 64 for a launch this small), across the
+    768-column split of the 64-lane buckets and past one 1536-column strip."""
+    from trajsimimeasures_amd import _lib
+    print(f"[case] column strips measure={measure} wide={wide}", flush=True)
+    la = [300, 420, 600, 1200, 257, 385, 1000, 193, 192, 400, 2000, 769, 768, 1537, 3100]
+    lb = [290, 500, 610, 1100, 300, 384, 999, 195, 192, 768, 1999, 800, 770, 1536, 3090]
     ts, pa, pb = fixed_lengths(la, lb, seed=5)
     d = _dev(ts)
     g = (0.5, 0.5)
-    for lanes in (0, 4, 8):
-        got, cnt = _gpu(measure, d, pa, pb, eps=0.01, gpoint=g, lanes=lanes)
-        _check(measure, ts, pa, pb, got, cnt, eps=0.01, gpoint=g)
+    _lib.check(_lib.lib().tsm_tune_set_wide_lanes(wide))
+    try:
+        for lanes in (0, 4, 8):
+            got, cnt = _gpu(measure, d, pa, pb, eps=0.01, gpoint=g, lanes=lanes)
+            _check(measure, ts, pa, pb, got, cnt, eps=0.01, gpoint=g)
+    finally:
+        _lib.check(_lib.lib().tsm_tune_set_wide_lanes(0))
 
 
 @pytest.mark.parametrize("measure", ["edr", "lcss"])

@@ -31,7 +31,7 @@ ERRORS = {-1: "TSM_E_ARG", -2: "TSM_E_HIP", -3: "TSM_E_NOMEM", -4: "TSM_E_UNSUP"
 EXPORTS = ("tsm_init", "tsm_last_error", "tsm_abi_version", "tsm_params_default", "tsm_xy_to_fp32",
            "tsm_xy_to_fp32_centred",
            "tsm_pairs", "tsm_pairs_grid", "tsm_pairs_triu", "tsm_pairs_host", "tsm_tune_set_lanes",
-           "tsm_tune_set_waves_per_cu", "tsm_graph_capture_pairs", "tsm_graph_launch", "tsm_graph_destroy",
+           "tsm_tune_set_waves_per_cu", "tsm_tune_set_wide_lanes", "tsm_graph_capture_pairs", "tsm_graph_launch", "tsm_graph_destroy",
            "tsm_profile_enable", "tsm_profile_read")
 # include/tsm_nn.h (argument types are declared by filter.py)
 NN_EXPORTS = ("tsm_nn_gemm_nt", "tsm_nn_cast_bf16", "tsm_nn_resample", "tsm_nn_conv_fwd", "tsm_nn_conv_bwd",
@@ -86,6 +86,7 @@ def _declare(L):
     L.tsm_square_from_triu.argtypes = [_P, _I64, _P, _P]
     L.tsm_kmedoids_fasterpam.argtypes = [_P, _I64, ctypes.c_int32, _P, ctypes.c_int32, _P, _P, _P, _P, _P]
     L.tsm_tune_set_waves_per_cu.argtypes = [ctypes.c_int]
+    L.tsm_tune_set_wide_lanes.argtypes = [ctypes.c_int]
     L.tsm_graph_capture_pairs.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _P, _P, _I64,
                                           ctypes.POINTER(ParamsC), _P, _P, _P, ctypes.POINTER(ctypes.c_void_p)]
     L.tsm_graph_launch.argtypes = [_P, _P]

@@ -397,6 +397,13 @@ int tsm_tune_set_waves_per_cu(int waves_per_cu) {
     return TSM_OK;
 }
 
+int tsm_tune_set_wide_lanes(int lanes) {
+    if (lanes != 0 && lanes != 16 && lanes != 64) return fail(TSM_E_ARG, "wide lanes must be 0 (by launch size), 16 or 64");
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_ctx.wide_lanes = lanes;
+    return TSM_OK;
+}
+
 }  // extern "C"
 
 // A captured batch: the planner and kernel launches of one call, replayable

@@ -61,6 +61,18 @@ __device__ __forceinline__ double shr1(double v) {
     return __builtin_bit_cast(double, ((int64_t)hi << 32) | (uint32_t)lo);
 }
 
+// Lane i <- lane i-1 across the whole wavefront (DPP wave_shr:1).  Lane 0
+// receives 0; callers overwrite it.
+__device__ __forceinline__ float shr1_wave(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double shr1_wave(double v) {
+    int64_t b = __builtin_bit_cast(int64_t, v);
+    int lo = __builtin_amdgcn_update_dpp(0, (int)(b & 0xffffffff), 0x138, 0xF, 0xF, false);
+    int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), 0x138, 0xF, 0xF, false);
+    return __builtin_bit_cast(double, ((int64_t)hi << 32) | (uint32_t)lo);
+}
+
 __device__ __forceinline__ int wave_max(int v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, kWave));

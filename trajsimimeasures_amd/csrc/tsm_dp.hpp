@@ -12,12 +12,13 @@
 // longer trajectory on the rows (streamed) and the shorter on the columns
 // (held in registers).
 //
-// Layout of the work: a "segment" of L lanes (L = 4, 8 or 16, so 16, 8 or 4
-// segments per wavefront) owns one pair.  Lane j of a segment holds W
+// Layout of the work: a "segment" of L lanes (L = 4, 8, 16 or 64, so 16, 8,
+// 4 or 1 segments per wavefront) owns one pair.  Lane j of a segment holds W
 // consecutive columns -- their points in VGPRs and the previous row of D for
 // them -- and sweeps the rows in skewed order: at step s it computes row
 // s - j.  The value of D at the lane's left edge comes from lane j-1,
-// computed one step earlier, through one DPP row_shr:1 move; the diagonal is
+// computed one step earlier, through one DPP row_shr:1 move (wave_shr:1 at
+// L = 64); the diagonal is
 // the previous step's left value.  No DP matrix touches memory.  The row
 // state is double-buffered across an unrolled pair of steps (even steps read
 // buffer A and write B, odd steps the reverse), so a row update needs no
@@ -55,20 +56,26 @@ constexpr int kTabMax = 257;    // bucket table covers column counts 1..256
 #endif
 template <typename CT> struct WMaxOf { static constexpr int value = sizeof(CT) == 4 ? TSM_WMAX32 : 24; };
 
-// Buckets, with H = WM / 2: b < H holds L = 16, W = WM - b (pairs wider
-// than 8 * WM columns; b = 0 also takes every pair wider than the table,
-// swept in strips of 16 * WM columns); b >= H holds b' = b - H:
+// Buckets, with H = WM / 2: b = 0 and 1 hold L = 64 with W = WM and W = H,
+// for pairs wider than 16 * WM columns in launches small enough that their
+// longest pairs, not the total work, set the time (a pair's steps run one
+// after another, and a wavefront takes a quarter of the steps of 16 lanes):
+// b = 1 up to 64 * H columns, b = 0 above, in strips of 64 * WM columns.
+// 2 <= b < 2 + H holds L = 16, W = WM - (b - 2) (pairs wider than 8 * WM
+// columns; b = 2 also takes the pairs wider than the table otherwise, in
+// strips of 16 * WM columns); b >= 2 + H holds b' = b - 2 - H:
 // L = (b' odd ? 4 : 8), W = WM - b' / 2.  Tasks run in bucket order, so the
-// widest pairs start first, and sixteen lanes per pair shorten their
-// critical path (the steps of one pair run one after another).
-__host__ __device__ constexpr int bucket_count(int WM) { return WM / 2 + 2 * WM; }
+// widest pairs start first.
+__host__ __device__ constexpr int bucket_count(int WM) { return 2 + WM / 2 + 2 * WM; }
 __host__ __device__ constexpr int bucket_of(int WM, int L, int W) {
-    return L == 16 ? WM - W : WM / 2 + 2 * (WM - W) + (L == 4 ? 1 : 0);
+    return L == 64 ? (W == WM ? 0 : 1) : L == 16 ? 2 + WM - W : 2 + WM / 2 + 2 * (WM - W) + (L == 4 ? 1 : 0);
 }
 __host__ __device__ constexpr int bucket_L(int WM, int b) {
-    return b < WM / 2 ? 16 : ((b - WM / 2) & 1) ? 4 : 8;
+    return b < 2 ? 64 : b < 2 + WM / 2 ? 16 : ((b - 2 - WM / 2) & 1) ? 4 : 8;
 }
-__host__ __device__ constexpr int bucket_W(int WM, int b) { return b < WM / 2 ? WM - b : WM - (b - WM / 2) / 2; }
+__host__ __device__ constexpr int bucket_W(int WM, int b) {
+    return b == 0 ? WM : b == 1 ? WM / 2 : b < 2 + WM / 2 ? WM - (b - 2) : WM - (b - 2 - WM / 2) / 2;
+}
 
 // Column count -> bucket table, plus the bucket of pairs wider than the
 // table (L = 16, W = WM, swept in strips of 16 * WM columns), and the lanes
@@ -77,7 +84,10 @@ struct BucketTable {
     uint8_t b[kTabMax];  // b[ncol] for 1 <= ncol < kTabMax
     uint8_t lanes[64];   // L of bucket b
     int ncmax;           // largest column count the table covers
-    int wide;            // bucket for ncol > ncmax
+    int wide;            // bucket for ncol > ncmax (16 lanes; strips when wider than wide16)
+    int wide16;          // with wide64: the column count up to which `wide` still applies (16 * WM)
+    int mid64;           // with wide64: bucket 1 (W = WM / 2) up to this column count, bucket 0 above
+    int wide64;          // nonzero: pairs wider than wide16 take 64-lane segments
 };
 
 struct DPArgs {
@@ -249,11 +259,17 @@ template <typename AT> __device__ __forceinline__ AT gsum_at(const DPArgs &a, in
 template <> __device__ __forceinline__ float gsum_at<float>(const DPArgs &a, int64_t t) { return a.gsum32[t]; }
 template <> __device__ __forceinline__ double gsum_at<double>(const DPArgs &a, int64_t t) { return a.gsum64[t]; }
 
-// Lane shift by one within rows of 16 lanes; out-of-row sources read 0.
-__device__ __forceinline__ float shl_in(float v) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x111, 0xF, 0xF, true));
+// Lane shift by one: within rows of 16 lanes for segments of up to 16
+// lanes, across the wavefront for 64; sources outside read 0 (every
+// segment's lane 0 replaces the value).
+template <int L> __device__ __forceinline__ float shl_in(float v) {
+    if constexpr (L > 16) return shr1_wave(v);
+    else return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x111, 0xF, 0xF, true));
 }
-__device__ __forceinline__ double shl_in(double v) { return shr1(v); }
+template <int L> __device__ __forceinline__ double shl_in(double v) {
+    if constexpr (L > 16) return shr1_wave(v);
+    else return shr1(v);
+}
 
 // Per-lane state of one strip sweep.
 template <typename CT, typename AT> struct RowPt {
@@ -266,7 +282,7 @@ template <typename CT, typename AT> struct RowPt {
 // row from `src` and writing the new one to `dst`.  `cur` holds the point of
 // row r, loaded one step earlier; `nxt` receives the point of the lane's
 // next row.  (Loading two steps ahead measured slower.)
-template <int M, bool F64, int W, bool kBnd>
+template <int M, bool F64, int L, int W, bool kBnd>
 __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t nrow_s, int rlast, int64_t orow,
                                         int64_t ocol, int col0, int ncol, bool lane0, typename Rec<M, F64>::AT left_fixed,
                                         typename Rec<M, F64>::AT *bnd, bool write_bnd,
@@ -282,7 +298,7 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
     using R = Rec<M, F64>;
     using CT = typename R::CT;
     using AT = typename R::AT;
-    const AT recv = shl_in(src[W - 1]);  // lane j-1's last column of row s - j
+    const AT recv = shl_in<L>(src[W - 1]);  // lane j-1's last column of row s - j
     // next row of this lane, kept inside the trajectory: a lane whose next
     // row is out of range does not use the point, so any row will do
     const uint32_t rn = min((uint32_t)(s + 1 - j), (uint32_t)rlast);
@@ -436,19 +452,19 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
         AT bl0 = 0, bl1 = 0;
         if (strip == 0) {
             for (int s = 0; s < nsteps; s += 2) {
-                dp_step<M, F64, W, false>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb, bt, A,
+                dp_step<M, F64, L, W, false>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb, bt, A,
                                           B, p0, p1, lpA, lpB, bl0, bl1);
                 if (s + 1 < nsteps)
-                    dp_step<M, F64, W, false>(a, s + 1, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by,
+                    dp_step<M, F64, L, W, false>(a, s + 1, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by,
                                               gb, bt, B, A, p1, p0, lpB, lpA, bl1, bl0);
             }
         } else {
             if (lane0) bl0 = bnd[r0];
             for (int s = 0; s < nsteps; s += 2) {
-                dp_step<M, F64, W, true>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb, bt, A,
+                dp_step<M, F64, L, W, true>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb, bt, A,
                                          B, p0, p1, lpA, lpB, bl0, bl1);
                 if (s + 1 < nsteps)
-                    dp_step<M, F64, W, true>(a, s + 1, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by,
+                    dp_step<M, F64, L, W, true>(a, s + 1, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by,
                                              gb, bt, B, A, p1, p0, lpB, lpA, bl1, bl0);
             }
         }

@@ -79,6 +79,7 @@ struct Context {
     uint8_t tuned_lanes[2][258] = {};
     bool tuned[2] = {false, false};
     int waves_per_cu = 8;  // resident waves per CU of the persistent DP kernel (autotuned)
+    int wide_lanes = 0;    // lanes for pairs wider than the bucket table: 0 = by launch size, 16 or 64
     // ordering of the shared workspace across streams: the last call's
     // stream and an event recorded after its kernels
     hipEvent_t ws_event = nullptr;

@@ -16,10 +16,17 @@ namespace tsm {
 // issue slots for 64/L pairs at once, so the per-pair cost is
 // L * (nrow + L - 1) * (c0 + c1 * W) / 64 with c0 ~ c1 (per-step bookkeeping
 // ~ one cell): L = 4 wins whenever its W fits in registers.
-inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint) {
+// Pairs wider than the table: 16-lane segments (strips of 16 * WM
+// columns), or with wide_lanes = 64, 16 lanes up to one such strip and a
+// whole wavefront beyond (W = WM / 2 up to 64 * WM / 2 columns, then W = WM
+// in strips of 64 * WM).
+inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint, int wide_lanes) {
     BucketTable t{};
     t.ncmax = 16 * WM < kTabMax - 1 ? 16 * WM : kTabMax - 1;
     t.wide = bucket_of(WM, 16, WM);
+    t.wide16 = 16 * WM;
+    t.mid64 = 64 * (WM / 2);
+    t.wide64 = wide_lanes == 64;
     for (int b = 0; b < bucket_count(WM); ++b) t.lanes[b] = (uint8_t)bucket_L(WM, b);
     const int k = coord_bits == 64 ? 1 : 0;
     const uint8_t *tuned = g_ctx.tuned[k] ? g_ctx.tuned_lanes[k] : nullptr;
@@ -41,6 +48,8 @@ inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint) {
     return t;
 }
 
+constexpr int64_t kSmallLaunchPairs = 1 << 20;
+
 template <int M, bool F64>
 int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr,
            double *out, int32_t *counts, hipStream_t st, double delta) {
@@ -49,7 +58,13 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     constexpr int WM = Rec<M, F64>::kWMax;
     constexpr int NB = bucket_count(WM);
     static_assert(NB <= kMaxBuckets && NB <= 64 && WM % 2 == 0, "bucket count");
-    const BucketTable tab = bucket_table(WM, sizeof(CT) * 8, lanes_hint);
+    // A launch of few pairs lasts as long as its longest pair's chain of
+    // steps: there, pairs wider than the table take a whole wavefront (a
+    // quarter of the steps of 16 lanes, at more lane-steps per pair); in a
+    // large launch the total work sets the time and 16-lane strips use the
+    // lanes better.
+    const int wide_lanes = g_ctx.wide_lanes ? g_ctx.wide_lanes : (npairs <= kSmallLaunchPairs ? 64 : 16);
+    const BucketTable tab = bucket_table(WM, sizeof(CT) * 8, lanes_hint, wide_lanes);
     const int64_t nslots = npairs + (int64_t)NB * kMaxSeg;
     int rc;
     if ((rc = c.hist.ensure(kMaxBins * 4)) || (rc = c.cursor.ensure(kMaxBins * 4)) ||

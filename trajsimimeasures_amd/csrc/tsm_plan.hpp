@@ -43,7 +43,10 @@ struct PlanArgs {
 __device__ __forceinline__ int bin_of(const PlanArgs &a, int64_t na, int64_t nb) {
     const int64_t nrow = na > nb ? na : nb;
     const int64_t ncol = na > nb ? nb : na;
-    const int b = ncol <= a.tab.ncmax ? (int)a.tab.b[ncol] : a.tab.wide;
+    const int b = ncol <= a.tab.ncmax                            ? (int)a.tab.b[ncol]
+                  : (!a.tab.wide64 || ncol <= a.tab.wide16) ? a.tab.wide
+                  : ncol <= a.tab.mid64                      ? 1
+                                                             : 0;
     const int rk = (int)((nrow < 511 ? nrow : 511) * kRowKeys / 512);
     return b * kRowKeys + (kRowKeys - 1 - rk);
 }
