@@ -288,6 +288,42 @@ def test_threshold_search_matches_full_search(gpu):
         assert (index.last_redo > 0) == expect_redo or (not expect_redo and index.last_redo < 3)
 
 
+def test_deferred_redo_matches_synchronous_search(gpu):
+    """search(defer_redo=True) queues the path without a host sync: rows
+    whose survivor list is short or overflowed come back flagged and hold
+    column 0 (a valid index), the others equal the synchronous search; the
+    flagged rows redone over their full distance rows give the synchronous
+    search's redo (as sets: that select has no fixed order), and
+    filtered_knn (which defers) finds the same 10 as a re-rank of the
+    synchronous candidates."""
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    from trajsimimeasures_amd.data import lengths_lognormal, random_walks
+    from trajsimimeasures_amd.engine import DeviceTrajSet
+    rng = np.random.default_rng(13)
+    nq, nd = 200, 70000
+    dset = DeviceTrajSet(random_walks(rng, lengths_lognormal(rng, nq + nd)))
+    index = F.FilterIndex(F.Encoder(), dset, torch.arange(nq, nq + nd, dtype=torch.int32, device="cuda"))
+    q = torch.arange(nq, dtype=torch.int32, device="cuda")
+    for ks in (1, 4000, None):
+        sync = index.search(q, 200, ks=ks).cpu().numpy()
+        cand, flag, A = index.search(q, 200, ks=ks, defer_redo=True)
+        cand, fl = cand.cpu().numpy(), flag.cpu().numpy()
+        if ks is not None:
+            assert fl.sum() > 0, ks
+        assert np.all(cand[fl == 1] == 0)
+        assert np.array_equal(cand[fl == 0], sync[fl == 0])
+        redo = np.nonzero(fl)[0]
+        if redo.size:  # the full-row select returns the C nearest in no fixed order
+            full = index._search_full(A[torch.as_tensor(redo, device="cuda")].contiguous(), 200).cpu().numpy()
+            assert all(set(full[i]) == set(sync[r]) for i, r in enumerate(redo))
+    best, _ = F.filtered_knn(index, 0, nq, k=10, C=200)
+    ref = F._rerank(index, q, torch.as_tensor(sync, device="cuda"), 10)
+    b, r = best.cpu().numpy(), ref.cpu().numpy()
+    assert all(set(b[i]) == set(r[i]) for i in range(nq))
+
+
 def test_filtered_knn_small_recall(gpu):
     """End to end on a small set: the untrained filter (resampled-L2
     embedding) shortlists candidates whose exact DTW re-rank recovers most of

@@ -416,7 +416,9 @@ __global__ __launch_bounds__(kSearchThreads, 2) void search_filter_kernel(GemmAr
 // The C smallest candidates of each row: keys (order-preserving bits of the
 // value, then the column) sorted by a bitonic network in LDS.  A row whose
 // list is short (fewer than C) or overflowed (more than cap) is flagged for
-// the caller's exact fallback.  One 256-thread block per row; cap <= 4096.
+// the caller's exact fallback and gets column 0 in every slot (a valid
+// index, so work queued on it before the fallback reads no garbage).  One
+// 256-thread block per row; cap <= 4096.
 __global__ __launch_bounds__(256) void select_candidates_kernel(const uint32_t *cnt, const float *key,
                                                                 const int32_t *idx, int cap, int C, int32_t *out,
                                                                 int32_t *flag) {
@@ -425,6 +427,7 @@ __global__ __launch_bounds__(256) void select_candidates_kernel(const uint32_t *
     const uint32_t n = cnt[row];
     if (n < (uint32_t)C || n > (uint32_t)cap) {
         if (threadIdx.x == 0) flag[row] = 1;
+        for (int t = threadIdx.x; t < C; t += blockDim.x) out[row * C + t] = 0;
         return;
     }
     if (threadIdx.x == 0) flag[row] = 0;

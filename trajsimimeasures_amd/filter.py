@@ -337,7 +337,8 @@ class FilterIndex:
         _lib.check(_nn().tsm_nn_split_bf16(_p(E), E.shape[0], D, int(query_side), _p(out), _p(norm), _s()))
         return out
 
-    def search(self, q_ids, C: int = 200, *, cap: int = 4096, sample: int = 16384, ks: int | None = None):
+    def search(self, q_ids, C: int = 200, *, cap: int = 4096, sample: int = 16384, ks: int | None = None,
+               defer_redo: bool = False):
         """Candidate database positions [nq, C] by embedding distance (the C
         smallest |e_d|^2 - 2 e_q . e_d of each query, ascending).
 
@@ -348,12 +349,20 @@ class FilterIndex:
         GEMM keeps only the distances below the threshold, and a per-row sort
         of those picks the C smallest.  A query whose list comes out shorter
         than C or longer than `cap` (rare by construction) is redone over the
-        full distance row."""
+        full distance row.
+
+        defer_redo=True returns (candidates, flags, query operand) without
+        that redo and without any host synchronisation: flagged rows hold
+        column 0, and the caller redoes them (``redo_rows``) once it
+        synchronises anyway."""
         Eq = self.enc.embed(self.dset, q_ids)
         A = self._split(Eq, query_side=True)
         nq, nd = A.shape[0], self.B.shape[0]
         if nd < 4 * sample or C > cap // 2:
-            return self._search_full(A, C)
+            cand = self._search_full(A, C)
+            if defer_redo:
+                return cand, torch.zeros(nq, dtype=torch.int32, device=A.device), A
+            return cand
         stride = nd // sample
         if ks is None:
             ks = int(round(4 * C * sample / nd))
@@ -374,6 +383,8 @@ class FilterIndex:
         cand = torch.empty((nq, C), dtype=torch.int32, device=A.device)
         flag = torch.empty(nq, dtype=torch.int32, device=A.device)
         _lib.check(L.tsm_nn_select_candidates(_p(cnt), _p(key), _p(idx), nq, cap, C, _p(cand), _p(flag), _s()))
+        if defer_redo:
+            return cand, flag, A
         redo = torch.nonzero(flag).reshape(-1)
         self.last_redo = int(redo.numel())
         if self.last_redo:
@@ -415,19 +426,31 @@ def exact_knn(dset: DeviceTrajSet, q0: int, nq: int, d0: int, nd: int, k: int = 
     return out, time.perf_counter() - t0
 
 
+def _rerank(index: FilterIndex, qids: torch.Tensor, cand: torch.Tensor, k: int) -> torch.Tensor:
+    """Exact DTW of each query against its candidates, the k smallest."""
+    nq, C = cand.shape
+    pa = qids.repeat_interleave(C)
+    pb = index.db_ids[cand.reshape(-1).long()]
+    d = dp_pairs("dtw", index.dset, pa, pb, validate=False).view(nq, C)
+    idx, _ = topk_smallest(d, k)
+    return torch.gather(cand, 1, idx.long())
+
+
 def filtered_knn(index: FilterIndex, q0: int, nq: int, k: int = 10, C: int = 200):
     """Embedding shortlist of C, exact DTW re-rank, k smallest: database
-    positions [nq, k] and the elapsed seconds (index build excluded)."""
-    dset = index.dset
+    positions [nq, k] and the elapsed seconds (index build excluded).  The
+    whole path is queued without a host synchronisation; the rare queries
+    whose shortlist needs the full distance row are redone after it."""
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     qids = torch.arange(q0, q0 + nq, dtype=torch.int32, device="cuda")
-    cand = index.search(qids, C)
-    pa = qids.repeat_interleave(C)
-    pb = index.db_ids[cand.reshape(-1).long()]
-    d = dp_pairs("dtw", dset, pa, pb, validate=False).view(nq, C)
-    idx, _ = topk_smallest(d, k)
-    best = torch.gather(cand, 1, idx.long())
+    cand, flag, A = index.search(qids, C, defer_redo=True)
+    best = _rerank(index, qids, cand, k)
+    redo = torch.nonzero(flag).reshape(-1)  # synchronises, after everything is queued
+    index.last_redo = int(redo.numel())
+    if index.last_redo:
+        cr = index._search_full(A[redo].contiguous(), C)
+        best[redo] = _rerank(index, qids[redo], cr, k)
     torch.cuda.synchronize()
     return best, time.perf_counter() - t0
 
