@@ -84,7 +84,9 @@ int tsm_nn_linear_f32(const float *X, int64_t n, int K, const float *W, int Dout
 /* The whole GRU forward over T steps (Gx: [n, T, 3H] input projections; Hs:
  * [T + 1, n, H] with Hs[0] the initial state; Hb: BF16 copy of the current h;
  * Gh: [n, 3H] scratch; r/z/nn/ghn: [T, n, H] when keep, else [1, n, H]):
- * per step the recurrent GEMM and tsm_nn_gru_step_fwd, launched from C. */
+ * per step the recurrent GEMM and tsm_nn_gru_step_fwd, launched from C.  With
+ * keep = 0 and H = 128 one fused kernel runs all T steps instead (bit-identical
+ * Hs and Hb; the scratch Gh/r/z/nn/ghn is then not written). */
 int tsm_nn_gru_forward(const float *Gx, int T, int n, int Hd, const uint16_t *Whh, const float *bhh, float *Hs,
                        uint16_t *Hb, float *Gh, float *r, float *z, float *nn, float *ghn, int keep, void *stream);
 int tsm_nn_split_bf16(const float *E, int64_t n, int D, int query_side, uint16_t *out, float *norm, void *stream);

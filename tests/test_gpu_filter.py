@@ -288,6 +288,29 @@ def test_threshold_search_matches_full_search(gpu):
         assert (index.last_redo > 0) == expect_redo or (not expect_redo and index.last_redo < 3)
 
 
+@pytest.mark.parametrize("n", [1, 15, 16, 17, 1000, 4099])
+def test_fused_gru_forward_matches_stepwise(gpu, n):
+    """The inference GRU (one fused kernel for all T steps, keep=False) gives
+    bit-identical embeddings to the stepwise recurrent GEMM + gate kernels
+    (keep=True), for row counts around its 16-row blocks."""
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    from trajsimimeasures_amd.data import lengths_lognormal, random_walks
+    from trajsimimeasures_amd.engine import DeviceTrajSet
+    rng = np.random.default_rng(n)
+    dset = DeviceTrajSet(random_walks(rng, lengths_lognormal(rng, n)))
+    p = F.init_params(seed=3)
+    p += np.random.default_rng(7).normal(0, 0.05, p.size).astype(np.float32)  # every weight nonzero
+    enc = F.Encoder(p)
+    ids = torch.arange(n, dtype=torch.int32, device="cuda")
+    X, lenf = enc.resample(dset, ids, n)
+    e_fused = enc.forward(X, lenf, keep=False).clone()
+    e_step = enc.forward(X, lenf, keep=True).clone()
+    assert torch.equal(e_fused, e_step)
+    assert torch.isfinite(e_fused).all() and float(e_fused.abs().max()) > 0
+
+
 def test_deferred_redo_matches_synchronous_search(gpu):
     """search(defer_redo=True) queues the path without a host sync: rows
     whose survivor list is short or overflowed come back flagged and hold

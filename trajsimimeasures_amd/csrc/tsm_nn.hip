@@ -598,6 +598,93 @@ __global__ void gru_step_fwd_kernel(const float *Gx, int64_t ldgx, const float *
     }
 }
 
+// The whole T-step GRU forward for inference (no activations kept), H = 128:
+// one 256-thread block per 16 trajectories runs every step, so T recurrent
+// GEMMs and T gate launches become one launch.  Wave w owns hidden units
+// [32 w, 32 w + 32): the three gate columns of unit j are the same lane's
+// outputs in three MFMA tiles, so the gates and the new state stay in
+// registers; the wave's W_hh fragments (6 tiles x 4 K chunks) stay in VGPRs
+// for all steps; only the BF16 state goes through LDS (double-buffered, one
+// barrier per step).  The arithmetic is the stepwise path's: each gate sum
+// is the same four MFMA K chunks in order plus the bias (alpha = 1), and
+// the gates use gru_step_fwd_kernel's expressions, so Hs and Hb are
+// bit-identical to it.
+constexpr int kGruH = 128, kGruRows = 16, kGruStride = kGruH + 16;  // LDS rows of 72 dwords
+__global__ __launch_bounds__(256) void gru_fwd_fused_kernel(const float *Gx, int T, int n, const uint16_t *Whh,
+                                                            const float *bhh, float *Hs, uint16_t *Hb) {
+    __shared__ __attribute__((aligned(16))) uint16_t hl[2][kGruRows * kGruStride];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, kb = (lane >> 4) * 8;
+    const int row0 = blockIdx.x * kGruRows;
+    const int64_t nh = (int64_t)n * kGruH;
+    // W_hh fragments of the wave's 6 output tiles: gate g, half q -> columns
+    // g * 128 + 32 w + 16 q + (0..15)
+    bf16x8 bw[6][4];
+    float bias[6];
+#pragma unroll
+    for (int tile = 0; tile < 6; ++tile) {
+        const int c = (tile >> 1) * kGruH + 32 * w + 16 * (tile & 1) + r16;
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch)
+            bw[tile][ch] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(Whh + (int64_t)c * kGruH + 32 * ch + kb));
+        bias[tile] = bhh[c];
+    }
+    // this lane's state: rows 4 * (lane >> 4) + e, units 32 w + 16 q + r16
+    float h[2][4];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            const int row = min(row0 + 4 * (lane >> 4) + e, n - 1);
+            h[q][e] = Hs[(int64_t)row * kGruH + 32 * w + 16 * q + r16];
+        }
+    // initial BF16 state into LDS buffer 0 (rows past n repeat row n - 1)
+    for (int i = threadIdx.x; i < kGruRows * kGruH / 8; i += 256) {
+        const int r = i / (kGruH / 8), k = (i % (kGruH / 8)) * 8;
+        const int row = min(row0 + r, n - 1);
+        *reinterpret_cast<uint4 *>(&hl[0][r * kGruStride + k]) = *reinterpret_cast<const uint4 *>(Hb + (int64_t)row * kGruH + k);
+    }
+    __syncthreads();
+    for (int t = 0; t < T; ++t) {
+        const int cur = t & 1;
+        bf16x8 af[4];
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch)
+            af[ch] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(&hl[cur][r16 * kGruStride + 32 * ch + kb]));
+        f32x4 acc[6];
+#pragma unroll
+        for (int tile = 0; tile < 6; ++tile) acc[tile] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ch = 0; ch < 4; ++ch)
+#pragma unroll
+            for (int tile = 0; tile < 6; ++tile)
+                acc[tile] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[ch], bw[tile][ch], acc[tile], 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int rl = 4 * (lane >> 4) + e, row = row0 + rl;
+                const int j = 32 * w + 16 * q + r16;
+                const float *gx = Gx + ((int64_t)min(row, n - 1) * T + t) * 3 * kGruH;
+                const float ghr = 1.f * acc[q][e] + bias[q];
+                const float ghz = 1.f * acc[2 + q][e] + bias[2 + q];
+                const float ghn = 1.f * acc[4 + q][e] + bias[4 + q];
+                const float r = sigm(gx[j] + ghr);
+                const float z = sigm(gx[kGruH + j] + ghz);
+                const float nn = tanhf(gx[2 * kGruH + j] + r * ghn);
+                const float hv = (1.f - z) * nn + z * h[q][e];
+                h[q][e] = hv;
+                const uint16_t hb = f2bf(hv);
+                hl[cur ^ 1][rl * kGruStride + j] = hb;
+                if (row < n) {
+                    Hs[(int64_t)(t + 1) * nh + (int64_t)row * kGruH + j] = hv;
+                    if (t == T - 1) Hb[(int64_t)row * kGruH + j] = hb;
+                }
+            }
+        __syncthreads();
+    }
+}
+
 // Backward of one GRU step.  dh: gradient w.r.t. h' (in), h: the step's
 // input state.  Writes dGx (pre-activation gradients of the input
 // projection, row stride lddgx), dGh [n, 3H] (fp32 + bf16) and
@@ -1129,6 +1216,11 @@ int tsm_nn_gru_forward(const float *Gx, int T, int n, int Hd, const uint16_t *Wh
                        uint16_t *Hb, float *Gh, float *r, float *z, float *nn, float *ghn, int keep, void *stream) {
     if (T < 1 || n < 0 || Hd < 1 || (Hd % 8) != 0) return fail(TSM_E_ARG, "gru_forward: bad arguments");
     if (n == 0) return TSM_OK;
+    if (!keep && Hd == kGruH && !(reinterpret_cast<uintptr_t>(Whh) % 16) && !(reinterpret_cast<uintptr_t>(Hb) % 16)) {
+        hipLaunchKernelGGL(gru_fwd_fused_kernel, dim3((n + kGruRows - 1) / kGruRows), dim3(256), 0, TSM_ST, Gx, T, n,
+                           Whh, bhh, Hs, Hb);
+        TSM_LAUNCHED();
+    }
     const int64_t nh = (int64_t)n * Hd;
     for (int t = 0; t < T; ++t) {
         int rc = tsm_nn_gemm_nt(n, 3 * Hd, Hd, Hb, Hd, Whh, Hd, Gh, 3 * Hd, 1.f, 0.f, bhh, stream);
