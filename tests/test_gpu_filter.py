@@ -75,10 +75,14 @@ def test_topk_smallest_matches_sort(gpu, dtype, rows, cols, k):
         assert np.array_equal(np.sort(val[r]), np.sort(v[r])[:k])
 
 
-@pytest.mark.parametrize("rows,cols,k", [(7, 16384, 13), (300, 16384, 1), (5, 1000, 1000), (64, 4097, 200)])
+@pytest.mark.parametrize("rows,cols,k", [(7, 16384, 13), (300, 16384, 1), (5, 1000, 1000), (64, 4097, 200),
+                                         (1000, 16384, 32), (20, 16384, 33), (9, 100, 32), (3, 1, 1),
+                                         (5, 16383, 17)])
 def test_kth_smallest_matches_kthvalue(gpu, rows, cols, k):
-    """The LDS radix select gives torch.kthvalue's value, with negative
-    values, exact ties and duplicated rows in the data."""
+    """Both selects (k <= 32: per-lane sorts and wave-minimum merges; else
+    the LDS radix select) give torch.kthvalue's value, with negative values,
+    exact ties and duplicated rows in the data, and rows narrower than one
+    key per lane."""
     import torch
 
     from trajsimimeasures_amd import filter as F

@@ -1046,6 +1046,78 @@ __global__ __launch_bounds__(kKthThreads) void kth_smallest_kernel(const float *
     }
 }
 
+// The same for small k (<= kKthSmallK), without histograms: each of 1024
+// lanes sorts its 16 keys (columns lane + 1024 i) in registers, each wave
+// pops its k smallest by k rounds of a wave minimum (the first lane holding
+// it pops), and one wave merges the 16 sorted lists the same way.  The
+// k-th smallest key is the radix kernel's, so the value is identical.  (In
+// the radix kernel's first pass almost every key of a row shares the top
+// byte, so its histogram atomics serialise within each wave.)
+constexpr int kKthSmallK = 32;
+
+__device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, kWave));
+    return v;
+}
+
+__global__ __launch_bounds__(kKthThreads) void kth_small_k_kernel(const float *V, int64_t cols, int64_t ld, int k,
+                                                                  float *out) {
+    __shared__ uint32_t wl[kKthThreads / 64][kKthSmallK];
+    const int t = threadIdx.x, wave = t >> 6, lane = t & 63;
+    const float *v = V + (int64_t)blockIdx.x * ld;
+    uint32_t a[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const int64_t c = t + (int64_t)kKthThreads * i;
+        a[i] = c < cols ? fkey(v[c]) : 0xffffffffu;  // past the row: larger than every key
+    }
+    // bitonic sort of the 16 keys, ascending
+#pragma unroll
+    for (int kk = 2; kk <= 16; kk <<= 1)
+#pragma unroll
+        for (int j = kk >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < 16; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const uint32_t x = a[i], y = a[l];
+                    const uint32_t lo = min(x, y), hi = max(x, y);
+                    if ((i & kk) == 0) {
+                        a[i] = lo;
+                        a[l] = hi;
+                    } else {
+                        a[i] = hi;
+                        a[l] = lo;
+                    }
+                }
+            }
+    for (int r = 0; r < k; ++r) {
+        const uint32_t m = wave_min_u32(a[0]);
+        const uint64_t b = __builtin_amdgcn_ballot_w64(a[0] == m);
+        const bool pop = lane == (int)__builtin_ctzll(b);
+#pragma unroll
+        for (int i = 0; i < 15; ++i) a[i] = pop ? a[i + 1] : a[i];
+        a[15] = pop ? 0xffffffffu : a[15];
+        if (lane == 0) wl[wave][r] = m;
+    }
+    __syncthreads();
+    if (wave == 0) {
+        int p = 0;
+        uint32_t m = 0;
+        for (int r = 0; r < k; ++r) {
+            const uint32_t head = (lane < kKthThreads / 64 && p < k) ? wl[lane][p] : 0xffffffffu;
+            m = wave_min_u32(head);
+            const uint64_t b = __builtin_amdgcn_ballot_w64(head == m);
+            if (lane == (int)__builtin_ctzll(b)) ++p;
+        }
+        if (lane == 0) {
+            const uint32_t u = (m & 0x80000000u) ? (m & 0x7fffffffu) : ~m;
+            out[blockIdx.x] = __builtin_bit_cast(float, u);
+        }
+    }
+}
+
 }  // namespace tsm
 
 // ---------------------------------------------------------------- C ABI (part)
@@ -1293,7 +1365,10 @@ int tsm_nn_kth_smallest_f32(const float *V, int64_t rows, int64_t cols, int64_t 
     if (cols > kKthCols) return fail(TSM_E_ARG, "kth_smallest: at most 16384 columns");
     if (rows == 0) return TSM_OK;
     if (rows > 0x7fffffff) return fail(TSM_E_ARG, "kth_smallest: too many rows");
-    hipLaunchKernelGGL(kth_smallest_kernel, dim3((unsigned)rows), dim3(kKthThreads), 0, TSM_ST, V, cols, ld, k, out);
+    if (k <= kKthSmallK)
+        hipLaunchKernelGGL(kth_small_k_kernel, dim3((unsigned)rows), dim3(kKthThreads), 0, TSM_ST, V, cols, ld, k, out);
+    else
+        hipLaunchKernelGGL(kth_smallest_kernel, dim3((unsigned)rows), dim3(kKthThreads), 0, TSM_ST, V, cols, ld, k, out);
     TSM_LAUNCHED();
 }
 
