@@ -177,9 +177,9 @@ int tsm_tune_set_lanes(int coord_bits, const uint8_t *lanes_of_ncol, int n);
 int tsm_tune_set_waves_per_cu(int waves_per_cu);
 
 /* Lanes per pair for pairs wider than one 16-lane strip (more than 384 columns
- * in the shorter trajectory): 64 (one wavefront: 12 columns per lane up to 768,
- * then 24 per lane in strips of 1536; a quarter of the steps of 16 lanes, so
- * the longest pairs finish sooner) or 16 (strips of 384 columns: fewer
+ * in the shorter trajectory): 64 (one wavefront: 12, 16, 20 or 24 columns per
+ * lane, the fewest that hold the pair, then 24 per lane in strips of 1536; a
+ * quarter of the steps of 16 lanes, so the longest pairs finish sooner) or 16 (strips of 384 columns: fewer
  * lane-steps per pair).  0 (default) picks 64 for launches of at most 2^20
  * pairs and 16 above.  No counterpart in the reference (it runs one thread per
  * column of one pair, core/dtw.py:33-45). */

@@ -84,12 +84,13 @@ def test_column_strips(gpu, measure, wide):
     """Pairs wider than one strip of L*W register columns (swept in strips
     with a boundary column in scratch): up to 2000 x 1999, with the pairs
     wider than 384 columns on 16-lane strips (wide = 16) or whole
-    wavefronts (64; 0 picks 64 for a launch this small), across the
-    768-column split of the 64-lane buckets and past one 1536-column strip."""
+    wavefronts (64; 0 picks 64 for a launch this small), across every
+    column split of the 64-lane buckets (W = 12, 16, 20, 24) and past one
+    1536-column strip."""
     from trajsimimeasures_amd import _lib
     print(f"[case] column strips measure={measure} wide={wide}", flush=True)
-    la = [300, 420, 600, 1200, 257, 385, 1000, 193, 192, 400, 2000, 769, 768, 1537, 3100]
-    lb = [290, 500, 610, 1100, 300, 384, 999, 195, 192, 768, 1999, 800, 770, 1536, 3090]
+    la = [300, 420, 600, 1200, 257, 385, 1000, 193, 192, 400, 2000, 769, 768, 1537, 3100, 1300, 1025]
+    lb = [290, 500, 610, 1100, 300, 384, 999, 195, 192, 768, 1999, 800, 770, 1536, 3090, 1290, 1030]
     ts, pa, pb = fixed_lengths(la, lb, seed=5)
     d = _dev(ts)
     g = (0.5, 0.5)

@@ -56,25 +56,26 @@ constexpr int kTabMax = 257;    // bucket table covers column counts 1..256
 #endif
 template <typename CT> struct WMaxOf { static constexpr int value = sizeof(CT) == 4 ? TSM_WMAX32 : 24; };
 
-// Buckets, with H = WM / 2: b = 0 and 1 hold L = 64 with W = WM and W = H,
-// for pairs wider than 16 * WM columns in launches small enough that their
-// longest pairs, not the total work, set the time (a pair's steps run one
-// after another, and a wavefront takes a quarter of the steps of 16 lanes):
-// b = 1 up to 64 * H columns, b = 0 above, in strips of 64 * WM columns.
-// 2 <= b < 2 + H holds L = 16, W = WM - (b - 2) (pairs wider than 8 * WM
-// columns; b = 2 also takes the pairs wider than the table otherwise, in
-// strips of 16 * WM columns); b >= 2 + H holds b' = b - 2 - H:
-// L = (b' odd ? 4 : 8), W = WM - b' / 2.  Tasks run in bucket order, so the
-// widest pairs start first.
-__host__ __device__ constexpr int bucket_count(int WM) { return 2 + WM / 2 + 2 * WM; }
+// Buckets, with H = WM / 2: b < 4 hold L = 64, W = WM - 4 b (24, 20, 16,
+// 12 at WM = 24), for pairs wider than 16 * WM columns in launches small
+// enough that their longest pairs, not the total work, set the time (a
+// pair's steps run one after another; a wavefront takes a quarter of the
+// steps of 16 lanes, and the smallest W that holds the pair the fewest
+// cells per step): a pair takes the smallest W with 64 W >= its columns,
+// b = 0 above 64 * WM, in strips of 64 * WM columns.  4 <= b < 4 + H hold
+// L = 16, W = WM - (b - 4) (pairs wider than 8 * WM columns; b = 4 also takes
+// the pairs wider than the table otherwise, in strips of 16 * WM columns);
+// b >= 4 + H hold b' = b - 4 - H: L = (b' odd ? 4 : 8), W = WM - b' / 2.
+// Tasks run in bucket order, so the widest pairs start first.
+__host__ __device__ constexpr int bucket_count(int WM) { return 4 + WM / 2 + 2 * WM; }
 __host__ __device__ constexpr int bucket_of(int WM, int L, int W) {
-    return L == 64 ? (W == WM ? 0 : 1) : L == 16 ? 2 + WM - W : 2 + WM / 2 + 2 * (WM - W) + (L == 4 ? 1 : 0);
+    return L == 64 ? (WM - W) / 4 : L == 16 ? 4 + WM - W : 4 + WM / 2 + 2 * (WM - W) + (L == 4 ? 1 : 0);
 }
 __host__ __device__ constexpr int bucket_L(int WM, int b) {
-    return b < 2 ? 64 : b < 2 + WM / 2 ? 16 : ((b - 2 - WM / 2) & 1) ? 4 : 8;
+    return b < 4 ? 64 : b < 4 + WM / 2 ? 16 : ((b - 4 - WM / 2) & 1) ? 4 : 8;
 }
 __host__ __device__ constexpr int bucket_W(int WM, int b) {
-    return b == 0 ? WM : b == 1 ? WM / 2 : b < 2 + WM / 2 ? WM - (b - 2) : WM - (b - 2 - WM / 2) / 2;
+    return b < 4 ? WM - 4 * b : b < 4 + WM / 2 ? WM - (b - 4) : WM - (b - 4 - WM / 2) / 2;
 }
 
 // Column count -> bucket table, plus the bucket of pairs wider than the
@@ -86,7 +87,8 @@ struct BucketTable {
     int ncmax;           // largest column count the table covers
     int wide;            // bucket for ncol > ncmax (16 lanes; strips when wider than wide16)
     int wide16;          // with wide64: the column count up to which `wide` still applies (16 * WM)
-    int mid64;           // with wide64: bucket 1 (W = WM / 2) up to this column count, bucket 0 above
+    int wm;              // with wide64: WM; a wider pair takes bucket (WM - W) / 4 with W the smallest
+                         // multiple of 4 (>= 12) with 64 W >= its columns, bucket 0 beyond 64 * WM
     int wide64;          // nonzero: pairs wider than wide16 take 64-lane segments
 };
 

@@ -18,14 +18,14 @@ namespace tsm {
 // ~ one cell): L = 4 wins whenever its W fits in registers.
 // Pairs wider than the table: 16-lane segments (strips of 16 * WM
 // columns), or with wide_lanes = 64, 16 lanes up to one such strip and a
-// whole wavefront beyond (W = WM / 2 up to 64 * WM / 2 columns, then W = WM
-// in strips of 64 * WM).
+// whole wavefront beyond (W = 12, 16, 20 or 24: the smallest that holds the
+// pair, then W = WM in strips of 64 * WM).
 inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint, int wide_lanes) {
     BucketTable t{};
     t.ncmax = 16 * WM < kTabMax - 1 ? 16 * WM : kTabMax - 1;
     t.wide = bucket_of(WM, 16, WM);
     t.wide16 = 16 * WM;
-    t.mid64 = 64 * (WM / 2);
+    t.wm = WM;
     t.wide64 = wide_lanes == 64;
     for (int b = 0; b < bucket_count(WM); ++b) t.lanes[b] = (uint8_t)bucket_L(WM, b);
     const int k = coord_bits == 64 ? 1 : 0;
@@ -57,7 +57,7 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     using AT = typename Rec<M, F64>::AT;
     constexpr int WM = Rec<M, F64>::kWMax;
     constexpr int NB = bucket_count(WM);
-    static_assert(NB <= kMaxBuckets && NB <= 64 && WM % 2 == 0, "bucket count");
+    static_assert(NB <= kMaxBuckets && NB <= 64 && WM % 4 == 0 && WM >= 16, "bucket count");
     // A launch of few pairs lasts as long as its longest pair's chain of
     // steps: there, pairs wider than the table take a whole wavefront (a
     // quarter of the steps of 16 lanes, at more lane-steps per pair); in a

@@ -43,10 +43,14 @@ struct PlanArgs {
 __device__ __forceinline__ int bin_of(const PlanArgs &a, int64_t na, int64_t nb) {
     const int64_t nrow = na > nb ? na : nb;
     const int64_t ncol = na > nb ? nb : na;
-    const int b = ncol <= a.tab.ncmax                            ? (int)a.tab.b[ncol]
-                  : (!a.tab.wide64 || ncol <= a.tab.wide16) ? a.tab.wide
-                  : ncol <= a.tab.mid64                      ? 1
-                                                             : 0;
+    int b;
+    if (ncol <= a.tab.ncmax) b = (int)a.tab.b[ncol];
+    else if (!a.tab.wide64 || ncol <= a.tab.wide16) b = a.tab.wide;
+    else {
+        int w = (int)((ncol + 255) / 256) * 4;  // 4 * ceil(ncol / 256): the smallest multiple of 4 with 64 w >= ncol
+        w = w < 12 ? 12 : w;
+        b = w >= a.tab.wm ? 0 : (a.tab.wm - w) / 4;
+    }
     const int rk = (int)((nrow < 511 ? nrow : 511) * kRowKeys / 512);
     return b * kRowKeys + (kRowKeys - 1 - rk);
 }
