@@ -58,8 +58,12 @@ def test_cast_bf16_rounds_to_nearest_even(gpu):
 
 
 @pytest.mark.parametrize("dtype", ["float32", "float64"])
-@pytest.mark.parametrize("rows,cols,k", [(3, 1000, 10), (5, 70000, 200), (2, 257, 257), (4, 5000, 1)])
+@pytest.mark.parametrize("rows,cols,k", [(3, 1000, 10), (5, 70000, 200), (2, 257, 257), (4, 5000, 1),
+                                         (1000, 200, 10), (7, 512, 32), (5, 4, 1), (9, 63, 5), (6, 300, 31)])
 def test_topk_smallest_matches_sort(gpu, dtype, rows, cols, k):
+    """The k smallest of each row (as a set, with their values): the radix
+    top-k and, for rows of at most 512 columns and k <= 32, one wave per row
+    popping wave minima."""
     import torch
 
     from trajsimimeasures_amd import filter as F
@@ -270,7 +274,8 @@ def test_threshold_search_matches_full_search(gpu):
     """The fused threshold search (sampled threshold, filtered GEMM, per-row
     sort) returns the same C nearest database rows as the full distance rows
     with a radix select, including queries forced onto the fallback (lists
-    too short with ks = 1, overflowing with ks = 4000).  By default the
+    too short with ks = 1, overflowing with ks = 4000) and lists longer than
+    the one-wave select takes (ks = 470).  By default the
     threshold aims at 4 C candidates per query, so fallbacks are rare."""
     import torch
 
@@ -285,7 +290,8 @@ def test_threshold_search_matches_full_search(gpu):
     q = torch.arange(nq, dtype=torch.int32, device="cuda")
     A = index._split(enc.embed(dset, q), query_side=True)
     full = index._search_full(A, 200).cpu().numpy()
-    for ks, expect_redo in ((None, False), (1, True), (4000, True)):
+    # ks = 470: ~2,000 survivors per query, past the one-wave select (1,024)
+    for ks, expect_redo in ((None, False), (1, True), (4000, True), (470, False)):
         got = index.search(q, 200, ks=ks).cpu().numpy()
         same = np.mean([len(set(got[i]) & set(full[i])) == 200 for i in range(nq)])
         assert same == 1.0, (ks, same)

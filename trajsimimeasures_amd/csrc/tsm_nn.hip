@@ -431,6 +431,54 @@ __global__ __launch_bounds__(256) void select_candidates_kernel(const uint32_t *
         return;
     }
     if (threadIdx.x == 0) flag[row] = 0;
+    if (n <= 1024) {
+        // one wave: each lane sorts its 16 keys (entries lane + 64 i) in
+        // registers, then C rounds of a wave minimum pop them in ascending
+        // order (keys are unique: the column is in the low bits); the same C
+        // keys in the same order as the bitonic network below
+        if (threadIdx.x >= kWave) return;  // block-uniform branch on n; no barrier follows
+        const int lane = threadIdx.x;
+        uint64_t a[16];
+#pragma unroll
+        for (int i = 0; i < 16; ++i) {
+            const int t = lane + kWave * i;
+            if (t < (int)n) {
+                const uint32_t u = __builtin_bit_cast(uint32_t, key[row * cap + t]);
+                const uint32_t kk = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+                a[i] = ((uint64_t)kk << 32) | (uint32_t)idx[row * cap + t];
+            } else {
+                a[i] = ~0ull;
+            }
+        }
+#pragma unroll
+        for (int kk = 2; kk <= 16; kk <<= 1)
+#pragma unroll
+            for (int j = kk >> 1; j > 0; j >>= 1)
+#pragma unroll
+                for (int i = 0; i < 16; ++i) {
+                    const int l = i ^ j;
+                    if (l > i) {
+                        const uint64_t x = a[i], y = a[l];
+                        const uint64_t lo = x < y ? x : y, hi = x < y ? y : x;
+                        a[i] = (i & kk) == 0 ? lo : hi;
+                        a[l] = (i & kk) == 0 ? hi : lo;
+                    }
+                }
+        for (int r = 0; r < C; ++r) {
+            uint64_t m = a[0];
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                const uint64_t y = __shfl_xor(m, o, kWave);
+                m = y < m ? y : m;
+            }
+            const bool pop = a[0] == m;
+#pragma unroll
+            for (int i = 0; i < 15; ++i) a[i] = pop ? a[i + 1] : a[i];
+            a[15] = pop ? ~0ull : a[15];
+            if (lane == 0) out[row * C + r] = (int32_t)(uint32_t)m;
+        }
+        return;
+    }
     int P = 1;
     while (P < (int)n) P <<= 1;
     for (int t = threadIdx.x; t < P; t += blockDim.x) {
@@ -1046,6 +1094,74 @@ __global__ __launch_bounds__(kKthThreads) void kth_smallest_kernel(const float *
     }
 }
 
+// Top-k of short rows (at most 512 columns, k <= kKthSmallK): one wave per
+// row; each lane sorts its 8 (key, column) pairs (columns lane + 64 i), and k
+// rounds of a wave minimum pop the k smallest in ascending (value, column)
+// order.  Used for the 10 nearest of each query's re-ranked candidates,
+// where the radix top-k's passes cost more than the row itself.
+constexpr int kTopkSmallCols = 512;
+template <typename T>
+__global__ __launch_bounds__(256) void topk_small_kernel(const T *V, int64_t rows, int64_t cols, int64_t ld, int k,
+                                                         int32_t *out_idx, T *out_val) {
+    using K_t = decltype(fkey((T)0));
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= rows) return;  // wave-uniform
+    const T *v = V + row * ld;
+    K_t key[8];
+    int col[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int c = lane + 64 * i;
+        key[i] = c < cols ? fkey(v[c]) : (K_t)~(K_t)0;
+        col[i] = c < cols ? c : 0x7fffffff;
+    }
+    auto less = [](K_t ka, int ca, K_t kb, int cb) { return ka < kb || (ka == kb && ca < cb); };
+#pragma unroll
+    for (int kk = 2; kk <= 8; kk <<= 1)
+#pragma unroll
+        for (int j = kk >> 1; j > 0; j >>= 1)
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int l = i ^ j;
+                if (l > i) {
+                    const bool up = (i & kk) == 0;
+                    const bool sw = up ? less(key[l], col[l], key[i], col[i]) : less(key[i], col[i], key[l], col[l]);
+                    const K_t ka = key[i];
+                    const int ca = col[i];
+                    key[i] = sw ? key[l] : key[i];
+                    col[i] = sw ? col[l] : col[i];
+                    key[l] = sw ? ka : key[l];
+                    col[l] = sw ? ca : col[l];
+                }
+            }
+    for (int r = 0; r < k; ++r) {
+        K_t mk = key[0];
+        int mc = col[0];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const K_t ok = __shfl_xor(mk, o, kWave);
+            const int oc = __shfl_xor(mc, o, kWave);
+            if (less(ok, oc, mk, mc)) {
+                mk = ok;
+                mc = oc;
+            }
+        }
+        const bool pop = col[0] == mc;  // columns are unique: one lane
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+            key[i] = pop ? key[i + 1] : key[i];
+            col[i] = pop ? col[i + 1] : col[i];
+        }
+        key[7] = pop ? (K_t)~(K_t)0 : key[7];
+        col[7] = pop ? 0x7fffffff : col[7];
+        if (lane == 0) {
+            out_idx[row * k + r] = mc;
+            if (out_val) out_val[row * k + r] = v[mc];
+        }
+    }
+}
+
 // The same for small k (<= kKthSmallK), without histograms: each of 1024
 // lanes sorts its 16 keys (columns lane + 1024 i) in registers, each wave
 // pops its k smallest by k rounds of a wave minimum (the first lane holding
@@ -1356,7 +1472,11 @@ int tsm_nn_topk_f32(const float *V, int64_t rows, int64_t cols, int64_t ld, int 
     if (rows < 0 || k < 1 || k > cols || ld < cols || !V || !idx) return fail(TSM_E_ARG, "topk: need 1 <= k <= cols");
     if (rows == 0) return TSM_OK;
     if (rows > 0x7fffffff) return fail(TSM_E_ARG, "topk: too many rows");
-    hipLaunchKernelGGL(topk_kernel<float>, dim3((unsigned)rows), dim3(256), 0, TSM_ST, V, rows, cols, ld, k, idx, val);
+    if (cols <= kTopkSmallCols && k <= kKthSmallK)
+        hipLaunchKernelGGL(topk_small_kernel<float>, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, TSM_ST, V, rows, cols,
+                           ld, k, idx, val);
+    else
+        hipLaunchKernelGGL(topk_kernel<float>, dim3((unsigned)rows), dim3(256), 0, TSM_ST, V, rows, cols, ld, k, idx, val);
     TSM_LAUNCHED();
 }
 
@@ -1377,7 +1497,11 @@ int tsm_nn_topk_f64(const double *V, int64_t rows, int64_t cols, int64_t ld, int
     if (rows < 0 || k < 1 || k > cols || ld < cols || !V || !idx) return fail(TSM_E_ARG, "topk: need 1 <= k <= cols");
     if (rows == 0) return TSM_OK;
     if (rows > 0x7fffffff) return fail(TSM_E_ARG, "topk: too many rows");
-    hipLaunchKernelGGL(topk_kernel<double>, dim3((unsigned)rows), dim3(256), 0, TSM_ST, V, rows, cols, ld, k, idx, val);
+    if (cols <= kTopkSmallCols && k <= kKthSmallK)
+        hipLaunchKernelGGL(topk_small_kernel<double>, dim3((unsigned)((rows + 3) / 4)), dim3(256), 0, TSM_ST, V, rows, cols,
+                           ld, k, idx, val);
+    else
+        hipLaunchKernelGGL(topk_kernel<double>, dim3((unsigned)rows), dim3(256), 0, TSM_ST, V, rows, cols, ld, k, idx, val);
     TSM_LAUNCHED();
 }
 
