@@ -274,8 +274,8 @@ def test_threshold_search_matches_full_search(gpu):
     """The fused threshold search (sampled threshold, filtered GEMM, per-row
     sort) returns the same C nearest database rows as the full distance rows
     with a radix select, including queries forced onto the fallback (lists
-    too short with ks = 1, overflowing with ks = 4000) and lists longer than
-    the one-wave select takes (ks = 470).  By default the
+    too short with ks = 1, overflowing with ks = 4000) and long lists
+    (ks = 470).  By default the
     threshold aims at 4 C candidates per query, so fallbacks are rare."""
     import torch
 
@@ -290,7 +290,7 @@ def test_threshold_search_matches_full_search(gpu):
     q = torch.arange(nq, dtype=torch.int32, device="cuda")
     A = index._split(enc.embed(dset, q), query_side=True)
     full = index._search_full(A, 200).cpu().numpy()
-    # ks = 470: ~2,000 survivors per query, past the one-wave select (1,024)
+    # ks = 470: ~2,000 survivors per query
     for ks, expect_redo in ((None, False), (1, True), (4000, True), (470, False)):
         got = index.search(q, 200, ks=ks).cpu().numpy()
         same = np.mean([len(set(got[i]) & set(full[i])) == 200 for i in range(nq)])

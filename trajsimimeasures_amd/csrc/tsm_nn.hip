@@ -431,54 +431,6 @@ __global__ __launch_bounds__(256) void select_candidates_kernel(const uint32_t *
         return;
     }
     if (threadIdx.x == 0) flag[row] = 0;
-    if (n <= 1024) {
-        // one wave: each lane sorts its 16 keys (entries lane + 64 i) in
-        // registers, then C rounds of a wave minimum pop them in ascending
-        // order (keys are unique: the column is in the low bits); the same C
-        // keys in the same order as the bitonic network below
-        if (threadIdx.x >= kWave) return;  // block-uniform branch on n; no barrier follows
-        const int lane = threadIdx.x;
-        uint64_t a[16];
-#pragma unroll
-        for (int i = 0; i < 16; ++i) {
-            const int t = lane + kWave * i;
-            if (t < (int)n) {
-                const uint32_t u = __builtin_bit_cast(uint32_t, key[row * cap + t]);
-                const uint32_t kk = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-                a[i] = ((uint64_t)kk << 32) | (uint32_t)idx[row * cap + t];
-            } else {
-                a[i] = ~0ull;
-            }
-        }
-#pragma unroll
-        for (int kk = 2; kk <= 16; kk <<= 1)
-#pragma unroll
-            for (int j = kk >> 1; j > 0; j >>= 1)
-#pragma unroll
-                for (int i = 0; i < 16; ++i) {
-                    const int l = i ^ j;
-                    if (l > i) {
-                        const uint64_t x = a[i], y = a[l];
-                        const uint64_t lo = x < y ? x : y, hi = x < y ? y : x;
-                        a[i] = (i & kk) == 0 ? lo : hi;
-                        a[l] = (i & kk) == 0 ? hi : lo;
-                    }
-                }
-        for (int r = 0; r < C; ++r) {
-            uint64_t m = a[0];
-#pragma unroll
-            for (int o = 32; o > 0; o >>= 1) {
-                const uint64_t y = __shfl_xor(m, o, kWave);
-                m = y < m ? y : m;
-            }
-            const bool pop = a[0] == m;
-#pragma unroll
-            for (int i = 0; i < 15; ++i) a[i] = pop ? a[i + 1] : a[i];
-            a[15] = pop ? ~0ull : a[15];
-            if (lane == 0) out[row * C + r] = (int32_t)(uint32_t)m;
-        }
-        return;
-    }
     int P = 1;
     while (P < (int)n) P <<= 1;
     for (int t = threadIdx.x; t < P; t += blockDim.x) {
