@@ -195,3 +195,24 @@ def test_fp32_origin_keeps_fp32_inputs_exact():
     assert fp32_origin(np.array([[0.0, 0.0], [0.0, 0.0]])) == ((0.0, 0.0), 0.0)
     o, cmax = fp32_origin(np.array([[0.25, -3.0], [0.75, 5.0]]))
     assert o == (0.5, 1.0) and cmax == 4.0
+
+
+def test_reader_refuses_truncated_and_corrupted_files(tmp_path):
+    """A file cut short, or with its root group's B-tree signature damaged,
+    raises H5FormatError (never an IndexError or a silent misread)."""
+    path, _ = _gold("h5_spatial")
+    raw = open(path, "rb").read()
+    cut = tmp_path / "cut.h5"
+    cut.write_bytes(raw[:2000])
+    with pytest.raises(H5FormatError):
+        with H5File(str(cut)) as f:
+            for i in range(12):
+                f["/trajs_merc/%d" % i][:]
+    bad = bytearray(raw)
+    i = raw.find(b"TREE")
+    bad[i:i + 4] = b"XXXX"
+    p = tmp_path / "bad.h5"
+    p.write_bytes(bytes(bad))
+    with pytest.raises(H5FormatError, match="B-tree"):
+        with H5File(str(p)) as f:
+            f.keys("/")

@@ -255,8 +255,12 @@ class H5File:
         size = self._uint(head, 8, 4)
         msgs = []
         blocks = [(addr + 16, size)]
+        seen = set()
         while blocks and len(msgs) < nmsg:
             a, n = blocks.pop(0)
+            if a in seen:
+                raise H5FormatError(f"{self.path}: object header continuation cycle at {a:#x}")
+            seen.add(a)
             buf = self._bytes(a, n)
             p = 0
             while p + 8 <= n and len(msgs) < nmsg:
@@ -284,8 +288,12 @@ class H5File:
         track = bool(flags & 0x04)
         msgs = []
         blocks = [(addr + p, size)]
+        seen = set()
         while blocks:
             a, n = blocks.pop(0)
+            if a in seen:
+                raise H5FormatError(f"{self.path}: object header continuation cycle at {a:#x}")
+            seen.add(a)
             buf = self._bytes(a, n)
             q = 0
             hdr = 4 + (2 if track else 0)
