@@ -263,7 +263,7 @@ __global__ __launch_bounds__(2 * BM) void gemm_nt_lds_kernel(GemmArgs g, FilterA
 }
 
 // Persistent search GEMM (the embedding search: M queries x N database rows,
-// K = 32 * KC <= 384).  A 512-thread block owns 128 query rows, whose whole A
+// K = 32 * KC <= 384).  A 1024-thread block owns 128 query rows, whose whole A
 // operand it stages in LDS once, and streams 128-column database tiles
 // through double-buffered LDS in K steps of 64, tile after tile, with no
 // per-tile prologue.  The database loads run a full tile ahead: step s of
@@ -285,12 +285,14 @@ constexpr int kSearchRows = 128;
 constexpr int kSearchAPad = 48, kSearchBStride = kLdsStride;
 
 // The one-tile-ahead register ring of the search kernel: six named tiles
-// picked at compile time (an array here ends up in scratch memory).
-constexpr int kSearchThreads = 512;
-constexpr int kRingVec = kBN * 8 / kSearchThreads;  // 16-byte loads per thread per step
-struct SearchRing {
-    u32x4 r0[kRingVec], r1[kRingVec], r2[kRingVec], r3[kRingVec], r4[kRingVec], r5[kRingVec];
-    template <int S> __device__ __forceinline__ u32x4 (&at())[kRingVec] {
+// picked at compile time (an array here ends up in scratch memory).  1024
+// threads: sixteen waves of 32 x 32 outputs (two column tiles each), 110
+// VGPRs, four waves per SIMD to cover each step's barrier and LDS waits
+// (8 % faster than eight waves of 32 x 64 at two per SIMD, same values).
+constexpr int kSearchThreads = 1024, kSearchColTiles = 2;
+template <int RV> struct SearchRingT {
+    u32x4 r0[RV], r1[RV], r2[RV], r3[RV], r4[RV], r5[RV];
+    template <int S> __device__ __forceinline__ u32x4 (&at())[RV] {
         if constexpr (S == 0) return r0;
         else if constexpr (S == 1) return r1;
         else if constexpr (S == 2) return r2;
@@ -302,36 +304,36 @@ struct SearchRing {
 
 // One K step s of tile t: store the ring slot (this tile's step s) into LDS,
 // reload it with tile tn's step s, then multiply from LDS.  A wave owns 32
-// rows x 64 columns (2 x 4 MFMA tiles).
-template <int s>
-__device__ __forceinline__ void search_step(const GemmArgs &g, SearchRing &ring, uint16_t *As, int AS,
+// rows x 16 JT columns (2 x JT MFMA tiles).
+template <int s, int NT, int JT>
+__device__ __forceinline__ void search_step(const GemmArgs &g, SearchRingT<kBN * 8 / NT> &ring, uint16_t *As, int AS,
                                            uint16_t (*Bs)[kBN * kSearchBStride], int &cur, int tn, int wm, int wn,
-                                           int r16, int kb, f32x4 (&acc)[2][4]) {
-    tile_store_v<kBN, kSearchThreads, kSearchBStride>(Bs[cur], ring.at<s>());
-    tile_load_rows<kBN, kSearchThreads>(g.B, g.ldb, tn * kBN, g.N, s * kBK, ring.at<s>());
+                                           int r16, int kb, f32x4 (&acc)[2][JT]) {
+    tile_store_v<kBN, NT, kSearchBStride>(Bs[cur], ring.template at<s>());
+    tile_load_rows<kBN, NT>(g.B, g.ldb, tn * kBN, g.N, s * kBK, ring.template at<s>());
     __syncthreads();
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk) {
-        bf16x8 af[2], bfr[4];
+        bf16x8 af[2], bfr[JT];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
             af[i] = __builtin_bit_cast(
                 bf16x8, *reinterpret_cast<const uint4 *>(&As[(wm + 16 * i + r16) * AS + 64 * s + 32 * kk + kb]));
 #pragma unroll
-        for (int j = 0; j < 4; ++j)
+        for (int j = 0; j < JT; ++j)
             bfr[j] = __builtin_bit_cast(
                 bf16x8, *reinterpret_cast<const uint4 *>(&Bs[cur][(wn + 16 * j + r16) * kSearchBStride + 32 * kk + kb]));
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j)
+            for (int j = 0; j < JT; ++j)
                 acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
     }
     cur ^= 1;
 }
 
-template <int KC>
-__global__ __launch_bounds__(kSearchThreads, 2) void search_filter_kernel(GemmArgs g, FilterArgs f, int qblocks,
+template <int KC, int NT, int JT>
+__global__ __launch_bounds__(NT, NT / 256) void search_filter_kernel(GemmArgs g, FilterArgs f, int qblocks,
                                                                           int chunks, int tiles_per_chunk) {
     static_assert(KC == 12, "the register ring holds six K steps of 64");
     constexpr int AS = 32 * KC + kSearchAPad;  // A row stride in LDS (BF16), padded
@@ -344,9 +346,10 @@ __global__ __launch_bounds__(kSearchThreads, 2) void search_filter_kernel(GemmAr
     const int ntiles = (g.N + kBN - 1) / kBN;
     const int t0 = c * tiles_per_chunk, t1 = min(ntiles, t0 + tiles_per_chunk);
     if (t0 >= t1) return;  // block-uniform
-    const int bm = q * kSearchRows, wm = (wave >> 1) * 32, wn = (wave & 1) * 64;
+    constexpr int CG = 128 / (16 * JT);  // column groups of waves
+    const int bm = q * kSearchRows, wm = (wave / CG) * 32, wn = (wave % CG) * (16 * JT);
     // the block's A rows, all of K, into LDS (16-byte chunks; zero past M / K)
-    for (int e = threadIdx.x; e < kSearchRows * 4 * KC; e += kSearchThreads) {
+    for (int e = threadIdx.x; e < kSearchRows * 4 * KC; e += NT) {
         const int row = e / (4 * KC), k = (e % (4 * KC)) * 8;
         uint4 v = make_uint4(0, 0, 0, 0);
         if (bm + row < g.M && k < g.K) v = *reinterpret_cast<const uint4 *>(g.A + (int64_t)(bm + row) * g.lda + k);
@@ -362,38 +365,38 @@ __global__ __launch_bounds__(kSearchThreads, 2) void search_filter_kernel(GemmAr
             const float tv = f.tau[row < g.M ? row : g.M - 1];
             taur[i][e] = row < g.M ? tv : -__builtin_huge_valf();
         }
-    SearchRing ring;  // step s of the next tile
-    tile_load_rows<kBN, kSearchThreads>(g.B, g.ldb, t0 * kBN, g.N, 0 * kBK, ring.at<0>());
-    tile_load_rows<kBN, kSearchThreads>(g.B, g.ldb, t0 * kBN, g.N, 1 * kBK, ring.at<1>());
-    tile_load_rows<kBN, kSearchThreads>(g.B, g.ldb, t0 * kBN, g.N, 2 * kBK, ring.at<2>());
-    tile_load_rows<kBN, kSearchThreads>(g.B, g.ldb, t0 * kBN, g.N, 3 * kBK, ring.at<3>());
-    tile_load_rows<kBN, kSearchThreads>(g.B, g.ldb, t0 * kBN, g.N, 4 * kBK, ring.at<4>());
-    tile_load_rows<kBN, kSearchThreads>(g.B, g.ldb, t0 * kBN, g.N, 5 * kBK, ring.at<5>());
+    SearchRingT<kBN * 8 / NT> ring;  // step s of the next tile
+    tile_load_rows<kBN, NT>(g.B, g.ldb, t0 * kBN, g.N, 0 * kBK, ring.template at<0>());
+    tile_load_rows<kBN, NT>(g.B, g.ldb, t0 * kBN, g.N, 1 * kBK, ring.template at<1>());
+    tile_load_rows<kBN, NT>(g.B, g.ldb, t0 * kBN, g.N, 2 * kBK, ring.template at<2>());
+    tile_load_rows<kBN, NT>(g.B, g.ldb, t0 * kBN, g.N, 3 * kBK, ring.template at<3>());
+    tile_load_rows<kBN, NT>(g.B, g.ldb, t0 * kBN, g.N, 4 * kBK, ring.template at<4>());
+    tile_load_rows<kBN, NT>(g.B, g.ldb, t0 * kBN, g.N, 5 * kBK, ring.template at<5>());
     int cur = 0;
     for (int t = t0; t < t1; ++t) {
-        f32x4 acc[2][4];
+        f32x4 acc[2][JT];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+            for (int j = 0; j < JT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
         const int tn = t + 1 < t1 ? t + 1 : t;  // the last tile reloads itself (unused)
-        search_step<0>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
-        search_step<1>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
-        search_step<2>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
-        search_step<3>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
-        search_step<4>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
-        search_step<5>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
+        search_step<0, NT, JT>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
+        search_step<1, NT, JT>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
+        search_step<2, NT, JT>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
+        search_step<3, NT, JT>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
+        search_step<4, NT, JT>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
+        search_step<5, NT, JT>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
         // filter epilogue of tile t: biases of the lane's 4 columns in one batch, then
         // compare (hits are rare: about 4 C per query over the whole database)
-        float bcol[4];
+        float bcol[JT];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < JT; ++j) {
             const int col = t * kBN + wn + 16 * j + r16;
             const float bv = g.bias ? g.bias[col < g.N ? col : g.N - 1] : 0.f;
             bcol[j] = bv;
         }
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
+        for (int j = 0; j < JT; ++j) {
             const int col = t * kBN + wn + 16 * j + r16;
 #pragma unroll
             for (int i = 0; i < 2; ++i)
@@ -1242,8 +1245,8 @@ int tsm_nn_gemm_nt_filter(int M, int N, int K, const uint16_t *A, int lda, const
         int chunks = (ncu / qblocks) / 8 * 8;
         if (chunks < 8) chunks = 8;
         const int tpc = (ntiles + chunks - 1) / chunks;
-        hipLaunchKernelGGL(search_filter_kernel<12>, dim3(qblocks * chunks), dim3(kSearchThreads), 0,
-                           reinterpret_cast<hipStream_t>(stream), g, fa, qblocks, chunks, tpc);
+        hipLaunchKernelGGL((search_filter_kernel<12, kSearchThreads, kSearchColTiles>), dim3(qblocks * chunks),
+                           dim3(kSearchThreads), 0, reinterpret_cast<hipStream_t>(stream), g, fa, qblocks, chunks, tpc);
     } else if (M > kBM) {  // 256 query rows per block: the database is streamed half as often
         dim3 grid((N + kBN - 1) / kBN, (M + 255) / 256);
         hipLaunchKernelGGL((gemm_nt_lds_kernel<1, 256>), grid, dim3(512), 0, reinterpret_cast<hipStream_t>(stream), g,
