@@ -2,7 +2,7 @@
 random walks (lognormal lengths, median 60, clipped to [10, 1000]), k = 10,
 200 candidates per query re-ranked by the exact DTW kernel.  Shows what the
 trained network gains over the same filter without training (the resampled-
-L2 embedding), with the exact all-pairs DTW kNN as the judge of recall."""
+L2 embedding), with the exact all-pairs DTW kNN as the reference for recall."""
 import numpy as np
 import pytest
 
