@@ -101,6 +101,8 @@ def main():
     ap.add_argument("--knn-db", type=int, default=1_000_000)
     ap.add_argument("--knn-queries", type=int, default=1000)
     ap.add_argument("--no-cluster", action="store_true", help="skip the all-pairs + k-medoids leg")
+    ap.add_argument("--no-host", action="store_true",
+                    help="skip the DTW rates from host buffers (padded drop-in and packed upload, PCIe included)")
     ap.add_argument("--cluster-n", type=int, default=10_000)
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one plain 256 MiB device copy (calibrates HBM byte counters under rocprofv3)")
@@ -212,6 +214,7 @@ def main():
     if args.pairs != 100_000:
         traffic = None
 
+    host = host_leg(ts, pa, pb, dev) if "dtw" in measures and not args.no_host else None
     knn = None if args.no_knn else knn_leg(args.knn_queries, args.knn_db)
     clu = None if args.no_cluster else cluster_leg(args.cluster_n, cpu_too=not args.no_cpu)
 
@@ -237,10 +240,55 @@ def main():
                      "note": "9 algorithmic FP32 FLOP per DTW cell over mean dp_kernel duration (HIP events)"},
         "cpu_baseline": cpu,
         "measures": report,
+        "host_buffers": host,
         "knn": knn,
         "cluster": clu,
     }
     print(json.dumps(line))
+
+
+def host_leg(ts, pa, pb, dev, reps: int = 3) -> dict:
+    """The same DTW pairs handed over in host memory, timed from the host
+    buffers to the results back in host memory (PCIe included), best of
+    `reps` after one warm-up:
+    - padded: the reference's boundary, core.dtw(t1, len1, t2, len2) on
+      zero-padded float64 [pairs, max length, 2] arrays (core/dtw.py:56-77);
+    - packed: the set uploaded once as one point buffer
+      (engine.DeviceTrajSet: FP64 copy + centred FP32 copy) and the pairs
+      given as index arrays, as drivers.pairwise does."""
+    import torch
+
+    from trajsimimeasures_amd import core, engine
+
+    la, lb = ts.lengths[pa], ts.lengths[pb]
+    width = int(max(la.max(), lb.max()))
+
+    def padded(idx, lens):
+        arr = np.zeros((len(idx), width, 2), dtype=np.float64)
+        rows = np.repeat(np.arange(len(idx)), lens)
+        cols = np.arange(int(lens.sum())) - np.repeat(np.cumsum(lens) - lens, lens)
+        arr[rows, cols] = ts.xy[ts.offsets[idx][rows] + cols]
+        return arr
+
+    t1, t2 = padded(pa, la), padded(pb, lb)
+
+    def best(fn):
+        fn()
+        ts_ = []
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            fn()
+            ts_.append(time.perf_counter() - t)
+        return min(ts_)
+
+    t_pad = best(lambda: core.dtw(t1, la, t2, lb))
+    t_pack = best(lambda: engine.pairs("dtw", engine.DeviceTrajSet(ts, device=dev), pa, pb).cpu().numpy())
+    n = len(pa)
+    out = {"padded_pairs_per_s": n / t_pad, "padded_s": t_pad, "padded_bytes": t1.nbytes + t2.nbytes,
+           "packed_pairs_per_s": n / t_pack, "packed_s": t_pack, "packed_bytes": ts.xy.nbytes + ts.offsets.nbytes}
+    print(f"[bench] host buffers (PCIe included): {out}", file=sys.stderr, flush=True)
+    return out
 
 
 def cluster_leg(n: int, k: int = 10, cpu_too: bool = True) -> dict:

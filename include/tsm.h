@@ -42,7 +42,7 @@
 extern "C" {
 #endif
 
-#define TSM_ABI_VERSION 3
+#define TSM_ABI_VERSION 4
 
 /* Measure ids. */
 #define TSM_DTW 0       /* core/dtw.py:10-77 */
@@ -161,6 +161,24 @@ int tsm_pairs_triu(int measure, const tsm_trajset *set, int64_t n, const tsm_par
 int tsm_pairs_host(int measure, const double *xy64, const double *t64, const int64_t *offsets, int64_t n,
                    int64_t npts, const int32_t *pa, const int32_t *pb, int64_t npairs, const tsm_params *prm,
                    double *out, int32_t *counts);
+
+/*
+ * The reference's padded batches as they are: t1 [B][L1][dim] and t2
+ * [B][L2][dim] (row-major float64, zero-padded; dim 2 = x, y or 3 = x, y, t)
+ * with lengths len1[B], len2[B] in [1, L]; pair k is (t1[k], t2[k]).  Both
+ * batches are copied to the device unchanged and compacted there (no host
+ * repacking); timestamps are read for STEDR/STLCSS/CDDS/SAR (dim must be 3)
+ * and ignored otherwise.  Non-finite values inside the lengths are an
+ * error; padding is never read.  Runs on the null stream, copies the B
+ * results (and counts) back and synchronises.  Replaces the body of the
+ * reference's per-batch functions, which take exactly these arguments:
+ * core/dtw.py:56-77 dtw(t1, len1, t2, len2), edr.py:55-76, lcss.py:56-77,
+ * erp.py:94-120, discret_frechet.py:53-74, hausdorff.py:97-123,
+ * stedr.py:65-86, stlcss.py:57-81, cdds.py:100-124, sar.py:247-286.
+ */
+int tsm_pairs_padded_host(int measure, const double *t1, const int64_t *len1, int64_t L1, const double *t2,
+                          const int64_t *len2, int64_t L2, int64_t B, int dim, const tsm_params *prm, double *out,
+                          int32_t *counts);
 
 /*
  * Autotuner hook: lanes per pair (4 or 8) for each column count 1..n-1

@@ -28,7 +28,7 @@ def test_library_exports_every_declared_function():
     for n in names:
         assert hasattr(L, n), n
     assert set(names) == set(_lib.EXPORTS)
-    assert L.tsm_abi_version() == 3
+    assert L.tsm_abi_version() == 4
     nn = header_functions("tsm_nn.h")
     for n in nn:
         assert hasattr(L, n), n
@@ -75,6 +75,51 @@ def test_argument_errors_without_device():
     assert rc == -1 and b"empty" in L.tsm_last_error()
     with pytest.raises(_lib.TsmError):
         _lib.check(-1)
+
+
+def test_padded_host_argument_errors_without_device():
+    """tsm_pairs_padded_host checks measure, point width, lengths and
+    pointers before any device work (the batch pointers here are never
+    dereferenced), and core.* turns these into ValueError."""
+    L = _lib.lib()
+    fake = ctypes.c_void_p(16)
+    prm = _lib.params(0.1)
+    lens = np.array([2, 3], dtype=np.int64)
+    out = np.zeros(2)
+
+    def call(measure=_lib.DTW, l1=lens, L1=3, l2=lens, L2=3, B=2, dim=2, t=fake, o=out):
+        return L.tsm_pairs_padded_host(measure, t, l1.ctypes.data, L1, t, l2.ctypes.data, L2, B, dim,
+                                       ctypes.byref(prm), o.ctypes.data if o is not None else None, None)
+
+    assert call(B=0) == 0
+    for kw, msg in ((dict(measure=99), b"unknown measure"), (dict(dim=4), b"2 (x, y) or 3"),
+                    (dict(measure=_lib.STEDR), b"need [batch, seq, 3]"), (dict(L1=2), b"every length"),
+                    (dict(l2=np.array([0, 1], dtype=np.int64)), b"every length"), (dict(L2=0), b"padded lengths"),
+                    (dict(t=None), b"null"), (dict(o=None), b"null"), (dict(B=-1), b"negative")):
+        assert call(**kw) == -1 and msg in L.tsm_last_error(), kw
+    from trajsimimeasures_amd import core
+    t = np.zeros((2, 3, 2))
+    with pytest.raises(ValueError, match="every length"):
+        core.dtw(t, [3, 4], t, [1, 1])
+    with pytest.raises(ValueError, match="every length"):
+        core.dtw(t, [3, 0], t, [1, 1])
+    with pytest.raises(ValueError, match="lengths for a batch"):
+        core.dtw(t, [3], t, [1, 1])
+    with pytest.raises(ValueError):
+        core.hausdorff(np.zeros((2, 3, 4)), [1, 1], np.zeros((2, 3, 4)), [1, 1])
+
+
+def test_nonfinite_origin_rejected_without_device():
+    """The FP32 conversion rejects a NaN or infinite origin before any
+    device work (the point pointers are never dereferenced).  The library is
+    built with -fno-honor-nans, under which a plain isfinite / bit-pattern
+    test of a double loses its NaN half; this pins the test it uses."""
+    L = _lib.lib()
+    fake = ctypes.c_void_p(16)
+    for ox, oy in ((float("nan"), 0.0), (0.0, float("nan")), (float("inf"), 0.0), (0.0, float("-inf")),
+                   (-float("nan"), 1.0)):
+        rc = L.tsm_xy_to_fp32_centred(fake, fake, 1, ox, oy, None)
+        assert rc == -1 and b"origin must be finite" in L.tsm_last_error(), (ox, oy)
 
 
 def test_sar_argument_errors_without_device():

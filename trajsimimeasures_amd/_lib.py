@@ -30,7 +30,8 @@ ERRORS = {-1: "TSM_E_ARG", -2: "TSM_E_HIP", -3: "TSM_E_NOMEM", -4: "TSM_E_UNSUP"
 # Every function include/tsm.h declares; the load check verifies all exist.
 EXPORTS = ("tsm_init", "tsm_last_error", "tsm_abi_version", "tsm_params_default", "tsm_xy_to_fp32",
            "tsm_xy_to_fp32_centred",
-           "tsm_pairs", "tsm_pairs_grid", "tsm_pairs_triu", "tsm_pairs_host", "tsm_tune_set_lanes",
+           "tsm_pairs", "tsm_pairs_grid", "tsm_pairs_triu", "tsm_pairs_host", "tsm_pairs_padded_host",
+           "tsm_tune_set_lanes",
            "tsm_tune_set_waves_per_cu", "tsm_tune_set_wide_lanes", "tsm_graph_capture_pairs", "tsm_graph_launch", "tsm_graph_destroy",
            "tsm_profile_enable", "tsm_profile_read")
 # include/tsm_nn.h (argument types are declared by filter.py)
@@ -82,6 +83,8 @@ def _declare(L):
                                  ctypes.POINTER(ParamsC), _P, _P, _P]
     L.tsm_pairs_triu.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _I64, ctypes.POINTER(ParamsC), _P, _P, _P]
     L.tsm_pairs_host.argtypes = [ctypes.c_int, _P, _P, _P, _I64, _I64, _P, _P, _I64, ctypes.POINTER(ParamsC), _P, _P]
+    L.tsm_pairs_padded_host.argtypes = [ctypes.c_int, _P, _P, _I64, _P, _P, _I64, _I64, ctypes.c_int,
+                                        ctypes.POINTER(ParamsC), _P, _P]
     L.tsm_tune_set_lanes.argtypes = [ctypes.c_int, _P, ctypes.c_int]
     L.tsm_square_from_triu.argtypes = [_P, _I64, _P, _P]
     L.tsm_kmedoids_fasterpam.argtypes = [_P, _I64, ctypes.c_int32, _P, ctypes.c_int32, _P, _P, _P, _P, _P]

@@ -276,7 +276,7 @@ int tsm_init(int device) {
 
 int tsm_xy_to_fp32_centred(const double *xy64, float *xy32, int64_t npts, double ox, double oy, void *stream) {
     if (npts < 0 || (npts > 0 && (!xy64 || !xy32))) return fail(TSM_E_ARG, "bad point buffers");
-    if (!std::isfinite(ox) || !std::isfinite(oy)) return fail(TSM_E_ARG, "the origin must be finite");
+    if (!finite64(ox) || !finite64(oy)) return fail(TSM_E_ARG, "the origin must be finite");
     if (npts == 0) return TSM_OK;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     hipLaunchKernelGGL(xy_to_f32_kernel, dim3(grid_for(2 * npts, 256)), dim3(256), 0, st, xy64, xy32, 2 * npts, ox,
@@ -370,6 +370,144 @@ int tsm_pairs_host(int measure, const double *xy64, const double *t64, const int
     if (rc) return rc;
     TSM_HIP(hipMemcpy(out, c.h_out.p, npairs * 8, hipMemcpyDeviceToHost));
     if (counts) TSM_HIP(hipMemcpy(counts, c.h_cnt.p, npairs * 4, hipMemcpyDeviceToHost));
+    return TSM_OK;
+}
+
+// Compaction of the reference's padded batches on the device: trajectory
+// k < B is row k of t1 [B, L1, dim], trajectory B + k row k of t2 [B, L2,
+// dim], staged one after the other in `pad`.  One wavefront per trajectory
+// (grid stride); point i goes to offs[k] + i of xy64 and, when t64 is not
+// null, its column 2 to t64.  Each block writes the bounding box of the
+// points it copied and whether any value it read was not finite:
+// box[5 b + 0..4] = lo x, lo y, hi x, hi y, 1.0 if not finite else 0.0.
+constexpr int kUnpadBlock = 256;
+__global__ __launch_bounds__(kUnpadBlock) void unpad_kernel(const double *pad, int64_t B, int64_t L1, int64_t L2,
+                                                             int dim, const int64_t *offs, double *xy64, double *t64,
+                                                             double *box) {
+    constexpr int kWaves = kUnpadBlock / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double lx = INFINITY, ly = INFINITY, hx = -INFINITY, hy = -INFINITY;
+    bool bad = false;
+    const int64_t stride = (int64_t)gridDim.x * kWaves;
+    for (int64_t k = (int64_t)blockIdx.x * kWaves + wave; k < 2 * B; k += stride) {
+        const int64_t o = offs[k], n = offs[k + 1] - o;
+        // moved as raw 64-bit words: the finite test needs the bits as loaded
+        const uint64_t *src = reinterpret_cast<const uint64_t *>(pad) + (k < B ? k * L1 : B * L1 + (k - B) * L2) * dim;
+        for (int64_t i = lane; i < n; i += 64) {
+            const uint64_t bx = src[i * dim], by = src[i * dim + 1];
+            reinterpret_cast<uint64_t *>(xy64)[2 * (o + i)] = bx;
+            reinterpret_cast<uint64_t *>(xy64)[2 * (o + i) + 1] = by;
+            bad |= !finite_bits(bx) || !finite_bits(by);
+            const double x = __builtin_bit_cast(double, bx), y = __builtin_bit_cast(double, by);
+            lx = fmin(lx, x);
+            ly = fmin(ly, y);
+            hx = fmax(hx, x);
+            hy = fmax(hy, y);
+            if (t64) {
+                const uint64_t bt = src[i * dim + 2];
+                reinterpret_cast<uint64_t *>(t64)[o + i] = bt;
+                bad |= !finite_bits(bt);
+            }
+        }
+    }
+    for (int m = 32; m >= 1; m >>= 1) {
+        lx = fmin(lx, __shfl_xor(lx, m));
+        ly = fmin(ly, __shfl_xor(ly, m));
+        hx = fmax(hx, __shfl_xor(hx, m));
+        hy = fmax(hy, __shfl_xor(hy, m));
+    }
+    const bool wbad = __any(bad);
+    __shared__ double part[kWaves][5];
+    if (lane == 0) {
+        part[wave][0] = lx;
+        part[wave][1] = ly;
+        part[wave][2] = hx;
+        part[wave][3] = hy;
+        part[wave][4] = wbad ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    if (threadIdx.x < 5) {
+        const int e = threadIdx.x;
+        double v = part[0][e];
+        for (int w = 1; w < kWaves; ++w) v = e < 2 ? fmin(v, part[w][e]) : fmax(v, part[w][e]);
+        box[5 * (int64_t)blockIdx.x + e] = v;
+    }
+}
+
+__global__ void iota_pairs_kernel(int32_t *pa, int32_t *pb, int64_t B) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < B; k += (int64_t)gridDim.x * blockDim.x) {
+        pa[k] = (int32_t)k;
+        pb[k] = (int32_t)(B + k);
+    }
+}
+
+int tsm_pairs_padded_host(int measure, const double *t1, const int64_t *len1, int64_t L1, const double *t2,
+                          const int64_t *len2, int64_t L2, int64_t B, int dim, const tsm_params *prm, double *out,
+                          int32_t *counts) {
+    if (measure < TSM_DTW || measure > TSM_SAR) return fail(TSM_E_ARG, "unknown measure id " + std::to_string(measure));
+    if (B < 0) return fail(TSM_E_ARG, "negative batch size");
+    if (B == 0) return TSM_OK;
+    if (B > 0x3FFFFFFFll) return fail(TSM_E_ARG, "at most 2^30 - 1 pairs per call");
+    if (!t1 || !t2 || !len1 || !len2 || !out) return fail(TSM_E_ARG, "null batch, length or output array");
+    if (dim != 2 && dim != 3) return fail(TSM_E_ARG, "points must have 2 (x, y) or 3 (x, y, t) values");
+    const bool timed = measure == TSM_STEDR || measure == TSM_STLCSS || measure == TSM_CDDS || measure == TSM_SAR;
+    if (timed && dim != 3) return fail(TSM_E_ARG, "STEDR/STLCSS/CDDS/SAR need [batch, seq, 3] points (x, y, t)");
+    if (L1 < 1 || L2 < 1) return fail(TSM_E_ARG, "padded lengths must be >= 1");
+    std::vector<int64_t> offs(2 * B + 1);
+    offs[0] = 0;
+    int64_t max_len = 0;
+    for (int64_t k = 0; k < 2 * B; ++k) {
+        const int64_t l = k < B ? len1[k] : len2[k - B], L = k < B ? L1 : L2;
+        if (l < 1 || l > L) return fail(TSM_E_ARG, "every length must be in [1, padded length]");
+        offs[k + 1] = offs[k] + l;
+        if (l > max_len) max_len = l;
+    }
+    const int64_t npts = offs[2 * B];
+    const int64_t n1 = B * L1 * dim, n2 = B * L2 * dim;
+    const int nblk = (int)std::min<int64_t>((2 * B + 3) / 4, 1024);
+    Context &c = g_ctx;
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        int rc = ensure_ctx();
+        if (rc) return rc;
+        if ((rc = c.h_pad.ensure((n1 + n2) * 8)) || (rc = c.h_box.ensure(nblk * 40)) ||
+            (rc = c.h_xy64.ensure(npts * 16)) || (rc = c.h_xy32.ensure(npts * 8)) ||
+            (rc = c.h_offs.ensure((2 * B + 1) * 8)) || (rc = c.h_pa.ensure(B * 4)) || (rc = c.h_pb.ensure(B * 4)) ||
+            (rc = c.h_out.ensure(B * 8)) || (rc = c.h_cnt.ensure(B * 4)) || (timed && (rc = c.h_t64.ensure(npts * 8))))
+            return rc;
+    }
+    double *pad = c.h_pad.as<double>();
+    TSM_HIP(hipMemcpy(pad, t1, n1 * 8, hipMemcpyHostToDevice));
+    TSM_HIP(hipMemcpy(pad + n1, t2, n2 * 8, hipMemcpyHostToDevice));
+    TSM_HIP(hipMemcpy(c.h_offs.p, offs.data(), (2 * B + 1) * 8, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(unpad_kernel, dim3(nblk), dim3(kUnpadBlock), 0, nullptr, pad, B, L1, L2, dim,
+                       c.h_offs.as<int64_t>(), c.h_xy64.as<double>(), timed ? c.h_t64.as<double>() : nullptr,
+                       c.h_box.as<double>());
+    TSM_HIP(hipGetLastError());
+    hipLaunchKernelGGL(iota_pairs_kernel, dim3(grid_for(B, 256)), dim3(256), 0, nullptr, c.h_pa.as<int32_t>(),
+                       c.h_pb.as<int32_t>(), B);
+    TSM_HIP(hipGetLastError());
+    std::vector<double> box(5 * (size_t)nblk);
+    TSM_HIP(hipMemcpy(box.data(), c.h_box.p, box.size() * 8, hipMemcpyDeviceToHost));
+    double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+    for (int b = 0; b < nblk; ++b) {
+        if (box[5 * b + 4] != 0.0) return fail(TSM_E_ARG, "coordinates and timestamps must be finite");
+        lo[0] = std::fmin(lo[0], box[5 * b]);
+        lo[1] = std::fmin(lo[1], box[5 * b + 1]);
+        hi[0] = std::fmax(hi[0], box[5 * b + 2]);
+        hi[1] = std::fmax(hi[1], box[5 * b + 3]);
+    }
+    double ox = 0.0, oy = 0.0, cmax = 0.0;
+    fp32_origin_of_box(lo, hi, ox, oy, cmax);
+    int rc = tsm_xy_to_fp32_centred(c.h_xy64.as<double>(), c.h_xy32.as<float>(), npts, ox, oy, nullptr);
+    if (rc) return rc;
+    tsm_trajset set{c.h_xy32.as<float>(), c.h_xy64.as<double>(), c.h_offs.as<int64_t>(), 2 * B, npts, max_len, cmax,
+                    timed ? c.h_t64.as<double>() : nullptr};
+    rc = tsm_pairs(measure, &set, c.h_pa.as<int32_t>(), c.h_pb.as<int32_t>(), B, prm, c.h_out.as<double>(),
+                   counts ? c.h_cnt.as<int32_t>() : nullptr, nullptr);
+    if (rc) return rc;
+    TSM_HIP(hipMemcpy(out, c.h_out.p, B * 8, hipMemcpyDeviceToHost));
+    if (counts) TSM_HIP(hipMemcpy(counts, c.h_cnt.p, B * 4, hipMemcpyDeviceToHost));
     return TSM_OK;
 }
 

@@ -73,8 +73,8 @@ struct Context {
     int device = -1;
     int num_cu = 0;
     Workspace ws;
-    // staging for tsm_pairs_host
-    DevBuf h_xy64, h_xy32, h_offs, h_pa, h_pb, h_out, h_cnt, h_t64;
+    // staging for tsm_pairs_host / tsm_pairs_padded_host
+    DevBuf h_xy64, h_xy32, h_offs, h_pa, h_pb, h_out, h_cnt, h_t64, h_pad, h_box;
     // autotuned lanes per column count (index 0: FP32 coordinates, 1: FP64)
     uint8_t tuned_lanes[2][258] = {};
     bool tuned[2] = {false, false};
@@ -117,6 +117,22 @@ inline double eps_threshold(double eps, int mode) {
     return t;
 }
 
+// Finite tests.  The library is built with -fno-honor-nans, under which
+// the compiler folds isnan / isfinite as if NaN never occurred, and also
+// rewrites an exponent-bits test of a double's bit pattern into a floating
+// class test whose NaN half it then drops (what remains is |v| == inf).  So
+// the bits must reach the test as an integer the compiler cannot trace back
+// to a double: loaded from memory as uint64_t (finite_bits), or, for a
+// double held in a register, passed through a volatile (finite64; host
+// code, a few scalars per call).
+__host__ __device__ inline bool finite_bits(uint64_t b) {
+    return (uint32_t)(b >> 52 & 0x7ffu) != 0x7ffu;
+}
+inline bool finite64(double v) {
+    volatile uint64_t b = __builtin_bit_cast(uint64_t, v);
+    return finite_bits(b);
+}
+
 // Guard band of the FP32 match test of EDR/LCSS.  With u = 2^-24, U a bound
 // on |coordinate| and r = sqrt(S), the FP32 squared distance d2' computed
 // from FP32-rounded coordinates (two subtractions, a product and an FMA)
@@ -136,7 +152,7 @@ inline double eps_threshold(double eps, int mode) {
 // 2^-53 |x - o| to the u |x - o| of the FP32 rounding; U is widened by
 // 2^-20 to cover it (u 2^-20 U = 2^-44 U > 2^-52 U).
 inline bool count_band(double thr, double coord_max, float &lo, float &hi) {
-    if (!(coord_max > 0.0) || !std::isfinite(coord_max) || coord_max > 1e15) return false;
+    if (!finite64(coord_max) || !(coord_max > 0.0) || coord_max > 1e15) return false;
     if (!(thr > 0.0)) {  // nothing matches
         lo = hi = 0.0f;
         return true;
@@ -162,20 +178,12 @@ inline bool count_band(double thr, double coord_max, float &lo, float &hi) {
 // Every FP32 value on the axis is a multiple of its own spacing, which
 // divides that one, so x - o is exact in FP32 whenever |x - o| stays below
 // 2^24 spacings of x (FP32 inputs of one binade per axis, as Mercator
-// coordinates are).  cmax bounds |x - ox| and |y - oy|.  Returns nonzero
-// when a coordinate is not finite.
-inline int fp32_origin(const double *xy, int64_t npts, double &ox, double &oy, double &cmax) {
-    double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
-    for (int64_t i = 0; i < npts; ++i)
-        for (int k = 0; k < 2; ++k) {
-            const double v = xy[2 * i + k];
-            if (!std::isfinite(v)) return 1;
-            lo[k] = std::fmin(lo[k], v);
-            hi[k] = std::fmax(hi[k], v);
-        }
+// coordinates are).  cmax bounds |x - ox| and |y - oy|.  From the box
+// lo[2], hi[2] of a non-empty set of finite points:
+inline void fp32_origin_of_box(const double lo[2], const double hi[2], double &ox, double &oy, double &cmax) {
     double o[2] = {0.0, 0.0};
     cmax = 0.0;
-    for (int k = 0; k < 2 && npts > 0; ++k) {
+    for (int k = 0; k < 2; ++k) {
         const double big = std::fmax(std::fabs(lo[k]), std::fabs(hi[k]));
         if (big > 0.0) {
             const double q = std::ldexp(1.0, std::ilogb(big) - 23);  // FP32 spacing at |big|
@@ -185,6 +193,23 @@ inline int fp32_origin(const double *xy, int64_t npts, double &ox, double &oy, d
     }
     ox = o[0];
     oy = o[1];
+}
+
+// The same from the points xy [npts][2]; returns nonzero when a coordinate
+// is not finite (or npts < 1).
+inline int fp32_origin(const double *xy, int64_t npts, double &ox, double &oy, double &cmax) {
+    if (npts < 1) return 1;
+    double lo[2] = {INFINITY, INFINITY}, hi[2] = {-INFINITY, -INFINITY};
+    const uint64_t *bits = reinterpret_cast<const uint64_t *>(xy);
+    for (int64_t i = 0; i < npts; ++i)
+        for (int k = 0; k < 2; ++k) {
+            const uint64_t b = bits[2 * i + k];
+            if (!finite_bits(b)) return 1;
+            const double v = __builtin_bit_cast(double, b);
+            lo[k] = std::fmin(lo[k], v);
+            hi[k] = std::fmax(hi[k], v);
+        }
+    fp32_origin_of_box(lo, hi, ox, oy, cmax);
     return 0;
 }
 

@@ -30,7 +30,9 @@ def fp32_origin(xy: np.ndarray):
     xy = np.asarray(xy, np.float64).reshape(-1, 2)
     if xy.size == 0:
         return (0.0, 0.0), 0.0
-    lo, hi = xy.min(axis=0), xy.max(axis=0)
+    # per column (numpy's axis-0 reduction of an [N, 2] array is ~5x slower)
+    lo = np.array([xy[:, 0].min(), xy[:, 1].min()])
+    hi = np.array([xy[:, 0].max(), xy[:, 1].max()])
     o = [0.0, 0.0]
     cmax = 0.0
     for k in range(2):
