@@ -28,7 +28,11 @@
  *     LCSS also report the integer DP count.
  *   - Trajectory lengths must be >= 1.
  *   - A pair whose trajectory index lies outside [0, n) is not computed: its
- *     result is NaN (count -1); no memory outside the set is read.
+ *     result is NaN (count -1); no memory outside the set is read.  So is a
+ *     pair of the DP measures longer than the set's max_len admits (a
+ *     max_len below a trajectory's length): NaN, count -1.
+ *   - Coordinates and timestamps must be finite; the host entry points
+ *     reject others with TSM_E_ARG.
  *   - SAR uses eps as distance_eps (a sample pair matches when its distance
  *     is <= eps) and time_step / target_length / replicas; timestamps must
  *     not be NaN.

@@ -23,6 +23,14 @@ def test_dispatch_and_collate():
     assert g.keywords == {"eps": 0.2, "delta": drivers.DEFAULT_DELTA, "threads_per_traj": 64}
     assert g.func is drivers.core.stlcss
     assert drivers.is_heuristic("frechet_cpu") and not drivers.is_heuristic("TrajCL")
+    # the default keywords are the reference's (method_helper.py:58-71 with
+    # config.py:45,52-53: eps 200, delta 60, 64 threads per trajectory)
+    for name in ("edr", "lcss", "cdds"):
+        assert drivers.heuristic_fn_wrapper(name).keywords == {"eps": 200.0, "threads_per_traj": 64}
+    for name in ("stedr", "stlcss"):
+        assert drivers.heuristic_fn_wrapper(name).keywords == {"eps": 200.0, "delta": 60.0, "threads_per_traj": 64}
+    for name in ("dtw", "erp", "frechet", "hausdorff"):
+        assert drivers.heuristic_fn_wrapper(name).keywords == {"threads_per_traj": 64}
     batch = [([[0.1, 0.2]], [[0.3, 0.4], [0.5, 0.6]]), ([[1 / 3, 0.0], [0.7, 0.8]], [[0.9, 1.0]])]
     t1, l1, t2, l2 = drivers.heuristic_colllate_fn(batch)
     assert t1.shape == (2, 2, 2) and t2.shape == (2, 2, 2) and l1 == [1, 2] and l2 == [2, 1]
@@ -52,17 +60,18 @@ def test_drivers_against_oracle(gpu):
     trajs = [ts.traj(i) for i in range(ts.n)]
     pairs = [(trajs[i], trajs[i + 35]) for i in range(35)]
     for m in ("dtw", "edr", "hausdorff"):
-        d, tm = drivers.pairwise(m, pairs)
-        ref, _ = O.pairs(m, ts.xy, ts.offsets, np.arange(35), np.arange(35) + 35, eps=drivers.DEFAULT_EPS)
+        eps = drivers.WORKLOAD_EPS
+        d, tm = drivers.pairwise(m, pairs, eps=eps)
+        ref, _ = O.pairs(m, ts.xy, ts.offsets, np.arange(35), np.arange(35) + 35, eps=eps)
         assert len(d) == 35 and len(tm) == 3
         np.testing.assert_allclose(d, ref, rtol=1e-5, atol=0)
-        mat, _ = drivers.query_db(m, trajs[:5], trajs[5:])
+        mat, _ = drivers.query_db(m, trajs[:5], trajs[5:], eps=eps)
         qa = np.repeat(np.arange(5), 65)
         qb = np.tile(np.arange(5, 70), 5)
-        ref, _ = O.pairs(m, ts.xy, ts.offsets, qa, qb, eps=drivers.DEFAULT_EPS)
+        ref, _ = O.pairs(m, ts.xy, ts.offsets, qa, qb, eps=eps)
         np.testing.assert_allclose(mat.reshape(-1), ref, rtol=1e-5, atol=0)
-        full, _ = drivers.all_pairs(m, trajs[:30])
+        full, _ = drivers.all_pairs(m, trajs[:30], eps=eps)
         iu = np.triu_indices(30, 1)
-        ref, _ = O.pairs(m, ts.xy, ts.offsets, iu[0], iu[1], eps=drivers.DEFAULT_EPS)
+        ref, _ = O.pairs(m, ts.xy, ts.offsets, iu[0], iu[1], eps=eps)
         np.testing.assert_allclose(full[iu], ref, rtol=1e-5, atol=0)
         assert np.array_equal(full, full.T)

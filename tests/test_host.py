@@ -122,6 +122,29 @@ def test_nonfinite_origin_rejected_without_device():
         assert rc == -1 and b"origin must be finite" in L.tsm_last_error(), (ox, oy)
 
 
+def test_packed_host_rejects_nonfinite_without_device():
+    """tsm_pairs_host refuses non-finite coordinates and timestamps before any
+    device work, as the padded entry point does: the kernels assume finite
+    inputs."""
+    L = _lib.lib()
+    prm = _lib.params()
+    offs = np.array([0, 2, 4], dtype=np.int64)
+    pa = np.zeros(1, np.int32)
+    pb = np.ones(1, np.int32)
+    out = np.zeros(1)
+    for bad in (float("nan"), float("inf"), -float("inf"), -float("nan")):
+        xy = np.zeros((4, 2))
+        xy[3, 1] = bad
+        rc = L.tsm_pairs_host(0, xy.ctypes.data, None, offs.ctypes.data, 2, 4, pa.ctypes.data, pb.ctypes.data, 1,
+                              ctypes.byref(prm), out.ctypes.data, None)
+        assert rc == -1 and b"coordinates must be finite" in L.tsm_last_error(), bad
+        t = np.arange(4.0)
+        t[1] = bad
+        rc = L.tsm_pairs_host(6, np.zeros((4, 2)).ctypes.data, t.ctypes.data, offs.ctypes.data, 2, 4,
+                              pa.ctypes.data, pb.ctypes.data, 1, ctypes.byref(prm), out.ctypes.data, None)
+        assert rc == -1 and b"timestamps must be finite" in L.tsm_last_error(), bad
+
+
 def test_sar_argument_errors_without_device():
     """SAR needs timestamps and positive time_step / target_length /
     replicas; the checks run before any device work (the pointers here are

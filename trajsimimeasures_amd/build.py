@@ -39,10 +39,28 @@ def sources():
     return sorted(os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith(".hip"))
 
 
-def _compile(src: str, build_dir: str = BUILD, defines=()) -> tuple[str, float, str]:
+def _up_to_date(src: str, obj: str) -> bool:
+    """True when obj is newer than src, this script and every header the
+    compiler listed in obj's dependency file (written by -MMD)."""
+    dep = obj[:-2] + ".d"
+    if not (os.path.exists(obj) and os.path.exists(dep)):
+        return False
+    with open(dep) as f:
+        text = f.read().replace("\\\n", " ")
+    files = text.split(":", 1)[1].split() if ":" in text else []
+    try:
+        newest = max([os.path.getmtime(src), os.path.getmtime(__file__)] + [os.path.getmtime(x) for x in files])
+    except OSError:
+        return False
+    return os.path.getmtime(obj) >= newest
+
+
+def _compile(src: str, build_dir: str = BUILD, defines=(), incremental: bool = False) -> tuple[str, float, str]:
     obj = os.path.join(build_dir, os.path.basename(src)[:-4] + ".o")
+    if incremental and _up_to_date(src, obj):
+        return obj, 0.0, "(up to date)"
     t0 = time.time()
-    cmd = [HIPCC, *CFLAGS, *[f"-D{d}" for d in defines], "-c", src, "-o", obj]
+    cmd = [HIPCC, *CFLAGS, *[f"-D{d}" for d in defines], "-MMD", "-MF", obj[:-2] + ".d", "-c", src, "-o", obj]
     p = subprocess.run(cmd, capture_output=True, text=True)
     out = "\n".join(l for l in (p.stdout + p.stderr).splitlines()
                     if "packed-fp32-ops' is not a recognized feature" not in l
@@ -52,9 +70,12 @@ def _compile(src: str, build_dir: str = BUILD, defines=()) -> tuple[str, float, 
     return obj, time.time() - t0, out
 
 
-def build(verbose: bool = True, jobs: int | None = None, out: str | None = None, defines=()) -> str:
+def build(verbose: bool = True, jobs: int | None = None, out: str | None = None, defines=(),
+          incremental: bool = False) -> str:
     """Compile every HIP source for gfx950 and link libtsm.so (or ``out``);
-    returns its path.  ``defines`` adds -D options (kernel variants)."""
+    returns its path.  ``defines`` adds -D options (kernel variants).
+    ``incremental`` reuses an object newer than its source and every header
+    (a full build is the default)."""
     lib_path = out or LIB
     build_dir = BUILD if not defines else BUILD + "_" + "_".join(d.replace("=", "") for d in defines)
     os.makedirs(build_dir, exist_ok=True)
@@ -62,7 +83,7 @@ def build(verbose: bool = True, jobs: int | None = None, out: str | None = None,
     jobs = jobs or min(len(srcs), max(1, os.cpu_count() or 1))
     t0 = time.time()
     with cf.ThreadPoolExecutor(max_workers=jobs) as ex:
-        results = list(ex.map(lambda src: _compile(src, build_dir, defines), srcs))
+        results = list(ex.map(lambda src: _compile(src, build_dir, defines, incremental), srcs))
     objs = [r[0] for r in results]
     tmp = lib_path + ".tmp"
     cmd = [HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", "-o", tmp, *objs]
@@ -82,9 +103,10 @@ if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=None)
     ap.add_argument("-D", dest="defines", action="append", default=[])
+    ap.add_argument("--incremental", action="store_true", help="reuse objects newer than their sources and headers")
     args = ap.parse_args()
     try:
-        build(out=args.out, defines=tuple(args.defines))
+        build(out=args.out, defines=tuple(args.defines), incremental=args.incremental)
     except RuntimeError as e:
         print(e, file=sys.stderr)
         sys.exit(1)

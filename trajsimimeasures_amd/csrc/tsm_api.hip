@@ -53,6 +53,10 @@ __global__ __launch_bounds__(256) void erp_gap_kernel(const double *xy, const in
 thread_local std::string g_err;
 std::mutex g_mu;
 Context g_ctx;
+// Held for the whole body of tsm_pairs_host / tsm_pairs_padded_host: their
+// staging buffers (Context::h_*) are shared, so two host calls must not
+// overlap.  Taken before g_mu (which tsm_pairs takes inside), never after.
+std::mutex g_host_mu;
 
 int fail(int code, const std::string &msg) {
     g_err = msg;
@@ -78,8 +82,8 @@ int choose_lanes(const tsm_params &prm) {
 int erp_prologue(Workspace &c, const tsm_trajset &set, double gx, double gy, hipStream_t st) {
     const int64_t npts = set.npts;
     int rc;
-    if ((rc = c.gap64.ensure(npts * 8 + 8)) || (rc = c.gap32.ensure(npts * 4 + 8)) ||
-        (rc = c.gsum64.ensure(set.n * 8 + 8)) || (rc = c.gsum32.ensure(set.n * 4 + 8)))
+    if ((rc = c.get(c.gap64, npts * 8 + 8)) || (rc = c.get(c.gap32, npts * 4 + 8)) ||
+        (rc = c.get(c.gsum64, set.n * 8 + 8)) || (rc = c.get(c.gsum32, set.n * 4 + 8)))
         return rc;
     hipLaunchKernelGGL(erp_gap_kernel, dim3(grid_for(set.n * kWave, 256)), dim3(256), 0, st, set.xy64, set.offsets,
                        set.n, gx, gy, c.gap64.as<double>(), c.gap32.as<float>(), c.gsum64.as<double>(),
@@ -87,6 +91,10 @@ int erp_prologue(Workspace &c, const tsm_trajset &set, double gx, double gy, hip
     TSM_HIP(hipGetLastError());
     return TSM_OK;
 }
+
+}  // namespace
+
+namespace tsm {
 
 // Argument checks shared by every entry point; fills prm with defaults
 // overridden by prm_in.
@@ -140,7 +148,7 @@ int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &p
         h.seg = nullptr;
         h.rseg = nullptr;
         if (!f64) {
-            if ((rc = w.seg.ensure(set->npts * sizeof(float4))) || (rc = w.rseg.ensure(set->npts * sizeof(float))))
+            if ((rc = w.get(w.seg, set->npts * sizeof(float4))) || (rc = w.get(w.rseg, set->npts * sizeof(float))))
                 return rc;
             h.seg = w.seg.as<float4>();
             h.rseg = w.rseg.as<float>();
@@ -220,6 +228,22 @@ int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &p
     }
     return fail(TSM_E_ARG, "unknown measure");
 }
+
+__global__ void clear_words_kernel(uint32_t *p, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+int clear_words(uint32_t *p, int64_t n, hipStream_t st) {
+    if (n <= 0) return TSM_OK;
+    hipLaunchKernelGGL(clear_words_kernel, dim3(grid_for(n, 256)), dim3(256), 0, st, p, n);
+    TSM_HIP(hipGetLastError());
+    return TSM_OK;
+}
+
+}  // namespace tsm
+
+namespace {
 
 int run(int measure, const tsm_trajset *set, const PairSpace &ps, int64_t npairs, const tsm_params *prm_in,
         double *out, int32_t *counts, void *stream) {
@@ -340,6 +364,16 @@ int tsm_pairs_host(int measure, const double *xy64, const double *t64, const int
     for (int64_t k = 0; k < npairs; ++k)
         if (pa[k] < 0 || pa[k] >= n || pb[k] < 0 || pb[k] >= n)
             return fail(TSM_E_ARG, "pair " + std::to_string(k) + " indexes outside the set");
+    // the FP32 copy relative to the centre of the set's bounding box; the
+    // kernels are built on the premise that every input is finite
+    double ox = 0.0, oy = 0.0, cmax = 0.0;
+    if (fp32_origin(xy64, npts, ox, oy, cmax)) return fail(TSM_E_ARG, "coordinates must be finite");
+    if (t64) {
+        const uint64_t *tb = reinterpret_cast<const uint64_t *>(t64);
+        for (int64_t i = 0; i < npts; ++i)
+            if (!finite_bits(tb[i])) return fail(TSM_E_ARG, "timestamps must be finite");
+    }
+    std::lock_guard<std::mutex> host_lk(g_host_mu);
     Context &c = g_ctx;
     {
         std::lock_guard<std::mutex> lk(g_mu);
@@ -355,12 +389,6 @@ int tsm_pairs_host(int measure, const double *xy64, const double *t64, const int
     TSM_HIP(hipMemcpy(c.h_offs.p, offsets, (n + 1) * 8, hipMemcpyHostToDevice));
     TSM_HIP(hipMemcpy(c.h_pa.p, pa, npairs * 4, hipMemcpyHostToDevice));
     TSM_HIP(hipMemcpy(c.h_pb.p, pb, npairs * 4, hipMemcpyHostToDevice));
-    // the FP32 copy relative to the centre of the set's bounding box
-    double ox = 0.0, oy = 0.0, cmax = 0.0;
-    if (fp32_origin(xy64, npts, ox, oy, cmax)) {
-        ox = oy = 0.0;
-        cmax = 0.0;  // non-finite coordinates: no bound, EDR/LCSS keep the FP64 match test
-    }
     int rc = tsm_xy_to_fp32_centred(c.h_xy64.as<double>(), c.h_xy32.as<float>(), npts, ox, oy, nullptr);
     if (rc) return rc;
     tsm_trajset set{c.h_xy32.as<float>(), c.h_xy64.as<double>(), c.h_offs.as<int64_t>(), n, npts, max_len, cmax,
@@ -465,6 +493,7 @@ int tsm_pairs_padded_host(int measure, const double *t1, const int64_t *len1, in
     const int64_t npts = offs[2 * B];
     const int64_t n1 = B * L1 * dim, n2 = B * L2 * dim;
     const int nblk = (int)std::min<int64_t>((2 * B + 3) / 4, 1024);
+    std::lock_guard<std::mutex> host_lk(g_host_mu);
     Context &c = g_ctx;
     {
         std::lock_guard<std::mutex> lk(g_mu);

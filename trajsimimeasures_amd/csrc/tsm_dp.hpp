@@ -411,9 +411,13 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
     const AT leftv = R::left0(sa);
 
     // a pair needing strips longer than the scratch column (max_len was
-    // understated by the caller) is not computed; its result is NaN
+    // understated by the caller) is not computed; its result is NaN and its
+    // count -1
     const bool fits = (ncol <= kSpan) || (nrow <= a.bnd_stride);
-    if (valid && !fits && lane0) a.out[slot] = __builtin_bit_cast(double, 0x7ff8000000000000ull);  // quiet NaN
+    if (valid && !fits && lane0) {
+        a.out[slot] = __builtin_bit_cast(double, 0x7ff8000000000000ull);  // quiet NaN
+        if ((M == kEDR || M == kLCSS || M == kSTEDR || M == kSTLCSS) && a.counts) a.counts[slot] = -1;
+    }
     const int nstrips = (valid && fits) ? (ncol + kSpan - 1) / kSpan : 0;
     const int maxrow = __builtin_amdgcn_readfirstlane(wave_max(nstrips > 0 ? nrow : 0));
     const int maxstrips = __builtin_amdgcn_readfirstlane(wave_max(nstrips));

@@ -57,17 +57,52 @@ struct DevBuf {
     }
 };
 
+// Fixed device memory handed out by a bump cursor in 256-byte aligned
+// pieces, in place of grow-only allocations (the no-allocation entry point).
+// Its contents are whatever the previous call left there.
+struct Arena {
+    char *base = nullptr;
+    size_t cap = 0, used = 0;
+};
+
 // Device scratch of one call sequence: the plan, the strip boundary columns
 // and the per-measure prologue arrays.  The context owns the shared one;
-// each captured graph owns its own, so replays never see reallocations.
+// each captured graph owns its own, so replays never see reallocations.  A
+// workspace over an arena allocates nothing: its buffers are pieces of the
+// arena, valid for one call, and must not be released.
 struct Workspace {
     DevBuf hist, cursor, slots, meta, counter, bnd;
     DevBuf gap64, gap32, gsum64, gsum32, seg, rseg;
+    Arena *arena = nullptr;
+    // b holding at least `bytes`: grown (fresh memory is zeroed), or the next
+    // piece of the arena (contents left by the previous call)
+    int get(DevBuf &b, size_t bytes) {
+        if (!arena) return b.ensure(bytes);
+        const size_t off = (arena->used + 255) & ~(size_t)255;
+        if (off > arena->cap || bytes > arena->cap - off)
+            return fail(TSM_E_ARG, "the call needs more device scratch than the " + std::to_string(arena->cap >> 20) +
+                                       " MiB of the no-allocation entry point (fewer pairs, points or shorter "
+                                       "trajectories per call)");
+        b.p = arena->base + off;
+        b.cap = bytes;
+        arena->used = off + bytes;
+        return TSM_OK;
+    }
+    // bytes of the arena still free (0 without one)
+    size_t arena_left() const {
+        if (!arena) return 0;
+        const size_t off = (arena->used + 255) & ~(size_t)255;
+        return off < arena->cap ? arena->cap - off : 0;
+    }
     void release() {
+        if (arena) return;
         for (DevBuf *b : {&hist, &cursor, &slots, &meta, &counter, &bnd, &gap64, &gap32, &gsum64, &gsum32, &seg, &rseg})
             b->release();
     }
 };
+
+// Zero n 32-bit words (a fill kernel: no memset node in a captured graph).
+int clear_words(uint32_t *p, int64_t n, hipStream_t st);
 
 struct Context {
     int device = -1;
@@ -85,7 +120,7 @@ struct Context {
     hipEvent_t ws_event = nullptr;
     hipStream_t ws_stream = nullptr;
     bool ws_used = false;
-    bool capturing = false;  // inside a graph capture: no profiling events
+    bool capturing = false;  // inside a graph capture or a no-allocation call: no profiling events
     // profiling
     bool prof = false;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
@@ -264,5 +299,15 @@ struct BucketTable;
 template <int M, bool F64>
 int run_dp(Workspace &w, int lanes_hint, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr,
            double *out, int32_t *counts, hipStream_t st, double delta = 0.0);
+
+// Argument checks shared by the entry points (tsm_api.hip); prm receives
+// the defaults overridden by prm_in.
+int validate(int measure, const tsm_trajset *set, int64_t npairs, const tsm_params *prm_in, const double *out,
+             tsm_params &prm);
+
+// The kernel launches of one call on workspace w, stream-ordered on st
+// (g_mu held).
+int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &ps, int64_t npairs,
+           const tsm_params &prm, double *out, int32_t *counts, hipStream_t st);
 
 }  // namespace tsm

@@ -67,18 +67,33 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     const BucketTable tab = bucket_table(WM, sizeof(CT) * 8, lanes_hint, wide_lanes);
     const int64_t nslots = npairs + (int64_t)NB * kMaxSeg;
     int rc;
-    if ((rc = c.hist.ensure(kMaxBins * 4)) || (rc = c.cursor.ensure(kMaxBins * 4)) ||
-        (rc = c.slots.ensure(nslots * 4)) || (rc = c.meta.ensure((1 + 2 * NB) * 8)) || (rc = c.counter.ensure(8)))
+    if ((rc = c.get(c.hist, kMaxBins * 4)) || (rc = c.get(c.cursor, kMaxBins * 4)) ||
+        (rc = c.get(c.slots, nslots * 4)) || (rc = c.get(c.meta, (1 + 2 * NB) * 8)) || (rc = c.get(c.counter, 8)))
+        return rc;
+    // the planner expects a zero histogram and done-counter (it leaves them
+    // zero behind itself; arena memory holds whatever the last call left)
+    if (c.arena && ((rc = clear_words(c.hist.as<uint32_t>(), kMaxBins, st)) ||
+                    (rc = clear_words(c.counter.as<uint32_t>(), 2, st))))
         return rc;
 
     void (*kern)(DPArgs);
     if constexpr (F64) kern = dp_kernel_wide<M, F64>;
     else kern = dp_kernel<M, F64>;
-    const int grid = persistent_grid((npairs + 3) / 4 + NB);
+    int grid = persistent_grid((npairs + 3) / 4 + NB);
     int64_t bnd_stride = 0;
     if (set.max_len > tab.ncmax) {  // some pair may need column strips
         bnd_stride = set.max_len;
-        if ((rc = c.bnd.ensure((size_t)grid * kMaxSeg * bnd_stride * sizeof(AT)))) return rc;
+        const size_t per_wave = (size_t)kMaxSeg * bnd_stride * sizeof(AT);
+        // a fixed arena holds the boundary columns of fewer resident waves
+        // (the kernel is persistent: any grid drains the queue)
+        if (c.arena && (size_t)grid * per_wave > c.arena_left()) {
+            const size_t fit = c.arena_left() / per_wave;
+            if (fit < 1)
+                return fail(TSM_E_ARG, "trajectories of " + std::to_string(set.max_len) +
+                                           " points need more boundary scratch than the no-allocation entry point holds");
+            grid = (int)fit;
+        }
+        if ((rc = c.get(c.bnd, (size_t)grid * per_wave))) return rc;
     }
 
     PlanArgs pa;

@@ -28,9 +28,13 @@ from .data import TrajSet, pack_list
 
 __all__ = ["heuristic_fn_wrapper", "heuristic_colllate_fn", "is_heuristic", "pairwise", "query_db", "all_pairs",
            "symmetric_from_triu", "fp64_drift", "clustering_kmedoids", "cluster_kmedoids", "DEFAULT_EPS",
-           "DEFAULT_DELTA", "DEFAULT_SAR", "DEFAULT_CENTROIDS"]
+           "WORKLOAD_EPS", "DEFAULT_DELTA", "DEFAULT_SAR", "DEFAULT_CENTROIDS"]
 
-DEFAULT_EPS = 0.0025  # EDR/LCSS epsilon of the 100k-pair workload
+# EDR/LCSS/CDDS/ST epsilon the reference binds (config.py:52
+# trajsimi_edr_lcss_eps = 200, metres on its Mercator datasets)
+DEFAULT_EPS = 200.0
+# epsilon of the random-walk workloads in the unit square (pass it explicitly)
+WORKLOAD_EPS = 0.0025
 DEFAULT_DELTA = 60.0  # STEDR/STLCSS time threshold, seconds (config.py:53)
 # SAR (distance_eps, time_eps, target_length, threads_per_traj): config.py:55-57
 # and core/sar.py:248 (the reference's wrapper leaves threads_per_traj at 64)

@@ -80,6 +80,35 @@ class DeviceTrajSet:
                 raise ValueError("timestamps must be finite")
             self.t64 = torch.as_tensor(ts.t, dtype=torch.float64, device=device).contiguous()
 
+    def refill(self, xy, t=None):
+        """Overwrite the coordinates (and timestamps) in place, keeping the
+        lengths, the FP32 origin and the coordinate bound: the FP32 copy is
+        regenerated relative to the same origin, and points farther than
+        ``coord_max`` from it are refused, since the EDR/LCSS guard band
+        (derived from that bound, also inside any graph captured on this
+        set) would no longer cover them.  Stream-ordered on torch's current
+        stream."""
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+        if xy.shape[0] != self.npts:
+            raise ValueError(f"refill needs {self.npts} points, got {xy.shape[0]}")
+        if not np.all(np.isfinite(xy)):
+            raise ValueError("coordinates must be finite")
+        dev = max(float(np.max(np.abs(xy[:, 0] - self.origin[0]))), float(np.max(np.abs(xy[:, 1] - self.origin[1]))))
+        if dev > self.coord_max:
+            raise ValueError(f"refilled points lie {dev:.6g} from the set's FP32 origin, beyond its bound "
+                             f"{self.coord_max:.6g}; build a new DeviceTrajSet (and recapture its graphs)")
+        if t is not None:
+            if self.t64 is None:
+                raise ValueError("this set has no timestamps")
+            t = np.ascontiguousarray(t, dtype=np.float64).reshape(-1)
+            if t.shape[0] != self.npts or not np.all(np.isfinite(t)):
+                raise ValueError(f"refill needs {self.npts} finite timestamps")
+            self.t64.copy_(torch.as_tensor(t), non_blocking=False)
+        self.xy64.copy_(torch.as_tensor(xy), non_blocking=False)
+        _lib.check(_lib.lib().tsm_xy_to_fp32_centred(self.xy64.data_ptr(), self.xy32.data_ptr(), self.npts,
+                                                     self.origin[0], self.origin[1], _stream_ptr()))
+        self.host = TrajSet(xy, self.host.offsets, t if t is not None else self.host.t)
+
     def cstruct(self) -> _lib.TrajSetC:
         return _lib.TrajSetC(self.xy32.data_ptr(), self.xy64.data_ptr(), self.offsets.data_ptr(), self.n,
                              self.npts, self.max_len, self.coord_max,
