@@ -139,6 +139,23 @@ int tsm_pairs(int measure, const tsm_trajset *set, const int32_t *pa, const int3
               int64_t npairs, const tsm_params *prm, double *out, int32_t *counts, void *stream);
 
 /*
+ * tsm_pairs for callers that cannot allocate device memory (code under
+ * stream capture, serving loops and harnesses that forbid hipMalloc).  No
+ * counterpart in the reference, which allocates per call (core/dtw.py:61-76).
+ * Same arguments, kernels and results as tsm_pairs; it calls no hipMalloc,
+ * hipFree or allocating routine.  The plan, the column-strip boundaries and
+ * the ERP / Hausdorff prologue arrays of one call live in a fixed 32 MiB
+ * device array that each device holds for the life of the process; a call
+ * needing more (roughly 8 million pairs, fewer with long trajectories or
+ * with ERP / Hausdorff over sets of more than ~1.5 million points) fails
+ * with TSM_E_ARG before launching anything.  Every call on a device shares
+ * that array: calls that overlap on one device (other streams, other
+ * threads) race and give undefined results.
+ */
+int tsm_pairs_noalloc(int measure, const tsm_trajset *set, const int32_t *pa, const int32_t *pb, int64_t npairs,
+                      const tsm_params *prm, double *out, int32_t *counts, void *stream);
+
+/*
  * Query x database grid: pair k = q * nd + d is (q0 + q, d0 + d) for
  * q < nq, d < nd, all indices into `set`.  Replaces the repeat/tile index
  * arrays and per-batch gathers of test_trajsimi_time_DQ.py:148-173.

@@ -30,7 +30,7 @@ ERRORS = {-1: "TSM_E_ARG", -2: "TSM_E_HIP", -3: "TSM_E_NOMEM", -4: "TSM_E_UNSUP"
 # Every function include/tsm.h declares; the load check verifies all exist.
 EXPORTS = ("tsm_init", "tsm_last_error", "tsm_abi_version", "tsm_params_default", "tsm_xy_to_fp32",
            "tsm_xy_to_fp32_centred",
-           "tsm_pairs", "tsm_pairs_grid", "tsm_pairs_triu", "tsm_pairs_host", "tsm_pairs_padded_host",
+           "tsm_pairs", "tsm_pairs_noalloc", "tsm_pairs_grid", "tsm_pairs_triu", "tsm_pairs_host", "tsm_pairs_padded_host",
            "tsm_tune_set_lanes",
            "tsm_tune_set_waves_per_cu", "tsm_tune_set_wide_lanes", "tsm_graph_capture_pairs", "tsm_graph_launch", "tsm_graph_destroy",
            "tsm_profile_enable", "tsm_profile_read")
@@ -79,6 +79,8 @@ def _declare(L):
     L.tsm_xy_to_fp32.argtypes = [_P, _P, _I64, _P]
     L.tsm_xy_to_fp32_centred.argtypes = [_P, _P, _I64, ctypes.c_double, ctypes.c_double, _P]
     L.tsm_pairs.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _P, _P, _I64, ctypes.POINTER(ParamsC), _P, _P, _P]
+    L.tsm_pairs_noalloc.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _P, _P, _I64, ctypes.POINTER(ParamsC), _P,
+                                    _P, _P]
     L.tsm_pairs_grid.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _I64, _I64, _I64, _I64,
                                  ctypes.POINTER(ParamsC), _P, _P, _P]
     L.tsm_pairs_triu.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _I64, ctypes.POINTER(ParamsC), _P, _P, _P]
