@@ -23,7 +23,9 @@ achieved = FLOP per launch / mean kernel time
 (HIP events on the launch stream); peak = 157.3 TFLOP/s (MI355X vector FP32).
 
 cpu_baseline: the FP64 full-matrix restatement of the DP (oracle/), one pair
-per task, OpenMP over all host cores, on the same pairs.
+per task, OpenMP over all host cores, on the same pairs: DTW (the headline
+baseline), each of the six measures on a prefix of the pairs, and the
+1,000-pair CPU configuration (DTW and discrete Frechet) in full.
 
 knn: the learned-filter leg.  1,000 queries over 1,000,000 database random
 walks (lognormal lengths, median 60, clipped to [10, 1000], seed 2024), k =
@@ -40,7 +42,18 @@ cluster: the clustering leg.  10,000 random walks (lengths 98 % uniform
 10..500, 2 % uniform 1000..2000), DTW over the upper triangle (49,995,000
 pairs), the symmetric matrix expanded on the device, FasterPAM k-medoids
 with k = 10; the oracle's FasterPAM runs on one host core beside it and must
-give the same result.
+give the same result.  The pairs touching a trajectory over 1,000 points are
+rerun in FP64 for DTW and ERP, and the largest relative FP32 drift reported.
+
+measures.<m>_fp64: DTW / ERP / Frechet with precision="fp64" (FP64
+coordinates and accumulators: the drift-check path), against the 78.6
+TFLOP/s FP64 vector peak; STEDR / STLCSS, whose arithmetic is FP64 too, are
+quoted against that peak as well.
+
+graph_replay: the headline step captured once into a hipGraph (planner +
+persistent kernel) and replayed.  graphs: eager calls against replays for
+the 512-pair batches of the reference's drivers and for the 1,000 x 100,000
+query x database grid (DTW and EDR), replays checked bit for bit.
 """
 from __future__ import annotations
 
@@ -57,7 +70,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3
+PEAK_FP64_TFLOPS = 78.6
 FLOP_PER_CELL = {"dtw": 9, "frechet": 9, "lcss": 9, "edr": 10, "erp": 11, "hausdorff": 24, "stedr": 13, "stlcss": 12}
+FP64_MEASURES = ("stedr", "stlcss")  # their DP arithmetic is FP64: quoted against the FP64 peak
+FP64_RERUN = ("dtw", "erp", "frechet")  # precision="fp64": the drift-check path
 EPS = 0.0025
 DELTA = 60.0
 SAR = dict(time_step=10, target_length=10, replicas=64)
@@ -98,6 +114,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of CPU baseline work at most")
     ap.add_argument("--no-knn", action="store_true", help="skip the learned kNN filter leg")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the graph leg (query x database grid and 512-pair batches, eager vs replayed)")
     ap.add_argument("--knn-db", type=int, default=1_000_000)
     ap.add_argument("--knn-queries", type=int, default=1000)
     ap.add_argument("--no-cluster", action="store_true", help="skip the all-pairs + k-medoids leg")
@@ -142,12 +160,12 @@ def main():
     out = torch.empty(args.pairs, dtype=torch.float64, device=dev)
     la, lb = ts.lengths[pa], ts.lengths[pb]
 
-    def step(measure):
+    def step(measure, precision="fp32"):
         if measure == "sar":
             engine.pairs(measure, dset, pa_d, pb_d, eps=EPS, validate=False, out=out, **SAR)
             return
         engine.pairs(measure, dset, pa_d, pb_d, eps=EPS, gpoint=gpoint, lanes=args.lanes, validate=False, out=out,
-                     delta=DELTA)
+                     delta=DELTA, precision=precision)
 
     def timed(measure, steps, warmup):
         for _ in range(warmup):
@@ -174,47 +192,75 @@ def main():
             "cells_per_s": cells / (kms * 1e-3),
         }
         if m in FLOP_PER_CELL:
-            report[m]["roofline_frac"] = cells * FLOP_PER_CELL[m] / (kms * 1e-3) / (PEAK_FP32_TFLOPS * 1e12)
+            peak = PEAK_FP64_TFLOPS if m in FP64_MEASURES else PEAK_FP32_TFLOPS
+            report[m]["roofline_frac"] = cells * FLOP_PER_CELL[m] / (kms * 1e-3) / (peak * 1e12)
+            report[m]["peak_tflops"] = peak
         elif m == "sar":  # linear in the points: HBM fraction of the input bytes
             report[m]["unit"] = "points/s"
             report[m]["hbm_frac"] = 24.0 * cells / (kms * 1e-3) / (PEAK_HBM_GBS * 1e9)
         else:  # CDDS: FP64, mostly a time-overlap test per segment pair; no FLOP roofline
             report[m]["unit"] = "segment pairs/s"
         print(f"[bench] {m}: {report[m]}", file=sys.stderr, flush=True)
+    # the FP64 DP (precision="fp64", the drift-check rerun), against the FP64 peak
+    for m in FP64_RERUN:
+        if m not in measures:
+            continue
+        steps = max(3, args.steps // 4)
+        for _ in range(args.warmup):
+            step(m, "fp64")
+        torch.cuda.synchronize()
+        with engine.KernelTimer() as kt:
+            t = time.perf_counter()
+            for _ in range(steps):
+                step(m, "fp64")
+            torch.cuda.synchronize()
+            el = time.perf_counter() - t
+        kms = kt.ms / max(kt.launches, 1)
+        cells = cells_of(m, la, lb)
+        report[m + "_fp64"] = {"pairs_per_s": args.pairs * steps / el, "ms_per_step": 1e3 * el / steps,
+                               "kernel_ms": kms, "cells_per_s": cells / (kms * 1e-3),
+                               "roofline_frac": cells * FLOP_PER_CELL[m] / (kms * 1e-3) / (PEAK_FP64_TFLOPS * 1e12),
+                               "peak_tflops": PEAK_FP64_TFLOPS}
+        print(f"[bench] {m} fp64: {report[m + '_fp64']}", file=sys.stderr, flush=True)
 
     el, kms, nl = timed("dtw", args.steps, args.warmup)
     cells = cells_of("dtw", la, lb)
     achieved = cells * FLOP_PER_CELL["dtw"] / (kms * 1e-3) / 1e12
 
-    cpu = None
-    if not args.no_cpu:
-        from oracle import oracle as O
-        threads = O.max_threads()
-        n = 2000
-        while True:
-            t = time.perf_counter()
-            O.pairs("dtw", ts.xy, ts.offsets, pa[:n], pb[:n], threads=threads)
-            dt = time.perf_counter() - t
-            if n >= args.pairs or dt * 2 * (1 + (n < args.pairs)) > args.cpu_budget or dt > args.cpu_budget / 4:
-                break
-            n = min(args.pairs, n * 4)
-        cpu = {"value": n / dt, "unit": "pairs/s", "cores": threads, "kind": "port",
-               "sample": f"first {n} of the {args.pairs} pairs, DTW, FP64 full-matrix restatement of the "
-                         f"traj_dist recurrence (oracle/tsm_oracle.c, gcc -O2), one pair per OpenMP task, "
-                         f"{threads} threads on {cpu_model()}; {n / dt * cells_of('dtw', la[:n], lb[:n]) / n:.3e} cells/s",
-               "seconds": dt}
-        print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
+    # the same step captured once into a hipGraph (planner + persistent
+    # kernel) and replayed: the launch model for repeated batches
+    g = engine.PairsGraph("dtw", dset, pa_d, pb_d, lanes=args.lanes)
+    for _ in range(args.warmup):
+        g.launch()
+    g.stream.synchronize()
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        g.launch()
+    g.stream.synchronize()
+    el_g = time.perf_counter() - t
+    replay_equal = bool(torch.equal(g.out, engine.pairs("dtw", dset, pa_d, pb_d, lanes=args.lanes, validate=False)))
+    g.close()
+    graph_step = {"pairs_per_s": args.pairs * args.steps / el_g, "ms_per_step": 1e3 * el_g / args.steps,
+                  "equal_to_eager": replay_equal}
+    print(f"[bench] dtw graph replay: {graph_step}", file=sys.stderr, flush=True)
 
-    traffic = None
+    cpu = None if args.no_cpu else cpu_leg(ts, pa, pb, gpoint, args.pairs, args.cpu_budget)
+
+    traffic, traffic_src = None, None
     try:  # HBM bytes per DTW launch from the committed rocprofv3 counter runs
         with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
-            traffic = json.load(f)["dp_kernel_dtw_fp32_100k_pairs"]["traffic_bytes"]
+            rec = json.load(f)["dp_kernel_dtw_fp32_100k_pairs"]
+        traffic = rec["traffic_bytes"]
+        traffic_src = (f"profiles/traffic.json: rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs of this workload, "
+                       f"scaled by a copy kernel's factors; library sha256 {rec.get('library_sha256', '?')[:16]} "
+                       f"({'this build' if rec.get('library_sha256') == library_sha256() else 'an earlier build'})")
     except (OSError, KeyError, ValueError):
         traffic = None
     if args.pairs != 100_000:
-        traffic = None
+        traffic, traffic_src = None, None
 
     host = host_leg(ts, pa, pb, dev) if "dtw" in measures and not args.no_host else None
+    graphs = None if args.no_graph else graph_leg(pa_d, pb_d, dset, args.pairs)
     knn = None if args.no_knn else knn_leg(args.knn_queries, args.knn_db)
     clu = None if args.no_cluster else cluster_leg(args.cluster_n, cpu_too=not args.no_cpu)
 
@@ -236,15 +282,158 @@ def main():
         "cells_per_s": cells / (kms * 1e-3),
         "kernel_ms": kms,
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic,
+                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": traffic, "traffic_source": traffic_src,
                      "note": "9 algorithmic FP32 FLOP per DTW cell over mean dp_kernel duration (HIP events)"},
+        "graph_replay": graph_step,
         "cpu_baseline": cpu,
         "measures": report,
         "host_buffers": host,
+        "graphs": graphs,
         "knn": knn,
         "cluster": clu,
     }
     print(json.dumps(line))
+
+
+def library_sha256() -> str:
+    import hashlib
+
+    from trajsimimeasures_amd import _lib
+    with open(_lib.LIB_PATH, "rb") as f:
+        return hashlib.sha256(f.read()).hexdigest()
+
+
+def cpu_leg(ts, pa, pb, gpoint, npairs: int, budget: float) -> dict:
+    """The FP64 oracle (the traj_dist restatement, oracle/tsm_oracle.c) on the
+    box's host cores, one pair per OpenMP task: DTW on a growing prefix of
+    the 100k pairs until about budget/4 seconds (the headline baseline); each
+    of the six measures on a prefix sized from DTW's rate to take about
+    budget/12 seconds; and the 1,000-pair CPU configuration (its own seeded
+    pairs, DTW and discrete Frechet) in full."""
+    from oracle import oracle as O
+    from trajsimimeasures_amd.data import lengths_uniform, random_walks
+
+    threads = O.max_threads()
+    la, lb = ts.lengths[pa], ts.lengths[pb]
+
+    def run(m, a, b, xy=ts.xy, offs=ts.offsets):
+        t = time.perf_counter()
+        O.pairs(m, xy, offs, a, b, eps=EPS, gpoint=gpoint, threads=threads)
+        return time.perf_counter() - t
+
+    n = 2000
+    while True:
+        dt = run("dtw", pa[:n], pb[:n])
+        if n >= npairs or dt * 2 * (1 + (n < npairs)) > budget or dt > budget / 4:
+            break
+        n = min(npairs, n * 4)
+    dtw_rate = n / dt
+    cpu = {"value": dtw_rate, "unit": "pairs/s", "cores": threads, "kind": "port",
+           "sample": f"first {n} of the {npairs} pairs, DTW, FP64 full-matrix restatement of the traj_dist "
+                     f"recurrence (oracle/tsm_oracle.c, gcc -O2), one pair per OpenMP task, {threads} threads on "
+                     f"{cpu_model()}; {dtw_rate * cells_of('dtw', la[:n], lb[:n]) / n:.3e} cells/s",
+           "seconds": dt, "measures": {}}
+    for m in ("dtw", "edr", "lcss", "erp", "frechet", "hausdorff"):
+        k = int(min(npairs, max(1000, dtw_rate * budget / 12 / (3 if m == "hausdorff" else 1))))
+        dt_m = run(m, pa[:k], pb[:k])
+        cpu["measures"][m] = {"pairs": k, "seconds": dt_m, "pairs_per_s": k / dt_m,
+                              "cells_per_s": cells_of(m, la[:k], lb[:k]) / dt_m}
+    rng = np.random.default_rng(0)  # the 1,000-pair CPU configuration, seed 0
+    c1 = random_walks(rng, lengths_uniform(rng, 2000, 20, 200))
+    a1 = np.arange(0, 2000, 2)
+    cpu["config1"] = {"workload": "1,000 random-walk pairs (seed 0), lengths uniform 20..200, FP64"}
+    for m in ("dtw", "frechet"):
+        dt_m = run(m, a1, a1 + 1, c1.xy, c1.offsets)
+        cpu["config1"][m] = {"seconds": dt_m, "pairs_per_s": 1000 / dt_m}
+    print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
+    return cpu
+
+
+def graph_leg(pa_d, pb_d, dset, npairs: int, reps: int = 3) -> dict:
+    """Eager calls against hipGraph replays of the same launches:
+    - the query x database configuration, 1,000 queries x 100,000 database
+      random walks (lognormal lengths, median 60, sigma 0.8, [10, 1000],
+      seed 0; 1e8 pairs) through grid addressing, DTW and EDR, best of
+      `reps`, replays checked bit for bit against the eager results;
+    - the reference's 512-pair batches (effi_batch_size_gpu, config.py:43)
+      over the 100k-pair DTW workload: one engine call per batch against
+      one captured 512-pair graph whose pair tensors are refilled per batch
+      on the graph's stream."""
+    import torch
+
+    from trajsimimeasures_amd import engine
+    from trajsimimeasures_amd.data import lengths_lognormal, random_walks
+
+    def log(msg):
+        print(f"[bench] graphs: {msg}", file=sys.stderr, flush=True)
+
+    res = {}
+    # 512-pair batches
+    B = 512
+    nb = npairs // B
+    out = torch.empty(nb * B, dtype=torch.float64, device=pa_d.device)
+
+    def eager():
+        for b in range(nb):
+            sl = slice(b * B, (b + 1) * B)
+            engine.pairs("dtw", dset, pa_d[sl], pb_d[sl], validate=False, out=out[sl])
+
+    g = engine.PairsGraph("dtw", dset, pa_d[:B], pb_d[:B])
+
+    def replay():
+        with torch.cuda.stream(g.stream):
+            for b in range(nb):
+                sl = slice(b * B, (b + 1) * B)
+                g.pa.copy_(pa_d[sl])
+                g.pb.copy_(pb_d[sl])
+                g.launch()
+                out[sl].copy_(g.out)
+
+    def best(fn, sync):
+        fn()
+        sync()
+        ts_ = []
+        for _ in range(reps):
+            t = time.perf_counter()
+            fn()
+            sync()
+            ts_.append(time.perf_counter() - t)
+        return min(ts_)
+
+    t_e = best(eager, torch.cuda.synchronize)
+    ref = out.clone()
+    t_r = best(replay, g.stream.synchronize)
+    same = bool(torch.equal(out, ref))
+    g.close()
+    res["batches_512"] = {"workload": f"{nb} batches of {B} pairs of the 100k-pair DTW workload",
+                          "eager_pairs_per_s": nb * B / t_e, "replay_pairs_per_s": nb * B / t_r,
+                          "equal": same}
+    log(str(res["batches_512"]))
+    # query x database, 1e8 pairs
+    rng = np.random.default_rng(0)
+    nq, nd = 1000, 100_000
+    ts = random_walks(rng, lengths_lognormal(rng, nq + nd))
+    qset = engine.DeviceTrajSet(ts)
+    lens = ts.lengths.astype(np.float64)
+    cells = float(lens[:nq].sum() * lens[nq:].sum())
+    res["query_db"] = {"workload": f"{nq} queries x {nd} database random walks (lognormal lengths, median 60, "
+                                   f"sigma 0.8, [10, 1000], seed 0): {nq * nd} pairs, {cells:.4e} cells"}
+    for m in ("dtw", "edr"):
+        holder = {}
+
+        def run_eager():
+            holder["out"] = engine.grid(m, qset, 0, nq, nq, nd, eps=EPS)
+
+        t_e = best(run_eager, torch.cuda.synchronize)
+        gg = engine.GridGraph(m, qset, 0, nq, nq, nd, eps=EPS)
+        t_r = best(gg.launch, gg.stream.synchronize)
+        same = bool(torch.equal(gg.out, holder["out"].reshape(-1)))
+        gg.close()
+        del gg, holder
+        res["query_db"][m] = {"eager_pairs_per_s": nq * nd / t_e, "replay_pairs_per_s": nq * nd / t_r,
+                              "eager_cells_per_s": cells / t_e, "replay_cells_per_s": cells / t_r, "equal": same}
+        log(f"{m}: {res['query_db'][m]}")
+    return res
 
 
 def host_leg(ts, pa, pb, dev, reps: int = 3) -> dict:
@@ -325,10 +514,26 @@ def cluster_leg(n: int, k: int = 10, cpu_too: bool = True) -> dict:
     torch.cuda.synchronize()
     t_gpu = time.perf_counter() - t
     log(f"FasterPAM k={k}: {t_gpu * 1e3:.2f} ms, {res.n_iter} passes, {res.n_swap} swaps, loss {res.loss:.6f}")
+    # FP64 rerun of the pairs touching a trajectory over 1,000 points: the
+    # FP32 drift of the cumulative-sum measures (DTW, and ERP with the set's
+    # centroid as gap point)
+    from trajsimimeasures_amd import drivers
+    iu0, iu1 = np.triu_indices(n, 1)
+    long = ts.lengths > 1000
+    sel = long[iu0] | long[iu1]
+    lpa, lpb = iu0[sel].astype(np.int32), iu1[sel].astype(np.int32)
+    del iu0, iu1, sel
+    drift = {"pairs": int(lpa.size)}
+    for m in ("dtw", "erp"):
+        t = time.perf_counter()
+        _, _, drift[m] = drivers.fp64_drift(m, dset, lpa, lpb, gpoint=ts.xy.mean(0) if m == "erp" else None)
+        log(f"FP64 rerun of {lpa.size} long-tail {m} pairs: max relative FP32 drift {drift[m]:.3e} "
+            f"({time.perf_counter() - t:.2f}s)")
     out = {"workload": f"{n} random walks (lengths 98% U(10..500), 2% U(1000..2000), seed 0), DTW upper "
                        f"triangle ({n * (n - 1) // 2} pairs), symmetric matrix on the device, FasterPAM k={k} "
                        f"from RandomState(0) initial medoids",
-           "matrix_s": t_mat, "fasterpam_s": t_gpu, "n_iter": res.n_iter, "n_swap": res.n_swap, "loss": res.loss}
+           "matrix_s": t_mat, "fasterpam_s": t_gpu, "n_iter": res.n_iter, "n_swap": res.n_swap, "loss": res.loss,
+           "fp64_drift_long_tail": drift}
     if cpu_too:
         from oracle import oracle as O
         Dh = D.cpu().numpy()

@@ -225,20 +225,38 @@ int tsm_tune_set_waves_per_cu(int waves_per_cu);
 int tsm_tune_set_wide_lanes(int lanes);
 
 /*
- * hipGraph capture of one explicit-pairs batch (ERP prologue, planner and
- * persistent DP kernel, or the Hausdorff kernels) on a non-null stream.  The
- * capture call runs the batch once eagerly (sizing the graph's private
- * workspace and producing results), then records the same launches; each
- * tsm_graph_launch replays them on the same device buffers, whose contents
- * (coordinates, pair indices) may change between launches as long as sizes
- * and lengths stay within those captured.  For the reference's repeated
+ * hipGraph capture of one batch (ERP prologue, planner and persistent DP
+ * kernel, or the Hausdorff kernels) on a non-null stream: explicit pairs
+ * (as tsm_pairs), a query x database grid (as tsm_pairs_grid: the per-batch
+ * relaunches of test_trajsimi_time_DQ.py:148-173) or an all-pairs triangle
+ * (as tsm_pairs_triu: test_clustering_time.py:219-254).  The capture call
+ * runs the batch once eagerly (sizing the graph's private workspace and
+ * producing results), then records the same launches; each tsm_graph_launch
+ * replays them on the same device buffers.  For the reference's repeated
  * fixed-size batches (effi_batch_size_gpu = 512, config.py:43) and serving
  * loops.  The graph holds no memsets.
+ *
+ * What a replay reads afresh and what the capture froze:
+ *   - read afresh: the pair index arrays, the offsets (lengths may change if
+ *     npts stays the same and no trajectory exceeds the captured max_len),
+ *     xy64, xy32 and t64;
+ *   - frozen: the pair count or grid / triangle ranges, every tsm_params
+ *     field, the set's pointers, and the EDR/LCSS match path chosen from the
+ *     set's coord_max: the FP32 test's guard band is derived from that bound.
+ * So refilled coordinates must keep xy32 = (float)(xy64 - origin) for the
+ * origin the FP32 copy was made with (tsm_xy_to_fp32_centred), with every
+ * |xy32| within the captured coord_max; beyond it EDR/LCSS counts are no
+ * longer guaranteed exact.  Refilling xy64 alone leaves the FP32 measures on
+ * the old points.
  */
 typedef struct tsm_graph tsm_graph;
 int tsm_graph_capture_pairs(int measure, const tsm_trajset *set, const int32_t *pa, const int32_t *pb,
                             int64_t npairs, const tsm_params *prm, double *out, int32_t *counts, void *stream,
                             tsm_graph **graph);
+int tsm_graph_capture_grid(int measure, const tsm_trajset *set, int64_t q0, int64_t nq, int64_t d0, int64_t nd,
+                           const tsm_params *prm, double *out, int32_t *counts, void *stream, tsm_graph **graph);
+int tsm_graph_capture_triu(int measure, const tsm_trajset *set, int64_t n, const tsm_params *prm, double *out,
+                           int32_t *counts, void *stream, tsm_graph **graph);
 int tsm_graph_launch(tsm_graph *graph, void *stream);
 int tsm_graph_destroy(tsm_graph *graph);
 

@@ -16,7 +16,9 @@ def test_graph_replays_match_eager(gpu, measure):
     print(f"[case] graph replay {measure}", flush=True)
     ts, pa, pb = uniform_pairs(4000, seed=31)
     d = engine.DeviceTrajSet(ts)
-    g = engine.PairsGraph(measure, d, pa, pb, eps=EPS, gpoint=(0.5, 0.5))
+    pa_dev = torch.as_tensor(pa, dtype=torch.int32, device="cuda")
+    pb_dev = torch.as_tensor(pb, dtype=torch.int32, device="cuda")
+    g = engine.PairsGraph(measure, d, pa_dev, pb_dev, eps=EPS, gpoint=(0.5, 0.5))
     eager = engine.pairs(measure, d, pa, pb, eps=EPS, gpoint=(0.5, 0.5)).cpu().numpy()
     for rep in range(3):
         g.out.fill_(-1.0)
@@ -34,6 +36,8 @@ def test_graph_replays_match_eager(gpu, measure):
     g.pa.copy_(torch.as_tensor(pb[perm], dtype=torch.int32))
     g.pb.copy_(torch.as_tensor(pa[perm], dtype=torch.int32))
     torch.cuda.synchronize()
+    # the graph owns its pair tensors: the caller's are untouched
+    assert np.array_equal(pa_dev.cpu().numpy(), pa) and np.array_equal(pb_dev.cpu().numpy(), pb)
     g.launch()
     g.stream.synchronize()
     ref, _ = O.pairs(measure, ts.xy, ts.offsets, pb[perm], pa[perm], eps=EPS, gpoint=(0.5, 0.5))

@@ -6,7 +6,11 @@ size-independent properties; plus the FP64 drift rerun on long-tail pairs.
     lengths lognormal (median 60, sigma 0.8) clipped to [10, 1000]; DTW, EDR
   all-pairs: upper triangle over 10,000 trajectories (49,995,000 pairs),
     lengths 98% uniform 10..500 and 2% uniform 1000..2000; DTW, Hausdorff;
-    FP64 GPU rerun of the DTW pairs that touch a trajectory over 1,000 points
+    FP64 GPU rerun of the DTW and ERP pairs that touch a trajectory over
+    1,000 points
+Both full-size workloads are also captured into a hipGraph (grid and
+triangle addressing, planner included) whose replays must equal the eager
+call bit for bit.
 """
 import time
 
@@ -27,6 +31,29 @@ def _sample_check(measure, ts, qa, qb, got, rng, n=1500, cnt=None):
     else:
         rel = np.abs(got[k] - ref) / np.maximum(ref, 1e-300)
         assert rel.max() <= 1e-5, (measure, rel.max())
+
+
+def _replays_equal_eager(g, eager, eager_counts=None, reps=2):
+    """Replays of graph g equal the eager result bit for bit, with eager
+    work between the replays; returns the best replay time in seconds."""
+    import torch
+    best = float("inf")
+    for _ in range(reps):
+        g.out.fill_(-1.0)
+        if g.counts is not None:
+            g.counts.fill_(-1)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        g.launch()
+        g.stream.synchronize()
+        best = min(best, time.perf_counter() - t)
+        assert torch.equal(g.out, eager.reshape(-1))
+        if eager_counts is not None:
+            assert torch.equal(g.counts, eager_counts.reshape(-1))
+        x = torch.randn(1 << 20, device="cuda").sum()
+        torch.cuda.synchronize()
+        del x
+    return best
 
 
 def test_query_by_database_1e8(gpu):
@@ -53,7 +80,13 @@ def test_query_by_database_1e8(gpu):
         qa = np.repeat(np.arange(nq), nd)
         qb = np.tile(np.arange(nq, nq + nd), nq)
         _sample_check(m, ts, qa, qb, v, rng)
-        del out
+        # the same grid captured into a hipGraph and replayed
+        cnt = engine.grid(m, dset, 0, nq, nq, nd, eps=EPS, return_counts=True)[1] if m == "edr" else None
+        g = engine.GridGraph(m, dset, 0, nq, nq, nd, eps=EPS)
+        dt_g = _replays_equal_eager(g, out, cnt)
+        print(f"[workload] query x db {m} graph replay: {nq * nd / dt_g:.3e} pairs/s", flush=True)
+        g.close()
+        del out, g, cnt
 
 
 def test_all_pairs_triangle_and_drift(gpu):
@@ -79,7 +112,11 @@ def test_all_pairs_triangle_and_drift(gpu):
         v = out.cpu().numpy()
         assert v.size == n * (n - 1) // 2 and np.all(np.isfinite(v)) and v.min() >= 0
         _sample_check(m, ts, iu0, iu1, v, rng)
-        del out
+        g = engine.TriuGraph(m, dset, n)
+        dt_g = _replays_equal_eager(g, out)
+        print(f"[workload] all-pairs {m} graph replay: {iu0.size / dt_g:.3e} pairs/s", flush=True)
+        g.close()
+        del out, g
     # FP64 rerun of the DTW pairs touching a trajectory longer than 1000 points
     long = ts.lengths > 1000
     sel = long[iu0] | long[iu1]
@@ -90,3 +127,11 @@ def test_all_pairs_triangle_and_drift(gpu):
     ref, _ = O.pairs("dtw", ts.xy, ts.offsets, pa[k], pb[k])
     assert np.max(np.abs(v64[k] - ref) / ref) <= 1e-12  # the FP64 path matches the oracle
     assert drift <= 1e-4
+    # the same for ERP, whose boundary is a sum of gap distances (core/erp.py:44-49)
+    g = ts.xy.mean(0)
+    e32, e64, edrift = drivers.fp64_drift("erp", dset, pa, pb, gpoint=g)
+    print(f"[workload] FP64 rerun of {pa.size} long-tail ERP pairs: max relative FP32 drift {edrift:.3e}", flush=True)
+    ref, _ = O.pairs("erp", ts.xy, ts.offsets, pa[k], pb[k], gpoint=g)
+    assert np.max(np.abs(e64[k] - ref) / ref) <= 1e-12
+    assert np.max(np.abs(e32[k] - ref) / ref) <= 1e-4
+    assert edrift <= 1e-4

@@ -32,7 +32,8 @@ EXPORTS = ("tsm_init", "tsm_last_error", "tsm_abi_version", "tsm_params_default"
            "tsm_xy_to_fp32_centred",
            "tsm_pairs", "tsm_pairs_noalloc", "tsm_pairs_grid", "tsm_pairs_triu", "tsm_pairs_host", "tsm_pairs_padded_host",
            "tsm_tune_set_lanes",
-           "tsm_tune_set_waves_per_cu", "tsm_tune_set_wide_lanes", "tsm_graph_capture_pairs", "tsm_graph_launch", "tsm_graph_destroy",
+           "tsm_tune_set_waves_per_cu", "tsm_tune_set_wide_lanes", "tsm_graph_capture_pairs", "tsm_graph_capture_grid",
+           "tsm_graph_capture_triu", "tsm_graph_launch", "tsm_graph_destroy",
            "tsm_profile_enable", "tsm_profile_read")
 # include/tsm_nn.h (argument types are declared by filter.py)
 NN_EXPORTS = ("tsm_nn_gemm_nt", "tsm_nn_cast_bf16", "tsm_nn_resample", "tsm_nn_conv_fwd", "tsm_nn_conv_bwd",
@@ -94,6 +95,10 @@ def _declare(L):
     L.tsm_tune_set_wide_lanes.argtypes = [ctypes.c_int]
     L.tsm_graph_capture_pairs.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _P, _P, _I64,
                                           ctypes.POINTER(ParamsC), _P, _P, _P, ctypes.POINTER(ctypes.c_void_p)]
+    L.tsm_graph_capture_grid.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _I64, _I64, _I64, _I64,
+                                         ctypes.POINTER(ParamsC), _P, _P, _P, ctypes.POINTER(ctypes.c_void_p)]
+    L.tsm_graph_capture_triu.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _I64, ctypes.POINTER(ParamsC), _P,
+                                         _P, _P, ctypes.POINTER(ctypes.c_void_p)]
     L.tsm_graph_launch.argtypes = [_P, _P]
     L.tsm_graph_destroy.argtypes = [_P]
     L.tsm_profile_enable.argtypes = [ctypes.c_int]

@@ -5,10 +5,18 @@ __all__ = ['pad_lists_to_array']
 
 
 def pad_lists_to_array(lst):
-    """Zero-pad a list of [n_i, d] point lists into float64 [len(lst), max n_i, d]."""
-    dim2 = len(lst[0][0])
-    dim1 = max(map(len, lst))
-    arr = np.zeros([len(lst), dim1, dim2], dtype=np.float64)
-    for i, j in enumerate(lst):
-        arr[i][0:len(j)] = j
-    return arr
+    """Zero-padded float64 [len(lst), max n_i, d] from a list of [n_i, d]
+    point lists, d being the width of the first list's first point (as the
+    reference reads it).  Built like the packed layout: the points of every
+    list one after another, then one scatter into the padded block through
+    their (list, position) indices."""
+    width = len(lst[0][0])
+    rows = [np.asarray(t, dtype=np.float64).reshape(-1, width) for t in lst]
+    lens = np.fromiter((r.shape[0] for r in rows), dtype=np.int64, count=len(rows))
+    out = np.zeros((len(rows), int(lens.max(initial=0)), width), dtype=np.float64)
+    if lens.sum():
+        which = np.repeat(np.arange(len(rows)), lens)
+        starts = np.cumsum(lens) - lens
+        pos = np.arange(int(lens.sum())) - np.repeat(starts, lens)
+        out[which, pos] = np.concatenate(rows, axis=0)
+    return out

@@ -581,29 +581,26 @@ struct tsm_graph {
     hipGraphExec_t exec = nullptr;
 };
 
-extern "C" {
+namespace {
 
-int tsm_graph_capture_pairs(int measure, const tsm_trajset *set, const int32_t *pa, const int32_t *pb,
-                            int64_t npairs, const tsm_params *prm_in, double *out, int32_t *counts, void *stream,
-                            tsm_graph **graph) {
+// Capture of one call over pair space ps: a first eager run sizes the
+// graph's private workspace (fresh buffers are zeroed there, outside the
+// capture) and produces the results once; the same launches are then
+// recorded.  The planner leaves its histogram and counters zero behind
+// itself, so replays need no clears and the graph holds no memset nodes.
+int capture(int measure, const tsm_trajset *set, const PairSpace &ps, int64_t npairs, const tsm_params *prm_in,
+            double *out, int32_t *counts, void *stream, tsm_graph **graph) {
     if (!graph) return fail(TSM_E_ARG, "null graph handle");
     *graph = nullptr;
     if (!stream) return fail(TSM_E_ARG, "graph capture needs a non-null stream");
     if (npairs < 1) return fail(TSM_E_ARG, "graph capture needs at least one pair");
-    if (!pa || !pb) return fail(TSM_E_ARG, "null pair index arrays");
     tsm_params prm;
     int rc = validate(measure, set, npairs, prm_in, out, prm);
     if (rc) return rc;
-    PairSpace ps{};
-    ps.mode = kExplicit;
-    ps.pa = pa;
-    ps.pb = pb;
     hipStream_t st = reinterpret_cast<hipStream_t>(stream);
     std::lock_guard<std::mutex> lk(g_mu);
     if ((rc = ensure_ctx())) return rc;
     tsm_graph *g = new tsm_graph();
-    // first run outside the capture: sizes the private workspace (and
-    // computes the results once)
     if ((rc = launch(g->ws, measure, set, ps, npairs, prm, out, counts, st))) {
         g->ws.release();
         delete g;
@@ -628,6 +625,45 @@ int tsm_graph_capture_pairs(int measure, const tsm_trajset *set, const int32_t *
     }
     *graph = g;
     return TSM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tsm_graph_capture_pairs(int measure, const tsm_trajset *set, const int32_t *pa, const int32_t *pb,
+                            int64_t npairs, const tsm_params *prm_in, double *out, int32_t *counts, void *stream,
+                            tsm_graph **graph) {
+    if (npairs > 0 && (!pa || !pb)) return fail(TSM_E_ARG, "null pair index arrays");
+    PairSpace ps{};
+    ps.mode = kExplicit;
+    ps.pa = pa;
+    ps.pb = pb;
+    return capture(measure, set, ps, npairs, prm_in, out, counts, stream, graph);
+}
+
+int tsm_graph_capture_grid(int measure, const tsm_trajset *set, int64_t q0, int64_t nq, int64_t d0, int64_t nd,
+                           const tsm_params *prm_in, double *out, int32_t *counts, void *stream, tsm_graph **graph) {
+    if (!set) return fail(TSM_E_ARG, "null trajectory set");
+    if (nq < 0 || nd < 0 || q0 < 0 || d0 < 0 || q0 + nq > set->n || d0 + nd > set->n)
+        return fail(TSM_E_ARG, "grid ranges outside the trajectory set");
+    PairSpace ps{};
+    ps.mode = kGrid;
+    ps.q0 = q0;
+    ps.nq = nq;
+    ps.d0 = d0;
+    ps.nd = nd;
+    return capture(measure, set, ps, nq * nd, prm_in, out, counts, stream, graph);
+}
+
+int tsm_graph_capture_triu(int measure, const tsm_trajset *set, int64_t n, const tsm_params *prm_in, double *out,
+                           int32_t *counts, void *stream, tsm_graph **graph) {
+    if (!set) return fail(TSM_E_ARG, "null trajectory set");
+    if (n < 0 || n > set->n) return fail(TSM_E_ARG, "triangle side outside the trajectory set");
+    PairSpace ps{};
+    ps.mode = kTriu;
+    ps.n = n;
+    return capture(measure, set, ps, n < 2 ? 0 : n * (n - 1) / 2, prm_in, out, counts, stream, graph);
 }
 
 int tsm_graph_launch(tsm_graph *graph, void *stream) {

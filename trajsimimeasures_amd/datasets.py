@@ -148,15 +148,17 @@ class DatasetSynthetic:
         self.merc_range = [0.0, 1.0, 0.0, 1.0]
 
     def __getitem__(self, index):
-        n1 = np.random.randint(self.traj_len_min, self.traj_len_max + 1)
-        traj = np.random.uniform(size=(n1, 2)).astype(np.float64).tolist()
-        n2 = np.random.randint(self.traj_len_min, self.traj_len_max + 1)
-        traj2 = np.random.uniform(size=(n2, 2)).astype(np.float64).tolist()
+        # the draws (and their order) define the pairs: length, points, length,
+        # points, then the two sets of time steps of 1..59 s
+        lens = []
+        pts = []
+        for _ in range(2):
+            lens.append(np.random.randint(self.traj_len_min, self.traj_len_max + 1))
+            pts.append(np.random.uniform(size=(lens[-1], 2)).astype(np.float64))
         if not self.use_temporal:
-            return (traj, traj2)
-        ts = np.cumsum(np.random.randint(1, 60, size=n1), axis=0).tolist()
-        ts2 = np.cumsum(np.random.randint(1, 60, size=n2), axis=0).tolist()
-        return ([[p[0], p[1], t] for p, t in zip(traj, ts)], [[p[0], p[1], t] for p, t in zip(traj2, ts2)])
+            return (pts[0].tolist(), pts[1].tolist())
+        stamps = [np.cumsum(np.random.randint(1, 60, size=n)) for n in lens]
+        return tuple(np.column_stack([p, t]).tolist() for p, t in zip(pts, stamps))
 
     def __len__(self):
         return self.num_trajs

@@ -18,7 +18,8 @@ import torch
 from . import _lib
 from .data import TrajSet
 
-__all__ = ["DeviceTrajSet", "pairs", "grid", "triu", "KernelTimer", "PairsGraph", "fp32_origin"]
+__all__ = ["DeviceTrajSet", "pairs", "grid", "triu", "KernelTimer", "PairsGraph", "GridGraph", "TriuGraph",
+           "fp32_origin"]
 
 
 def fp32_origin(xy: np.ndarray):
@@ -209,36 +210,31 @@ class KernelTimer:
         return False
 
 
-class PairsGraph:
-    """A batch of explicit pairs captured once into a hipGraph (planner +
-    persistent kernel launches) and replayed with ``launch()``.  The device
-    tensors ``pa``, ``pb``, ``out`` and the set's coordinates may be refilled
-    in place between launches; their sizes may not change.  Capture and
-    replay run on a dedicated stream (default priority) unless one is
-    given."""
+class _Graph:
+    """A batch captured once into a hipGraph (planner + persistent kernel
+    launches, no memsets) and replayed with ``launch()`` on its own stream
+    (default priority) unless one is given.  Results land in ``self.out``
+    (``self.counts`` for the counting measures).  Between launches the set's
+    coordinates may be refilled in place (``DeviceTrajSet.refill``), and for
+    explicit pairs the pair tensors; sizes may not change."""
 
-    def __init__(self, measure, dset: DeviceTrajSet, pa, pb, *, eps=0.0, eps_mode="sqrt", gpoint=None, lanes=0,
-                 precision="fp32", stream=None, delta=0.0, time_step=None, target_length=None, replicas=None):
+    def _setup(self, measure, dset, npairs, stream, prm):
         m = _lib.measure_id(measure)
         dev = dset.xy64.device
         self.dset = dset
-        self.pa = torch.as_tensor(pa, dtype=torch.int32, device=dev).contiguous()
-        self.pb = torch.as_tensor(pb, dtype=torch.int32, device=dev).contiguous()
-        n = self.pa.numel()
-        if self.pb.numel() != n or n < 1:
-            raise ValueError("pa and pb must be non-empty and of equal length")
-        self.out = torch.empty(n, dtype=torch.float64, device=dev)
-        self.counts = torch.empty(n, dtype=torch.int32, device=dev) if m in _lib.COUNTED else None
+        self.measure = m
+        self.out = torch.empty(npairs, dtype=torch.float64, device=dev)
+        self.counts = torch.empty(npairs, dtype=torch.int32, device=dev) if m in _lib.COUNTED else None
         self.stream = stream if stream is not None else torch.cuda.Stream(device=dev)
-        self._prm = _lib.params(eps, eps_mode, gpoint, lanes, precision, delta, time_step, target_length,
-                                replicas)
+        self._prm = prm
         self._set = dset.cstruct()
         self._h = ctypes.c_void_p()
         torch.cuda.current_stream().synchronize()
-        _lib.check(_lib.lib().tsm_graph_capture_pairs(
-            m, ctypes.byref(self._set), self.pa.data_ptr(), self.pb.data_ptr(), n, ctypes.byref(self._prm),
-            self.out.data_ptr(), self.counts.data_ptr() if self.counts is not None else None,
-            ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(self._h)))
+
+    def _args_tail(self):
+        return (ctypes.byref(self._prm), self.out.data_ptr(),
+                self.counts.data_ptr() if self.counts is not None else None,
+                ctypes.c_void_p(self.stream.cuda_stream), ctypes.byref(self._h))
 
     def launch(self):
         """Replay the batch on the graph's stream; results land in self.out."""
@@ -258,3 +254,57 @@ class PairsGraph:
             self.close()
         except Exception:
             pass
+
+
+class PairsGraph(_Graph):
+    """Explicit pairs (pa[k], pb[k]) of ``dset`` captured into a hipGraph;
+    ``pa``/``pb`` (device int32) may be refilled in place between launches."""
+
+    def __init__(self, measure, dset: DeviceTrajSet, pa, pb, *, eps=0.0, eps_mode="sqrt", gpoint=None, lanes=0,
+                 precision="fp32", stream=None, delta=0.0, time_step=None, target_length=None, replicas=None):
+        dev = dset.xy64.device
+        # the graph's own copies: refilling them must not write into the caller's tensors
+        self.pa = torch.as_tensor(pa, dtype=torch.int32, device=dev).clone(memory_format=torch.contiguous_format)
+        self.pb = torch.as_tensor(pb, dtype=torch.int32, device=dev).clone(memory_format=torch.contiguous_format)
+        n = self.pa.numel()
+        if self.pb.numel() != n or n < 1:
+            raise ValueError("pa and pb must be non-empty and of equal length")
+        self._setup(measure, dset, n, stream, _lib.params(eps, eps_mode, gpoint, lanes, precision, delta, time_step,
+                                                          target_length, replicas))
+        _lib.check(_lib.lib().tsm_graph_capture_pairs(self.measure, ctypes.byref(self._set), self.pa.data_ptr(),
+                                                      self.pb.data_ptr(), n, *self._args_tail()))
+
+
+class GridGraph(_Graph):
+    """The query x database grid (q0 + q, d0 + d) of ``dset`` captured into a
+    hipGraph; ``out`` is [nq * nd] row-major (``result()`` gives [nq, nd])."""
+
+    def __init__(self, measure, dset: DeviceTrajSet, q0: int, nq: int, d0: int, nd: int, *, eps=0.0,
+                 eps_mode="sqrt", gpoint=None, lanes=0, precision="fp32", stream=None, delta=0.0, time_step=None,
+                 target_length=None, replicas=None):
+        if nq < 1 or nd < 1:
+            raise ValueError("the grid needs at least one query and one database trajectory")
+        self.shape = (int(nq), int(nd))
+        self._setup(measure, dset, nq * nd, stream, _lib.params(eps, eps_mode, gpoint, lanes, precision, delta,
+                                                                time_step, target_length, replicas))
+        _lib.check(_lib.lib().tsm_graph_capture_grid(self.measure, ctypes.byref(self._set), q0, nq, d0, nd,
+                                                     *self._args_tail()))
+
+    def result(self):
+        return self.out.view(*self.shape)
+
+
+class TriuGraph(_Graph):
+    """The upper triangle over trajectories [0, n) of ``dset`` (pairs in
+    itertools.combinations order) captured into a hipGraph."""
+
+    def __init__(self, measure, dset: DeviceTrajSet, n: int | None = None, *, eps=0.0, eps_mode="sqrt",
+                 gpoint=None, lanes=0, precision="fp32", stream=None, delta=0.0, time_step=None, target_length=None,
+                 replicas=None):
+        n = dset.n if n is None else int(n)
+        if n < 2:
+            raise ValueError("the triangle needs at least two trajectories")
+        self.n = n
+        self._setup(measure, dset, n * (n - 1) // 2, stream, _lib.params(eps, eps_mode, gpoint, lanes, precision,
+                                                                         delta, time_step, target_length, replicas))
+        _lib.check(_lib.lib().tsm_graph_capture_triu(self.measure, ctypes.byref(self._set), n, *self._args_tail()))
