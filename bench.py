@@ -31,12 +31,14 @@ knn: the learned-filter leg.  1,000 queries over 1,000,000 database random
 walks (lognormal lengths, median 60, clipped to [10, 1000], seed 2024), k =
 10.  Exact kNN = DTW of every query against every database trajectory (the
 DP kernel over the grid) plus a per-row top-k.  The filter is trained in the
-run (1,000 random anchors x their 200 nearest by the untrained embedding =
+run (2,000 random anchors x their 100 nearest by the untrained embedding =
 200,000 DTW labels from the DP kernel, 15 epochs of Adam through the
 encoder's HIP forward/backward kernels); filtered kNN = embed the queries,
-BF16-MFMA distance GEMM against the embedded database, 200 candidates per
+BF16-MFMA distance GEMM against the embedded database, 100 candidates per
 query, exact DTW re-rank, top 10.  Recall@10 is judged against the exact
 kNN; q/s excludes the one-off database embedding (reported as index_build_s).
+Recall and q/s are also reported for 25..200 candidates, untrained, trained,
+and trained on 200,000 random-pair labels instead.
 
 cluster: the clustering leg.  10,000 random walks (lengths 98 % uniform
 10..500, 2 % uniform 1000..2000), DTW over the upper triangle (49,995,000
@@ -549,6 +551,12 @@ def cluster_leg(n: int, k: int = 10, cpu_too: bool = True) -> dict:
 
 
 def knn_leg(nq: int, nd: int) -> dict:
+    """Exact DTW kNN against the learned filter: recall@10 and q/s at
+    C in {25, 50, 100, 150, 200} candidates for the untrained encoder (the
+    resampled-L2 embedding), the trained one (the default training run: each
+    of 2,000 anchors with its 100 nearest by the untrained embedding) and one
+    trained on 200,000 random-pair DTW labels (2,000 anchors x 100 random
+    database trajectories).  The headline pair is the default C."""
     import torch
 
     from trajsimimeasures_amd import filter as F
@@ -565,10 +573,18 @@ def knn_leg(nq: int, nd: int) -> dict:
     truth, t_exact = F.exact_knn(dset, 0, nq, nq, nd, k=10)
     log(f"exact kNN {t_exact:.3f}s ({nq / t_exact:.1f} q/s)")
     db = torch.arange(nq, nq + nd, dtype=torch.int32, device="cuda")
-    base = F.FilterIndex(F.Encoder(), dset, db)
-    found0, _ = F.filtered_knn(base, 0, nq, k=10, C=200)
-    r0 = F.recall_at(found0, truth)
-    del base
+    Cs = (25, 50, 100, 150, 200)
+
+    def sweep(index):
+        row = {}
+        for C in Cs:
+            F.filtered_knn(index, 0, nq, k=10, C=C)  # warm-up
+            runs = [F.filtered_knn(index, 0, nq, k=10, C=C) for _ in range(3)]
+            row[C] = {"recall_at_10": F.recall_at(runs[-1][0], truth), "qps": nq / min(r[1] for r in runs)}
+        return row
+
+    untrained = sweep(F.FilterIndex(F.Encoder(), dset, db))
+    log(f"untrained: {untrained}")
     enc, info = F.train_filter(dset, db)
     log(f"trained: {info}")
     torch.cuda.synchronize()
@@ -576,16 +592,22 @@ def knn_leg(nq: int, nd: int) -> dict:
     index = F.FilterIndex(enc, dset, db)
     torch.cuda.synchronize()
     t_index = time.perf_counter() - t
-    F.filtered_knn(index, 0, nq, k=10, C=200)  # warm-up
-    runs = [F.filtered_knn(index, 0, nq, k=10, C=200) for _ in range(3)]
-    t_f = min(r[1] for r in runs)
-    r1 = F.recall_at(runs[-1][0], truth)
+    trained = sweep(index)
+    log(f"trained: {trained}")
+    del index
+    enc_r, info_r = F.train_filter(dset, db, sampling="random")
+    rnd = sweep(F.FilterIndex(enc_r, dset, db))
+    log(f"random-pair labels: {info_r} {rnd}")
+    C = F.DEFAULT_C
     out = {"workload": f"{nq} queries x {nd} database random walks, lognormal lengths (median 60, sigma 0.8, "
-                       f"[10, 1000]), seed 2024; k=10, 200 candidates, exact DTW re-rank",
-           "exact_qps": nq / t_exact, "exact_s": t_exact, "filtered_qps": nq / t_f, "filtered_s": t_f,
-           "speedup": t_exact / t_f, "recall_at_10": r1, "recall_at_10_untrained": r0,
-           "index_build_s": t_index, "train": info}
-    log(str(out))
+                       f"[10, 1000]), seed 2024; k=10, {C} candidates, exact DTW re-rank",
+           "candidates": C, "exact_qps": nq / t_exact, "exact_s": t_exact,
+           "filtered_qps": trained[C]["qps"], "filtered_s": nq / trained[C]["qps"],
+           "speedup": trained[C]["qps"] * t_exact / nq, "recall_at_10": trained[C]["recall_at_10"],
+           "recall_at_10_untrained": untrained[C]["recall_at_10"], "index_build_s": t_index, "train": info,
+           "by_candidates": {"untrained": untrained, "trained": trained, "trained_on_random_pairs": rnd},
+           "train_random_pairs": info_r}
+    log(str({k: v for k, v in out.items() if k != "by_candidates"}))
     return out
 
 

@@ -1,14 +1,16 @@
 """The learned kNN filter at full size: 1,000 queries over 1,000,000 database
 random walks (lognormal lengths, median 60, clipped to [10, 1000]), k = 10,
-200 candidates per query re-ranked by the exact DTW kernel.  Shows what the
+100 candidates per query re-ranked by the exact DTW kernel.  Shows what the
 trained network gains over the same filter without training (the resampled-
-L2 embedding), with the exact all-pairs DTW kNN as the reference for recall."""
+L2 embedding), with the exact all-pairs DTW kNN as the reference for recall:
+at 100 candidates the trained encoder holds recall@10 >= 0.95 and the
+untrained one does not."""
 import numpy as np
 import pytest
 
 pytestmark = pytest.mark.gpu
 
-NQ, ND, K, C = 1000, 1_000_000, 10, 200
+NQ, ND, K, C = 1000, 1_000_000, 10, 100
 
 
 @pytest.fixture(scope="module")
@@ -50,6 +52,7 @@ def test_trained_filter_gains_over_untrained(knn):
     r0 = F.recall_at(found0, truth)
     enc, info = F.train_filter(dset, db)
     assert info["labels"] == 200_000 and info["loss_last"] < info["loss_first"]
+    assert C == F.DEFAULT_C
     index = F.FilterIndex(enc, dset, db)
     F.filtered_knn(index, 0, NQ, k=K, C=C)
     found, t_f = F.filtered_knn(index, 0, NQ, k=K, C=C)
@@ -57,6 +60,7 @@ def test_trained_filter_gains_over_untrained(knn):
     torch.cuda.synchronize()
     print(f"recall@10 untrained {r0:.4f} trained {r1:.4f}; exact {NQ / t_exact:.0f} q/s, "
           f"filtered {NQ / t_f:.0f} q/s", flush=True)
-    assert r1 >= 0.95
-    assert r1 > r0
+    # the network is what lifts the filter over the bar at this candidate count
+    assert r1 >= 0.95 and r0 < 0.95
+    assert r1 - r0 >= 0.01, (r0, r1)
     assert t_exact / t_f >= 10.0

@@ -4,9 +4,9 @@
 // Host flow of one batched call (the body of the reference's per-measure
 // functions, e.g. core/dtw.py:56-77):
 //   1. validate, pick the lanes-per-pair L (the caller's hint or a model);
-//   2. ERP only: per-point gap distances and per-trajectory sums;
-//   3. plan: counting sort of pairs into tasks (tsm_plan.hpp);
-//   4. one persistent DP kernel (tsm_dp.hpp), or the Hausdorff kernel.
+//   2. plan: counting sort of pairs into tasks (tsm_plan.hpp);
+//   3. one persistent DP kernel (tsm_dp.hpp; ERP's gap distances are
+//      computed inside it), or the Hausdorff kernel.
 // Everything is stream-ordered on the caller's stream; the only host sync
 // is a workspace grow.
 #include <cmath>
@@ -23,31 +23,6 @@ namespace tsm {
 __global__ void xy_to_f32_kernel(const double *in, float *out, int64_t n, double ox, double oy) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         out[i] = (float)(in[i] - ((i & 1) ? oy : ox));
-}
-
-// ERP gap distances |p - g| per point (core/erp.py:32-40) in FP64 and FP32,
-// and their per-trajectory sums (core/erp.py:44-49): one wavefront per
-// trajectory, lanes over its points, an FP64 wave reduction for the sum.
-__global__ __launch_bounds__(256) void erp_gap_kernel(const double *xy, const int64_t *offs, int64_t n, double gx,
-                                                      double gy, double *g64, float *g32, double *s64, float *s32) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t waves = (int64_t)gridDim.x * (256 / kWave);
-    for (int64_t t = ((int64_t)blockIdx.x * 256 + threadIdx.x) / kWave; t < n; t += waves) {
-        const int64_t o0 = offs[t], o1 = offs[t + 1];
-        double acc = 0.0;
-        for (int64_t i = o0 + lane; i < o1; i += kWave) {
-            const double d = sqrt(dist2_f64(gx, gy, xy[2 * i], xy[2 * i + 1]));
-            g64[i] = d;
-            g32[i] = (float)d;
-            acc += d;
-        }
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
-        if (lane == 0) {
-            s64[t] = acc;
-            s32[t] = (float)acc;
-        }
-    }
 }
 
 thread_local std::string g_err;
@@ -77,19 +52,6 @@ int choose_lanes(const tsm_params &prm) {
     if (prm.lanes == 4 || prm.lanes == 8) return prm.lanes;
     if (prm.lanes == 16) return 8;
     return 0;
-}
-
-int erp_prologue(Workspace &c, const tsm_trajset &set, double gx, double gy, hipStream_t st) {
-    const int64_t npts = set.npts;
-    int rc;
-    if ((rc = c.get(c.gap64, npts * 8 + 8)) || (rc = c.get(c.gap32, npts * 4 + 8)) ||
-        (rc = c.get(c.gsum64, set.n * 8 + 8)) || (rc = c.get(c.gsum32, set.n * 4 + 8)))
-        return rc;
-    hipLaunchKernelGGL(erp_gap_kernel, dim3(grid_for(set.n * kWave, 256)), dim3(256), 0, st, set.xy64, set.offsets,
-                       set.n, gx, gy, c.gap64.as<double>(), c.gap32.as<float>(), c.gsum64.as<double>(),
-                       c.gsum32.as<float>());
-    TSM_HIP(hipGetLastError());
-    return TSM_OK;
 }
 
 }  // namespace
@@ -135,7 +97,6 @@ int validate(int measure, const tsm_trajset *set, int64_t npairs, const tsm_para
 int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &ps, int64_t npairs,
            const tsm_params &prm, double *out, int32_t *counts, hipStream_t st) {
     const bool f64 = prm.precision == TSM_PREC_FP64;
-    int rc;
     if (measure == TSM_HAUSDORFF) {
         HArgs h;
         h.xy32 = set->xy32;
@@ -145,16 +106,6 @@ int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &p
         h.npairs = npairs;
         h.nset = set->n;
         h.out = out;
-        h.seg = nullptr;
-        h.rseg = nullptr;
-        if (!f64) {
-            if ((rc = w.get(w.seg, set->npts * sizeof(float4))) || (rc = w.get(w.rseg, set->npts * sizeof(float))))
-                return rc;
-            h.seg = w.seg.as<float4>();
-            h.rseg = w.rseg.as<float>();
-            hipLaunchKernelGGL(seg_prep_kernel, dim3(grid_for(set->npts, 256)), dim3(256), 0, st, set->xy32, set->npts,
-                               w.seg.as<float4>(), w.rseg.as<float>());
-        }
         int grid = (int)(npairs < 1048576 ? npairs : 1048576);
         ProfScope p_(st);
         if (f64) hipLaunchKernelGGL(hausdorff_kernel<double>, dim3(grid), dim3(kWave), 0, st, h);
@@ -197,8 +148,6 @@ int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &p
         TSM_HIP(hipGetLastError());
         return TSM_OK;
     }
-    if (measure == TSM_ERP && (rc = erp_prologue(w, *set, prm.gx, prm.gy, st))) return rc;
-
     const double thr = eps_threshold(prm.eps, prm.eps_mode);
     const int L = choose_lanes(prm);
     switch (measure) {
@@ -209,8 +158,8 @@ int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &p
         return f64 ? run_dp<kFrechet, true>(w, L, *set, ps, npairs, thr, out, counts, st)
                    : run_dp<kFrechet, false>(w, L, *set, ps, npairs, thr, out, counts, st);
     case TSM_ERP:
-        return f64 ? run_dp<kERP, true>(w, L, *set, ps, npairs, thr, out, counts, st)
-                   : run_dp<kERP, false>(w, L, *set, ps, npairs, thr, out, counts, st);
+        return f64 ? run_dp<kERP, true>(w, L, *set, ps, npairs, thr, out, counts, st, 0.0, prm.gx, prm.gy)
+                   : run_dp<kERP, false>(w, L, *set, ps, npairs, thr, out, counts, st, 0.0, prm.gx, prm.gy);
     case TSM_EDR:
     case TSM_LCSS: {
         // FP32 match test with a guard band when the set's coordinate bound

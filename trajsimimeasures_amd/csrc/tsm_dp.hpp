@@ -96,10 +96,7 @@ struct DPArgs {
     const float *xy32;
     const double *xy64;
     const int64_t *offs;
-    const float *gap32;      // ERP: per-point gap distance, FP32
-    const double *gap64;     // ERP: per-point gap distance, FP64
-    const float *gsum32;     // ERP: per-trajectory sum of gap distances
-    const double *gsum64;
+    double gx, gy;           // ERP: the gap point g
     PairSpace ps;
     const uint32_t *slots;   // pair slots grouped by task, kEmptySlot = idle segment
     const int64_t *meta;     // [0] ntasks, [1 + b] task end of bucket b, [1 + nb + b] slot base of bucket b
@@ -254,12 +251,47 @@ template <> struct Pt<double> {
         y = v.y;
     }
 };
-template <typename AT> __device__ __forceinline__ AT gap_at(const DPArgs &a, int64_t i);
-template <> __device__ __forceinline__ float gap_at<float>(const DPArgs &a, int64_t i) { return a.gap32[i]; }
-template <> __device__ __forceinline__ double gap_at<double>(const DPArgs &a, int64_t i) { return a.gap64[i]; }
-template <typename AT> __device__ __forceinline__ AT gsum_at(const DPArgs &a, int64_t t);
-template <> __device__ __forceinline__ float gsum_at<float>(const DPArgs &a, int64_t t) { return a.gsum32[t]; }
-template <> __device__ __forceinline__ double gsum_at<double>(const DPArgs &a, int64_t t) { return a.gsum64[t]; }
+// ERP gap distance |p - g| (core/erp.py:32-40), computed where the point is
+// used instead of in a pass over the whole set.  FP32: p is a point of the
+// FP32 copy and g its position in that copy's frame (gap_point_f32); FP64:
+// the oracle's expression on the FP64 coordinates.
+__device__ __forceinline__ float gap_of(float x, float y, float gx, float gy) { return dist_f32(x, y, gx, gy); }
+__device__ __forceinline__ double gap_of(double x, double y, double gx, double gy) {
+    return sqrt(dist2_f64(x, y, gx, gy));
+}
+
+// The gap point in the frame of the coordinates the kernel reads.  The FP32
+// copy is xy64 minus an origin the kernel is not told, so g is placed
+// relative to a point r of the set: g32 = r32 - (float)(r64 - g), with the
+// FP64 subtraction exact to rounding; |p32 - g32| then differs from
+// |p - g| by FP32 roundings of the set's extent, as the cell distances do.
+template <typename CT> __device__ __forceinline__ void gap_point(const DPArgs &a, int64_t r, CT &gx, CT &gy);
+template <> __device__ __forceinline__ void gap_point<float>(const DPArgs &a, int64_t r, float &gx, float &gy) {
+    const float2 r32 = reinterpret_cast<const float2 *>(a.xy32)[r];
+    const double2 r64 = reinterpret_cast<const double2 *>(a.xy64)[r];
+    gx = r32.x - (float)(r64.x - a.gx);
+    gy = r32.y - (float)(r64.y - a.gy);
+}
+template <> __device__ __forceinline__ void gap_point<double>(const DPArgs &a, int64_t, double &gx, double &gy) {
+    gx = a.gx;
+    gy = a.gy;
+}
+
+// Sum of the gap distances of the n points from o (core/erp.py:44-49): the
+// L lanes of a segment take every L-th point, accumulate in FP64 and
+// reduce across the segment (its lanes are an aligned group of L).
+template <typename CT, int L>
+__device__ __forceinline__ double gap_sum(const DPArgs &a, int64_t o, int n, int j, CT gx, CT gy) {
+    double s = 0.0;
+    for (int i = j; i < n; i += L) {
+        CT x, y;
+        Pt<CT>::load(a, o + i, x, y);
+        s += (double)gap_of(x, y, gx, gy);
+    }
+#pragma unroll
+    for (int m = 1; m < L; m <<= 1) s += __shfl_xor(s, m, kWave);
+    return s;
+}
 
 // Lane shift by one: within rows of 16 lanes for segments of up to 16
 // lanes, across the wavefront for 64; sources outside read 0 (every
@@ -276,7 +308,6 @@ template <int L> __device__ __forceinline__ double shl_in(double v) {
 // Per-lane state of one strip sweep.
 template <typename CT, typename AT> struct RowPt {
     CT x, y;
-    AT g;
     CT t;  // timestamp (STEDR/STLCSS)
 };
 
@@ -296,7 +327,8 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
                                         const RowPt<typename Rec<M, F64>::CT, typename Rec<M, F64>::AT> &cur,
                                         RowPt<typename Rec<M, F64>::CT, typename Rec<M, F64>::AT> &nxt,
                                         typename Rec<M, F64>::AT &lp_in, typename Rec<M, F64>::AT &lp_out,
-                                        typename Rec<M, F64>::AT cur_bl, typename Rec<M, F64>::AT &nxt_bl) {
+                                        typename Rec<M, F64>::AT cur_bl, typename Rec<M, F64>::AT &nxt_bl,
+                                        typename Rec<M, F64>::CT gx, typename Rec<M, F64>::CT gy) {
     using R = Rec<M, F64>;
     using CT = typename R::CT;
     using AT = typename R::AT;
@@ -305,7 +337,6 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
     // row is out of range does not use the point, so any row will do
     const uint32_t rn = min((uint32_t)(s + 1 - j), (uint32_t)rlast);
     Pt<CT>::load(a, orow + rn, nxt.x, nxt.y);
-    if constexpr (R::kGap) nxt.g = gap_at<AT>(a, orow + rn);
     if constexpr (R::kTime) nxt.t = a.t64[orow + rn];
     if constexpr (kBnd) {
         if (lane0) nxt_bl = bnd[rn];
@@ -317,6 +348,8 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
         else left_in = lane0 ? left_fixed : recv;
         AT diag = lp_in;  // D[r][col0]: the corner value before the first row
         lp_out = left_in;
+        AT ga = 0;  // ERP: the row point's gap distance
+        if constexpr (R::kGap) ga = (AT)gap_of(cur.x, cur.y, gx, gy);
         AT lf = left_in;
         if constexpr (R::kBand) {
             // FP32 match test; `amb` collects the lanes that met a squared
@@ -349,7 +382,13 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
                     for (int c = 0; c < W; ++c) {
                         if (c < nvalid) {
                             const double2 q = pc[c];
-                            const AT v = R::pick(dist2_f64(pr.x, pr.y, q.x, q.y) < a.thr, dg, src[c], lx);
+                            // an opaque copy of the previous row's value:
+                            // min(diag, up) here must not be shared with the
+                            // FP32 pass above, where sharing it splits every
+                            // cell's v_min3 into two v_min (one VALU more)
+                            AT up = src[c];
+                            asm volatile("" : "+v"(up));
+                            const AT v = R::pick(dist2_f64(pr.x, pr.y, q.x, q.y) < a.thr, dg, up, lx);
                             dst[c] = v;
                             lx = v;
                         }
@@ -368,7 +407,7 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
                 if constexpr (R::kTime)
                     v = R::cell_t(cur.x, cur.y, cur.t, bx[c], by[c], bt[c], diag, src[c], lf, a.thr, a.delta);
                 else
-                    v = R::cell(cur.x, cur.y, cur.g, bx[c], by[c], gb[c], diag, src[c], lf, a.thr);
+                    v = R::cell(cur.x, cur.y, ga, bx[c], by[c], gb[c], diag, src[c], lf, a.thr);
                 diag = src[c];
                 dst[c] = v;
                 lf = v;
@@ -400,12 +439,13 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
     const int64_t orow = swp ? ob : oa, ocol = swp ? oa : ob;
     const int nrow = valid ? (int)(swp ? nb : na) : 0;
     const int ncol = valid ? (int)(swp ? na : nb) : 0;
-    const int64_t trow = swp ? ib : ia, tcol = swp ? ia : ib;
 
     AT sa = 0, sb = 0;
+    CT gx = 0, gy = 0;
     if constexpr (R::kGap) {
-        sa = gsum_at<AT>(a, trow);
-        sb = gsum_at<AT>(a, tcol);
+        gap_point<CT>(a, orow, gx, gy);
+        sa = (AT)gap_sum<CT, L>(a, orow, nrow, j, gx, gy);
+        sb = (AT)gap_sum<CT, L>(a, ocol, ncol, j, gx, gy);
     }
     const AT topv = R::top(sb);
     const AT leftv = R::left0(sa);
@@ -435,7 +475,7 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
             int col = col0 + c;
             col = col < ncol ? col : (ncol > 0 ? ncol - 1 : 0);
             Pt<CT>::load(a, ocol + col, bx[c], by[c]);
-            if constexpr (R::kGap) gb[c] = gap_at<AT>(a, ocol + col);
+            if constexpr (R::kGap) gb[c] = (AT)gap_of(bx[c], by[c], gx, gy);
             else gb[c] = 0;
             if constexpr (R::kTime) bt[c] = a.t64[ocol + col];
             else bt[c] = 0;
@@ -449,29 +489,26 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
         RowPt<CT, AT> p0, p1;  // points of the lane's rows at even / odd steps
         const uint32_t r0 = min((uint32_t)(-j), (uint32_t)rlast);
         Pt<CT>::load(a, orow + r0, p0.x, p0.y);
-        p0.g = 0;
-        p1.g = 0;
         p0.t = 0;
         p1.t = 0;
-        if constexpr (R::kGap) p0.g = gap_at<AT>(a, orow + r0);
         if constexpr (R::kTime) p0.t = a.t64[orow + r0];
         AT bl0 = 0, bl1 = 0;
         if (strip == 0) {
             for (int s = 0; s < nsteps; s += 2) {
                 dp_step<M, F64, L, W, false>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb, bt, A,
-                                          B, p0, p1, lpA, lpB, bl0, bl1);
+                                          B, p0, p1, lpA, lpB, bl0, bl1, gx, gy);
                 if (s + 1 < nsteps)
                     dp_step<M, F64, L, W, false>(a, s + 1, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by,
-                                              gb, bt, B, A, p1, p0, lpB, lpA, bl1, bl0);
+                                              gb, bt, B, A, p1, p0, lpB, lpA, bl1, bl0, gx, gy);
             }
         } else {
             if (lane0) bl0 = bnd[r0];
             for (int s = 0; s < nsteps; s += 2) {
                 dp_step<M, F64, L, W, true>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb, bt, A,
-                                         B, p0, p1, lpA, lpB, bl0, bl1);
+                                         B, p0, p1, lpA, lpB, bl0, bl1, gx, gy);
                 if (s + 1 < nsteps)
                     dp_step<M, F64, L, W, true>(a, s + 1, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by,
-                                             gb, bt, B, A, p1, p0, lpB, lpA, bl1, bl0);
+                                             gb, bt, B, A, p1, p0, lpB, lpA, bl1, bl0, gx, gy);
             }
         }
 

@@ -2,5 +2,5 @@
 #include "tsm_launch.hpp"
 
 namespace tsm {
-template int run_dp<kERP, true>(Workspace &, int, const tsm_trajset &, const PairSpace &, int64_t, double, double *, int32_t *, hipStream_t, double);
+template int run_dp<kERP, true>(Workspace &, int, const tsm_trajset &, const PairSpace &, int64_t, double, double *, int32_t *, hipStream_t, double, double, double);
 }  // namespace tsm

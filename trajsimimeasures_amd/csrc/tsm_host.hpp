@@ -72,7 +72,7 @@ struct Arena {
 // arena, valid for one call, and must not be released.
 struct Workspace {
     DevBuf hist, cursor, slots, meta, counter, bnd;
-    DevBuf gap64, gap32, gsum64, gsum32, seg, rseg;
+    DevBuf seg, rseg;
     Arena *arena = nullptr;
     // b holding at least `bytes`: grown (fresh memory is zeroed), or the next
     // piece of the arena (contents left by the previous call)
@@ -96,7 +96,7 @@ struct Workspace {
     }
     void release() {
         if (arena) return;
-        for (DevBuf *b : {&hist, &cursor, &slots, &meta, &counter, &bnd, &gap64, &gap32, &gsum64, &gsum32, &seg, &rseg})
+        for (DevBuf *b : {&hist, &cursor, &slots, &meta, &counter, &bnd, &seg, &rseg})
             b->release();
     }
 };
@@ -298,7 +298,7 @@ struct BucketTable;
 // Defined in tsm_dp_<measure>.hip.
 template <int M, bool F64>
 int run_dp(Workspace &w, int lanes_hint, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr,
-           double *out, int32_t *counts, hipStream_t st, double delta = 0.0);
+           double *out, int32_t *counts, hipStream_t st, double delta = 0.0, double gx = 0.0, double gy = 0.0);
 
 // Argument checks shared by the entry points (tsm_api.hip); prm receives
 // the defaults overridden by prm_in.

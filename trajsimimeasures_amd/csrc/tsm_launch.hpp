@@ -52,7 +52,7 @@ constexpr int64_t kSmallLaunchPairs = 1 << 20;
 
 template <int M, bool F64>
 int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr,
-           double *out, int32_t *counts, hipStream_t st, double delta) {
+           double *out, int32_t *counts, hipStream_t st, double delta, double gx, double gy) {
     using CT = typename Rec<M, F64>::CT;
     using AT = typename Rec<M, F64>::AT;
     constexpr int WM = Rec<M, F64>::kWMax;
@@ -121,10 +121,8 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     a.xy32 = set.xy32;
     a.xy64 = set.xy64;
     a.offs = set.offsets;
-    a.gap32 = c.gap32.as<float>();
-    a.gap64 = c.gap64.as<double>();
-    a.gsum32 = c.gsum32.as<float>();
-    a.gsum64 = c.gsum64.as<double>();
+    a.gx = gx;
+    a.gy = gy;
     a.ps = ps;
     a.slots = pa.slots;
     a.meta = pa.meta;

@@ -15,7 +15,7 @@ Weights are FP32 masters with BF16 copies for the GEMMs (FP32 accumulation
 on the matrix cores).  Training minimises a pairwise ranking loss on anchor
 lists: each anchor's candidates (its nearest by the current embedding) carry
 exact DTW labels from the DP kernel, and every true top-10 candidate is
-pushed closer than every candidate ranked 20th or worse.
+pushed closer than every candidate ranked 15th or worse.
 
 Search: distances |e_d|^2 - 2 e_q . e_d through the MFMA GEMM with the
 embeddings split into BF16 hi + lo parts (three products in one K = 384
@@ -40,13 +40,17 @@ from .data import TrajSet, lengths_lognormal, random_walks
 from .engine import DeviceTrajSet, pairs as dp_pairs
 
 __all__ = ["Encoder", "FilterIndex", "exact_knn", "filtered_knn", "recall_at", "train_encoder", "hard_pairs",
-           "train_filter", "knn_workload"]
+           "random_pairs", "train_filter", "knn_workload", "DEFAULT_C"]
 
 T = 16
 C_CONV = 64
 H = 128
 D = 128
 G3 = 3 * H
+# candidates per query of the filtered kNN: the smallest count at which the
+# trained encoder keeps recall@10 >= 0.95 on the 1,000 x 1,000,000 workload
+# and the untrained one does not (0.956 against 0.938)
+DEFAULT_C = 100
 
 
 def _nn():
@@ -337,7 +341,7 @@ class FilterIndex:
         _lib.check(_nn().tsm_nn_split_bf16(_p(E), E.shape[0], D, int(query_side), _p(out), _p(norm), _s()))
         return out
 
-    def search(self, q_ids, C: int = 200, *, cap: int = 4096, sample: int = 16384, ks: int | None = None,
+    def search(self, q_ids, C: int = DEFAULT_C, *, cap: int = 4096, sample: int = 16384, ks: int | None = None,
                defer_redo: bool = False):
         """Candidate database positions [nq, C] by embedding distance (the C
         smallest |e_d|^2 - 2 e_q . e_d of each query, ascending).
@@ -365,7 +369,12 @@ class FilterIndex:
             return cand
         stride = nd // sample
         if ks is None:
-            ks = int(round(4 * C * sample / nd))
+            # about 4 C rows below the threshold, and never fewer than 12
+            # sample rows under it: the count below the ks-th sample order
+            # statistic spreads like a Gamma(ks) variable, so a small ks
+            # (ks = 3 at C = 50) leaves ~5 % of the queries short of C and
+            # sends them to the full-row redo
+            ks = max(int(round(4 * C * sample / nd)), 12)
         ks = max(1, min(ks, sample))
         Bs = self.B[::stride][:sample]
         ns = torch.empty(sample, dtype=torch.float32, device=A.device)
@@ -436,7 +445,7 @@ def _rerank(index: FilterIndex, qids: torch.Tensor, cand: torch.Tensor, k: int) 
     return torch.gather(cand, 1, idx.long())
 
 
-def filtered_knn(index: FilterIndex, q0: int, nq: int, k: int = 10, C: int = 200):
+def filtered_knn(index: FilterIndex, q0: int, nq: int, k: int = 10, C: int = DEFAULT_C):
     """Embedding shortlist of C, exact DTW re-rank, k smallest: database
     positions [nq, k] and the elapsed seconds (index build excluded).  The
     whole path is queued without a host synchronisation; the rare queries
@@ -486,7 +495,8 @@ def hard_pairs(enc: Encoder, dset: DeviceTrajSet, anchors, pool_ids, K: int = 20
 
 
 def train_encoder(enc: Encoder, dset: DeviceTrajSet, anchors, cands, labels, epochs: int = 20, batch: int = 32,
-                  lr: float = 1e-3, tau: float = 0.05, seed: int = 0, log=None, callback=None):
+                  lr: float = 1e-3, tau: float = 0.05, seed: int = 0, log=None, callback=None, P: int = 10,
+                  Q0: int = 20):
     """Adam on the ranking loss over anchor lists (cands sorted by label).
     ``callback(epoch)`` runs after every epoch; returns the mean loss per epoch."""
     NA, K = cands.shape
@@ -502,7 +512,7 @@ def train_encoder(enc: Encoder, dset: DeviceTrajSet, anchors, cands, labels, epo
             ids = torch.cat([anchors[sel][:, None], cands[sel]], dim=1).reshape(-1).contiguous()
             X, lenf = enc.resample(dset, ids, ids.numel())
             E = enc.forward(X, lenf, keep=True)
-            loss, dE = rank_loss(E, batch, K, tau=tau)
+            loss, dE = rank_loss(E, batch, K, P=P, Q0=Q0, tau=tau)
             grad = enc.backward(dE)
             opt.update(enc.flat, grad)
             enc.refresh()
@@ -524,27 +534,62 @@ def knn_workload(seed: int = 2024, nq: int = 1000, nd: int = 1_000_000) -> TrajS
     return random_walks(rng, lengths_lognormal(rng, nq + nd))
 
 
-def train_filter(dset: DeviceTrajSet, db_ids, n_anchors: int = 1000, K: int = 200, epochs: int = 15,
-                 lr: float = 1e-3, tau: float = 0.05, batch: int = 32, seed: int = 0, log=None):
+def random_pairs(dset: DeviceTrajSet, anchors, pool_ids, K: int, seed: int = 0):
+    """For each anchor, K pool trajectories drawn uniformly at random (with
+    replacement), labelled with exact DTW by the DP kernel and sorted by
+    label: the random-pair labels of the reference's training data
+    (preprocessing/trajsimi.py:123-149 samples its trajectories at random)."""
+    anchors = torch.as_tensor(anchors, dtype=torch.int32, device="cuda")
+    pool = torch.as_tensor(pool_ids, dtype=torch.int32, device="cuda")
+    rng = np.random.default_rng(seed + 1)
+    pick = torch.as_tensor(rng.integers(0, pool.numel(), (anchors.numel(), K)), device="cuda")
+    cids = pool[pick].contiguous()
+    y = dp_pairs("dtw", dset, anchors.repeat_interleave(K), cids.reshape(-1), validate=False).view(anchors.numel(), K)
+    order = torch.argsort(y, dim=1)
+    return anchors, torch.gather(cids, 1, order), torch.gather(y, 1, order)
+
+
+# Training defaults, chosen by recall@10 over 1,000 held-out queries of the
+# 1,000 x 1,000,000 kNN workload (200,000 labels in every configuration).
+# 2,000 anchors x 100 neighbours, lr 3e-4, tau 0.1, negatives from rank 15:
+# recall@10 at C = 100 candidates 0.956 against 0.938 untrained (C = 50:
+# 0.907 / 0.883).  1,000 x 200 at lr 1e-3, tau 0.05, negatives from rank 20
+# gave 0.944; lr 3e-3 or 8,000 x 25 anchors lost recall; random pairs give
+# lists whose top 10 are not near neighbours at all, and training on them
+# drops recall@10 to ~0.2.
+TRAIN_ANCHORS, TRAIN_K, TRAIN_LR, TRAIN_TAU, TRAIN_P, TRAIN_Q0, TRAIN_EPOCHS = 2000, 100, 3e-4, 0.1, 10, 15, 15
+
+
+def train_filter(dset: DeviceTrajSet, db_ids, n_anchors: int = TRAIN_ANCHORS, K: int = TRAIN_K,
+                 epochs: int = TRAIN_EPOCHS, lr: float = TRAIN_LR, tau: float = TRAIN_TAU, batch: int = 32,
+                 seed: int = 0, log=None, P: int = TRAIN_P, Q0: int = TRAIN_Q0, sampling: str = "nearest"):
     """The filter's training run, all on the device: n_anchors database
-    trajectories drawn at random, each with its K nearest database
-    trajectories by the untrained (resampled-L2) embedding, labelled with
-    exact DTW by the DP kernel (n_anchors * K distances), then ``epochs`` of
-    Adam on the ranking loss.  Returns (encoder, info dict)."""
+    trajectories drawn at random, each with K database trajectories --
+    its K nearest by the untrained (resampled-L2) embedding
+    (``sampling="nearest"``) or K drawn at random (``"random"``) -- labelled
+    with exact DTW by the DP kernel (n_anchors * K distances), then
+    ``epochs`` of Adam on the ranking loss (the P best of each list pushed
+    ahead of every candidate ranked Q0 or worse).  Returns (encoder, info)."""
     db_ids = torch.as_tensor(db_ids, dtype=torch.int32, device="cuda")
     if db_ids.numel() <= K:
         raise ValueError("the database must hold more than K trajectories")
+    if sampling not in ("nearest", "random"):
+        raise ValueError("sampling must be 'nearest' or 'random'")
     rng = np.random.default_rng(seed)
     pick = torch.as_tensor(rng.choice(db_ids.numel(), n_anchors, replace=False), device="cuda")
     anchors = db_ids[pick]
     enc = Encoder()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    A, Cn, Y = hard_pairs(enc, dset, anchors, db_ids, K=K)
+    if sampling == "nearest":
+        A, Cn, Y = hard_pairs(enc, dset, anchors, db_ids, K=K)
+    else:
+        A, Cn, Y = random_pairs(dset, anchors, db_ids, K=K, seed=seed)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    hist = train_encoder(enc, dset, A, Cn, Y, epochs=epochs, batch=batch, lr=lr, tau=tau, seed=seed, log=log)
+    hist = train_encoder(enc, dset, A, Cn, Y, epochs=epochs, batch=batch, lr=lr, tau=tau, seed=seed, log=log, P=P,
+                         Q0=Q0)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    return enc, dict(labels=int(Cn.numel()), anchors=n_anchors, K=K, epochs=epochs, label_s=t1 - t0,
-                     train_s=t2 - t1, loss_first=hist[0], loss_last=hist[-1])
+    return enc, dict(labels=int(Cn.numel()), anchors=n_anchors, K=K, epochs=epochs, sampling=sampling,
+                     label_s=t1 - t0, train_s=t2 - t1, loss_first=hist[0], loss_last=hist[-1])
