@@ -110,6 +110,8 @@ struct DPArgs {
     double delta;            // STEDR/STLCSS: time threshold
     double *out;
     int32_t *counts;
+    uint32_t *plan_zero;     // the planner's histogram and fill counters, zeroed here for the next call
+    int plan_zero_n;         // their words
 };
 
 // Per-measure policy: coordinate type CT, accumulator type AT, boundary
@@ -543,12 +545,20 @@ __device__ __forceinline__ void dispatch_bucket(const DPArgs &a, int bk, int64_t
     }
 }
 
+// Waves of the DP kernel that clear the planner's counters first.
+constexpr int kPlanZeroWaves = 16;
+
 // Persistent kernel: each wavefront pulls tasks until the queue is drained.
 template <int M, bool F64> __device__ __forceinline__ void dp_body(const DPArgs &a) {
     constexpr int WM = Rec<M, F64>::kWMax;
     const int64_t ntasks = a.meta[0];
     const int64_t wave_id = blockIdx.x;
     const int lane = threadIdx.x;
+    // the planner that ran before this kernel left its histogram and fill
+    // counters set; the next call's planner expects them zero
+    const int nz = gridDim.x < kPlanZeroWaves ? (int)gridDim.x : kPlanZeroWaves;
+    if ((int)blockIdx.x < nz)
+        for (int i = blockIdx.x * kWave + lane; i < a.plan_zero_n; i += nz * kWave) a.plan_zero[i] = 0u;
     while (true) {
         unsigned int t = 0;
         if (lane == 0) t = atomicAdd(a.counter, 1u);

@@ -65,14 +65,13 @@ struct Arena {
     size_t cap = 0, used = 0;
 };
 
-// Device scratch of one call sequence: the plan, the strip boundary columns
-// and the per-measure prologue arrays.  The context owns the shared one;
+// Device scratch of one call sequence: the plan and the strip boundary
+// columns.  The context owns the shared one;
 // each captured graph owns its own, so replays never see reallocations.  A
 // workspace over an arena allocates nothing: its buffers are pieces of the
 // arena, valid for one call, and must not be released.
 struct Workspace {
-    DevBuf hist, cursor, slots, meta, counter, bnd;
-    DevBuf seg, rseg;
+    DevBuf hist, slots, meta, counter, bnd;  // hist: the planner's histogram, then its fill counters
     Arena *arena = nullptr;
     // b holding at least `bytes`: grown (fresh memory is zeroed), or the next
     // piece of the arena (contents left by the previous call)
@@ -96,7 +95,7 @@ struct Workspace {
     }
     void release() {
         if (arena) return;
-        for (DevBuf *b : {&hist, &cursor, &slots, &meta, &counter, &bnd, &seg, &rseg})
+        for (DevBuf *b : {&hist, &slots, &meta, &counter, &bnd})
             b->release();
     }
 };

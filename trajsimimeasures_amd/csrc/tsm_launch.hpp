@@ -67,13 +67,9 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     const BucketTable tab = bucket_table(WM, sizeof(CT) * 8, lanes_hint, wide_lanes);
     const int64_t nslots = npairs + (int64_t)NB * kMaxSeg;
     int rc;
-    if ((rc = c.get(c.hist, kMaxBins * 4)) || (rc = c.get(c.cursor, kMaxBins * 4)) ||
-        (rc = c.get(c.slots, nslots * 4)) || (rc = c.get(c.meta, (1 + 2 * NB) * 8)) || (rc = c.get(c.counter, 8)))
-        return rc;
-    // the planner expects a zero histogram and done-counter (it leaves them
-    // zero behind itself; arena memory holds whatever the last call left)
-    if (c.arena && ((rc = clear_words(c.hist.as<uint32_t>(), kMaxBins, st)) ||
-                    (rc = clear_words(c.counter.as<uint32_t>(), 2, st))))
+    // hist holds the planner's histogram and then its fill counters
+    if ((rc = c.get(c.hist, 2 * kMaxBins * 4)) || (rc = c.get(c.slots, nslots * 4)) ||
+        (rc = c.get(c.meta, (1 + 2 * NB) * 8)) || (rc = c.get(c.counter, 8)))
         return rc;
 
     void (*kern)(DPArgs);
@@ -101,21 +97,15 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     pa.offs = set.offsets;
     pa.npairs = npairs;
     pa.hist = c.hist.as<uint32_t>();
-    pa.cursor = c.cursor.as<uint32_t>();
+    pa.fill = c.hist.as<uint32_t>() + kMaxBins;
     pa.slots = c.slots.as<uint32_t>();
     pa.meta = c.meta.as<int64_t>();
     pa.counter = c.counter.as<unsigned int>();
-    pa.done = c.counter.as<unsigned int>() + 1;
     pa.nset = set.n;
     pa.out = out;
     pa.counts = counts;
     pa.nb = NB;
     pa.tab = tab;
-
-    const int pblocks = (int)((npairs + kPairsPerBlock - 1) / kPairsPerBlock);
-    hipLaunchKernelGGL(plan_hist_kernel, dim3(pblocks), dim3(kPlanBlock), 0, st, pa);
-    hipLaunchKernelGGL(plan_scatter_kernel, dim3(pblocks), dim3(kPlanBlock), 0, st, pa);
-    TSM_HIP(hipGetLastError());
 
     DPArgs a;
     a.xy32 = set.xy32;
@@ -141,6 +131,17 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     }
     a.out = out;
     a.counts = counts;
+    a.plan_zero = pa.hist;
+    a.plan_zero_n = 2 * kMaxBins;
+
+    // The planner expects zero counters: fresh buffers start zeroed and every
+    // DP launch re-zeroes them behind its planner; arena memory holds
+    // whatever the last call left, so it is cleared first.  Every check that
+    // can fail ran above: once the planner is queued its DP kernel follows.
+    if (c.arena && (rc = clear_words(pa.hist, 2 * kMaxBins, st))) return rc;
+    const int pblocks = (int)((npairs + kPairsPerBlock - 1) / kPairsPerBlock);
+    hipLaunchKernelGGL(plan_hist_kernel, dim3(pblocks), dim3(kPlanBlock), 0, st, pa);
+    hipLaunchKernelGGL(plan_scatter_kernel, dim3(pblocks), dim3(kPlanBlock), 0, st, pa);
     {
         ProfScope ps_(st);
         hipLaunchKernelGGL(kern, dim3(grid), dim3(kWave), 0, st, a);

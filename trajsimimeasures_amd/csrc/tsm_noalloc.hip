@@ -2,7 +2,7 @@
 // allocation, for callers that cannot allocate (code under stream capture,
 // serving loops that forbid hipMalloc).
 //
-// The planner's histogram, cursors, task slots and queue counters, the
+// The planner's histogram, fill counters, task slots and queue counters, the
 // column-strip boundary vectors and the ERP / Hausdorff prologue arrays of
 // one call are carved from `scratch`, a fixed device array of kNoAllocBytes.
 // Each device holds its own copy of the array for the life of the process,

@@ -8,10 +8,17 @@
 // Each bucket is padded to whole tasks of 64/L slots so every task has one
 // (L, W); the persistent kernel takes tasks in bin order, largest work first.
 //
-// Two kernels: a histogram (LDS-aggregated per block) whose last block to
-// finish lays the buckets out (scan, task counts, cursors) and re-zeroes the
-// histogram for the next call, and a scatter that reserves each block's range
-// per bin with one global atomic and ranks pairs inside it through LDS.
+// Two kernels: a histogram (LDS-aggregated per block, one global atomic per
+// bin a block touches), and a scatter in which every block lays the buckets
+// out itself from that histogram (a block scan in LDS: no block waits for
+// another and no serial layout step sits between the kernels), reserves its
+// range in each bin with one global atomic on the bin's fill counter and
+// ranks its pairs inside it through LDS.  Block 0 also writes the task
+// table and the padding slots.  Nothing here waits on another block: the
+// histogram and the fill counters are re-zeroed for the next call by the
+// first waves of the DP kernel that follows (a kernel boundary orders it
+// after every read here), since a device-scope fence per block to find the
+// last one costs more than the whole scatter on gfx950.
 #pragma once
 
 #include "tsm_dp.hpp"
@@ -33,12 +40,11 @@ struct PlanArgs {
     PairSpace ps;
     const int64_t *offs;
     int64_t npairs;
-    uint32_t *hist;     // [kMaxBins] counts; zero between calls
-    uint32_t *cursor;   // [kMaxBins] first slot of each bin, advanced by the scatter
+    uint32_t *hist;     // [kMaxBins] pairs per bin; zero between calls
+    uint32_t *fill;     // [kMaxBins] pairs of each bin placed so far; zero between calls
     uint32_t *slots;
     int64_t *meta;      // see DPArgs::meta
     unsigned int *counter;
-    unsigned int *done;  // histogram blocks finished; zero between calls
     int64_t nset;        // trajectories in the set: explicit pair indices must lie in [0, nset)
     double *out;         // results: NaN for a pair with an index outside the set
     int32_t *counts;     // EDR/LCSS counts (or null): -1 for such a pair
@@ -107,18 +113,29 @@ __device__ __forceinline__ void pair_bins(const PlanArgs &a, int64_t p0, int64_t
     }
 }
 
-// The bucket layout, run by the last histogram block once every block's
-// counts are in (a done-counter; no block waits for another).  Thread t owns
-// the kScanBins bins [kScanBins * (t % 4), + kScanBins) of bucket t / 4.  A
+// The bucket layout, computed by every scatter block from the global
+// histogram: base[bin] = the first slot of the bin.  Thread t owns the
+// kScanBins bins [kScanBins * (t % 4), + kScanBins) of bucket t / 4.  A
 // block scan of the per-thread sums gives each bin's position within its
 // bucket; buckets are laid out one after another, each padded to whole tasks
-// of 64/L slots whose unused slots are marked empty.  The histogram is
-// re-zeroed and the counters reset for the next call.
+// of 64/L slots whose unused slots are marked empty.  The block passed
+// `writer` also writes the task table (meta), the padding slots and resets
+// the task counter.
 constexpr int kScanTPB = kPlanBlock / kMaxBuckets;  // threads per bucket
 constexpr int kScanBins = kRowKeys / kScanTPB;      // bins per thread
 static_assert(kScanTPB * kMaxBuckets == kPlanBlock && kScanBins * kScanTPB == kRowKeys, "scan layout");
 
-__device__ __forceinline__ void plan_layout_block(const PlanArgs &a) {
+// The histogram words thread threadIdx.x owns in the layout, loaded ahead of
+// the layout so the loads overlap the pair lookups.
+__device__ __forceinline__ void layout_load(const PlanArgs &a, uint32_t (&v)[kScanBins]) {
+    const int b = threadIdx.x / kScanTPB;
+    const int bin0 = b * kRowKeys + (threadIdx.x % kScanTPB) * kScanBins;
+#pragma unroll
+    for (int i = 0; i < kScanBins; ++i) v[i] = b < a.nb ? a.hist[bin0 + i] : 0u;
+}
+
+__device__ __forceinline__ void plan_layout(const PlanArgs &a, uint32_t *base, bool writer,
+                                            const uint32_t (&v)[kScanBins]) {
     __shared__ uint32_t wsum[kPlanBlock / kWave];
     __shared__ uint32_t incl[kPlanBlock];
     __shared__ int64_t slot_base[kMaxBuckets];
@@ -128,16 +145,9 @@ __device__ __forceinline__ void plan_layout_block(const PlanArgs &a) {
     const int b = t / kScanTPB;
     const int bin0 = b * kRowKeys + (t % kScanTPB) * kScanBins;
     const bool live = b < a.nb;
-    uint32_t v[kScanBins];
     uint32_t sum = 0;
 #pragma unroll
-    for (int i = 0; i < kScanBins; ++i) {
-        v[i] = live ? __hip_atomic_load(&a.hist[bin0 + i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
-        sum += v[i];
-    }
-    if (live)
-#pragma unroll
-        for (int i = 0; i < kScanBins; ++i) a.hist[bin0 + i] = 0u;  // ready for the next call
+    for (int i = 0; i < kScanBins; ++i) sum += v[i];
     // inclusive scan of `sum`: within each wavefront, then across the four
     uint32_t x = sum;
 #pragma unroll
@@ -175,33 +185,34 @@ __device__ __forceinline__ void plan_layout_block(const PlanArgs &a) {
             }
         }
         if (k < a.nb) {
-            const int64_t base = (int64_t)(si - ns);
-            slot_base[k] = base;
-            slot_fill[k] = base + cnt;
+            const int64_t sb = (int64_t)(si - ns);
+            slot_base[k] = sb;
+            slot_fill[k] = sb + cnt;
             slot_end[k] = (int64_t)si;
-            a.meta[1 + a.nb + k] = base;
-            a.meta[1 + k] = (int64_t)ti;
-            if (k == a.nb - 1) a.meta[0] = (int64_t)ti;
+            if (writer) {
+                a.meta[1 + a.nb + k] = sb;
+                a.meta[1 + k] = (int64_t)ti;
+                if (k == a.nb - 1) a.meta[0] = (int64_t)ti;
+            }
         }
-        if (lane == 0) {
-            *a.counter = 0u;
-            *a.done = 0u;
-        }
+        if (writer && lane == 0) *a.counter = 0u;
     }
     __syncthreads();
     if (live) {
         uint32_t pos = (uint32_t)slot_base[b] + (excl_t - excl_first);
 #pragma unroll
         for (int i = 0; i < kScanBins; ++i) {
-            a.cursor[bin0 + i] = pos;
+            base[bin0 + i] = pos;
             pos += v[i];
         }
     }
-    // padding slots after each bucket's pairs: at most 64/L - 1 < 16 per bucket
-    for (int q = t; q < a.nb * 16; q += kPlanBlock) {
-        const int k = q / 16;
-        const int64_t s = slot_fill[k] + q % 16;
-        if (s < slot_end[k]) a.slots[s] = kEmptySlot;
+    if (writer) {
+        // padding slots after each bucket's pairs: at most 64/L - 1 < 16 per bucket
+        for (int q = t; q < a.nb * 16; q += kPlanBlock) {
+            const int k = q / 16;
+            const int64_t s = slot_fill[k] + q % 16;
+            if (s < slot_end[k]) a.slots[s] = kEmptySlot;
+        }
     }
 }
 
@@ -220,28 +231,22 @@ static __global__ __launch_bounds__(kPlanBlock) void plan_hist_kernel(PlanArgs a
     __syncthreads();
     for (int i = threadIdx.x; i < nbins; i += kPlanBlock)
         if (h[i]) atomicAdd(&a.hist[i], h[i]);
-    // the last block to finish lays the buckets out
-    __shared__ bool last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) last = atomicAdd(a.done, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    plan_layout_block(a);
 }
 
 static __global__ __launch_bounds__(kPlanBlock) void plan_scatter_kernel(PlanArgs a) {
     __shared__ uint32_t cnt[kMaxBins];
     __shared__ uint32_t base[kMaxBins];
     const int nbins = a.nb * kRowKeys;
-    for (int i = threadIdx.x; i < nbins; i += kPlanBlock) cnt[i] = 0;
-    __syncthreads();
+    uint32_t hv[kScanBins];
+    layout_load(a, hv);  // in flight during the pair lookups
     const int64_t p0 = (int64_t)blockIdx.x * kPairsPerBlock;
     const int64_t p1 = p0 + kPairsPerBlock < a.npairs ? p0 + kPairsPerBlock : a.npairs;
     int bins[kPer];
     uint32_t rank[kPer];
     pair_bins(a, p0, p1, bins);
+    plan_layout(a, base, blockIdx.x == 0, hv);
+    for (int i = threadIdx.x; i < nbins; i += kPlanBlock) cnt[i] = 0;
+    __syncthreads();
 #pragma unroll
     for (int k = 0; k < kPer; ++k) {
         rank[k] = bins[k] >= 0 ? atomicAdd(&cnt[bins[k]], 1u) : 0u;
@@ -260,12 +265,12 @@ static __global__ __launch_bounds__(kPlanBlock) void plan_scatter_kernel(PlanArg
     for (int k = 0; k < kBinsPerThread; ++k) {
         const int i = threadIdx.x + k * kPlanBlock;
         const uint32_t c = i < nbins ? cnt[i] : 0u;
-        got[k] = c ? atomicAdd(&a.cursor[i], c) : 0u;
+        got[k] = c ? atomicAdd(&a.fill[i], c) : 0u;
     }
 #pragma unroll
     for (int k = 0; k < kBinsPerThread; ++k) {
         const int i = threadIdx.x + k * kPlanBlock;
-        if (i < nbins) base[i] = got[k];
+        if (i < nbins) base[i] += got[k];
     }
     __syncthreads();
 #pragma unroll
