@@ -79,8 +79,8 @@ int tsm_nn_adam(float *p, const float *g, float *m, float *v, int64_t n, float l
 /* out[n, Dout] = X[n, K] . W[Dout, K]^T + beta * out, FP32. */
 int tsm_nn_linear_f32(const float *X, int64_t n, int K, const float *W, int Dout, float *out, float beta,
                       void *stream);
-/* BF16 hi/lo split of FP32 rows into [hi, lo, hi] (query_side = 0) or
- * [hi, hi, lo] (query_side = 1), out [n, 3D]; norm[n] (or NULL) = |E_r|^2. */
+/* BF16 hi/lo split of FP32 rows into [hi, lo] (query_side = 0) or
+ * [hi, hi] (query_side = 1), out [n, 2D]; norm[n] (or NULL) = |E_r|^2. */
 /* The whole GRU forward over T steps (Gx: [n, T, 3H] input projections; Hs:
  * [T + 1, n, H] with Hs[0] the initial state; Hb: BF16 copy of the current h;
  * Gh: [n, 3H] scratch; r/z/nn/ghn: [T, n, H] when keep, else [1, n, H]):

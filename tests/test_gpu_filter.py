@@ -213,8 +213,11 @@ def test_adam_step_matches_formula(gpu):
 
 
 def test_split_bf16_search_distances(gpu):
-    """|e_d|^2 - 2 e_q . e_d through the hi/lo split GEMM agrees with the
-    FP64 squared distance minus |e_q|^2 to about 2^-16 of the scale."""
+    """|e_d|^2 - 2 e_q . e_d through the hi/lo split GEMM (the query rounded
+    to BF16, the database to ~16 mantissa bits as hi + lo) agrees with the
+    FP64 value of |e_d|^2 - 2 bf16(e_q) . e_d to about 2^-16 of the scale;
+    the rounding of e_q shifts every distance of a row by the same
+    -2 (e_q - bf16(e_q)) . e_d term, small between near candidates."""
     import torch
 
     from trajsimimeasures_amd import filter as F
@@ -226,26 +229,28 @@ def test_split_bf16_search_distances(gpu):
     Am = F.FilterIndex._split(torch.tensor(Eq, device="cuda"), True)
     dist = torch.empty((40, 300), dtype=torch.float32, device="cuda")
     F.gemm_nt(Am, Bm, dist, alpha=-2.0, bias=norm)
-    eq, ed = Eq.astype(np.float64), Ed.astype(np.float64)
-    want = ((eq[:, None] - ed[None]) ** 2).sum(-1) - (eq ** 2).sum(1)[:, None]
+    eq = torch.tensor(Eq).to(torch.bfloat16).to(torch.float64).numpy()
+    ed = Ed.astype(np.float64)
+    want = (ed ** 2).sum(1)[None] - 2.0 * eq @ ed.T
     scale = (eq ** 2).sum(1)[:, None] + (ed ** 2).sum(1)[None]
     assert np.all(np.abs(dist.cpu().numpy() - want) <= 5e-5 * scale)
 
 
+@pytest.mark.parametrize("K", [256, 384])
 @pytest.mark.parametrize("M,N", [(1, 1000), (100, 3000), (128, 70001), (300, 20000), (1000, 70001), (1300, 9000)])
-def test_search_filter_gemm_matches_dense(gpu, M, N):
-    """The fused filter GEMM (K = 384: the persistent search kernel) appends
-    exactly the (column, value) pairs of the dense GEMM that lie below each
-    row's threshold, with bit-identical values (same MFMA accumulation
-    order)."""
+def test_search_filter_gemm_matches_dense(gpu, M, N, K):
+    """The fused filter GEMM (K = 256 or 384: the persistent search kernel)
+    appends exactly the (column, value) pairs of the dense GEMM that lie
+    below each row's threshold, with bit-identical values (same MFMA
+    accumulation order)."""
     import torch
 
     from trajsimimeasures_amd import _lib
     from trajsimimeasures_amd import filter as F
-    print(f"[case] search filter gemm M={M} N={N}", flush=True)
+    print(f"[case] search filter gemm M={M} N={N} K={K}", flush=True)
     g = torch.Generator(device="cuda").manual_seed(M + N)
-    A = torch.randn((M, 384), device="cuda", generator=g).to(torch.bfloat16)
-    B = torch.randn((N, 384), device="cuda", generator=g).to(torch.bfloat16)
+    A = torch.randn((M, K), device="cuda", generator=g).to(torch.bfloat16)
+    B = torch.randn((N, K), device="cuda", generator=g).to(torch.bfloat16)
     bias = torch.randn(N, device="cuda", generator=g)
     dense = torch.empty((M, N), dtype=torch.float32, device="cuda")
     F.gemm_nt(A.view(torch.int16), B.view(torch.int16), dense, alpha=-2.0, bias=bias)
@@ -257,7 +262,7 @@ def test_search_filter_gemm_matches_dense(gpu, M, N):
     idx = torch.empty((M, cap), dtype=torch.int32, device="cuda")
     L = F._nn()
     P = lambda t: t.data_ptr()
-    _lib.check(L.tsm_nn_gemm_nt_filter(M, N, 384, P(A), 384, P(B), 384, -2.0, P(bias), P(tau), cap, P(cnt), P(key),
+    _lib.check(L.tsm_nn_gemm_nt_filter(M, N, K, P(A), K, P(B), K, -2.0, P(bias), P(tau), cap, P(cnt), P(key),
                                        P(idx), F._s()))
     torch.cuda.synchronize()
     d, t, c = dense.cpu().numpy(), tau.cpu().numpy(), cnt.cpu().numpy()

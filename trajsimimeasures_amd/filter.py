@@ -18,8 +18,8 @@ exact DTW labels from the DP kernel, and every true top-10 candidate is
 pushed closer than every candidate ranked 15th or worse.
 
 Search: distances |e_d|^2 - 2 e_q . e_d through the MFMA GEMM with the
-embeddings split into BF16 hi + lo parts (three products in one K = 384
-GEMM, ~16-bit mantissa), the C smallest per query by a radix select, then
+embeddings split into BF16 hi + lo parts (the query's hi part against the
+database's hi and lo parts: two products in one K = 256 GEMM), the C smallest per query by a radix select, then
 the exact DTW of those C pairs by the DP kernel and the k smallest of them.
 
 The reference's learned kNN path (test_knn_time.py:123-231) embeds with a
@@ -322,7 +322,7 @@ def kth_smallest(V: torch.Tensor, k: int) -> torch.Tensor:
 
 class FilterIndex:
     """A database embedded once (the index build), searched per query batch:
-    BF16 hi/lo split embeddings so one K = 384 MFMA GEMM gives
+    BF16 hi/lo split embeddings so one K = 256 MFMA GEMM gives
     |e_d|^2 - 2 e_q . e_d to ~16 mantissa bits."""
 
     def __init__(self, enc: Encoder, dset: DeviceTrajSet, db_ids):
@@ -335,9 +335,9 @@ class FilterIndex:
 
     @staticmethod
     def _split(E: torch.Tensor, query_side: bool, norm=None) -> torch.Tensor:
-        # database rows [hi, lo, hi], query rows [hi, hi, lo]: one K = 3D product
-        # gives hi.hi + hi.lo + lo.hi
-        out = torch.empty((E.shape[0], 3 * D), dtype=torch.int16, device=E.device)
+        # database rows [hi, lo], query rows [hi, hi]: one K = 2D product gives
+        # q_hi . (d_hi + d_lo), the database side to ~16 mantissa bits
+        out = torch.empty((E.shape[0], 2 * D), dtype=torch.int16, device=E.device)
         _lib.check(_nn().tsm_nn_split_bf16(_p(E), E.shape[0], D, int(query_side), _p(out), _p(norm), _s()))
         return out
 
@@ -387,7 +387,7 @@ class FilterIndex:
         key = torch.empty((nq, cap), dtype=torch.float32, device=A.device)
         idx = torch.empty((nq, cap), dtype=torch.int32, device=A.device)
         L = _nn()
-        _lib.check(L.tsm_nn_gemm_nt_filter(nq, nd, 3 * D, _p(A), A.stride(0), _p(self.B), self.B.stride(0), -2.0,
+        _lib.check(L.tsm_nn_gemm_nt_filter(nq, nd, 2 * D, _p(A), A.stride(0), _p(self.B), self.B.stride(0), -2.0,
                                            _p(self.norm), _p(tau), cap, _p(cnt), _p(key), _p(idx), _s()))
         cand = torch.empty((nq, C), dtype=torch.int32, device=A.device)
         flag = torch.empty(nq, dtype=torch.int32, device=A.device)
