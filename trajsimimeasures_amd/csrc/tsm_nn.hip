@@ -266,12 +266,12 @@ __global__ __launch_bounds__(2 * BM) void gemm_nt_lds_kernel(GemmArgs g, FilterA
 // K = 32 * KC, 256 or 384).  A 1024-thread block owns 128 query rows, whose whole A
 // operand it stages in LDS once, and streams 128-column database tiles
 // through double-buffered LDS in K steps of 64, tile after tile, with no
-// per-tile prologue.  The database loads run a full tile ahead: step s of
-// tile t stores the registers loaded during step s of tile t - 1 and reloads
-// them with step s of tile t + 1, so each load has the six steps of a tile to
-// arrive (a load issued one step ahead would stall every step on HBM / L2
-// latency at this occupancy).  The filter epilogue runs after each tile's
-// last step.  The grid has one block per CU-sized share: block b runs on XCD
+// per-tile prologue.  The database loads run a full tile ahead: each step
+// stores the ring slot of the step after it (loaded one tile earlier) and
+// reloads that slot one tile further on, so each load has a whole tile of
+// steps to arrive (a load issued one step ahead would stall every step on
+// HBM / L2 latency at this occupancy).  The filter epilogue runs after each
+// tile's last step and queues its hits in LDS.  The grid has one block per CU-sized share: block b runs on XCD
 // b % 8 (round-robin dispatch), and the blocks of one XCD take consecutive
 // database chunks with all query blocks of a chunk on that XCD, so a database
 // tile comes from HBM once per XCD and from its L2 for the other query
@@ -287,9 +287,14 @@ constexpr int kSearchAPad = 48, kSearchBStride = kLdsStride;
 
 // The one-tile-ahead register ring of the search kernel: six named tiles
 // picked at compile time (an array here ends up in scratch memory).  1024
-// threads: sixteen waves of 32 x 32 outputs (two column tiles each), 110
-// VGPRs, four waves per SIMD to cover each step's barrier and LDS waits
-// (8 % faster than eight waves of 32 x 64 at two per SIMD, same values).
+// threads: sixteen waves of 32 x 32 outputs (two column tiles each), 128
+// VGPRs, four waves per SIMD to cover each step's barrier and LDS waits.
+// Measured at K = 256 on the 1,000 x 1M search against eight waves per
+// block at two per SIMD: 32 x 64 outputs per wave 3 - 12 % slower; 64 x 32
+// 4 - 13 % slower; 32 x 64 with the query fragments held in registers (half
+// the LDS reads) 3 % faster to 7 % slower from box to box.  LDS was busy
+// 49 % of the cycles and the matrix cores 38 %: each step's barrier, not
+// LDS bandwidth, sets the pace.
 constexpr int kSearchThreads = 1024, kSearchColTiles = 2;
 template <int RV> struct SearchRingT {
     u32x4 r0[RV], r1[RV], r2[RV], r3[RV], r4[RV], r5[RV];
@@ -303,34 +308,66 @@ template <int RV> struct SearchRingT {
     }
 };
 
-// One K step s of tile t: store the ring slot (this tile's step s) into LDS,
-// reload it with tile tn's step s, then multiply from LDS.  A wave owns 32
-// rows x 16 JT columns (2 x JT MFMA tiles).
-template <int s, int NT, int JT>
-__device__ __forceinline__ void search_step(const GemmArgs &g, SearchRingT<kBN * 8 / NT> &ring, uint16_t *As, int AS,
-                                           uint16_t (*Bs)[kBN * kSearchBStride], int &cur, int tn, int wm, int wn,
-                                           int r16, int kb, f32x4 (&acc)[2][JT]) {
-    tile_store_v<kBN, NT, kSearchBStride>(Bs[cur], ring.template at<s>());
-    tile_load_rows<kBN, NT>(g.B, g.ldb, tn * kBN, g.N, s * kBK, ring.template at<s>());
-    __syncthreads();
+// One K step s of tile t (NS steps of 64 per tile).  The fragments of step
+// s come from Bs[cur] (stored and made visible in the step before); then the
+// ring slot of the next step goes to Bs[cur ^ 1] (last read in the step
+// before, finished at its barrier) and is reloaded from tile tn (tn2 for the
+// next tile's step 0); then the products; then the step's one barrier.  No
+// wave waits on its own LDS store before reading.  A wave owns 32 rows x
+// 16 JT columns (2 x JT MFMA tiles).
+template <int s, int NS, int NT, int JT>
+__device__ __forceinline__ void search_step(const GemmArgs &g, SearchRingT<kBN * 8 / NT> &ring, const uint16_t *As,
+                                           int AS, uint16_t (*Bs)[kBN * kSearchBStride], int &cur, int tn, int tn2,
+                                           int wm, int wn, int r16, int kb, f32x4 (&acc)[2][JT]) {
+    bf16x8 af[2][2], bfr[2][JT];
 #pragma unroll
     for (int kk = 0; kk < 2; ++kk) {
-        bf16x8 af[2], bfr[JT];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
-            af[i] = __builtin_bit_cast(
+            af[kk][i] = __builtin_bit_cast(
                 bf16x8, *reinterpret_cast<const uint4 *>(&As[(wm + 16 * i + r16) * AS + 64 * s + 32 * kk + kb]));
 #pragma unroll
         for (int j = 0; j < JT; ++j)
-            bfr[j] = __builtin_bit_cast(
+            bfr[kk][j] = __builtin_bit_cast(
                 bf16x8, *reinterpret_cast<const uint4 *>(&Bs[cur][(wn + 16 * j + r16) * kSearchBStride + 32 * kk + kb]));
+    }
+    constexpr int sn = (s + 1) % NS;
+    tile_store_v<kBN, NT, kSearchBStride>(Bs[cur ^ 1], ring.template at<sn>());
+    tile_load_rows<kBN, NT>(g.B, g.ldb, (s + 1 < NS ? tn : tn2) * kBN, g.N, sn * kBK, ring.template at<sn>());
+#pragma unroll
+    for (int kk = 0; kk < 2; ++kk)
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < JT; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i], bfr[j], acc[i][j], 0, 0, 0);
-    }
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[kk][i], bfr[kk][j], acc[i][j], 0, 0, 0);
+    __syncthreads();
     cur ^= 1;
+}
+
+// Candidates found by the search kernel go to a block queue in LDS (row,
+// column, value; slots taken with LDS atomics) and reach the caller's lists
+// in flushes: when the queue is half full at a tile boundary, and at the
+// end.  A global atomic returns through the vector-memory counter, whose
+// waits are in order: one in the tile epilogue would also wait for the
+// next tile's database loads in flight.  Hits past a full queue take the
+// global path at once.
+template <int QN> struct SearchQueue {
+    int row[QN], col[QN];
+    float val[QN];
+    unsigned n;
+};
+template <int QN>
+__device__ __forceinline__ void search_flush(const SearchQueue<QN> &q, unsigned n, const FilterArgs &f, int NT) {
+    if (n > (unsigned)QN) n = QN;
+    for (int e = threadIdx.x; e < (int)n; e += NT) {
+        const int row = q.row[e];
+        const uint32_t pos = atomicAdd(&f.cnt[row], 1u);
+        if (pos < (uint32_t)f.cap) {
+            f.key[(int64_t)row * f.cap + pos] = q.val[e];
+            f.idx[(int64_t)row * f.cap + pos] = q.col[e];
+        }
+    }
 }
 
 template <int KC, int NT, int JT>
@@ -339,8 +376,10 @@ __global__ __launch_bounds__(NT, NT / 256) void search_filter_kernel(GemmArgs g,
     static_assert(KC == 8 || KC == 12, "the register ring holds four or six K steps of 64");
     constexpr int NS = KC / 2;  // K steps of 64 per tile
     constexpr int AS = 32 * KC + kSearchAPad;  // A row stride in LDS (BF16), padded
+    constexpr int QN = KC == 8 ? 2048 : 512;  // queue slots within the 160 KB of LDS
     __shared__ __attribute__((aligned(16))) uint16_t As[kSearchRows * AS];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[2][kBN * kSearchBStride];
+    __shared__ SearchQueue<QN> hq;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, kb = (lane >> 4) * 8;
     const int xcd = blockIdx.x & 7, jx = blockIdx.x >> 3;
@@ -349,7 +388,9 @@ __global__ __launch_bounds__(NT, NT / 256) void search_filter_kernel(GemmArgs g,
     const int t0 = c * tiles_per_chunk, t1 = min(ntiles, t0 + tiles_per_chunk);
     if (t0 >= t1) return;  // block-uniform
     constexpr int CG = 128 / (16 * JT);  // column groups of waves
+    static_assert((NT / 64) * 32 * 16 * JT == kSearchRows * kBN, "waves tile the block");
     const int bm = q * kSearchRows, wm = (wave / CG) * 32, wn = (wave % CG) * (16 * JT);
+    if (threadIdx.x == 0) hq.n = 0;  // visible after the first barrier below
     // the block's A rows, all of K, into LDS (16-byte chunks; zero past M / K)
     for (int e = threadIdx.x; e < kSearchRows * 4 * KC; e += NT) {
         const int row = e / (4 * KC), k = (e % (4 * KC)) * 8;
@@ -376,31 +417,47 @@ __global__ __launch_bounds__(NT, NT / 256) void search_filter_kernel(GemmArgs g,
         tile_load_rows<kBN, NT>(g.B, g.ldb, t0 * kBN, g.N, 4 * kBK, ring.template at<4>());
         tile_load_rows<kBN, NT>(g.B, g.ldb, t0 * kBN, g.N, 5 * kBK, ring.template at<5>());
     }
+    // step 0 of tile t0 into Bs[0], its ring slot reloaded with tile t0 + 1
     int cur = 0;
+    tile_store_v<kBN, NT, kSearchBStride>(Bs[0], ring.template at<0>());
+    tile_load_rows<kBN, NT>(g.B, g.ldb, (t0 + 1 < t1 ? t0 + 1 : t0) * kBN, g.N, 0, ring.template at<0>());
+    __syncthreads();
     for (int t = t0; t < t1; ++t) {
         f32x4 acc[2][JT];
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < JT; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-        const int tn = t + 1 < t1 ? t + 1 : t;  // the last tile reloads itself (unused)
-        search_step<0, NT, JT>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
-        search_step<1, NT, JT>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
-        search_step<2, NT, JT>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
-        search_step<3, NT, JT>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
-        if constexpr (NS > 4) {
-            search_step<4, NT, JT>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
-            search_step<5, NT, JT>(g, ring, As, AS, Bs, cur, tn, wm, wn, r16, kb, acc);
-        }
-        // filter epilogue of tile t: biases of the lane's 4 columns in one batch, then
-        // compare (hits are rare: about 4 C per query over the whole database)
+        // biases of the lane's columns, issued before the tile's database loads
+        // (vector-memory waits are in order: loaded in the epilogue, the wait
+        // for them would also wait for the next tile's loads in flight)
         float bcol[JT];
 #pragma unroll
         for (int j = 0; j < JT; ++j) {
             const int col = t * kBN + wn + 16 * j + r16;
-            const float bv = g.bias ? g.bias[col < g.N ? col : g.N - 1] : 0.f;
-            bcol[j] = bv;
+            bcol[j] = g.bias ? g.bias[col < g.N ? col : g.N - 1] : 0.f;
         }
+        // the last tile reloads itself (unused)
+        const int tn = t + 1 < t1 ? t + 1 : t, tn2 = t + 2 < t1 ? t + 2 : t1 - 1;
+        search_step<0, NS, NT, JT>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
+        // the queue count as every wave sees it: no wave adds to the queue
+        // between step 0's barrier and the last step's
+        const unsigned qn = hq.n;
+        search_step<1, NS, NT, JT>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
+        search_step<2, NS, NT, JT>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
+        search_step<3, NS, NT, JT>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
+        if constexpr (NS > 4) {
+            search_step<4, NS, NT, JT>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
+            search_step<5, NS, NT, JT>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
+        }
+        if (qn >= (unsigned)QN / 2) {  // block-uniform
+            search_flush(hq, qn, f, NT);
+            __syncthreads();
+            if (threadIdx.x == 0) hq.n = 0;
+            __syncthreads();
+        }
+        // filter epilogue of tile t (hits are rare: about 4 C per query over
+        // the whole database)
 #pragma unroll
         for (int j = 0; j < JT; ++j) {
             const int col = t * kBN + wn + 16 * j + r16;
@@ -411,15 +468,24 @@ __global__ __launch_bounds__(NT, NT / 256) void search_filter_kernel(GemmArgs g,
                     const float v = g.alpha * acc[i][j][e] + bcol[j];
                     if (col < g.N && v < taur[i][e]) {
                         const int row = bm + wm + 16 * i + 4 * (lane >> 4) + e;
-                        const uint32_t pos = atomicAdd(&f.cnt[row], 1u);
-                        if (pos < (uint32_t)f.cap) {
-                            f.key[(int64_t)row * f.cap + pos] = v;
-                            f.idx[(int64_t)row * f.cap + pos] = col;
+                        const unsigned slot = atomicAdd(&hq.n, 1u);
+                        if (slot < (unsigned)QN) {
+                            hq.row[slot] = row;
+                            hq.col[slot] = col;
+                            hq.val[slot] = v;
+                        } else {
+                            const uint32_t pos = atomicAdd(&f.cnt[row], 1u);
+                            if (pos < (uint32_t)f.cap) {
+                                f.key[(int64_t)row * f.cap + pos] = v;
+                                f.idx[(int64_t)row * f.cap + pos] = col;
+                            }
                         }
                     }
                 }
         }
     }
+    __syncthreads();
+    search_flush(hq, hq.n, f, NT);
 }
 
 // The C smallest candidates of each row: keys (order-preserving bits of the
