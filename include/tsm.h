@@ -260,9 +260,15 @@ int tsm_graph_capture_triu(int measure, const tsm_trajset *set, int64_t n, const
 int tsm_graph_launch(tsm_graph *graph, void *stream);
 int tsm_graph_destroy(tsm_graph *graph);
 
-/* Kernel timing of the DP / Hausdorff kernels (HIP events on the launch
- * stream), for roofline reporting.  No reference counterpart. */
-int tsm_profile_enable(int on);
+/* Kernel timing of the DP / Hausdorff / CDDS / SAR kernels (HIP events
+ * stamped by the kernel's own dispatch, on its stream), for roofline
+ * reporting.  tsm_profile_enable(n): time every n-th launch from now on
+ * (n = 1: all; n = 0: off; a timed dispatch costs a few microseconds, so a
+ * stride keeps repeated identical steps near their untimed speed).
+ * tsm_profile_read: total milliseconds and number of the timed launches
+ * since the last read or enable (waits for them).  No reference
+ * counterpart. */
+int tsm_profile_enable(int every);
 int tsm_profile_read(double *total_ms, int64_t *launches);
 
 #ifdef __cplusplus

@@ -107,9 +107,9 @@ int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &p
         h.nset = set->n;
         h.out = out;
         int grid = (int)(npairs < 1048576 ? npairs : 1048576);
-        ProfScope p_(st);
-        if (f64) hipLaunchKernelGGL(hausdorff_kernel<double>, dim3(grid), dim3(kWave), 0, st, h);
-        else hipLaunchKernelGGL(hausdorff_kernel<float>, dim3(grid), dim3(kWave), 0, st, h);
+        const ProfEvents pe;
+        if (f64) hipExtLaunchKernelGGL(hausdorff_kernel<double>, dim3(grid), dim3(kWave), 0, st, pe.start, pe.stop, 0, h);
+        else hipExtLaunchKernelGGL(hausdorff_kernel<float>, dim3(grid), dim3(kWave), 0, st, pe.start, pe.stop, 0, h);
         TSM_HIP(hipGetLastError());
         return TSM_OK;
     }
@@ -124,8 +124,8 @@ int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &p
         c.eps = prm.eps;
         c.out = out;
         int grid = (int)(npairs < 1048576 ? npairs : 1048576);
-        ProfScope p_(st);
-        hipLaunchKernelGGL(cdds_kernel, dim3(grid), dim3(kWave), 0, st, c);
+        const ProfEvents pe;
+        hipExtLaunchKernelGGL(cdds_kernel, dim3(grid), dim3(kWave), 0, st, pe.start, pe.stop, 0, c);
         TSM_HIP(hipGetLastError());
         return TSM_OK;
     }
@@ -143,8 +143,8 @@ int launch(Workspace &w, int measure, const tsm_trajset *set, const PairSpace &p
         s.R = prm.replicas;
         s.out = out;
         int grid = (int)(npairs < 1048576 ? npairs : 1048576);
-        ProfScope p_(st);
-        hipLaunchKernelGGL(sar_kernel, dim3(grid), dim3(kWave), 0, st, s);
+        const ProfEvents pe;
+        hipExtLaunchKernelGGL(sar_kernel, dim3(grid), dim3(kWave), 0, st, pe.start, pe.stop, 0, s);
         TSM_HIP(hipGetLastError());
         return TSM_OK;
     }
@@ -630,9 +630,12 @@ int tsm_graph_destroy(tsm_graph *graph) {
     return TSM_OK;
 }
 
-int tsm_profile_enable(int on) {
+int tsm_profile_enable(int every) {
     std::lock_guard<std::mutex> lk(g_mu);
-    g_ctx.prof = on != 0;
+    if (every < 0) return fail(TSM_E_ARG, "profile stride must be >= 0");
+    g_ctx.prof = every != 0;
+    g_ctx.prof_every = every > 0 ? every : 1;
+    g_ctx.prof_seen = 0;
     g_ctx.ev_used = 0;
     return TSM_OK;
 }

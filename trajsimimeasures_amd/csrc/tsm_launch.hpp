@@ -142,10 +142,8 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     const int pblocks = (int)((npairs + kPairsPerBlock - 1) / kPairsPerBlock);
     hipLaunchKernelGGL(plan_hist_kernel, dim3(pblocks), dim3(kPlanBlock), 0, st, pa);
     hipLaunchKernelGGL(plan_scatter_kernel, dim3(pblocks), dim3(kPlanBlock), 0, st, pa);
-    {
-        ProfScope ps_(st);
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(kWave), 0, st, a);
-    }
+    const ProfEvents pe;
+    hipExtLaunchKernelGGL(kern, dim3(grid), dim3(kWave), 0, st, pe.start, pe.stop, 0, a);
     TSM_HIP(hipGetLastError());
     return TSM_OK;
 }

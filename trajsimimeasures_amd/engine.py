@@ -193,11 +193,18 @@ def triu(measure, dset: DeviceTrajSet, n: int | None = None, *, eps=0.0, eps_mod
 
 
 class KernelTimer:
-    """HIP-event timing of the DP / Hausdorff kernels inside libtsm.so, on the
-    stream they were launched on."""
+    """HIP-event timing of the DP / Hausdorff / CDDS / SAR kernels inside
+    libtsm.so, on the stream they were launched on: every ``every``-th launch
+    is timed (a timed dispatch costs a few microseconds).  After the block,
+    ``ms`` is the timed launches' total and ``launches`` their number."""
+
+    def __init__(self, every: int = 1):
+        if every < 1:
+            raise ValueError("every must be >= 1")
+        self.every = int(every)
 
     def __enter__(self):
-        _lib.check(_lib.lib().tsm_profile_enable(1))
+        _lib.check(_lib.lib().tsm_profile_enable(self.every))
         return self
 
     def __exit__(self, *exc):
