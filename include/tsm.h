@@ -97,7 +97,9 @@ typedef struct tsm_params {
     double eps;     /* EDR/LCSS/STEDR/STLCSS/CDDS distance threshold (core/edr.py:55 `eps`) */
     int eps_mode;   /* TSM_EPS_SQRT (default) or TSM_EPS_SQUARED */
     double gx, gy;  /* ERP gap point g (core/erp.py:94 `gpoint`, default (0,0) erp.py:103) */
-    int lanes;      /* lanes per pair: 0 = autotuned per column count, else 4 or 8 (threads_per_traj hint) */
+    int lanes;      /* lanes per pair: 0 = autotuned per column count, else 2, 4, 8 or 16 (threads_per_traj
+                       hint; used where that many lanes hold the shorter trajectory's columns: at most 12
+                       per lane with 2 lanes (16 for STEDR/STLCSS: 8), 24 otherwise) */
     int precision;  /* TSM_PREC_FP32 or TSM_PREC_FP64 */
     double delta;   /* STEDR/STLCSS time threshold (core/stedr.py:62 `delta`): match needs |t1 - t2| < delta */
     int64_t time_step;     /* SAR time slice (core/sar.py:247 `time_eps`), a positive integer; default 10 */
@@ -202,9 +204,11 @@ int tsm_pairs_padded_host(int measure, const double *t1, const int64_t *len1, in
                           int32_t *counts);
 
 /*
- * Autotuner hook: lanes per pair (4 or 8) for each column count 1..n-1
- * (entry 0 unused) of the DP kernels with FP32 (coord_bits 32) or FP64
- * (coord_bits 64) coordinates; NULL restores the built-in cost model.  The
+ * Autotuner hook: lanes per pair (2, 4, 8 or 16; each lane then holds
+ * ceil(ncol / lanes) register columns) for each column count 1..n-1 (entry 0
+ * unused) of the DP kernels with FP32 (coord_bits 32) or FP64 (coord_bits
+ * 64) coordinates; an entry whose lanes cannot hold its columns falls back
+ * to the cost model, and NULL restores the cost model everywhere.  The
  * Python autotuner (trajsimimeasures_amd/autotune.py) measures and sets it.
  * No reference counterpart (the reference fixes 64 threads per pair,
  * config.py:45).
@@ -225,8 +229,8 @@ int tsm_tune_set_waves_per_cu(int waves_per_cu);
 int tsm_tune_set_wide_lanes(int lanes);
 
 /*
- * hipGraph capture of one batch (ERP prologue, planner and persistent DP
- * kernel, or the Hausdorff kernels) on a non-null stream: explicit pairs
+ * hipGraph capture of one batch (planner and persistent DP kernel, or the
+ * Hausdorff kernel) on a non-null stream: explicit pairs
  * (as tsm_pairs), a query x database grid (as tsm_pairs_grid: the per-batch
  * relaunches of test_trajsimi_time_DQ.py:148-173) or an all-pairs triangle
  * (as tsm_pairs_triu: test_clustering_time.py:219-254).  The capture call

@@ -51,11 +51,19 @@ def test_graph_replays_match_eager(gpu, measure):
 
 def test_autotune_small(gpu):
     from trajsimimeasures_amd import autotune, engine
-    t = autotune.tune(npairs_per_bucket=1024, waves=(4, 8), verbose=False)
-    # a measured lanes table is kept only where it beats the built-in cost model
+    t = autotune.tune(npairs_per_bucket=1024, waves=(4, 8), ncol_max=64, workload_pairs=4096, verbose=False)
+    # every bucket timed every lane count that holds its columns, and reports the model's margin
+    for prec in ("fp32", "fp64"):
+        rows = t["buckets"][prec]
+        assert [r["ncol"][0] for r in rows] == list(range(1, 65, autotune.BUCKET))
+        for r in rows:
+            assert {int(c) for c in r["ms"]} == {c for c in autotune.LANES if autotune.fits(24, c, r["ncol"][1])}
+            assert r["best"] in (2, 4, 8, 16) and r["model_over_best"] >= 0.0
+        assert t["workloads"][prec]["kept"] in ("measured table", "cost model")
+    # a measured lanes table is kept only where it beat the built-in cost model
     for key in ("lanes_fp32", "lanes_fp64"):
         if key in t:
-            assert set(t[key][1:97]) <= {4, 8}
+            assert all(autotune.fits(24, t[key][nc], nc) for nc in range(1, 257))
     assert t["waves_per_cu"] in (4, 8)
     ts, pa, pb = uniform_pairs(1500, seed=2)
     d = engine.DeviceTrajSet(ts)

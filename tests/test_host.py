@@ -309,3 +309,14 @@ def test_st_dropin_rejects_2d_points():
     from trajsimimeasures_amd import core
     with pytest.raises(ValueError):
         core.stedr(np.zeros((2, 3, 2)), [1, 1], np.zeros((2, 3, 2)), [1, 1], 0.1, 60)
+
+
+def test_autotune_cost_model_mirror():
+    """The tuner's copy of the library's lanes cost model: the fewest lanes
+    whose register columns (24 per lane, 12 with two lanes) hold the pair."""
+    from trajsimimeasures_amd import autotune as A
+    got = [A.model_lanes(24, nc) for nc in range(1, 257)]
+    want = [2 if nc <= 24 else 4 if nc <= 96 else 8 if nc <= 192 else 16 for nc in range(1, 257)]
+    assert got == want
+    assert [A.model_lanes(16, nc) for nc in (16, 17, 64, 65, 128, 129)] == [2, 4, 4, 8, 8, 16]
+    assert all(A.fits(24, A.model_lanes(24, nc), nc) for nc in range(1, 385))

@@ -1,17 +1,23 @@
-"""Autotuner for the DP kernel's launch parameters, measured on the device.
+"""Autotuner for the DP kernel's launch shape, measured on the device.
 
-For each column-count bucket (the shorter trajectory's length, in steps of
-8) it times the kernel with 4 and with 8 lanes per pair on synthetic
-random-walk pairs of that shape and keeps the faster; each bucket's batch is
-large enough to fill the GPU at either width (a batch too small for 16 pairs
-per task makes 4 lanes look slow for lack of tasks, not per pair).  Then it
-times the persistent kernel at several resident-wave counts per CU on the
-mixed 100k-pair workload, and finally keeps the measured table only if it
-beats the built-in cost model on that workload.  W (register columns per lane) follows from the lanes choice as
-ceil(ncol / L); the DP kernel uses no LDS, so there is no LDS footprint to
-tune.  Results are applied through tsm_tune_set_lanes /
-tsm_tune_set_waves_per_cu and can be saved as JSON; the package ships the
-table measured on MI355X (tuning_gfx950.json) and applies it at import.
+The DP kernel runs a pair on a segment of L lanes (L = 2, 4, 8 or 16), each
+lane holding W = ceil(ncol / L) columns of the shorter trajectory in
+registers (at most 24, 12 with two lanes), so choosing L per column count
+chooses the tile width.  For each column-count bucket (BUCKET columns) the
+tuner times DTW with every L that holds the bucket's columns, on synthetic
+random-walk pairs of that shape (rows from the bucket up to 200), keeps the
+fastest, and records by how much the built-in cost model's choice there
+(model_lanes, the same rule as the library's) wins or loses.  It does so for
+the FP32-coordinate kernels and for the FP64 ones (precision="fp64"), which
+keep separate tables.  Then it times the persistent kernel at several
+resident-wave counts per CU, and finally keeps a measured lanes table only
+if it beats the cost model on whole workloads: the 100,000-pair batch with
+lengths uniform on 20..200 and one with lognormal lengths (median 60).  The
+DP kernel uses no LDS, so there is no LDS footprint to tune.  Tables go to
+tsm_tune_set_lanes / tsm_tune_set_waves_per_cu and can be saved as JSON;
+the package applies tuning_gfx950.json at import when it is present.
+Reference counterpart: the fixed 64 threads per pair of config.py:45 /
+method_helper.py:69-70.
 """
 from __future__ import annotations
 
@@ -27,6 +33,32 @@ PKG = os.path.dirname(os.path.abspath(__file__))
 SHIPPED = os.path.join(PKG, "tuning_gfx950.json")
 WMAX = {32: 24, 64: 24}  # register columns per lane of the FP32 / FP64-coordinate kernels
 BUCKET = 8
+LANES = (2, 4, 8, 16)
+NCOL_MAX = 256  # column counts the lanes table covers
+
+
+def wmax(wm: int, lanes: int) -> int:
+    """Largest W of an L-lane segment."""
+    return wm // 2 if lanes == 2 else wm
+
+
+def fits(wm: int, lanes: int, ncol: int) -> bool:
+    return lanes in LANES and -(-ncol // lanes) <= wmax(wm, lanes)
+
+
+def model_lanes(wm: int, ncol: int) -> int:
+    """The library's cost model (tsm_launch.hpp model_lanes): per pair
+    L * (rows + L - 1) * (1 + W) issue slots with rows = max(ncol, 32), the
+    cheapest L that holds the columns, ties to fewer lanes."""
+    rows = max(ncol, 32)
+    best, cost = 16, 0.0
+    for L in (16, 8, 4, 2):
+        if not fits(wm, L, ncol):
+            continue
+        c = L * (rows + L - 1) * (1 + -(-ncol // L))
+        if L == 16 or c <= cost:
+            best, cost = L, c
+    return best
 
 
 def apply(table: dict) -> None:
@@ -55,7 +87,7 @@ def load_shipped() -> dict | None:
 def _pairs_with_ncol(rng, lo, hi, npairs, rows_hi=200):
     from .data import random_walks
     ncol = rng.integers(lo, hi + 1, npairs)
-    nrow = np.maximum(ncol, rng.integers(lo, rows_hi + 1, npairs))
+    nrow = np.maximum(ncol, rng.integers(lo, max(rows_hi, hi) + 1, npairs))
     lens = np.empty(2 * npairs, dtype=np.int64)
     lens[0::2] = nrow
     lens[1::2] = ncol
@@ -78,67 +110,98 @@ def _kernel_ms(measure, dset, pa, pb, reps=3, **kw):
     return kt.ms / max(kt.launches, 1)
 
 
-def tune(npairs_per_bucket: int = 65536, seed: int = 0, measures=(("dtw", 32), ("edr", 64)),
-         waves=(4, 8, 12, 16), verbose: bool = True) -> dict:
-    """Measure and apply a tuning table; returns it."""
+def _workloads(rng, npairs):
+    """The whole-workload checks: (name, set, pa, pb) on the device."""
+    import torch
+
+    from . import engine
+    from .data import lengths_lognormal, lengths_uniform, random_walks
+    out = []
+    for name, lens in (("uniform 20..200", lengths_uniform(rng, 2 * npairs, 20, 200)),
+                       ("lognormal median 60", lengths_lognormal(rng, 2 * npairs))):
+        d = engine.DeviceTrajSet(random_walks(rng, lens))
+        pa = torch.arange(0, 2 * npairs, 2, dtype=torch.int32, device="cuda")
+        out.append((name, d, pa, pa + 1))
+    return out
+
+
+def tune(npairs_per_bucket: int = 65536, seed: int = 0, precisions=("fp32", "fp64"), waves=(4, 8, 12, 16),
+         ncol_max: int = NCOL_MAX, workload_pairs: int = 100_000, verbose: bool = True) -> dict:
+    """Measure and apply a tuning table; returns it (with the per-bucket
+    timings under "buckets" and the whole-workload comparison under
+    "workloads")."""
     import torch
 
     from . import engine
     rng = np.random.default_rng(seed)
-    table = {"gpu": torch.cuda.get_device_name(0), "bucket": BUCKET}
-    for measure, bits in measures:
+    L = _lib.lib()
+    table = {"gpu": torch.cuda.get_device_name(0), "bucket": BUCKET, "measure": "dtw", "buckets": {},
+             "workloads": {}}
+    measured = {}
+    for prec in precisions:
+        bits = 32 if prec == "fp32" else 64
         wm = WMAX[bits]
-        lanes = [8] * 257
-        limit4 = 4 * wm  # beyond this only L = 8 fits in registers
-        for lo in range(1, limit4 + 1, BUCKET):
-            hi = min(lo + BUCKET - 1, limit4)
+        _lib.check(L.tsm_tune_set_lanes(bits, None, 0))
+        lanes = [model_lanes(wm, max(nc, 1)) for nc in range(257)]
+        rows = []
+        for lo in range(1, ncol_max + 1, BUCKET):
+            hi = min(lo + BUCKET - 1, ncol_max)
+            cands = [c for c in LANES if fits(wm, c, hi)]
             ts, pa, pb = _pairs_with_ncol(rng, lo, hi, npairs_per_bucket)
             d = engine.DeviceTrajSet(ts)
             pa_d = torch.as_tensor(pa, device="cuda")
             pb_d = torch.as_tensor(pb, device="cuda")
-            t4 = _kernel_ms(measure, d, pa_d, pb_d, lanes=4, eps=0.0025)
-            t8 = _kernel_ms(measure, d, pa_d, pb_d, lanes=8, eps=0.0025)
-            best = 4 if t4 <= t8 else 8
+            ms = {c: _kernel_ms("dtw", d, pa_d, pb_d, lanes=c, precision=prec) for c in cands}
+            best = min(ms, key=ms.get)
+            model = model_lanes(wm, hi)
             for nc in range(lo, hi + 1):
                 lanes[nc] = best
+            rows.append({"ncol": [lo, hi], "ms": {str(c): round(t, 5) for c, t in ms.items()}, "best": best,
+                         "model": model, "model_over_best": round(ms[model] / ms[best] - 1.0, 4)})
             if verbose:
-                print(f"[tune] {measure} ncol {lo:3d}-{hi:3d}: L=4 {t4:.4f} ms, L=8 {t8:.4f} ms -> {best}", flush=True)
-        table[f"lanes_fp{bits}"] = lanes
-    apply(table)
-    # resident waves per CU on the mixed 100k-pair shape
+                print(f"[tune] {prec} ncol {lo:3d}-{hi:3d}: " + ", ".join(f"L={c} {t:.4f} ms" for c, t in ms.items())
+                      + f" -> {best} (model {model}, +{100 * (ms[model] / ms[best] - 1):.1f} %)", flush=True)
+        table["buckets"][prec] = rows
+        measured[bits] = lanes
+    # resident waves per CU on the 100k-pair uniform batch (cost-model tables)
     from .data import lengths_uniform, random_walks
-    ts = random_walks(rng, lengths_uniform(rng, 2 * 100_000, 20, 200))
-    d = engine.DeviceTrajSet(ts)
+    d = engine.DeviceTrajSet(random_walks(rng, lengths_uniform(rng, 2 * 100_000, 20, 200)))
     pa = torch.arange(0, 200_000, 2, dtype=torch.int32, device="cuda")
-    pb = pa + 1
     best_w, best_t = None, None
     for w in waves:
-        _lib.check(_lib.lib().tsm_tune_set_waves_per_cu(int(w)))
-        t = _kernel_ms("dtw", d, pa, pb, reps=5)
+        _lib.check(L.tsm_tune_set_waves_per_cu(int(w)))
+        t = _kernel_ms("dtw", d, pa, pa + 1, reps=5)
         if verbose:
             print(f"[tune] waves per CU {w}: {t:.4f} ms", flush=True)
         if best_t is None or t < best_t:
             best_w, best_t = w, t
     table["waves_per_cu"] = int(best_w)
-    apply(table)
-    # keep the measured lanes tables only where they beat the cost model
-    L = _lib.lib()
-    for measure, bits in measures:
-        t_tuned = _kernel_ms(measure, d, pa, pb, reps=5, eps=0.0025)
-        _lib.check(L.tsm_tune_set_lanes(bits, None, 0))
-        t_model = _kernel_ms(measure, d, pa, pb, reps=5, eps=0.0025)
+    _lib.check(L.tsm_tune_set_waves_per_cu(int(best_w)))
+    # keep a measured lanes table only where it beats the cost model on the workloads
+    loads = _workloads(rng, workload_pairs)
+    for prec in precisions:
+        bits = 32 if prec == "fp32" else 64
+        arr = (ctypes.c_uint8 * 257)(*[int(x) for x in measured[bits]])
+        res = {}
+        for name, dd, pa_w, pb_w in loads:
+            _lib.check(L.tsm_tune_set_lanes(bits, arr, 257))
+            t_tuned = _kernel_ms("dtw", dd, pa_w, pb_w, reps=5, precision=prec)
+            _lib.check(L.tsm_tune_set_lanes(bits, None, 0))
+            t_model = _kernel_ms("dtw", dd, pa_w, pb_w, reps=5, precision=prec)
+            res[name] = {"tuned_ms": round(t_tuned, 5), "model_ms": round(t_model, 5)}
+        keep = sum(r["tuned_ms"] for r in res.values()) < sum(r["model_ms"] for r in res.values())
+        table["workloads"][prec] = {**res, "kept": "measured table" if keep else "cost model"}
+        if keep:
+            table[f"lanes_fp{bits}"] = measured[bits]
+            _lib.check(L.tsm_tune_set_lanes(bits, arr, 257))
         if verbose:
-            print(f"[tune] {measure}: measured table {t_tuned:.4f} ms, cost model {t_model:.4f} ms", flush=True)
-        if t_model <= t_tuned:
-            table.pop(f"lanes_fp{bits}")
-        else:
-            apply({f"lanes_fp{bits}": table[f"lanes_fp{bits}"]})
+            print(f"[tune] {prec} workloads: {res} -> {table['workloads'][prec]['kept']}", flush=True)
     return table
 
 
 def save(table: dict, path: str) -> None:
     with open(path, "w") as f:
-        json.dump(table, f)
+        json.dump(table, f, indent=1)
 
 
 if __name__ == "__main__":
@@ -150,4 +213,4 @@ if __name__ == "__main__":
     t = tune(npairs_per_bucket=args.pairs)
     if args.out:
         save(t, args.out)
-    print(json.dumps({k: v for k, v in t.items() if not k.startswith("lanes")}))
+    print(json.dumps({k: v for k, v in t.items() if k in ("gpu", "waves_per_cu", "workloads")}))

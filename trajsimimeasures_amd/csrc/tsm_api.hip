@@ -49,9 +49,7 @@ using namespace tsm;
 // plus column padding up to a multiple of L; both shrink with smaller L,
 // while register columns (and so occupancy) grow.
 int choose_lanes(const tsm_params &prm) {
-    if (prm.lanes == 4 || prm.lanes == 8) return prm.lanes;
-    if (prm.lanes == 16) return 8;
-    return 0;
+    return prm.lanes == 2 || prm.lanes == 4 || prm.lanes == 8 || prm.lanes == 16 ? prm.lanes : 0;
 }
 
 }  // namespace
@@ -499,7 +497,8 @@ int tsm_tune_set_lanes(int coord_bits, const uint8_t *lanes_of_ncol, int n) {
     }
     if (n < 2 || n > 257) return fail(TSM_E_ARG, "lanes table must cover column counts 1..n-1 with 2 <= n <= 257");
     for (int i = 1; i < n; ++i)
-        if (lanes_of_ncol[i] != 4 && lanes_of_ncol[i] != 8) return fail(TSM_E_ARG, "lanes must be 4 or 8");
+        if (lanes_of_ncol[i] != 2 && lanes_of_ncol[i] != 4 && lanes_of_ncol[i] != 8 && lanes_of_ncol[i] != 16)
+            return fail(TSM_E_ARG, "lanes must be 2, 4, 8 or 16");
     std::memset(g_ctx.tuned_lanes[k], 0, sizeof(g_ctx.tuned_lanes[k]));
     std::memcpy(g_ctx.tuned_lanes[k], lanes_of_ncol, (size_t)n);
     g_ctx.tuned[k] = true;

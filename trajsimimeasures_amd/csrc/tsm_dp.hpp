@@ -46,7 +46,8 @@ namespace tsm {
 #endif
 constexpr int kRowKeys = TSM_ROW_KEYS;
 static_assert(kRowKeys >= 16 && kRowKeys <= 128 && (kRowKeys & (kRowKeys - 1)) == 0, "row keys: power of two <= 128");
-constexpr int kMaxSeg = 16;     // segments per wave at L = 4
+constexpr int kMaxSeg = 16;     // segments per wave at L = 4 (the strip scratch; 2-lane pairs never need strips)
+constexpr int kMaxBuckets = 96;  // the planner's bucket capacity
 constexpr int kTabMax = 257;    // bucket table covers column counts 1..256
 
 // Register columns per lane: 24 with FP32 coordinates (TSM_WMAX32; measured
@@ -56,39 +57,51 @@ constexpr int kTabMax = 257;    // bucket table covers column counts 1..256
 #endif
 template <typename CT> struct WMaxOf { static constexpr int value = sizeof(CT) == 4 ? TSM_WMAX32 : 24; };
 
-// Buckets, with H = WM / 2: b < 4 hold L = 64, W = WM - 4 b (24, 20, 16,
-// 12 at WM = 24), for pairs wider than 16 * WM columns in launches small
-// enough that their longest pairs, not the total work, set the time (a
-// pair's steps run one after another; a wavefront takes a quarter of the
-// steps of 16 lanes, and the smallest W that holds the pair the fewest
-// cells per step): a pair takes the smallest W with 64 W >= its columns,
-// b = 0 above 64 * WM, in strips of 64 * WM columns.  4 <= b < 4 + H hold
-// L = 16, W = WM - (b - 4) (pairs wider than 8 * WM columns; b = 4 also takes
-// the pairs wider than the table otherwise, in strips of 16 * WM columns);
-// b >= 4 + H hold b' = b - 4 - H: L = (b' odd ? 4 : 8), W = WM - b' / 2.
+// Buckets (L lanes per pair, W register columns per lane):
+//   b < 4: L = 64, W = WM - 4 b (24, 20, 16, 12 at WM = 24), for pairs wider
+//     than 16 * WM columns in launches small enough that their longest
+//     pairs, not the total work, set the time (a pair's steps run one after
+//     another; a wavefront takes a quarter of the steps of 16 lanes, and the
+//     smallest W that holds the pair the fewest cells per step): a pair
+//     takes the smallest W with 64 W >= its columns, b = 0 above 64 * WM, in
+//     strips of 64 * WM columns;
+//   4 <= b < 4 + WM: L = 16, W = WM - (b - 4) (b = 4 also takes the pairs
+//     wider than the table otherwise, in strips of 16 * WM columns);
+//   4 + WM <= b < 4 + 3 WM: b' = b - 4 - WM, L = (b' odd ? 4 : 8),
+//     W = WM - b' / 2;
+//   4 + 3 WM <= b: L = 2, W = WM / 2 - (b - 4 - 3 WM) (short columns).
 // Tasks run in bucket order, so the widest pairs start first.
-__host__ __device__ constexpr int bucket_count(int WM) { return 4 + WM / 2 + 2 * WM; }
+__host__ __device__ constexpr int bucket_count(int WM) { return 4 + 3 * WM + WM / 2; }
 __host__ __device__ constexpr int bucket_of(int WM, int L, int W) {
-    return L == 64 ? (WM - W) / 4 : L == 16 ? 4 + WM - W : 4 + WM / 2 + 2 * (WM - W) + (L == 4 ? 1 : 0);
+    return L == 64   ? (WM - W) / 4
+           : L == 16 ? 4 + WM - W
+           : L == 2  ? 4 + 3 * WM + (WM / 2 - W)
+                     : 4 + WM + 2 * (WM - W) + (L == 4 ? 1 : 0);
 }
 __host__ __device__ constexpr int bucket_L(int WM, int b) {
-    return b < 4 ? 64 : b < 4 + WM / 2 ? 16 : ((b - 4 - WM / 2) & 1) ? 4 : 8;
+    return b < 4 ? 64 : b < 4 + WM ? 16 : b < 4 + 3 * WM ? (((b - 4 - WM) & 1) ? 4 : 8) : 2;
 }
 __host__ __device__ constexpr int bucket_W(int WM, int b) {
-    return b < 4 ? WM - 4 * b : b < 4 + WM / 2 ? WM - (b - 4) : WM - (b - 4 - WM / 2) / 2;
+    return b < 4 ? WM - 4 * b : b < 4 + WM ? WM - (b - 4) : b < 4 + 3 * WM ? WM - (b - 4 - WM) / 2 : WM / 2 - (b - 4 - 3 * WM);
 }
+// Largest W of an L-lane bucket.
+__host__ __device__ constexpr int bucket_wmax(int WM, int L) { return L == 2 ? WM / 2 : WM; }
 
-// Column count -> bucket table, plus the bucket of pairs wider than the
-// table (L = 16, W = WM, swept in strips of 16 * WM columns), and the lanes
-// per pair of every bucket.
+// Column count -> bucket table for one launch.  The planner and the task
+// queue work on the launch's used buckets only, numbered 0 .. nb - 1 in
+// schedule order (widest W first: the longest tasks start first); kid maps
+// that number to the (L, W) kernel variant above.
 struct BucketTable {
-    uint8_t b[kTabMax];  // b[ncol] for 1 <= ncol < kTabMax
-    uint8_t lanes[64];   // L of bucket b
+    uint8_t b[kTabMax];  // used-bucket number of a pair whose shorter side has ncol columns, 1 <= ncol <= ncmax
+    uint8_t lanes[kMaxBuckets];  // L of used bucket k
+    uint8_t kid[kMaxBuckets];    // (L, W) variant of used bucket k (bucket_L / bucket_W)
+    uint8_t w64[4];      // with wide64: used-bucket number of L = 64, W = 24, 20, 16, 12 (WM - 4 i)
+    int nb;              // used buckets
     int ncmax;           // largest column count the table covers
-    int wide;            // bucket for ncol > ncmax (16 lanes; strips when wider than wide16)
+    int wide;            // used bucket for ncol > ncmax (16 lanes, W = WM; strips when wider than wide16)
     int wide16;          // with wide64: the column count up to which `wide` still applies (16 * WM)
-    int wm;              // with wide64: WM; a wider pair takes bucket (WM - W) / 4 with W the smallest
-                         // multiple of 4 (>= 12) with 64 W >= its columns, bucket 0 beyond 64 * WM
+    int wm;              // with wide64: WM; a wider pair takes w64[(WM - W) / 4] with W the smallest
+                         // multiple of 4 (>= 12) with 64 W >= its columns, W = WM beyond 64 * WM
     int wide64;          // nonzero: pairs wider than wide16 take 64-lane segments
 };
 
@@ -99,8 +112,9 @@ struct DPArgs {
     double gx, gy;           // ERP: the gap point g
     PairSpace ps;
     const uint32_t *slots;   // pair slots grouped by task, kEmptySlot = idle segment
-    const int64_t *meta;     // [0] ntasks, [1 + b] task end of bucket b, [1 + nb + b] slot base of bucket b
-    int nb;                  // number of buckets
+    const int64_t *meta;     // [0] ntasks, [1 + b] task end of used bucket b, [1 + nb + b] its slot base
+    int nb;                  // number of used buckets
+    uint8_t kid[kMaxBuckets];  // (L, W) variant of used bucket b
     unsigned int *counter;   // task queue head
     void *bnd;               // strip boundary scratch (AT), kMaxSeg segments per wave
     int64_t bnd_stride;
@@ -455,7 +469,7 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
     // a pair needing strips longer than the scratch column (max_len was
     // understated by the caller) is not computed; its result is NaN and its
     // count -1
-    const bool fits = (ncol <= kSpan) || (nrow <= a.bnd_stride);
+    const bool fits = (ncol <= kSpan) || (L >= 4 && nrow <= a.bnd_stride);
     if (valid && !fits && lane0) {
         a.out[slot] = __builtin_bit_cast(double, 0x7ff8000000000000ull);  // quiet NaN
         if ((M == kEDR || M == kLCSS || M == kSTEDR || M == kSTLCSS) && a.counts) a.counts[slot] = -1;
@@ -464,7 +478,8 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
     const int maxrow = __builtin_amdgcn_readfirstlane(wave_max(nstrips > 0 ? nrow : 0));
     const int maxstrips = __builtin_amdgcn_readfirstlane(wave_max(nstrips));
     const int nsteps = maxrow + L - 1;
-    AT *bnd = reinterpret_cast<AT *>(a.bnd) + (wave_id * kMaxSeg + seg) * a.bnd_stride;
+    // (2-lane segments hold at most WM columns: never strips, no scratch)
+    AT *bnd = L >= 4 ? reinterpret_cast<AT *>(a.bnd) + (wave_id * kMaxSeg + seg) * a.bnd_stride : nullptr;
     const int rlast = nrow > 0 ? nrow - 1 : 0;
 
     for (int strip = 0; strip < maxstrips; ++strip) {
@@ -572,9 +587,10 @@ template <int M, bool F64> __device__ __forceinline__ void dp_body(const DPArgs 
             else lo = mid + 1;
         }
         const int bk = __builtin_amdgcn_readfirstlane(lo);
+        const int kid = a.kid[bk];
         const int64_t tstart = bk > 0 ? a.meta[bk] : 0;
-        const int64_t slot0 = a.meta[1 + a.nb + bk] + ((int64_t)t - tstart) * (kWave / bucket_L(WM, bk));
-        dispatch_bucket<M, F64, WM, 0>(a, bk, slot0, wave_id);
+        const int64_t slot0 = a.meta[1 + a.nb + bk] + ((int64_t)t - tstart) * (kWave / bucket_L(WM, kid));
+        dispatch_bucket<M, F64, WM, 0>(a, kid, slot0, wave_id);
     }
 }
 

@@ -3,48 +3,98 @@
 // translation units, which instantiate run_dp for their measure.
 #pragma once
 
+#include <cstring>
+#include <utility>
+
 #include "tsm_host.hpp"
 #include "tsm_plan.hpp"
 
 namespace tsm {
 
-// Column count -> (L, W) bucket.  Sixteen lanes where eight would need more
-// than WM columns each, eight where four would; otherwise, by priority, the
+// Whether an L-lane segment holds nc columns in registers.
+inline bool lanes_fit(int WM, int L, int nc) {
+    return (L == 2 || L == 4 || L == 8 || L == 16) && (nc + L - 1) / L <= bucket_wmax(WM, L);
+}
+
+// Cost-model lanes for nc columns: per pair, a segment of L lanes takes
+// (nrow + L - 1) steps of (c0 + c1 * W) issue slots for 64/L pairs at once,
+// so the per-pair cost is L * (nrow + L - 1) * (c0 + c1 * W) / 64 with
+// c0 ~ c1 (per-step bookkeeping ~ one cell) and nrow >= nc (taken as
+// max(nc, 32)): the fewest lanes whose W fits, nearly always.
+inline int model_lanes(int WM, int nc) {
+    const double rows = nc > 32 ? nc : 32;
+    int best = 16;
+    double bc = 0.0;
+    for (int L = 16; L >= 2; L >>= 1) {
+        if (!lanes_fit(WM, L, nc)) continue;
+        const double c = L * (rows + L - 1) * (1.0 + (nc + L - 1) / L);
+        if (L == 16 || c <= bc) {
+            best = L;
+            bc = c;
+        }
+    }
+    return best;
+}
+
+// Column count -> (L, W) variant, W = ceil(nc / L): by priority, the
 // caller's lanes hint, the autotuned lanes table (tsm_tune_set_lanes), then
-// a cost model.  Per
-// pair, a segment of L lanes takes (nrow + L - 1) steps of (c0 + c1 * W)
-// issue slots for 64/L pairs at once, so the per-pair cost is
-// L * (nrow + L - 1) * (c0 + c1 * W) / 64 with c0 ~ c1 (per-step bookkeeping
-// ~ one cell): L = 4 wins whenever its W fits in registers.
-// Pairs wider than the table: 16-lane segments (strips of 16 * WM
-// columns), or with wide_lanes = 64, 16 lanes up to one such strip and a
-// whole wavefront beyond (W = 12, 16, 20 or 24: the smallest that holds the
-// pair, then W = WM in strips of 64 * WM).
-inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint, int wide_lanes) {
+// the cost model, each where its L holds the columns (16 lanes always do
+// within the table).  Pairs wider than the table: 16-lane segments (strips
+// of 16 * WM columns), or with wide_lanes = 64, 16 lanes up to one such
+// strip and a whole wavefront beyond (W = 12, 16, 20 or 24: the smallest
+// that holds the pair, then W = WM in strips of 64 * WM).
+// Only the variants that the set's lengths (shorter sides up to max_len)
+// can reach become buckets of the launch, numbered in schedule order: the
+// 64-lane ones, then by W and L descending (per step, a task's time grows
+// with W; per task, its steps with L).
+inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint, int wide_lanes, int64_t max_len) {
     BucketTable t{};
     t.ncmax = 16 * WM < kTabMax - 1 ? 16 * WM : kTabMax - 1;
-    t.wide = bucket_of(WM, 16, WM);
     t.wide16 = 16 * WM;
     t.wm = WM;
     t.wide64 = wide_lanes == 64;
-    for (int b = 0; b < bucket_count(WM); ++b) t.lanes[b] = (uint8_t)bucket_L(WM, b);
     const int k = coord_bits == 64 ? 1 : 0;
     const uint8_t *tuned = g_ctx.tuned[k] ? g_ctx.tuned_lanes[k] : nullptr;
+    bool used[kMaxBuckets] = {};
+    uint8_t var[kTabMax] = {};
     for (int nc = 1; nc <= t.ncmax; ++nc) {
-        const int w4 = (nc + 3) / 4, w8 = (nc + 7) / 8;
         int L;
-        if (w8 > WM) L = 16;
-        else if (w4 > WM) L = 8;
-        else if (lanes_hint == 4 || lanes_hint == 8) L = lanes_hint;
-        else if (tuned && (tuned[nc] == 4 || tuned[nc] == 8)) L = tuned[nc];
-        else {
-            const double rows = nc > 32 ? nc : 32;
-            const double c4 = 4.0 * (rows + 3) * (1.0 + w4), c8 = 8.0 * (rows + 7) * (1.0 + w8);
-            L = c4 <= c8 ? 4 : 8;
-        }
-        t.b[nc] = (uint8_t)bucket_of(WM, L, (nc + L - 1) / L);
+        if (lanes_fit(WM, lanes_hint, nc)) L = lanes_hint;
+        else if (tuned && lanes_fit(WM, tuned[nc], nc)) L = tuned[nc];
+        else L = model_lanes(WM, nc);
+        var[nc] = (uint8_t)bucket_of(WM, L, (nc + L - 1) / L);
+        if (nc <= max_len || nc == 1) used[var[nc]] = true;
     }
+    const int wide_id = bucket_of(WM, 16, WM);
+    if (max_len > t.ncmax) {
+        used[wide_id] = true;
+        if (t.wide64 && max_len > t.wide16)
+            for (int i = 0; i < 4; ++i) used[i] = true;
+    }
+    int order[kMaxBuckets], n = 0;
+    for (int b = 0; b < bucket_count(WM); ++b)
+        if (used[b]) order[n++] = b;
+    auto before = [WM](int x, int y) {
+        const int lx = bucket_L(WM, x), ly = bucket_L(WM, y);
+        if ((lx == 64) != (ly == 64)) return lx == 64;
+        if (bucket_W(WM, x) != bucket_W(WM, y)) return bucket_W(WM, x) > bucket_W(WM, y);
+        return lx > ly;
+    };
+    for (int i = 1; i < n; ++i)  // insertion sort: at most kMaxBuckets entries
+        for (int j = i; j > 0 && before(order[j], order[j - 1]); --j) std::swap(order[j], order[j - 1]);
+    uint8_t num[kMaxBuckets] = {};
+    for (int i = 0; i < n; ++i) {
+        num[order[i]] = (uint8_t)i;
+        t.kid[i] = (uint8_t)order[i];
+        t.lanes[i] = (uint8_t)bucket_L(WM, order[i]);
+    }
+    t.nb = n;
+    // column counts beyond max_len occur only with an understated max_len
+    // (their pairs land in bucket 0 and are computed or reported unfit there)
+    for (int nc = 1; nc <= t.ncmax; ++nc) t.b[nc] = used[var[nc]] ? num[var[nc]] : 0;
     t.b[0] = t.b[1];
+    t.wide = used[wide_id] ? num[wide_id] : 0;
+    for (int i = 0; i < 4; ++i) t.w64[i] = used[i] ? num[i] : 0;
     return t;
 }
 
@@ -56,16 +106,16 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     using CT = typename Rec<M, F64>::CT;
     using AT = typename Rec<M, F64>::AT;
     constexpr int WM = Rec<M, F64>::kWMax;
-    constexpr int NB = bucket_count(WM);
-    static_assert(NB <= kMaxBuckets && NB <= 64 && WM % 4 == 0 && WM >= 16, "bucket count");
+    static_assert(bucket_count(WM) <= kMaxBuckets && WM % 4 == 0 && WM >= 16, "bucket count");
     // A launch of few pairs lasts as long as its longest pair's chain of
     // steps: there, pairs wider than the table take a whole wavefront (a
     // quarter of the steps of 16 lanes, at more lane-steps per pair); in a
     // large launch the total work sets the time and 16-lane strips use the
     // lanes better.
     const int wide_lanes = g_ctx.wide_lanes ? g_ctx.wide_lanes : (npairs <= kSmallLaunchPairs ? 64 : 16);
-    const BucketTable tab = bucket_table(WM, sizeof(CT) * 8, lanes_hint, wide_lanes);
-    const int64_t nslots = npairs + (int64_t)NB * kMaxSeg;
+    const BucketTable tab = bucket_table(WM, sizeof(CT) * 8, lanes_hint, wide_lanes, set.max_len);
+    const int NB = tab.nb;
+    const int64_t nslots = npairs + (int64_t)NB * (kWave / 2);  // each bucket padded to whole tasks
     int rc;
     // hist holds the planner's histogram and then its fill counters
     if ((rc = c.get(c.hist, 2 * kMaxBins * 4)) || (rc = c.get(c.slots, nslots * 4)) ||
@@ -117,6 +167,7 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     a.slots = pa.slots;
     a.meta = pa.meta;
     a.nb = NB;
+    std::memcpy(a.kid, tab.kid, sizeof(a.kid));
     a.counter = pa.counter;
     a.bnd = c.bnd.p;
     a.bnd_stride = bnd_stride;

@@ -25,16 +25,15 @@
 
 namespace tsm {
 
-constexpr int kMaxBuckets = 64;
-constexpr int kMaxBins = kMaxBuckets * kRowKeys;                   // 8192
-constexpr int kPlanBlock = 256;
+constexpr int kMaxBins = kMaxBuckets * kRowKeys;                   // 12288
+constexpr int kPlanBlock = 384;  // four threads per bucket in the layout scan
 // Pairs per planner block.  Measured on 100k pairs (DTW step minus kernel):
 // 256 adds ~22 us and 512 ~5 us to the planner; 2048 is the same as 1024.
 #ifndef TSM_PAIRS_PER_BLOCK
-#define TSM_PAIRS_PER_BLOCK 1024
+#define TSM_PAIRS_PER_BLOCK 1536
 #endif
 constexpr int kPairsPerBlock = TSM_PAIRS_PER_BLOCK;
-static_assert(kPairsPerBlock % 256 == 0, "pairs per planner block: a multiple of the block size");
+static_assert(kPairsPerBlock % kPlanBlock == 0, "pairs per planner block: a multiple of the block size");
 
 struct PlanArgs {
     PairSpace ps;
@@ -61,7 +60,7 @@ __device__ __forceinline__ int bin_of(const PlanArgs &a, int64_t na, int64_t nb)
     else {
         int w = (int)((ncol + 255) / 256) * 4;  // 4 * ceil(ncol / 256): the smallest multiple of 4 with 64 w >= ncol
         w = w < 12 ? 12 : w;
-        b = w >= a.tab.wm ? 0 : (a.tab.wm - w) / 4;
+        b = a.tab.w64[w >= a.tab.wm ? 0 : (a.tab.wm - w) / 4];
     }
     const int rk = (int)((nrow < 511 ? nrow : 511) * kRowKeys / 512);
     return b * kRowKeys + (kRowKeys - 1 - rk);
@@ -164,36 +163,44 @@ __device__ __forceinline__ void plan_layout(const PlanArgs &a, uint32_t *base, b
     const uint32_t excl_t = x - sum;
     const uint32_t excl_first = first > 0 ? incl[first - 1] : 0u;
     if (wave == 0) {
-        // per bucket (one lane each): pairs, tasks of 64/L slots, then an
-        // inclusive scan of tasks and slots across the buckets
-        const int k = lane;
-        uint32_t cnt = 0, nt = 0, ns = 0;
-        if (k < a.nb) {
-            const uint32_t start = k > 0 ? incl[k * kScanTPB - 1] : 0u;
-            cnt = incl[(k + 1) * kScanTPB - 1] - start;
-            const uint32_t S = (uint32_t)(kWave / a.tab.lanes[k]);
-            nt = (cnt + S - 1) / S;
-            ns = nt * S;
-        }
-        uint32_t ti = nt, si = ns;
+        // per bucket (one lane each, 64 buckets a pass): pairs, tasks of
+        // 64/L slots, then an inclusive scan of tasks and slots across the
+        // buckets, carried from pass to pass
+        uint32_t tcarry = 0, scarry = 0;
+        for (int h = 0; h < kMaxBuckets; h += kWave) {
+            const int k = h + lane;
+            uint32_t cnt = 0, nt = 0, ns = 0;
+            if (k < a.nb) {
+                const uint32_t start = k > 0 ? incl[k * kScanTPB - 1] : 0u;
+                cnt = incl[(k + 1) * kScanTPB - 1] - start;
+                const uint32_t S = (uint32_t)(kWave / a.tab.lanes[k]);
+                nt = (cnt + S - 1) / S;
+                ns = nt * S;
+            }
+            uint32_t ti = nt, si = ns;
 #pragma unroll
-        for (int o = 1; o < kWave; o <<= 1) {
-            const uint32_t tu = __shfl_up(ti, o, kWave), su = __shfl_up(si, o, kWave);
-            if (lane >= o) {
-                ti += tu;
-                si += su;
+            for (int o = 1; o < kWave; o <<= 1) {
+                const uint32_t tu = __shfl_up(ti, o, kWave), su = __shfl_up(si, o, kWave);
+                if (lane >= o) {
+                    ti += tu;
+                    si += su;
+                }
             }
-        }
-        if (k < a.nb) {
-            const int64_t sb = (int64_t)(si - ns);
-            slot_base[k] = sb;
-            slot_fill[k] = sb + cnt;
-            slot_end[k] = (int64_t)si;
-            if (writer) {
-                a.meta[1 + a.nb + k] = sb;
-                a.meta[1 + k] = (int64_t)ti;
-                if (k == a.nb - 1) a.meta[0] = (int64_t)ti;
+            ti += tcarry;
+            si += scarry;
+            if (k < a.nb) {
+                const int64_t sb = (int64_t)(si - ns);
+                slot_base[k] = sb;
+                slot_fill[k] = sb + cnt;
+                slot_end[k] = (int64_t)si;
+                if (writer) {
+                    a.meta[1 + a.nb + k] = sb;
+                    a.meta[1 + k] = (int64_t)ti;
+                    if (k == a.nb - 1) a.meta[0] = (int64_t)ti;
+                }
             }
+            tcarry = __shfl(ti, kWave - 1, kWave);
+            scarry = __shfl(si, kWave - 1, kWave);
         }
         if (writer && lane == 0) *a.counter = 0u;
     }
@@ -207,10 +214,10 @@ __device__ __forceinline__ void plan_layout(const PlanArgs &a, uint32_t *base, b
         }
     }
     if (writer) {
-        // padding slots after each bucket's pairs: at most 64/L - 1 < 16 per bucket
-        for (int q = t; q < a.nb * 16; q += kPlanBlock) {
-            const int k = q / 16;
-            const int64_t s = slot_fill[k] + q % 16;
+        // padding slots after each bucket's pairs: at most 64/L - 1 < 32 per bucket
+        for (int q = t; q < a.nb * 32; q += kPlanBlock) {
+            const int k = q / 32;
+            const int64_t s = slot_fill[k] + q % 32;
             if (s < slot_end[k]) a.slots[s] = kEmptySlot;
         }
     }
