@@ -414,8 +414,19 @@ def graph_leg(pa_d, pb_d, dset, npairs: int, reps: int = 3) -> dict:
     t_r = best(replay, g.stream.synchronize)
     same = bool(torch.equal(out, ref))
     g.close()
+    # the same batches through the planner (direct launches switched off)
+    from trajsimimeasures_amd import _lib
+    _lib.check(_lib.lib().tsm_tune_set_direct_pairs(0))
+    try:
+        t_p = best(eager, torch.cuda.synchronize)
+    finally:
+        _lib.check(_lib.lib().tsm_tune_set_direct_pairs(-1))
+    same = same and bool(torch.equal(out, ref))
     res["batches_512"] = {"workload": f"{nb} batches of {B} pairs of the 100k-pair DTW workload",
                           "eager_pairs_per_s": nb * B / t_e, "replay_pairs_per_s": nb * B / t_r,
+                          "eager_planned_pairs_per_s": nb * B / t_p,
+                          "note": "eager and replayed calls of this size are direct launches (one wavefront per "
+                                  "pair, no planner); eager_planned: the same calls through the planner",
                           "equal": same}
     log(str(res["batches_512"]))
     # query x database, 1e8 pairs
@@ -561,9 +572,11 @@ def knn_leg(nq: int, nd: int) -> dict:
     """Exact DTW kNN against the learned filter: recall@10 and q/s at
     C in {25, 50, 100, 150, 200} candidates for the untrained encoder (the
     resampled-L2 embedding), the trained one (the default training run: each
-    of 2,000 anchors with its 100 nearest by the untrained embedding) and one
-    trained on 200,000 random-pair DTW labels (2,000 anchors x 100 random
-    database trajectories).  The headline pair is the default C."""
+    of 2,000 anchors with its 100 nearest by the untrained embedding, ranking
+    loss) and one trained on 200,000 random-pair DTW labels (2,000 anchors x
+    100 random database trajectories, log-space distance regression as the
+    reference fits random pairs' distances).  The headline pair is the
+    default C."""
     import torch
 
     from trajsimimeasures_amd import filter as F

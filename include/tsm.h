@@ -228,6 +228,19 @@ int tsm_tune_set_waves_per_cu(int waves_per_cu);
  * column of one pair, core/dtw.py:33-45). */
 int tsm_tune_set_wide_lanes(int lanes);
 
+/* Direct DP launches: a call of at most max_pairs pairs, whose set needs no
+ * column strips (max_len <= 64 * 24 columns with FP32 coordinates, 64 * 16 for
+ * STEDR/STLCSS) and which passes no lanes hint, skips the planner and runs
+ * one wavefront per pair (64 lanes of 4 to 24 columns, the fewest that hold
+ * it): a small batch then lasts as long as its longest pair's chain of about
+ * rows + columns / 4 steps instead of rows + 15 steps of wider lanes, and
+ * pays no planner kernels.  -1 (default): three times the resident waves of
+ * the persistent grid (3 x CUs x waves per CU, 6,144 on MI355X; the autotuner
+ * measures the crossover); 0: never.  Results equal the planned launch's.  No counterpart in the reference, whose
+ * batches of 512 pairs run 64 threads per pair (config.py:45,
+ * method_helper.py:69-70). */
+int tsm_tune_set_direct_pairs(int64_t max_pairs);
+
 /*
  * hipGraph capture of one batch (planner and persistent DP kernel, or the
  * Hausdorff kernel) on a non-null stream: explicit pairs

@@ -73,6 +73,13 @@ int tsm_nn_colsum(const float *M, int64_t rows, int cols, int64_t ld, float *out
  * distance: positives are ranks < P, negatives ranks >= Q0. */
 int tsm_nn_rank_loss(const float *E, int B, int K, int D, int P, int Q0, float tau, float *dE, float *loss,
                      void *stream);
+/* Log-space distance regression over B anchor lists of K candidates with
+ * exact distances Y [B, K]: mean over each list of (log(d + c) - log(s y +
+ * c))^2, d the embedding distance to the anchor, s = sum d / sum y of the
+ * list (held constant), c = 1e-3 * mean d; dE includes the 1/B of the mean
+ * over lists.  Counterpart of the MSE fit of embedding distances to
+ * ground-truth ones in nn/MLP_trajsimi_train.py:78-82 (random-pair labels). */
+int tsm_nn_dist_loss(const float *E, const float *Y, int B, int K, int D, float *dE, float *loss, void *stream);
 /* Adam step `step` (>= 1) on a flat FP32 parameter vector. */
 int tsm_nn_adam(float *p, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2, float eps,
                 int64_t step, void *stream);

@@ -1,5 +1,7 @@
 """TEST INFRASTRUCTURE ONLY -- FP64 numpy restatement of the learned kNN
-filter's encoder and ranking loss, forward and backward: the checker for the
+filter's encoder and its two losses (ranking; log-space distance regression,
+after the reference's MSE fit of embedding distances in
+nn/MLP_trajsimi_train.py:78-82), forward and backward: the checker for the
 HIP kernels of trajsimimeasures_amd/filter.py (csrc/tsm_nn.hip).  Only tests
 import it.
 
@@ -152,3 +154,25 @@ def rank_loss(E: np.ndarray, B: int, K: int, P: int, Q0: int, tau: float, scales
     dEb[:, 1:] = 2 * diff * g[:, :, None]
     dEb[:, 0] = -dEb[:, 1:].sum(1)
     return loss, dEb.reshape(E.shape), scales
+
+
+def dist_loss(E: np.ndarray, Y: np.ndarray, B: int, K: int, consts=None):
+    """Mean over anchors of mean_k (log(d_k + c) - log(s y_k + c))^2 with
+    d_k = |e_k - e_a|, s = sum d / sum y and c = 1e-3 mean d per anchor,
+    both held constant (pass `consts` = (s, c) arrays to freeze them).
+    Returns (loss, dE, consts)."""
+    Dm = E.shape[1]
+    Eb = E.reshape(B, 1 + K, Dm)
+    Y = np.asarray(Y, dtype=np.float64).reshape(B, K)
+    diff = Eb[:, 1:] - Eb[:, :1]
+    d = np.sqrt((diff ** 2).sum(-1))
+    if consts is None:
+        consts = (d.sum(1) / Y.sum(1), np.maximum(1e-3 * d.mean(1), 1e-12))
+    s, c = consts
+    r = np.log(d + c[:, None]) - np.log(s[:, None] * Y + c[:, None])
+    loss = float((r ** 2).mean(1).mean())
+    g = 2 * r / ((d + c[:, None]) * K * B * d)
+    dEb = np.zeros_like(Eb)
+    dEb[:, 1:] = diff * g[:, :, None]
+    dEb[:, 0] = -dEb[:, 1:].sum(1)
+    return loss, dEb.reshape(E.shape), consts

@@ -1,0 +1,187 @@
+"""Direct DP launches (few pairs, one wavefront per pair, no planner) against
+the planned launch and the FP64 oracle.
+
+Bars (written in each assertion):
+  - direct and planned launches of DTW, EDR, LCSS, Frechet, STEDR, STLCSS
+    (FP32 and FP64 modes) are equal bit for bit: every cell follows the
+    same recurrence in the same order whichever lanes hold it;
+  - ERP's boundary sums are reduced over the segment's lanes, so the two
+    launches may differ by an FP64 rounding of the sum: equal within 1e-6
+    relative in FP32 mode, 1e-13 in FP64 mode;
+  - against the oracle: FP32 max relative error <= 1e-5, counts bit-exact,
+    FP64 mode within 1e-13.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.workloads import fixed_lengths, uniform_pairs
+
+pytestmark = pytest.mark.gpu
+
+EPS = 0.01
+DELTA = 60.0
+
+
+def _set_direct(n):
+    from trajsimimeasures_amd import _lib
+    _lib.check(_lib.lib().tsm_tune_set_direct_pairs(n))
+
+
+def _run(measure, d, pa, pb, **kw):
+    from trajsimimeasures_amd import engine
+    counting = measure in ("edr", "lcss", "stedr", "stlcss")
+    out = engine.pairs(measure, d, pa, pb, return_counts=measure in ("edr", "lcss"), **kw)
+    if isinstance(out, tuple):
+        return out[0].cpu().numpy(), out[1].cpu().numpy()
+    return out.cpu().numpy(), (None if not counting else out.cpu().numpy())
+
+
+def _both(measure, d, pa, pb, **kw):
+    """(direct, planned) results of one call."""
+    _set_direct(-1)
+    direct = _run(measure, d, pa, pb, **kw)
+    _set_direct(0)
+    try:
+        planned = _run(measure, d, pa, pb, **kw)
+    finally:
+        _set_direct(-1)
+    return direct, planned
+
+
+def _lens_every_w(wm: int, seed: int):
+    """Pairs whose shorter side falls in every 64-lane W bucket (4 .. wm
+    columns per lane) at and around its boundaries, plus short and unequal
+    pairs."""
+    rng = np.random.default_rng(seed)
+    cols = [1, 2, 3, 63, 64, 65, 200, 255, 256, 257]
+    for w in range(8, wm + 1, 4):
+        cols += [64 * w - 1, 64 * w]
+    cols += list(rng.integers(1, 64 * wm + 1, 24))
+    la = [int(c) + int(rng.integers(0, 40)) for c in cols]
+    lb = list(cols)
+    swap = rng.random(len(la)) < 0.5  # the longer side on either trajectory
+    la2 = [b if s else a for a, b, s in zip(la, lb, swap)]
+    lb2 = [a if s else b for a, b, s in zip(la, lb, swap)]
+    return la2, lb2
+
+
+@pytest.mark.parametrize("measure", ["dtw", "edr", "lcss", "erp", "frechet"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_direct_equals_planned_every_w(gpu, measure, precision):
+    la, lb = _lens_every_w(24, seed=7)
+    print(f"[case] direct vs planned measure={measure} precision={precision} pairs={len(la)}", flush=True)
+    ts, pa, pb = fixed_lengths(la, lb, seed=8)
+    from trajsimimeasures_amd import engine
+    d = engine.DeviceTrajSet(ts)
+    assert ts.lengths.max() <= 64 * 24 + 40
+    g = ts.xy.mean(0)
+    kw = dict(eps=EPS, gpoint=g, precision=precision)
+    (dv, dc), (pv, pc) = _both(measure, d, pa, pb, **kw)
+    if measure == "erp":
+        tol = 1e-6 if precision == "fp32" else 1e-13
+        assert np.max(np.abs(dv - pv) / np.maximum(np.abs(pv), 1e-300)) <= tol
+    else:
+        assert np.array_equal(dv, pv), (measure, np.nonzero(dv != pv)[0][:5])
+    if dc is not None:
+        assert np.array_equal(dc, pc)
+    ref, rcnt = O.pairs(measure, ts.xy, ts.offsets, pa, pb, eps=EPS, gpoint=g)
+    if measure in ("edr", "lcss"):
+        assert np.array_equal(dc, rcnt) and np.array_equal(dv, ref)
+    else:
+        rtol = 1e-5 if precision == "fp32" else 1e-13
+        rel = np.abs(dv - ref) / np.maximum(np.abs(ref), 1e-300)
+        assert rel.max() <= rtol, (measure, precision, float(rel.max()))
+
+
+@pytest.mark.parametrize("measure", ["stedr", "stlcss"])
+def test_direct_equals_planned_st(gpu, measure):
+    """STEDR/STLCSS (16 FP64 columns per lane): 64-lane W of 4 .. 16."""
+    from trajsimimeasures_amd import engine
+    from trajsimimeasures_amd.data import with_timestamps
+    la, lb = _lens_every_w(16, seed=9)
+    print(f"[case] direct vs planned measure={measure} pairs={len(la)}", flush=True)
+    ts, pa, pb = fixed_lengths(la, lb, seed=10)
+    ts = with_timestamps(np.random.default_rng(11), ts)
+    d = engine.DeviceTrajSet(ts)
+    (dv, _), (pv, _) = _both(measure, d, pa, pb, eps=EPS, delta=DELTA)
+    assert np.array_equal(dv, pv)
+    ref, _ = O.pairs_st(measure, ts.xy, ts.t, ts.offsets, pa, pb, EPS, DELTA)
+    assert np.array_equal(dv, ref)
+
+
+@pytest.mark.parametrize("npairs", [1, 64, 512, 2048, 6144])
+def test_direct_batch_sizes(gpu, npairs):
+    """Batches up to the default direct limit (6,144 pairs on MI355X), DTW,
+    EDR, ERP and Frechet against the oracle."""
+    from trajsimimeasures_amd import engine
+    print(f"[case] direct batch npairs={npairs}", flush=True)
+    ts, pa, pb = uniform_pairs(npairs, seed=40 + npairs)
+    d = engine.DeviceTrajSet(ts)
+    g = ts.xy.mean(0)
+    for m in ("dtw", "edr", "erp", "frechet"):
+        got, cnt = _run(m, d, pa, pb, eps=EPS, gpoint=g)
+        ref, rcnt = O.pairs(m, ts.xy, ts.offsets, pa, pb, eps=EPS, gpoint=g)
+        if m == "edr":
+            assert np.array_equal(cnt, rcnt)
+        else:
+            assert np.max(np.abs(got - ref) / ref) <= 1e-5, m
+
+
+def test_direct_bad_indices(gpu):
+    """Pair indices outside the set: NaN and count -1 in a direct launch,
+    as in a planned one; the other pairs are computed."""
+    from trajsimimeasures_amd import engine
+    ts, pa, pb = uniform_pairs(40, seed=12)
+    d = engine.DeviceTrajSet(ts)
+    pa = pa.copy()
+    pb = pb.copy()
+    pa[3] = -1
+    pb[7] = ts.n
+    pa[11] = ts.n + 5
+    bad = np.zeros(pa.size, bool)
+    bad[[3, 7, 11]] = True
+    for direct in (-1, 0):
+        _set_direct(direct)
+        try:
+            v, c = _run("edr", d, pa, pb, eps=EPS, validate=False)
+        finally:
+            _set_direct(-1)
+        assert np.all(np.isnan(v[bad])) and np.all(c[bad] == -1), direct
+        ref, rcnt = O.pairs("edr", ts.xy, ts.offsets, pa[~bad], pb[~bad], eps=EPS)
+        assert np.array_equal(c[~bad], rcnt), direct
+
+
+def test_direct_long_set_takes_planner(gpu):
+    """A set longer than 64 x 24 points needs column strips, which only the
+    planned launch has: a small call on it is planned, and correct."""
+    from trajsimimeasures_amd import engine
+    ts, pa, pb = fixed_lengths([1700, 30, 200], [1600, 40, 1650], seed=13)
+    d = engine.DeviceTrajSet(ts)
+    got, _ = _run("dtw", d, pa, pb)
+    ref, _ = O.pairs("dtw", ts.xy, ts.offsets, pa, pb)
+    assert np.max(np.abs(got - ref) / ref) <= 1e-5
+
+
+def test_direct_graph_replay(gpu):
+    """A captured direct launch (512 pairs) replays bit-identical to eager."""
+    import torch
+
+    from trajsimimeasures_amd import engine
+    ts, pa, pb = uniform_pairs(512, seed=14)
+    d = engine.DeviceTrajSet(ts)
+    pa_dev = torch.as_tensor(pa, dtype=torch.int32, device="cuda")
+    pb_dev = torch.as_tensor(pb, dtype=torch.int32, device="cuda")
+    eager = _run("dtw", d, pa, pb)[0]
+    g = engine.PairsGraph("dtw", d, pa_dev, pb_dev)
+    for rep in range(3):
+        g.out.fill_(-1.0)
+        g.stream.synchronize()
+        torch.cuda.synchronize()
+        g.launch()
+        g.stream.synchronize()
+        assert np.array_equal(g.out.cpu().numpy(), eager), rep
+        x = torch.randn(1 << 18, device="cuda").sum()
+        torch.cuda.synchronize()
+        del x
+    g.close()

@@ -382,3 +382,19 @@ def test_filtered_knn_small_recall(gpu):
     assert F.recall_at(found, truth) == 1.0
     found, _ = F.filtered_knn(index, 0, nq, k=10, C=200)
     assert F.recall_at(found, truth) > 0.8
+
+
+def test_dist_loss_kernel_matches_reference(gpu):
+    """The distance-regression loss kernel against its FP64 restatement
+    (itself pinned by finite differences): loss and dE within 1e-4."""
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    rng = np.random.default_rng(5)
+    for B, K in ((4, 100), (3, 7), (2, 1024)):
+        E = rng.standard_normal((B * (1 + K), F.D)).astype(np.float32)
+        Y = rng.uniform(0.01, 5.0, (B, K)).astype(np.float32)
+        loss, dE = F.dist_loss(torch.as_tensor(E, device="cuda"), torch.as_tensor(Y, device="cuda"), B, K)
+        lr_, dEr, _ = R.dist_loss(E.astype(np.float64), Y.astype(np.float64), B, K)
+        assert abs(float(loss.mean()) - lr_) <= 1e-4 * abs(lr_) + 1e-7, (B, K)
+        assert _rel(dE.cpu().numpy().astype(np.float64), dEr) < 1e-4, (B, K)

@@ -51,7 +51,8 @@ def test_graph_replays_match_eager(gpu, measure):
 
 def test_autotune_small(gpu):
     from trajsimimeasures_amd import autotune, engine
-    t = autotune.tune(npairs_per_bucket=1024, waves=(4, 8), ncol_max=64, workload_pairs=4096, verbose=False)
+    t = autotune.tune(npairs_per_bucket=1024, waves=(4, 8), ncol_max=64, workload_pairs=4096, verbose=False,
+                      direct_sizes=(256, 1024))
     # every bucket timed every lane count that holds its columns, and reports the model's margin
     for prec in ("fp32", "fp64"):
         rows = t["buckets"][prec]
@@ -65,6 +66,11 @@ def test_autotune_small(gpu):
         if key in t:
             assert all(autotune.fits(24, t[key][nc], nc) for nc in range(1, 257))
     assert t["waves_per_cu"] in (4, 8)
+    # the direct-launch crossover: the largest timed batch size at which direct won (0: none)
+    assert set(t["direct"]) == {"256", "1024"} and t["direct_pairs"] in (0, 256, 1024)
+    if t["direct_pairs"] > 0:
+        r = t["direct"][str(t["direct_pairs"])]
+        assert r["direct_ms"] < r["planned_ms"]
     ts, pa, pb = uniform_pairs(1500, seed=2)
     d = engine.DeviceTrajSet(ts)
     for m in ("dtw", "lcss"):
@@ -78,6 +84,7 @@ def test_autotune_small(gpu):
     _lib.check(_lib.lib().tsm_tune_set_lanes(32, None, 0))
     _lib.check(_lib.lib().tsm_tune_set_lanes(64, None, 0))
     _lib.check(_lib.lib().tsm_tune_set_waves_per_cu(0))
+    _lib.check(_lib.lib().tsm_tune_set_direct_pairs(-1))
     autotune.load_shipped()
 
 

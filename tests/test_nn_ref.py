@@ -52,3 +52,27 @@ def test_forward_zero_head_is_resampled_l2():
     d_e = np.linalg.norm(E[0] - E[1])
     d_x = np.linalg.norm(X[0].reshape(-1) - X[1].reshape(-1))
     assert abs(d_e - d_x) < 1e-12
+
+
+def test_dist_loss_finite_differences():
+    """d loss / d E of the distance regression by central differences in E
+    (s and c frozen, as the kernel holds them), and the closed form of a
+    perfect fit: embedding distances proportional to the labels give 0."""
+    rng = np.random.default_rng(4)
+    B, K = 3, 6
+    E = rng.standard_normal((B * (1 + K), R.D))
+    Y = rng.uniform(0.5, 3.0, (B, K))
+    loss, dE, consts = R.dist_loss(E, Y, B, K)
+    h = 1e-6
+    for _ in range(40):
+        i, j = rng.integers(0, E.shape[0]), rng.integers(0, R.D)
+        e = np.zeros_like(E)
+        e[i, j] = h
+        fd = (R.dist_loss(E + e, Y, B, K, consts)[0] - R.dist_loss(E - e, Y, B, K, consts)[0]) / (2 * h)
+        assert abs(fd - dE[i, j]) <= 1e-6 * max(1.0, abs(fd)) + 1e-10, (i, j, fd, dE[i, j])
+    # anchors at 0, candidates at distance 2 y_k along axis 0: s = 2, residual 0
+    E0 = np.zeros((B * (1 + K), R.D))
+    for b in range(B):
+        E0[b * (1 + K) + 1: (b + 1) * (1 + K), 0] = 2.0 * Y[b]
+    l0, dE0, _ = R.dist_loss(E0, Y, B, K)
+    assert l0 < 1e-24 and np.abs(dE0).max() < 1e-12

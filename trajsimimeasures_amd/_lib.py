@@ -32,13 +32,13 @@ EXPORTS = ("tsm_init", "tsm_last_error", "tsm_abi_version", "tsm_params_default"
            "tsm_xy_to_fp32_centred",
            "tsm_pairs", "tsm_pairs_noalloc", "tsm_pairs_grid", "tsm_pairs_triu", "tsm_pairs_host", "tsm_pairs_padded_host",
            "tsm_tune_set_lanes",
-           "tsm_tune_set_waves_per_cu", "tsm_tune_set_wide_lanes", "tsm_graph_capture_pairs", "tsm_graph_capture_grid",
+           "tsm_tune_set_waves_per_cu", "tsm_tune_set_wide_lanes", "tsm_tune_set_direct_pairs", "tsm_graph_capture_pairs", "tsm_graph_capture_grid",
            "tsm_graph_capture_triu", "tsm_graph_launch", "tsm_graph_destroy",
            "tsm_profile_enable", "tsm_profile_read")
 # include/tsm_nn.h (argument types are declared by filter.py)
 NN_EXPORTS = ("tsm_nn_gemm_nt", "tsm_nn_cast_bf16", "tsm_nn_resample", "tsm_nn_conv_fwd", "tsm_nn_conv_bwd",
               "tsm_nn_gru_h0", "tsm_nn_gru_h0_bwd", "tsm_nn_gru_step_fwd", "tsm_nn_gru_step_bwd", "tsm_nn_colsum",
-              "tsm_nn_rank_loss", "tsm_nn_adam", "tsm_nn_rownorm_bf16", "tsm_nn_topk_f32", "tsm_nn_topk_f64",
+              "tsm_nn_rank_loss", "tsm_nn_dist_loss", "tsm_nn_adam", "tsm_nn_rownorm_bf16", "tsm_nn_topk_f32", "tsm_nn_topk_f64",
               "tsm_nn_linear_f32", "tsm_nn_split_bf16", "tsm_nn_gemm_nt_filter", "tsm_nn_select_candidates",
               "tsm_nn_gemm_nt_acc", "tsm_nn_gru_forward", "tsm_nn_kth_smallest_f32")
 # include/tsm_cluster.h
@@ -93,6 +93,7 @@ def _declare(L):
     L.tsm_kmedoids_fasterpam.argtypes = [_P, _I64, ctypes.c_int32, _P, ctypes.c_int32, _P, _P, _P, _P, _P]
     L.tsm_tune_set_waves_per_cu.argtypes = [ctypes.c_int]
     L.tsm_tune_set_wide_lanes.argtypes = [ctypes.c_int]
+    L.tsm_tune_set_direct_pairs.argtypes = [ctypes.c_int64]
     L.tsm_graph_capture_pairs.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _P, _P, _I64,
                                           ctypes.POINTER(ParamsC), _P, _P, _P, ctypes.POINTER(ctypes.c_void_p)]
     L.tsm_graph_capture_grid.argtypes = [ctypes.c_int, ctypes.POINTER(TrajSetC), _I64, _I64, _I64, _I64,

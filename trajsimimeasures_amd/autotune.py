@@ -12,10 +12,14 @@ the FP32-coordinate kernels and for the FP64 ones (precision="fp64"), which
 keep separate tables.  Then it times the persistent kernel at several
 resident-wave counts per CU, and finally keeps a measured lanes table only
 if it beats the cost model on whole workloads: the 100,000-pair batch with
-lengths uniform on 20..200 and one with lognormal lengths (median 60).  The
+lengths uniform on 20..200 and one with lognormal lengths (median 60).
+Last, it times batches of 2,048 to 12,288 pairs (a caller's batch loop,
+planner included) as direct launches (one wavefront per pair, no planner)
+and as planned ones, and keeps the largest size at which direct wins.  The
 DP kernel uses no LDS, so there is no LDS footprint to tune.  Tables go to
-tsm_tune_set_lanes / tsm_tune_set_waves_per_cu and can be saved as JSON;
-the package applies tuning_gfx950.json at import when it is present.
+tsm_tune_set_lanes / tsm_tune_set_waves_per_cu / tsm_tune_set_direct_pairs
+and can be saved as JSON; the package applies tuning_gfx950.json at import
+when it is present.
 Reference counterpart: the fixed 64 threads per pair of config.py:45 /
 method_helper.py:69-70.
 """
@@ -73,6 +77,8 @@ def apply(table: dict) -> None:
             _lib.check(L.tsm_tune_set_lanes(bits, arr, 257))
     if table.get("waves_per_cu"):
         _lib.check(L.tsm_tune_set_waves_per_cu(int(table["waves_per_cu"])))
+    if table.get("direct_pairs") is not None:
+        _lib.check(L.tsm_tune_set_direct_pairs(int(table["direct_pairs"])))
 
 
 def load_shipped() -> dict | None:
@@ -126,18 +132,89 @@ def _workloads(rng, npairs):
 
 
 def tune(npairs_per_bucket: int = 65536, seed: int = 0, precisions=("fp32", "fp64"), waves=(4, 8, 12, 16),
-         ncol_max: int = NCOL_MAX, workload_pairs: int = 100_000, verbose: bool = True) -> dict:
+         ncol_max: int = NCOL_MAX, workload_pairs: int = 100_000, verbose: bool = True,
+         direct_sizes=(2048, 4096, 6144, 8192, 12288)) -> dict:
     """Measure and apply a tuning table; returns it (with the per-bucket
-    timings under "buckets" and the whole-workload comparison under
-    "workloads")."""
+    timings under "buckets", the whole-workload comparison under
+    "workloads", and the batch sizes up to which direct launches win under
+    "direct_pairs" / "direct")."""
     import torch
 
-    from . import engine
     rng = np.random.default_rng(seed)
     L = _lib.lib()
     table = {"gpu": torch.cuda.get_device_name(0), "bucket": BUCKET, "measure": "dtw", "buckets": {},
              "workloads": {}}
     measured = {}
+    # every timing below is of planned launches (a small call would
+    # otherwise be a direct one, whatever its lanes table)
+    _lib.check(L.tsm_tune_set_direct_pairs(0))
+    try:
+        _tune_planned(table, measured, rng, L, precisions, waves, npairs_per_bucket, ncol_max, workload_pairs,
+                      verbose)
+        table["direct_pairs"], table["direct"] = _direct_crossover(rng, direct_sizes, verbose)
+    finally:
+        _lib.check(L.tsm_tune_set_direct_pairs(int(table.get("direct_pairs", -1))))
+    return table
+
+
+def _call_ms(measure, dset, pa, pb, nb, reps=3):
+    """Mean time of one call over nb consecutive batches of pa.numel() // nb
+    pairs (planner and DP kernels, as a caller's batch loop runs them), best
+    of reps, from events on the current stream."""
+    import torch
+
+    from . import engine
+    B = pa.numel() // nb
+    out = torch.empty(nb * B, dtype=torch.float64, device=pa.device)
+
+    def run():
+        for b in range(nb):
+            sl = slice(b * B, (b + 1) * B)
+            engine.pairs(measure, dset, pa[sl], pb[sl], validate=False, out=out[sl])
+
+    run()
+    best = None
+    for _ in range(reps):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        run()
+        e1.record()
+        e1.synchronize()
+        t = e0.elapsed_time(e1) / nb
+        best = t if best is None else min(best, t)
+    return best
+
+
+def _direct_crossover(rng, sizes, verbose):
+    """Largest batch size at which a direct launch beats the planned one on
+    20..200-point DTW pairs (0 if none does), with the timings per size."""
+    import torch
+
+    from . import engine
+    from .data import lengths_uniform, random_walks
+    L = _lib.lib()
+    n = max(sizes) * 8
+    d = engine.DeviceTrajSet(random_walks(rng, lengths_uniform(rng, 2 * n, 20, 200)))
+    pa = torch.arange(0, 2 * n, 2, dtype=torch.int32, device="cuda")
+    best, rows = 0, {}
+    for N in sizes:
+        nb = n // N
+        _lib.check(L.tsm_tune_set_direct_pairs(N))
+        t_d = _call_ms("dtw", d, pa[:nb * N], pa[:nb * N] + 1, nb)
+        _lib.check(L.tsm_tune_set_direct_pairs(0))
+        t_p = _call_ms("dtw", d, pa[:nb * N], pa[:nb * N] + 1, nb)
+        rows[str(N)] = {"direct_ms": round(t_d, 5), "planned_ms": round(t_p, 5)}
+        if t_d < t_p:
+            best = N
+        if verbose:
+            print(f"[tune] batch of {N} pairs: direct {t_d:.4f} ms, planned {t_p:.4f} ms", flush=True)
+    return best, rows
+
+
+def _tune_planned(table, measured, rng, L, precisions, waves, npairs_per_bucket, ncol_max, workload_pairs, verbose):
+    import torch
+
+    from . import engine
     for prec in precisions:
         bits = 32 if prec == "fp32" else 64
         wm = WMAX[bits]
@@ -196,7 +273,6 @@ def tune(npairs_per_bucket: int = 65536, seed: int = 0, precisions=("fp32", "fp6
             _lib.check(L.tsm_tune_set_lanes(bits, arr, 257))
         if verbose:
             print(f"[tune] {prec} workloads: {res} -> {table['workloads'][prec]['kept']}", flush=True)
-    return table
 
 
 def save(table: dict, path: str) -> None:

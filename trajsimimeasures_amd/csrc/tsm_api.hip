@@ -519,6 +519,13 @@ int tsm_tune_set_wide_lanes(int lanes) {
     return TSM_OK;
 }
 
+int tsm_tune_set_direct_pairs(int64_t max_pairs) {
+    if (max_pairs < -1) return fail(TSM_E_ARG, "direct pairs must be -1 (default), 0 (never) or a pair count");
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_ctx.direct_pairs = max_pairs;
+    return TSM_OK;
+}
+
 }  // extern "C"
 
 // A captured batch: the planner and kernel launches of one call, replayable

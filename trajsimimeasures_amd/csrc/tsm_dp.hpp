@@ -57,32 +57,43 @@ constexpr int kTabMax = 257;    // bucket table covers column counts 1..256
 #endif
 template <typename CT> struct WMaxOf { static constexpr int value = sizeof(CT) == 4 ? TSM_WMAX32 : 24; };
 
-// Buckets (L lanes per pair, W register columns per lane):
-//   b < 4: L = 64, W = WM - 4 b (24, 20, 16, 12 at WM = 24), for pairs wider
-//     than 16 * WM columns in launches small enough that their longest
-//     pairs, not the total work, set the time (a pair's steps run one after
-//     another; a wavefront takes a quarter of the steps of 16 lanes, and the
-//     smallest W that holds the pair the fewest cells per step): a pair
-//     takes the smallest W with 64 W >= its columns, b = 0 above 64 * WM, in
-//     strips of 64 * WM columns;
-//   4 <= b < 4 + WM: L = 16, W = WM - (b - 4) (b = 4 also takes the pairs
-//     wider than the table otherwise, in strips of 16 * WM columns);
-//   4 + WM <= b < 4 + 3 WM: b' = b - 4 - WM, L = (b' odd ? 4 : 8),
+// Buckets (L lanes per pair, W register columns per lane), with N64 = WM / 4
+// 64-lane buckets:
+//   b < N64: L = 64, W = WM - 4 b (24, 20, ..., 4 at WM = 24).  A pair on a
+//     whole wavefront takes a quarter of the steps of 16 lanes and the
+//     smallest W that holds it the fewest cells per step, which is what
+//     counts wherever a pair's own chain of steps, not the total work, sets
+//     the time: pairs wider than 16 * WM columns in launches of at most
+//     2^20 pairs (W the smallest with 64 W >= its columns, W = WM beyond
+//     64 * WM in strips of 64 * WM columns), and every pair of a direct
+//     launch (few pairs, one wavefront each, no planner);
+//   N64 <= b < N64 + WM: L = 16, W = WM - (b - N64) (b = N64 also takes the
+//     pairs wider than the table otherwise, in strips of 16 * WM columns);
+//   N64 + WM <= b < N64 + 3 WM: b' = b - N64 - WM, L = (b' odd ? 4 : 8),
 //     W = WM - b' / 2;
-//   4 + 3 WM <= b: L = 2, W = WM / 2 - (b - 4 - 3 WM) (short columns).
+//   N64 + 3 WM <= b: L = 2, W = WM / 2 - (b - N64 - 3 WM) (short columns).
 // Tasks run in bucket order, so the widest pairs start first.
-__host__ __device__ constexpr int bucket_count(int WM) { return 4 + 3 * WM + WM / 2; }
+__host__ __device__ constexpr int bucket_n64(int WM) { return WM / 4; }
+__host__ __device__ constexpr int bucket_count(int WM) { return WM / 4 + 3 * WM + WM / 2; }
 __host__ __device__ constexpr int bucket_of(int WM, int L, int W) {
     return L == 64   ? (WM - W) / 4
-           : L == 16 ? 4 + WM - W
-           : L == 2  ? 4 + 3 * WM + (WM / 2 - W)
-                     : 4 + WM + 2 * (WM - W) + (L == 4 ? 1 : 0);
+           : L == 16 ? WM / 4 + WM - W
+           : L == 2  ? WM / 4 + 3 * WM + (WM / 2 - W)
+                     : WM / 4 + WM + 2 * (WM - W) + (L == 4 ? 1 : 0);
 }
 __host__ __device__ constexpr int bucket_L(int WM, int b) {
-    return b < 4 ? 64 : b < 4 + WM ? 16 : b < 4 + 3 * WM ? (((b - 4 - WM) & 1) ? 4 : 8) : 2;
+    return b < WM / 4 ? 64 : b < WM / 4 + WM ? 16 : b < WM / 4 + 3 * WM ? (((b - WM / 4 - WM) & 1) ? 4 : 8) : 2;
 }
 __host__ __device__ constexpr int bucket_W(int WM, int b) {
-    return b < 4 ? WM - 4 * b : b < 4 + WM ? WM - (b - 4) : b < 4 + 3 * WM ? WM - (b - 4 - WM) / 2 : WM / 2 - (b - 4 - 3 * WM);
+    return b < WM / 4            ? WM - 4 * b
+           : b < WM / 4 + WM     ? WM - (b - WM / 4)
+           : b < WM / 4 + 3 * WM ? WM - (b - WM / 4 - WM) / 2
+                                 : WM / 2 - (b - WM / 4 - 3 * WM);
+}
+// The 64-lane bucket of a pair with nc columns: the smallest W (a multiple
+// of 4) with 64 W >= nc, W = WM beyond 64 * WM (strips).
+__host__ __device__ constexpr int bucket_w64(int WM, int64_t nc) {
+    return nc > 64 * (int64_t)WM ? 0 : (WM - 4 * (int)((nc + 255) / 256)) / 4;
 }
 // Largest W of an L-lane bucket.
 __host__ __device__ constexpr int bucket_wmax(int WM, int L) { return L == 2 ? WM / 2 : WM; }
@@ -95,13 +106,12 @@ struct BucketTable {
     uint8_t b[kTabMax];  // used-bucket number of a pair whose shorter side has ncol columns, 1 <= ncol <= ncmax
     uint8_t lanes[kMaxBuckets];  // L of used bucket k
     uint8_t kid[kMaxBuckets];    // (L, W) variant of used bucket k (bucket_L / bucket_W)
-    uint8_t w64[4];      // with wide64: used-bucket number of L = 64, W = 24, 20, 16, 12 (WM - 4 i)
+    uint8_t w64[8];      // with wide64: used-bucket number of L = 64, W = WM - 4 i (i < WM / 4)
     int nb;              // used buckets
     int ncmax;           // largest column count the table covers
     int wide;            // used bucket for ncol > ncmax (16 lanes, W = WM; strips when wider than wide16)
     int wide16;          // with wide64: the column count up to which `wide` still applies (16 * WM)
-    int wm;              // with wide64: WM; a wider pair takes w64[(WM - W) / 4] with W the smallest
-                         // multiple of 4 (>= 12) with 64 W >= its columns, W = WM beyond 64 * WM
+    int wm;              // with wide64: WM; a wider pair takes w64[bucket_w64(WM, ncol)]
     int wide64;          // nonzero: pairs wider than wide16 take 64-lane segments
 };
 
@@ -111,7 +121,9 @@ struct DPArgs {
     const int64_t *offs;
     double gx, gy;           // ERP: the gap point g
     PairSpace ps;
-    const uint32_t *slots;   // pair slots grouped by task, kEmptySlot = idle segment
+    const uint32_t *slots;   // pair slots grouped by task, kEmptySlot = idle segment; null: a direct
+                             // launch (wave w computes pair w on 64 lanes, no planner ran)
+    int64_t nset;            // direct launch: trajectories in the set (pair indices outside it give NaN)
     const int64_t *meta;     // [0] ntasks, [1 + b] task end of used bucket b, [1 + nb + b] its slot base
     int nb;                  // number of used buckets
     uint8_t kid[kMaxBuckets];  // (L, W) variant of used bucket b
@@ -445,7 +457,8 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
     const int j = lane % L;
     const bool lane0 = j == 0;
 
-    const uint32_t slot = a.slots[slot0 + seg];
+    // a direct launch passes the pair itself (one segment per wave)
+    const uint32_t slot = a.slots ? a.slots[slot0 + seg] : (uint32_t)slot0;
     const bool valid = slot != kEmptySlot;
     int64_t ia = 0, ib = 0;
     if (valid) decode_pair(a.ps, (int64_t)slot, ia, ib);
@@ -475,9 +488,7 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
         if ((M == kEDR || M == kLCSS || M == kSTEDR || M == kSTLCSS) && a.counts) a.counts[slot] = -1;
     }
     const int nstrips = (valid && fits) ? (ncol + kSpan - 1) / kSpan : 0;
-    const int maxrow = __builtin_amdgcn_readfirstlane(wave_max(nstrips > 0 ? nrow : 0));
     const int maxstrips = __builtin_amdgcn_readfirstlane(wave_max(nstrips));
-    const int nsteps = maxrow + L - 1;
     // (2-lane segments hold at most WM columns: never strips, no scratch)
     AT *bnd = L >= 4 ? reinterpret_cast<AT *>(a.bnd) + (wave_id * kMaxSeg + seg) * a.bnd_stride : nullptr;
     const int rlast = nrow > 0 ? nrow - 1 : 0;
@@ -485,6 +496,11 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
     for (int strip = 0; strip < maxstrips; ++strip) {
         const bool in_strip = strip < nstrips;
         const int col0 = strip * kSpan + j * W;  // first column (0-based) of this lane
+        // a segment's lanes past its last column in this strip compute
+        // nothing that is read: the sweep ends when the last lane holding a
+        // column has done its last row
+        const int used = in_strip ? min(L, (ncol - strip * kSpan + W - 1) / W) : 0;
+        const int nsteps = __builtin_amdgcn_readfirstlane(wave_max(in_strip ? nrow + used - 1 : 0));
         CT bx[W], by[W], bt[W];
         AT gb[W], A[W], B[W];
 #pragma unroll
@@ -566,7 +582,6 @@ constexpr int kPlanZeroWaves = 16;
 // Persistent kernel: each wavefront pulls tasks until the queue is drained.
 template <int M, bool F64> __device__ __forceinline__ void dp_body(const DPArgs &a) {
     constexpr int WM = Rec<M, F64>::kWMax;
-    const int64_t ntasks = a.meta[0];
     const int64_t wave_id = blockIdx.x;
     const int lane = threadIdx.x;
     // the planner that ran before this kernel left its histogram and fill
@@ -574,22 +589,50 @@ template <int M, bool F64> __device__ __forceinline__ void dp_body(const DPArgs 
     const int nz = gridDim.x < kPlanZeroWaves ? (int)gridDim.x : kPlanZeroWaves;
     if ((int)blockIdx.x < nz)
         for (int i = blockIdx.x * kWave + lane; i < a.plan_zero_n; i += nz * kWave) a.plan_zero[i] = 0u;
+    const bool direct = a.slots == nullptr;
+    bool first = true;
+    // one call site of the bucket dispatch (every (L, W) variant is inlined
+    // there): a direct launch passes through the loop once, a persistent
+    // wave until the task queue is drained
     while (true) {
-        unsigned int t = 0;
-        if (lane == 0) t = atomicAdd(a.counter, 1u);
-        t = __builtin_amdgcn_readfirstlane(t);
-        if ((int64_t)t >= ntasks) break;
-        // bucket of task t: first b with task_end[b] > t (binary search)
-        int lo = 0, hi = a.nb - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi) >> 1;
-            if (a.meta[1 + mid] > (int64_t)t) hi = mid;
-            else lo = mid + 1;
+        int kid;
+        int64_t slot0;
+        if (direct) {
+            // this wave's pair on all 64 lanes, in the 64-lane bucket of its
+            // column count (the host chose this mode only for sets whose
+            // pairs need no column strips)
+            if (!first) break;
+            first = false;
+            int64_t ia = 0, ib = 0;
+            decode_pair(a.ps, wave_id, ia, ib);
+            if (ia < 0 || ia >= a.nset || ib < 0 || ib >= a.nset) {
+                if (lane == 0) {
+                    a.out[wave_id] = __builtin_bit_cast(double, 0x7ff8000000000000ull);  // quiet NaN
+                    if ((M == kEDR || M == kLCSS || M == kSTEDR || M == kSTLCSS) && a.counts) a.counts[wave_id] = -1;
+                }
+                break;
+            }
+            const int64_t na = a.offs[ia + 1] - a.offs[ia], nb = a.offs[ib + 1] - a.offs[ib];
+            const int64_t nc = na < nb ? na : nb;
+            kid = __builtin_amdgcn_readfirstlane(bucket_w64(WM, nc < 1 ? 1 : nc));
+            slot0 = wave_id;
+        } else {
+            unsigned int t = 0;
+            if (lane == 0) t = atomicAdd(a.counter, 1u);
+            t = __builtin_amdgcn_readfirstlane(t);
+            if ((int64_t)t >= a.meta[0]) break;
+            // bucket of task t: first b with task_end[b] > t (binary search)
+            int lo = 0, hi = a.nb - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi) >> 1;
+                if (a.meta[1 + mid] > (int64_t)t) hi = mid;
+                else lo = mid + 1;
+            }
+            const int bk = __builtin_amdgcn_readfirstlane(lo);
+            kid = a.kid[bk];
+            const int64_t tstart = bk > 0 ? a.meta[bk] : 0;
+            slot0 = a.meta[1 + a.nb + bk] + ((int64_t)t - tstart) * (kWave / bucket_L(WM, kid));
         }
-        const int bk = __builtin_amdgcn_readfirstlane(lo);
-        const int kid = a.kid[bk];
-        const int64_t tstart = bk > 0 ? a.meta[bk] : 0;
-        const int64_t slot0 = a.meta[1 + a.nb + bk] + ((int64_t)t - tstart) * (kWave / bucket_L(WM, kid));
         dispatch_bucket<M, F64, WM, 0>(a, kid, slot0, wave_id);
     }
 }
@@ -606,5 +649,13 @@ __global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(TSM_DP_WA
     dp_body<M, F64>(a);
 }
 template <int M, bool F64> __global__ __launch_bounds__(kWave) void dp_kernel_wide(DPArgs a) { dp_body<M, F64>(a); }
+// STEDR / STLCSS (W <= 16 FP64 columns with timestamps) fit 256 VGPRs
+// without spilling: held there, two waves share each SIMD (measured 0.56 ms
+// per 100k pairs against 0.78 ms at the 272 registers the compiler picks
+// unbounded, one wave per SIMD).
+template <int M, bool F64>
+__global__ __launch_bounds__(kWave) __attribute__((amdgpu_waves_per_eu(2))) void dp_kernel_st(DPArgs a) {
+    dp_body<M, F64>(a);
+}
 
 }  // namespace tsm

@@ -115,6 +115,8 @@ struct Context {
     bool tuned[2] = {false, false};
     int waves_per_cu = 8;  // resident waves per CU of the persistent DP kernel (autotuned)
     int wide_lanes = 0;    // lanes for pairs wider than the bucket table: 0 = by launch size, 16 or 64
+    int64_t direct_pairs = -1;  // direct DP launches up to this many pairs: -1 = kDirectRounds * num_cu *
+                                // waves_per_cu, 0 = never
     // ordering of the shared workspace across streams: the last call's
     // stream and an event recorded after its kernels
     hipEvent_t ws_event = nullptr;

@@ -69,7 +69,10 @@ inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint, int wide
     if (max_len > t.ncmax) {
         used[wide_id] = true;
         if (t.wide64 && max_len > t.wide16)
-            for (int i = 0; i < 4; ++i) used[i] = true;
+            for (int64_t nc = t.wide16 + 1; nc <= max_len; nc = ((nc - 1) / 256 + 1) * 256 + 1) {  // each 64-lane W
+                used[bucket_w64(WM, nc)] = true;
+                if (nc > 64 * (int64_t)WM) break;
+            }
     }
     int order[kMaxBuckets], n = 0;
     for (int b = 0; b < bucket_count(WM); ++b)
@@ -94,11 +97,16 @@ inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint, int wide
     for (int nc = 1; nc <= t.ncmax; ++nc) t.b[nc] = used[var[nc]] ? num[var[nc]] : 0;
     t.b[0] = t.b[1];
     t.wide = used[wide_id] ? num[wide_id] : 0;
-    for (int i = 0; i < 4; ++i) t.w64[i] = used[i] ? num[i] : 0;
+    for (int i = 0; i < bucket_n64(WM); ++i) t.w64[i] = used[i] ? num[i] : 0;
     return t;
 }
 
 constexpr int64_t kSmallLaunchPairs = 1 << 20;
+// Direct launches by default up to three rounds of the persistent grid's
+// resident waves: measured on MI355X with batches of 20..200-point pairs
+// (DTW, EDR, ERP, Frechet), a direct launch takes 0.40-0.45x the planned
+// one's time at 512 pairs, 0.70-0.76x at 4,096 and 1.09-1.17x at 8,192.
+constexpr int64_t kDirectRounds = 3;
 
 template <int M, bool F64>
 int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace &ps, int64_t npairs, double thr,
@@ -106,7 +114,46 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     using CT = typename Rec<M, F64>::CT;
     using AT = typename Rec<M, F64>::AT;
     constexpr int WM = Rec<M, F64>::kWMax;
-    static_assert(bucket_count(WM) <= kMaxBuckets && WM % 4 == 0 && WM >= 16, "bucket count");
+    static_assert(bucket_count(WM) <= kMaxBuckets && WM % 4 == 0 && WM >= 16 && WM / 4 <= 8, "bucket count");
+    void (*kern)(DPArgs);
+    if constexpr (Rec<M, F64>::kTime) kern = dp_kernel_st<M, F64>;
+    else if constexpr (F64) kern = dp_kernel_wide<M, F64>;
+    else kern = dp_kernel<M, F64>;
+
+    DPArgs a{};
+    a.xy32 = set.xy32;
+    a.xy64 = set.xy64;
+    a.offs = set.offsets;
+    a.gx = gx;
+    a.gy = gy;
+    a.ps = ps;
+    a.nset = set.n;
+    a.thr = thr;
+    a.t64 = set.t64;
+    a.delta = delta;
+    a.thr_lo = 0.0f;
+    a.thr_hi = 0.0f;
+    if constexpr (Rec<M, F64>::kBand) {
+        if (!count_band(thr, set.coord_max, a.thr_lo, a.thr_hi))
+            return fail(TSM_E_ARG, "internal: FP32 match test chosen without a valid guard band");
+    }
+    a.out = out;
+    a.counts = counts;
+
+    // Few pairs: one wavefront per pair, no planner (a launch that size
+    // lasts as long as its longest pair's chain of steps, and the planner's
+    // two kernels would cost about as much as the pairs themselves).
+    const int64_t direct_max = g_ctx.direct_pairs >= 0
+                                   ? g_ctx.direct_pairs
+                                   : kDirectRounds * (int64_t)g_ctx.num_cu * g_ctx.waves_per_cu;
+    if (lanes_hint == 0 && npairs <= direct_max && set.max_len <= 64 * (int64_t)WM) {
+        a.slots = nullptr;  // direct: wave w computes pair w
+        const ProfEvents pe;
+        hipExtLaunchKernelGGL(kern, dim3((unsigned)npairs), dim3(kWave), 0, st, pe.start, pe.stop, 0, a);
+        TSM_HIP(hipGetLastError());
+        return TSM_OK;
+    }
+
     // A launch of few pairs lasts as long as its longest pair's chain of
     // steps: there, pairs wider than the table take a whole wavefront (a
     // quarter of the steps of 16 lanes, at more lane-steps per pair); in a
@@ -122,9 +169,6 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
         (rc = c.get(c.meta, (1 + 2 * NB) * 8)) || (rc = c.get(c.counter, 8)))
         return rc;
 
-    void (*kern)(DPArgs);
-    if constexpr (F64) kern = dp_kernel_wide<M, F64>;
-    else kern = dp_kernel<M, F64>;
     int grid = persistent_grid((npairs + 3) / 4 + NB);
     int64_t bnd_stride = 0;
     if (set.max_len > tab.ncmax) {  // some pair may need column strips
@@ -157,13 +201,6 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     pa.nb = NB;
     pa.tab = tab;
 
-    DPArgs a;
-    a.xy32 = set.xy32;
-    a.xy64 = set.xy64;
-    a.offs = set.offsets;
-    a.gx = gx;
-    a.gy = gy;
-    a.ps = ps;
     a.slots = pa.slots;
     a.meta = pa.meta;
     a.nb = NB;
@@ -171,17 +208,6 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     a.counter = pa.counter;
     a.bnd = c.bnd.p;
     a.bnd_stride = bnd_stride;
-    a.thr = thr;
-    a.t64 = set.t64;
-    a.delta = delta;
-    a.thr_lo = 0.0f;
-    a.thr_hi = 0.0f;
-    if constexpr (Rec<M, F64>::kBand) {
-        if (!count_band(thr, set.coord_max, a.thr_lo, a.thr_hi))
-            return fail(TSM_E_ARG, "internal: FP32 match test chosen without a valid guard band");
-    }
-    a.out = out;
-    a.counts = counts;
     a.plan_zero = pa.hist;
     a.plan_zero_n = 2 * kMaxBins;
 

@@ -905,6 +905,77 @@ __global__ __launch_bounds__(256) void rank_loss_kernel(const float *E, int B, i
     }
 }
 
+// Distance regression over B anchor lists of K candidates with exact
+// distances Y [B, K] (the reference's trainer fits embedding distances to the
+// ground-truth ones, nn/MLP_trajsimi_train.py:78-82), in log space so near
+// and far pairs weigh alike.  Per list, with d_k = |e_k - e_a|:
+//   loss = (1/K) sum_k (log(d_k + c) - log(s y_k + c))^2,
+// s = sum_k d_k / sum_k y_k held constant (the embedding keeps its own
+// units), c = 1e-3 * mean d_k.  dE carries the 1/B of the mean over lists.
+__global__ __launch_bounds__(256) void dist_loss_kernel(const float *E, const float *Y, int B, int K, int D,
+                                                        float *dE, float *loss) {
+    const int b = blockIdx.x;
+    const float *ea = E + (int64_t)b * (1 + K) * D;
+    const float *y = Y + (int64_t)b * K;
+    __shared__ float dist[1024];
+    __shared__ float g[1024];
+    __shared__ float red[2][256];
+    float pd = 0.f, py = 0.f;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        const float *ek = ea + (int64_t)(1 + k) * D;
+        float s = 0.f;
+        for (int q = 0; q < D; ++q) {
+            const float t = ek[q] - ea[q];
+            s = fmaf(t, t, s);
+        }
+        dist[k] = sqrtf(s);
+        pd += dist[k];
+        py += y[k];
+    }
+    red[0][threadIdx.x] = pd;
+    red[1][threadIdx.x] = py;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) {
+            red[0][threadIdx.x] += red[0][threadIdx.x + o];
+            red[1][threadIdx.x] += red[1][threadIdx.x + o];
+        }
+        __syncthreads();
+    }
+    const float sd = red[0][0], sy = red[1][0];
+    const float sc = sy > 0.f ? sd / sy : 0.f;
+    const float c = fmaxf(1e-3f * sd / (float)K, 1e-12f);
+    __syncthreads();
+    float lsum = 0.f;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        const float r = logf(dist[k] + c) - logf(sc * y[k] + c);
+        lsum += r * r;
+        // dL/dd_k, then / d_k for the direction (e_k - e_a)
+        g[k] = dist[k] > 0.f ? 2.f * r / ((dist[k] + c) * (float)K * (float)B * dist[k]) : 0.f;
+    }
+    red[0][threadIdx.x] = lsum;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[0][threadIdx.x] += red[0][threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss[b] = red[0][0] / (float)K;
+    __syncthreads();
+    // dE_k = (e_k - e_a) g_k ; dE_a = -sum_k dE_k
+    float *dea = dE + (int64_t)b * (1 + K) * D;
+    for (int k = threadIdx.x; k < K; k += blockDim.x) {
+        const float *ek = ea + (int64_t)(1 + k) * D;
+        float *dek = dea + (int64_t)(1 + k) * D;
+        for (int q = 0; q < D; ++q) dek[q] = (ek[q] - ea[q]) * g[k];
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < D; q += blockDim.x) {
+        float s = 0.f;
+        for (int k = 0; k < K; ++k) s += dea[(int64_t)(1 + k) * D + q];
+        dea[q] = -s;
+    }
+}
+
 // Adam on a flat FP32 parameter vector (bias-corrected, no weight decay).
 __global__ void adam_kernel(float *p, const float *g, float *m, float *v, int64_t n, float lr, float b1, float b2,
                             float eps, float c1, float c2) {
@@ -1487,6 +1558,13 @@ int tsm_nn_rank_loss(const float *E, int B, int K, int D, int P, int Q0, float t
     if (B < 1 || K < 2 || K > 1024 || D < 1 || P < 1 || Q0 < P || Q0 >= K || !(tau > 0.f) || !E || !dE || !loss)
         return fail(TSM_E_ARG, "rank_loss: need 1 <= P <= Q0 < K <= 1024, tau > 0");
     hipLaunchKernelGGL(rank_loss_kernel, dim3(B), dim3(256), 0, TSM_ST, E, B, K, D, P, Q0, tau, dE, loss);
+    TSM_LAUNCHED();
+}
+
+int tsm_nn_dist_loss(const float *E, const float *Y, int B, int K, int D, float *dE, float *loss, void *stream) {
+    if (B < 1 || K < 1 || K > 1024 || D < 1 || !E || !Y || !dE || !loss)
+        return fail(TSM_E_ARG, "dist_loss: need B >= 1, 1 <= K <= 1024, D >= 1");
+    hipLaunchKernelGGL(dist_loss_kernel, dim3(B), dim3(256), 0, TSM_ST, E, Y, B, K, D, dE, loss);
     TSM_LAUNCHED();
 }
 

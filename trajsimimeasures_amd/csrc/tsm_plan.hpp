@@ -57,11 +57,7 @@ __device__ __forceinline__ int bin_of(const PlanArgs &a, int64_t na, int64_t nb)
     int b;
     if (ncol <= a.tab.ncmax) b = (int)a.tab.b[ncol];
     else if (!a.tab.wide64 || ncol <= a.tab.wide16) b = a.tab.wide;
-    else {
-        int w = (int)((ncol + 255) / 256) * 4;  // 4 * ceil(ncol / 256): the smallest multiple of 4 with 64 w >= ncol
-        w = w < 12 ? 12 : w;
-        b = a.tab.w64[w >= a.tab.wm ? 0 : (a.tab.wm - w) / 4];
-    }
+    else b = a.tab.w64[bucket_w64(a.tab.wm, ncol)];
     const int rk = (int)((nrow < 511 ? nrow : 511) * kRowKeys / 512);
     return b * kRowKeys + (kRowKeys - 1 - rk);
 }

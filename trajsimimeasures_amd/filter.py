@@ -69,6 +69,7 @@ def _nn():
         L.tsm_nn_gru_step_bwd.argtypes = [P, P, P, P, P, P, i, i, P, I64, P, P, P, P]
         L.tsm_nn_colsum.argtypes = [P, I64, i, I64, P, f, P]
         L.tsm_nn_rank_loss.argtypes = [P, i, i, i, i, i, f, P, P, P]
+        L.tsm_nn_dist_loss.argtypes = [P, P, i, i, i, P, P, P]
         L.tsm_nn_adam.argtypes = [P, P, P, P, I64, f, f, f, f, I64, P]
         L.tsm_nn_rownorm_bf16.argtypes = [P, I64, i, P, P]
         L.tsm_nn_topk_f32.argtypes = [P, I64, I64, I64, i, P, P, P]
@@ -81,7 +82,7 @@ def _nn():
         L.tsm_nn_gru_forward.argtypes = [P, i, i, i, P, P, P, P, P, P, P, P, P, i, P]
         L.tsm_nn_kth_smallest_f32.argtypes = [P, I64, I64, I64, i, P, P]
         for n in ("gemm_nt", "cast_bf16", "resample", "conv_fwd", "conv_bwd", "gru_h0", "gru_h0_bwd",
-                  "gru_step_fwd", "gru_step_bwd", "colsum", "rank_loss", "adam", "rownorm_bf16", "topk_f32",
+                  "gru_step_fwd", "gru_step_bwd", "colsum", "rank_loss", "dist_loss", "adam", "rownorm_bf16", "topk_f32",
                   "topk_f64", "linear_f32", "split_bf16", "gemm_nt_filter", "select_candidates",
                   "gemm_nt_acc", "gru_forward", "kth_smallest_f32"):
             getattr(L, "tsm_nn_" + n).restype = ctypes.c_int
@@ -297,6 +298,15 @@ def rank_loss(E: torch.Tensor, B: int, K: int, P: int = 10, Q0: int = 20, tau: f
     return loss, dE
 
 
+def dist_loss(E: torch.Tensor, Y: torch.Tensor, B: int, K: int):
+    """Log-space regression of anchor-candidate embedding distances on the
+    exact ones (Y [B, K] FP32); returns (per-list loss, dE)."""
+    dE = torch.empty_like(E)
+    loss = torch.empty(B, dtype=torch.float32, device=E.device)
+    _lib.check(_nn().tsm_nn_dist_loss(_p(E), _p(Y), B, K, D, _p(dE), _p(loss), _s()))
+    return loss, dE
+
+
 def topk_smallest(V: torch.Tensor, k: int):
     """Indices (int32) and values of the k smallest entries of each row."""
     rows, cols = V.shape
@@ -496,10 +506,14 @@ def hard_pairs(enc: Encoder, dset: DeviceTrajSet, anchors, pool_ids, K: int = 20
 
 def train_encoder(enc: Encoder, dset: DeviceTrajSet, anchors, cands, labels, epochs: int = 20, batch: int = 32,
                   lr: float = 1e-3, tau: float = 0.05, seed: int = 0, log=None, callback=None, P: int = 10,
-                  Q0: int = 20):
-    """Adam on the ranking loss over anchor lists (cands sorted by label).
+                  Q0: int = 20, loss: str = "rank"):
+    """Adam over anchor lists (cands sorted by label) on the ranking loss
+    (``loss="rank"``) or the log-space distance regression (``"dist"``).
     ``callback(epoch)`` runs after every epoch; returns the mean loss per epoch."""
+    if loss not in ("rank", "dist"):
+        raise ValueError("loss must be 'rank' or 'dist'")
     NA, K = cands.shape
+    y32 = labels.to(torch.float32).contiguous() if loss == "dist" else None
     opt = Adam(NPARAM, enc.device, lr=lr)
     g = torch.Generator(device="cpu").manual_seed(seed)
     hist = []
@@ -512,11 +526,14 @@ def train_encoder(enc: Encoder, dset: DeviceTrajSet, anchors, cands, labels, epo
             ids = torch.cat([anchors[sel][:, None], cands[sel]], dim=1).reshape(-1).contiguous()
             X, lenf = enc.resample(dset, ids, ids.numel())
             E = enc.forward(X, lenf, keep=True)
-            loss, dE = rank_loss(E, batch, K, P=P, Q0=Q0, tau=tau)
+            if loss == "dist":
+                lb, dE = dist_loss(E, y32[sel].contiguous(), batch, K)
+            else:
+                lb, dE = rank_loss(E, batch, K, P=P, Q0=Q0, tau=tau)
             grad = enc.backward(dE)
             opt.update(enc.flat, grad)
             enc.refresh()
-            tot += float(loss.mean())
+            tot += float(lb.mean())
             nb += 1
         hist.append(tot / max(nb, 1))
         if log:
@@ -554,27 +571,36 @@ def random_pairs(dset: DeviceTrajSet, anchors, pool_ids, K: int, seed: int = 0):
 # 2,000 anchors x 100 neighbours, lr 3e-4, tau 0.1, negatives from rank 15:
 # recall@10 at C = 100 candidates 0.956 against 0.938 untrained (C = 50:
 # 0.907 / 0.883).  1,000 x 200 at lr 1e-3, tau 0.05, negatives from rank 20
-# gave 0.944; lr 3e-3 or 8,000 x 25 anchors lost recall; random pairs give
-# lists whose top 10 are not near neighbours at all, and training on them
-# drops recall@10 to ~0.2.
+# gave 0.944; lr 3e-3 or 8,000 x 25 anchors lost recall.  Random pairs (the
+# reference's training data) give lists whose best entries are nowhere near
+# a trajectory's 10 nearest of 1,000,000: trained on them, by the ranking
+# loss or by the log-space distance regression (lr 3e-4 to 3e-3), recall@10
+# at C = 100 falls to 0.06-0.10, and the regression on nearest lists to
+# 0.80: only a loss on the near-neighbour order keeps and improves it.
 TRAIN_ANCHORS, TRAIN_K, TRAIN_LR, TRAIN_TAU, TRAIN_P, TRAIN_Q0, TRAIN_EPOCHS = 2000, 100, 3e-4, 0.1, 10, 15, 15
 
 
 def train_filter(dset: DeviceTrajSet, db_ids, n_anchors: int = TRAIN_ANCHORS, K: int = TRAIN_K,
                  epochs: int = TRAIN_EPOCHS, lr: float = TRAIN_LR, tau: float = TRAIN_TAU, batch: int = 32,
-                 seed: int = 0, log=None, P: int = TRAIN_P, Q0: int = TRAIN_Q0, sampling: str = "nearest"):
+                 seed: int = 0, log=None, P: int = TRAIN_P, Q0: int = TRAIN_Q0, sampling: str = "nearest",
+                 loss: str | None = None):
     """The filter's training run, all on the device: n_anchors database
     trajectories drawn at random, each with K database trajectories --
     its K nearest by the untrained (resampled-L2) embedding
     (``sampling="nearest"``) or K drawn at random (``"random"``) -- labelled
     with exact DTW by the DP kernel (n_anchors * K distances), then
     ``epochs`` of Adam on the ranking loss (the P best of each list pushed
-    ahead of every candidate ranked Q0 or worse).  Returns (encoder, info)."""
+    ahead of every candidate ranked Q0 or worse; ``loss="rank"``, the
+    default with nearest lists) or on the log-space regression of embedding
+    distances on the labels (``"dist"``, the default with random lists, as
+    the reference fits random pairs' distances).  Returns (encoder, info)."""
     db_ids = torch.as_tensor(db_ids, dtype=torch.int32, device="cuda")
     if db_ids.numel() <= K:
         raise ValueError("the database must hold more than K trajectories")
     if sampling not in ("nearest", "random"):
         raise ValueError("sampling must be 'nearest' or 'random'")
+    if loss is None:
+        loss = "rank" if sampling == "nearest" else "dist"
     rng = np.random.default_rng(seed)
     pick = torch.as_tensor(rng.choice(db_ids.numel(), n_anchors, replace=False), device="cuda")
     anchors = db_ids[pick]
@@ -588,8 +614,8 @@ def train_filter(dset: DeviceTrajSet, db_ids, n_anchors: int = TRAIN_ANCHORS, K:
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     hist = train_encoder(enc, dset, A, Cn, Y, epochs=epochs, batch=batch, lr=lr, tau=tau, seed=seed, log=log, P=P,
-                         Q0=Q0)
+                         Q0=Q0, loss=loss)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
-    return enc, dict(labels=int(Cn.numel()), anchors=n_anchors, K=K, epochs=epochs, sampling=sampling,
+    return enc, dict(labels=int(Cn.numel()), anchors=n_anchors, K=K, epochs=epochs, sampling=sampling, loss=loss,
                      label_s=t1 - t0, train_s=t2 - t1, loss_first=hist[0], loss_last=hist[-1])
