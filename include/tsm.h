@@ -145,12 +145,12 @@ int tsm_pairs(int measure, const tsm_trajset *set, const int32_t *pa, const int3
  * stream capture, serving loops and harnesses that forbid hipMalloc).  No
  * counterpart in the reference, which allocates per call (core/dtw.py:61-76).
  * Same arguments, kernels and results as tsm_pairs; it calls no hipMalloc,
- * hipFree or allocating routine.  The plan, the column-strip boundaries and
- * the ERP / Hausdorff prologue arrays of one call live in a fixed 32 MiB
- * device array that each device holds for the life of the process; a call
- * needing more (roughly 8 million pairs, fewer with long trajectories or
- * with ERP / Hausdorff over sets of more than ~1.5 million points) fails
- * with TSM_E_ARG before launching anything.  Every call on a device shares
+ * hipFree or allocating routine.  The plan and the column-strip boundaries
+ * of one call live in a fixed 32 MiB device array that each device holds for
+ * the life of the process (a direct launch, tsm_tune_set_direct_pairs, and
+ * the Hausdorff, CDDS and SAR kernels use none of it); a call needing more
+ * (roughly 8 million pairs, fewer with trajectories longer than 256 points)
+ * fails with TSM_E_ARG before launching anything.  Every call on a device shares
  * that array: calls that overlap on one device (other streams, other
  * threads) race and give undefined results.
  */

@@ -2,9 +2,10 @@
 // allocation, for callers that cannot allocate (code under stream capture,
 // serving loops that forbid hipMalloc).
 //
-// The planner's histogram, fill counters, task slots and queue counters, the
-// column-strip boundary vectors and the ERP / Hausdorff prologue arrays of
-// one call are carved from `scratch`, a fixed device array of kNoAllocBytes.
+// The planner's histogram, fill counters, task slots and queue counters and
+// the column-strip boundary vectors of one call are carved from `scratch`, a
+// fixed device array of kNoAllocBytes (a direct DP launch and the Hausdorff,
+// CDDS and SAR kernels take none of it).
 // Each device holds its own copy of the array for the life of the process,
 // at its own address: the host finds it with hipGetSymbolAddress on the
 // entry point's first call on a device (with that device current) and uses
