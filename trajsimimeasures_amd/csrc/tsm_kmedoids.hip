@@ -76,8 +76,10 @@ __device__ __forceinline__ void km_block_partials(int k, bool ok, uint32_t ni, d
 }
 
 // One wavefront: rl[m] = sum over blocks of part[b][m] in block order, loss likewise.
+// With ctl: only after a pending swap, which it then clears.
 __global__ __launch_bounds__(kWave) void km_blocksum_kernel(const double *part, int64_t nblk, int k, double *rl,
-                                                            double *loss) {
+                                                            double *loss, int64_t *pend) {
+    if (pend && *pend < 0) return;
     for (int v = threadIdx.x; v <= k; v += kWave) {
         double s = 0.0;
         constexpr int U = 16;
@@ -92,6 +94,7 @@ __global__ __launch_bounds__(kWave) void km_blocksum_kernel(const double *part, 
         if (v < k) rl[v] = s;
         else *loss = s;
     }
+    if (pend && threadIdx.x == 0) *pend = -1;
 }
 
 __global__ __launch_bounds__(kKmBlock) void km_assign_kernel(const double *D, int64_t n, int k, const int64_t *med,
@@ -122,84 +125,140 @@ __global__ __launch_bounds__(kKmBlock) void km_assign_kernel(const double *D, in
     km_block_partials(k, ok, ni, nd, sd, part);
 }
 
-// Candidates j0 + blockIdx.x: loss change of the best swap bringing j in and
-// the medoid it replaces (+inf for a current medoid).
+// The scan's control block, kept on the device so that the host enqueues
+// whole rounds (evaluate a batch, pick, swap, sums) without waiting on each
+// batch's outcome.  Every kernel of a round reads it; only km_pick_kernel
+// (one wavefront, between the batch's evaluation and the swap) writes it,
+// and km_blocksum_kernel clears the pending swap.
+struct KmCtl {
+    int64_t j;         // first candidate of the next batch
+    int64_t end;       // the pass scans candidates [0, end): the last swap's candidate, n after a swap
+    int64_t batch;     // current batch size
+    int64_t cnt;       // candidates in the next batch
+    int64_t lastswap;  // candidate of the last swap (n before any)
+    int64_t swaps;
+    int64_t iter;      // passes started
+    int64_t before;    // swaps when the current pass started
+    int64_t max_iter;
+    int64_t pend_b;    // pending swap: medoid position (-1: none) ...
+    int64_t pend_c;    // ... and the candidate that replaces it
+    int64_t done;
+};
+constexpr int64_t kKmMinBatch = 512;
+
+// Candidates ctl.j + i, i < ctl.cnt (blocks loop over i): loss change of the
+// best swap bringing candidate j in and the medoid it replaces (+inf for a
+// current medoid).
 __global__ __launch_bounds__(kWave) void km_eval_kernel(const double *D, int64_t n, int k, KmState s,
-                                                        const double *rl, const int64_t *med, int64_t j0,
+                                                        const double *rl, const int64_t *med, const KmCtl *ctl,
                                                         double *change, int32_t *best) {
     extern __shared__ double q[];  // k * 64 doubles, lane-private columns
+    if (ctl->done) return;
     const int l = threadIdx.x;
-    const int64_t j = j0 + blockIdx.x;
-    if (j == med[s.ni[j]]) {
+    const int64_t j0 = ctl->j, cnt = ctl->cnt;
+    for (int64_t i = blockIdx.x; i < cnt; i += gridDim.x) {
+        const int64_t j = j0 + i;
+        if (j == med[s.ni[j]]) {
+            if (l == 0) {
+                change[i] = __builtin_huge_val();
+                best[i] = 0;
+            }
+            continue;
+        }
+        for (int m = 0; m < k; ++m) q[m * kWave + l] = 0.0;
+        const double *row = D + j * n;
+        double acc = 0.0;
+        constexpr int U = 16;  // points per lane whose loads are in flight together
+        for (int64_t base = l; base < n; base += U * kWave) {
+            double dj[U], nd[U], sd[U];
+            uint32_t ni[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {  // unconditional loads (index clamped in bounds; unused past n)
+                const int64_t o = base + u * kWave < n ? base + u * kWave : n - 1;
+                dj[u] = row[o];
+                nd[u] = s.nd[o];
+                sd[u] = s.sd[o];
+                ni[u] = s.ni[o];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                // j would be o's nearest: shared gain, and n(o)'s removal term is void; j would be o's
+                // second: n(o)'s removal term shrinks.  Branch-free: otherwise (and past n) both terms
+                // are +0, which leaves the partials as they are (they are never -0: every term is <= 0).
+                const bool ok = base + u * kWave < n;
+                const bool nearer = ok && dj[u] < nd[u], second = ok && !nearer && dj[u] < sd[u];
+                acc += nearer ? dj[u] - nd[u] : 0.0;
+                q[ni[u] * kWave + l] += nearer ? nd[u] - sd[u] : (second ? dj[u] - sd[u] : 0.0);
+            }
+        }
+        acc = km_tree(acc);
+        double bl = 0.0;
+        int b = 0;
+        for (int m = 0; m < k; ++m) {
+            const double v = rl[m] + km_tree(q[m * kWave + l]);
+            if (m == 0 || v < bl) {
+                bl = v;
+                b = m;
+            }
+        }
         if (l == 0) {
-            change[blockIdx.x] = __builtin_huge_val();
-            best[blockIdx.x] = 0;
+            change[i] = bl + acc;
+            best[i] = b;
         }
-        return;
-    }
-    for (int m = 0; m < k; ++m) q[m * kWave + l] = 0.0;
-    const double *row = D + j * n;
-    double acc = 0.0;
-    constexpr int U = 16;  // points per lane whose loads are in flight together
-    for (int64_t base = l; base < n; base += U * kWave) {
-        double dj[U], nd[U], sd[U];
-        uint32_t ni[U];
-#pragma unroll
-        for (int u = 0; u < U; ++u) {  // unconditional loads (index clamped in bounds; unused past n)
-            const int64_t o = base + u * kWave < n ? base + u * kWave : n - 1;
-            dj[u] = row[o];
-            nd[u] = s.nd[o];
-            sd[u] = s.sd[o];
-            ni[u] = s.ni[o];
-        }
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-            // j would be o's nearest: shared gain, and n(o)'s removal term is void; j would be o's
-            // second: n(o)'s removal term shrinks.  Branch-free: otherwise (and past n) both terms
-            // are +0, which leaves the partials as they are (they are never -0: every term is <= 0).
-            const bool ok = base + u * kWave < n;
-            const bool nearer = ok && dj[u] < nd[u], second = ok && !nearer && dj[u] < sd[u];
-            acc += nearer ? dj[u] - nd[u] : 0.0;
-            q[ni[u] * kWave + l] += nearer ? nd[u] - sd[u] : (second ? dj[u] - sd[u] : 0.0);
-        }
-    }
-    acc = km_tree(acc);
-    double bl = 0.0;
-    int b = 0;
-    for (int m = 0; m < k; ++m) {
-        const double v = rl[m] + km_tree(q[m * kWave + l]);
-        if (m == 0 || v < bl) {
-            bl = v;
-            b = m;
-        }
-    }
-    if (l == 0) {
-        change[blockIdx.x] = bl + acc;
-        best[blockIdx.x] = b;
     }
 }
 
-// First candidate of the batch whose change is negative: res = {index or -1, medoid}.
-__global__ __launch_bounds__(kWave) void km_pick_kernel(const double *change, const int32_t *best, int64_t count,
-                                                        int64_t *res) {
+// The first candidate of the batch whose change is negative becomes the
+// pending swap (its medoid position updated in med); then the scan moves on
+// exactly as the sequential algorithm's: behind the swap with a halved batch,
+// or past the batch with a doubled one; a pass that reaches its end starts
+// the next (ending at the last swap) unless it made no swap or max_iter
+// passes ran.
+__global__ __launch_bounds__(kWave) void km_pick_kernel(const double *change, const int32_t *best, int64_t n,
+                                                        int64_t *med, KmCtl *ctl) {
+    if (ctl->done) return;
     const int l = threadIdx.x;
-    for (int64_t c0 = 0; c0 < count; c0 += kWave) {
+    const int64_t j0 = ctl->j, cnt = ctl->cnt;
+    int64_t f = -1;
+    for (int64_t c0 = 0; c0 < cnt; c0 += kWave) {
         const int64_t c = c0 + l;
-        const bool neg = c < count && change[c] < 0.0;
+        const bool neg = c < cnt && change[c] < 0.0;
         const uint64_t mask = __builtin_amdgcn_ballot_w64(neg);
         if (mask) {
-            const int64_t f = c0 + __builtin_ctzll(mask);
-            if (l == 0) {
-                res[0] = f;
-                res[1] = best[f];
-            }
-            return;
+            f = c0 + __builtin_ctzll(mask);
+            break;
         }
     }
-    if (l == 0) {
-        res[0] = -1;
-        res[1] = 0;
+    if (l != 0) return;
+    const int64_t minb = kKmMinBatch < n ? kKmMinBatch : n;
+    KmCtl c = *ctl;
+    if (f >= 0) {
+        const int64_t cand = j0 + f;
+        const int b = best[f];
+        med[b] = cand;
+        c.pend_b = b;
+        c.pend_c = cand;
+        ++c.swaps;
+        c.lastswap = cand;
+        c.end = n;
+        c.j = cand + 1;
+        c.batch = c.batch / 2 > minb ? c.batch / 2 : minb;
+    } else {
+        c.j += cnt;
+        c.batch = 2 * c.batch < n ? 2 * c.batch : n;
     }
+    while (!c.done && c.j >= c.end) {  // the pass is over
+        if (c.swaps == c.before || c.iter >= c.max_iter) {
+            c.done = 1;
+        } else {
+            ++c.iter;
+            c.before = c.swaps;
+            c.end = c.lastswap;
+            c.j = 0;
+        }
+    }
+    c.cnt = c.done ? 0 : (c.batch < c.end - c.j ? c.batch : c.end - c.j);
+    *ctl = c;
 }
 
 __device__ __forceinline__ void km_second(const double *D, int64_t n, int k, const int64_t *med, uint32_t nn,
@@ -221,7 +280,10 @@ __device__ __forceinline__ void km_second(const double *D, int64_t n, int k, con
 // Medoid b is now point j (med already updated): every point's nearest and
 // second nearest.
 __global__ __launch_bounds__(kKmBlock) void km_swap_kernel(const double *D, int64_t n, int k, const int64_t *med,
-                                                            KmState s, uint32_t b, int64_t j, double *part) {
+                                                            KmState s, const KmCtl *ctl, double *part) {
+    if (ctl->pend_b < 0) return;
+    const uint32_t b = (uint32_t)ctl->pend_b;
+    const int64_t j = ctl->pend_c;
     const int64_t o = blockIdx.x * (int64_t)kKmBlock + threadIdx.x;
     const bool ok = o < n;
     uint32_t ni = 0, si = 0;
@@ -296,9 +358,15 @@ __global__ void square_from_triu_kernel(const double *tri, int64_t n, double *D)
 }
 
 struct KmBuffers {
-    DevBuf ni, si, nd, sd, rl, loss, med, change, best, res, sums, part;
-    int64_t *h_pin = nullptr;  // pinned: [0, 2) batch outcome, [2] loss, [8, 8 + 64) medoids
+    DevBuf ni, si, nd, sd, rl, loss, med, change, best, ctl, sums, part;
+    int64_t *h_pin = nullptr;  // pinned: [0, 16) the control block, [16] loss, [32, 32 + 64) medoids
 };
+constexpr int kKmPinWords = 96;
+static_assert(sizeof(KmCtl) <= 16 * sizeof(int64_t), "control block in the pinned words [0, 16)");
+
+// Rounds enqueued between two reads of the control block: a round after the
+// scan ended costs four no-op launches, a read one host round trip.
+constexpr int kKmRoundsPerCheck = 16;
 std::mutex g_km_mu;
 KmBuffers g_km;
 
@@ -343,10 +411,11 @@ extern "C" int tsm_kmedoids_fasterpam(const double *diss, int64_t n, int32_t k, 
     std::lock_guard<std::mutex> lk(g_km_mu);
     KmBuffers &w = g_km;
     int rc;
-    if ((rc = w.med.ensure(sizeof(int64_t) * k)) || (rc = w.res.ensure(2 * sizeof(int64_t)))) return rc;
-    if (!w.h_pin) TSM_HIP(hipHostMalloc((void **)&w.h_pin, 80 * sizeof(int64_t), hipHostMallocDefault));
-    int64_t *h_res = w.h_pin, *h_med = w.h_pin + 8;
-    double *h_lossp = reinterpret_cast<double *>(w.h_pin + 2);
+    if ((rc = w.med.ensure(sizeof(int64_t) * k)) || (rc = w.ctl.ensure(sizeof(KmCtl)))) return rc;
+    if (!w.h_pin) TSM_HIP(hipHostMalloc((void **)&w.h_pin, kKmPinWords * sizeof(int64_t), hipHostMallocDefault));
+    KmCtl *h_ctl = reinterpret_cast<KmCtl *>(w.h_pin);
+    int64_t *h_med = w.h_pin + 32;
+    double *h_lossp = reinterpret_cast<double *>(w.h_pin + 16);
     std::vector<int64_t> med(medoids, medoids + k);
     double h_loss = 0.0;
     int iter = 0, swaps = 0;
@@ -389,54 +458,52 @@ extern "C" int tsm_kmedoids_fasterpam(const double *diss, int64_t n, int32_t k, 
         hipLaunchKernelGGL(km_assign_kernel, dim3((unsigned)nblk), dim3(kKmBlock), 0, st, diss, n, (int)k, dmed, s,
                            part);
         hipLaunchKernelGGL(km_blocksum_kernel, dim3(1), dim3(kWave), 0, st, part, nblk, (int)k, w.rl.as<double>(),
-                           w.loss.as<double>());
+                           w.loss.as<double>(), nullptr);
         TSM_HIP(hipGetLastError());
-        const int64_t kMinBatch = 512;
-        int64_t batch = kMinBatch < n ? kMinBatch : n;
-        int64_t lastswap = n;
-        while (iter < max_iter) {
-            ++iter;
-            const int before = swaps;
-            int64_t end = lastswap;  // a pass stops at the candidate of the last swap ...
-            int64_t j = 0;
-            while (j < end) {
-                const int64_t cnt = batch < end - j ? batch : end - j;
-                hipLaunchKernelGGL(km_eval_kernel, dim3((unsigned)cnt), dim3(kWave), lds, st, diss, n, (int)k, s,
-                                   w.rl.as<double>(), dmed, j, w.change.as<double>(), w.best.as<int32_t>());
+        // The scan runs on the device (km_pick_kernel steers it through the
+        // control block): the host enqueues rounds of evaluate / pick / swap /
+        // sums and reads the control block once per kKmRoundsPerCheck rounds
+        // instead of each batch's outcome.
+        KmCtl c0{};
+        c0.j = 0;
+        c0.end = n;
+        c0.batch = kKmMinBatch < n ? kKmMinBatch : n;
+        c0.cnt = c0.batch;
+        c0.lastswap = n;
+        c0.max_iter = max_iter;
+        c0.iter = max_iter > 0 ? 1 : 0;
+        c0.done = max_iter > 0 ? 0 : 1;
+        c0.pend_b = -1;
+        *h_ctl = c0;
+        KmCtl *dctl = w.ctl.as<KmCtl>();
+        TSM_HIP(hipMemcpyAsync(dctl, h_ctl, sizeof(KmCtl), hipMemcpyHostToDevice, st));
+        const int eval_grid = (int)(n < 4096 ? n : 4096);  // blocks loop over a batch's candidates
+        while (true) {
+            for (int r = 0; r < kKmRoundsPerCheck; ++r) {
+                hipLaunchKernelGGL(km_eval_kernel, dim3((unsigned)eval_grid), dim3(kWave), lds, st, diss, n, (int)k, s,
+                                   w.rl.as<double>(), dmed, dctl, w.change.as<double>(), w.best.as<int32_t>());
                 hipLaunchKernelGGL(km_pick_kernel, dim3(1), dim3(kWave), 0, st, w.change.as<double>(),
-                                   w.best.as<int32_t>(), cnt, w.res.as<int64_t>());
-                TSM_HIP(hipGetLastError());
-                TSM_HIP(hipMemcpyAsync(h_res, w.res.p, 2 * sizeof(int64_t), hipMemcpyDeviceToHost, st));
-                TSM_HIP(hipStreamSynchronize(st));
-                if (h_res[0] < 0) {
-                    j += cnt;
-                    batch = 2 * batch < n ? 2 * batch : n;
-                    continue;
-                }
-                const int64_t c = j + h_res[0];
-                const uint32_t b = (uint32_t)h_res[1];
-                ++swaps;
-                lastswap = c;
-                end = n;  // ... unless it made a swap itself
-                med[b] = c;
-                h_med[b] = c;  // the previous copies finished at the last synchronisation
-                TSM_HIP(hipMemcpyAsync(dmed + b, h_med + b, sizeof(int64_t), hipMemcpyHostToDevice, st));
+                                   w.best.as<int32_t>(), n, dmed, dctl);
                 hipLaunchKernelGGL(km_swap_kernel, dim3((unsigned)nblk), dim3(kKmBlock), 0, st, diss, n, (int)k, dmed,
-                                   s, b, c, part);
+                                   s, dctl, part);
                 hipLaunchKernelGGL(km_blocksum_kernel, dim3(1), dim3(kWave), 0, st, part, nblk, (int)k,
-                                   w.rl.as<double>(), w.loss.as<double>());
-                TSM_HIP(hipGetLastError());
-                j = c + 1;
-                batch = batch / 2 > kMinBatch ? batch / 2 : (kMinBatch < n ? kMinBatch : n);
+                                   w.rl.as<double>(), w.loss.as<double>(), &dctl->pend_b);
             }
-            if (swaps == before) break;
+            TSM_HIP(hipGetLastError());
+            TSM_HIP(hipMemcpyAsync(h_ctl, dctl, sizeof(KmCtl), hipMemcpyDeviceToHost, st));
+            TSM_HIP(hipStreamSynchronize(st));
+            if (h_ctl->done) break;
         }
+        iter = (int)h_ctl->iter;
+        swaps = (int)h_ctl->swaps;
+        TSM_HIP(hipMemcpyAsync(h_med, dmed, sizeof(int64_t) * k, hipMemcpyDeviceToHost, st));
         TSM_HIP(hipMemcpyAsync(h_lossp, w.loss.p, sizeof(double), hipMemcpyDeviceToHost, st));
         if (labels)
             hipLaunchKernelGGL(km_labels_kernel, dim3(km_grid(n)), dim3(256), 0, st, w.ni.as<uint32_t>(), n, labels);
         TSM_HIP(hipGetLastError());
         TSM_HIP(hipStreamSynchronize(st));
         h_loss = *h_lossp;
+        for (int i = 0; i < k; ++i) med[i] = h_med[i];
     }
     for (int i = 0; i < k; ++i) medoids[i] = med[i];
     if (loss) *loss = h_loss;
