@@ -462,6 +462,9 @@ def host_leg(ts, pa, pb, dev, reps: int = 3) -> dict:
     `reps` after one warm-up:
     - padded: the reference's boundary, core.dtw(t1, len1, t2, len2) on
       zero-padded float64 [pairs, max length, 2] arrays (core/dtw.py:56-77);
+    - padded in the reference's batches: the same arrays in 512-pair
+      slices (effi_batch_size_gpu, config.py:43), one core.dtw call each, as
+      the reference's pairwise driver loops (test_trajsimi_time.py:222-276);
     - packed: the set uploaded once as one point buffer
       (engine.DeviceTrajSet: FP64 copy + centred FP32 copy) and the pairs
       given as index arrays, as drivers.pairwise does."""
@@ -492,9 +495,17 @@ def host_leg(ts, pa, pb, dev, reps: int = 3) -> dict:
         return min(ts_)
 
     t_pad = best(lambda: core.dtw(t1, la, t2, lb))
+    B = 512
+
+    def batches():
+        for o in range(0, len(pa), B):
+            core.dtw(t1[o:o + B], la[o:o + B], t2[o:o + B], lb[o:o + B])
+
+    t_bat = best(batches)
     t_pack = best(lambda: engine.pairs("dtw", engine.DeviceTrajSet(ts, device=dev), pa, pb).cpu().numpy())
     n = len(pa)
     out = {"padded_pairs_per_s": n / t_pad, "padded_s": t_pad, "padded_bytes": t1.nbytes + t2.nbytes,
+           "padded_512_batches_pairs_per_s": n / t_bat, "padded_512_batches_s": t_bat,
            "packed_pairs_per_s": n / t_pack, "packed_s": t_pack, "packed_bytes": ts.xy.nbytes + ts.offsets.nbytes}
     print(f"[bench] host buffers (PCIe included): {out}", file=sys.stderr, flush=True)
     return out

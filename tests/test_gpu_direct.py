@@ -185,3 +185,40 @@ def test_direct_graph_replay(gpu):
         torch.cuda.synchronize()
         del x
     g.close()
+
+
+@pytest.mark.parametrize("measure", ["dtw", "edr", "erp"])
+def test_direct_grid_and_triangle(gpu, measure):
+    """Grid (7 x 40) and triangle (n = 50) addressing decoded in a direct
+    launch: equal to the planned launch and to the oracle."""
+    from trajsimimeasures_amd import engine
+    from tests.workloads import lognormal_set
+    ts = lognormal_set(90, seed=15, lo=5, hi=300)
+    d = engine.DeviceTrajSet(ts)
+    g = ts.xy.mean(0)
+    counted = measure == "edr"
+    outs = {}
+    for direct in (-1, 0):
+        _set_direct(direct)
+        try:
+            gr = engine.grid(measure, d, 0, 7, 7, 40, eps=EPS, gpoint=g, return_counts=counted)
+            tr = engine.triu(measure, d, 50, eps=EPS, gpoint=g, return_counts=counted)
+        finally:
+            _set_direct(-1)
+        if counted:
+            gr, tr = gr[1], tr[1]
+        outs[direct] = (gr.cpu().numpy().reshape(-1), tr.cpu().numpy().reshape(-1))
+    for a, b in zip(outs[-1], outs[0]):
+        if measure == "erp":
+            assert np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)) <= 1e-6
+        else:
+            assert np.array_equal(a, b)
+    qa = np.repeat(np.arange(7), 40)
+    qb = np.tile(np.arange(7, 47), 7)
+    ia, ib = np.triu_indices(50, 1)
+    for (pa, pb), got in (((qa, qb), outs[-1][0]), ((ia, ib), outs[-1][1])):
+        ref, rcnt = O.pairs(measure, ts.xy, ts.offsets, pa, pb, eps=EPS, gpoint=g)
+        if counted:
+            assert np.array_equal(got, rcnt)
+        else:
+            assert np.max(np.abs(got - ref) / np.maximum(ref, 1e-300)) <= 1e-5

@@ -45,15 +45,28 @@ def _same(a, b):
 
 
 @pytest.mark.parametrize("measure", MEASURES)
-def test_noalloc_equals_tsm_pairs(gpu, measure):
+@pytest.mark.parametrize("direct", [0, -1])
+def test_noalloc_equals_tsm_pairs(gpu, measure, direct):
+    """Two calls in a row, the second smaller.  With direct DP launches off
+    (direct = 0) both DP calls plan through the fixed scratch; by default
+    the smaller one is a direct launch, which needs none of it."""
     import torch
 
     from trajsimimeasures_amd import _lib, engine
     from trajsimimeasures_amd.data import with_timestamps
     L = _lib.lib()
     noalloc = _decl(L)
-    print(f"[case] noalloc {measure}", flush=True)
-    ts, pa, pb = uniform_pairs(6000, seed=61)  # ERP gaps and Hausdorff segments of 1.3M points fit
+    print(f"[case] noalloc {measure} direct={direct}", flush=True)
+    _lib.check(L.tsm_tune_set_direct_pairs(direct))
+    try:
+        _noalloc_two_calls(L, noalloc, measure, torch, engine, with_timestamps)
+    finally:
+        _lib.check(L.tsm_tune_set_direct_pairs(-1))
+
+
+def _noalloc_two_calls(L, noalloc, measure, torch, engine, with_timestamps):
+    from trajsimimeasures_amd import _lib
+    ts, pa, pb = uniform_pairs(8000, seed=61)  # more pairs than a direct launch takes
     ts = with_timestamps(np.random.default_rng(62), ts)
     d = engine.DeviceTrajSet(ts)
     if measure == "sar":

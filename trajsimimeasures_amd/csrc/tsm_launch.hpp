@@ -146,7 +146,7 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
     const int64_t direct_max = g_ctx.direct_pairs >= 0
                                    ? g_ctx.direct_pairs
                                    : kDirectRounds * (int64_t)g_ctx.num_cu * g_ctx.waves_per_cu;
-    if (lanes_hint == 0 && npairs <= direct_max && set.max_len <= 64 * (int64_t)WM) {
+    if (lanes_hint == 0 && npairs <= direct_max && npairs <= 0x7fffffff && set.max_len <= 64 * (int64_t)WM) {
         a.slots = nullptr;  // direct: wave w computes pair w
         const ProfEvents pe;
         hipExtLaunchKernelGGL(kern, dim3((unsigned)npairs), dim3(kWave), 0, st, pe.start, pe.stop, 0, a);
