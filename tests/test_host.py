@@ -320,3 +320,12 @@ def test_autotune_cost_model_mirror():
     assert got == want
     assert [A.model_lanes(16, nc) for nc in (16, 17, 64, 65, 128, 129)] == [2, 4, 4, 8, 8, 16]
     assert all(A.fits(24, A.model_lanes(24, nc), nc) for nc in range(1, 385))
+
+
+def test_dropin_lanes_hint():
+    """threads_per_traj of the reference's drop-in calls: 2, 4, 8, 16 are
+    lanes hints, the reference default 64 (and anything else) leaves the
+    launch shape to the library."""
+    from trajsimimeasures_amd.core._batch import lanes_hint
+    assert [lanes_hint(t) for t in (2, 4, 8, 16)] == [2, 4, 8, 16]
+    assert [lanes_hint(t) for t in (64, 32, 1, 0, None, "x")] == [0] * 6

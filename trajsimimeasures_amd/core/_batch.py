@@ -13,14 +13,16 @@ __all__ = ["run_padded", "lanes_hint"]
 
 
 def lanes_hint(threads_per_traj) -> int:
-    """The reference's block size per pair (config.py:45) is a hint here: 4
-    and 8 select that many lanes per pair (16 maps to 8), anything else
-    (the reference default 64 included) the autotuned table."""
+    """The reference's block size per pair (config.py:45) is a hint here: 2,
+    4, 8 and 16 select that many lanes per pair where they hold the pair's
+    columns; anything else (the reference default 64 included) leaves the
+    launch shape to the library (a direct launch for a small batch, else the
+    planner with the tuned lanes table)."""
     try:
         t = int(threads_per_traj)
     except (TypeError, ValueError):
         return 0
-    return {4: 4, 8: 8, 16: 8}.get(t, 0)
+    return t if t in (2, 4, 8, 16) else 0
 
 
 def run_padded(measure: int, t1, len1, t2, len2, *, eps=0.0, eps_mode="sqrt", gpoint=None, threads_per_traj=64,
