@@ -222,3 +222,28 @@ def test_direct_grid_and_triangle(gpu, measure):
             assert np.array_equal(got, rcnt)
         else:
             assert np.max(np.abs(got - ref) / np.maximum(ref, 1e-300)) <= 1e-5
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_random_launch_shapes(gpu, seed):
+    """Random batch sizes (1 .. 12,000 pairs: direct and planned launches)
+    and lengths (lognormal, median 80, up to 1,700 points: 64-lane buckets,
+    column strips) for a random DP measure, against the oracle."""
+    from trajsimimeasures_amd import engine
+    from trajsimimeasures_amd.data import random_walks
+    rng = np.random.default_rng(100 + seed)
+    npairs = int(np.exp(rng.uniform(0, np.log(12000))))
+    measure = ["dtw", "edr", "lcss", "erp", "frechet"][seed % 5]
+    lens = np.clip(np.round(np.exp(rng.normal(np.log(80), 1.0, 2 * npairs))), 1, 1700).astype(np.int64)
+    ts = random_walks(rng, lens)
+    pa = np.arange(0, 2 * npairs, 2)
+    pb = pa + 1
+    print(f"[case] random shapes seed={seed} measure={measure} pairs={npairs} max_len={lens.max()}", flush=True)
+    d = engine.DeviceTrajSet(ts)
+    g = ts.xy.mean(0)
+    got, cnt = _run(measure, d, pa, pb, eps=EPS, gpoint=g)
+    ref, rcnt = O.pairs(measure, ts.xy, ts.offsets, pa, pb, eps=EPS, gpoint=g)
+    if measure in ("edr", "lcss"):
+        assert np.array_equal(cnt, rcnt)
+    else:
+        assert np.max(np.abs(got - ref) / np.maximum(ref, 1e-300)) <= 1e-5
