@@ -478,7 +478,13 @@ extern "C" int tsm_kmedoids_fasterpam(const double *diss, int64_t n, int32_t k, 
         KmCtl *dctl = w.ctl.as<KmCtl>();
         TSM_HIP(hipMemcpyAsync(dctl, h_ctl, sizeof(KmCtl), hipMemcpyHostToDevice, st));
         const int eval_grid = (int)(n < 4096 ? n : 4096);  // blocks loop over a batch's candidates
+        // every round moves the scan past at least one candidate, so a pass
+        // takes at most n rounds: a scan that has not ended by then is a fault
+        const double round_cap = ((double)max_iter + 1.0) * ((double)n + 1.0) + kKmRoundsPerCheck;
+        double rounds = 0.0;
         while (true) {
+            if (rounds > round_cap) return fail(TSM_E_HIP, "kmedoids: the device scan did not finish");
+            rounds += kKmRoundsPerCheck;
             for (int r = 0; r < kKmRoundsPerCheck; ++r) {
                 hipLaunchKernelGGL(km_eval_kernel, dim3((unsigned)eval_grid), dim3(kWave), lds, st, diss, n, (int)k, s,
                                    w.rl.as<double>(), dmed, dctl, w.change.as<double>(), w.best.as<int32_t>());
