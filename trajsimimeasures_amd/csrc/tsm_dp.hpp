@@ -131,7 +131,7 @@ struct DPArgs {
     void *bnd;               // strip boundary scratch (AT), kMaxSeg segments per wave
     int64_t bnd_stride;
     double thr;              // EDR/LCSS squared-distance threshold
-    float thr_lo, thr_hi;    // EDR/LCSS FP32 guard band: below lo a match, at or above hi none
+    float thr_lo, thr_hi;    // EDR/LCSS FP32 guard band, 0 <= lo <= hi: below lo a match, at or above hi none
     const double *t64;       // STEDR/STLCSS: per-point timestamps
     double delta;            // STEDR/STLCSS: time threshold
     double *out;
@@ -151,6 +151,7 @@ template <bool F64> struct RecDTW {
     static constexpr bool kTime = false;
     static constexpr bool kBand = false;
     static constexpr bool kGap = false;
+    static constexpr bool kIdxBnd = false;  // boundaries are top() / left0()
     __device__ static AT top(AT) { return (AT)__builtin_inf(); }
     __device__ static AT left0(AT) { return (AT)__builtin_inf(); }
     __device__ static AT cell(CT ax, CT ay, AT, CT bx, CT by, AT, AT diag, AT up, AT left, double) {
@@ -168,6 +169,7 @@ template <bool F64> struct RecFrechet {
     static constexpr bool kTime = false;
     static constexpr bool kBand = false;
     static constexpr bool kGap = false;
+    static constexpr bool kIdxBnd = false;  // boundaries are top() / left0()
     __device__ static AT top(AT) { return (AT)__builtin_inf(); }
     __device__ static AT left0(AT) { return (AT)__builtin_inf(); }
     // Squared domain: sqrt is monotone, so max/min commute with it and one
@@ -187,6 +189,7 @@ template <bool F64> struct RecERP {
     static constexpr bool kTime = false;
     static constexpr bool kBand = false;
     static constexpr bool kGap = true;
+    static constexpr bool kIdxBnd = false;  // boundaries are top() / left0()
     __device__ static AT top(AT sb) { return sb; }
     __device__ static AT left0(AT sa) { return sa; }
     __device__ static AT cell(CT ax, CT ay, AT ga, CT bx, CT by, AT gb, AT diag, AT up, AT left, double) {
@@ -197,47 +200,75 @@ template <bool F64> struct RecERP {
 };
 template <bool F64> struct Rec<kERP, F64> : RecERP<F64> {};
 
-// EDR / LCSS: integer counts carried in FP32 (exact below 2^24).  The match
-// decision must equal the FP64 oracle's, (dx*dx + dy*dy) < thr on FP64
-// coordinates, bit for bit.  Two forms:
-//   F64 = true:  the test itself runs on the FP64 coordinates.
+// EDR / LCSS counts.  The match decision must equal the FP64 oracle's,
+// (dx*dx + dy*dy) < thr on FP64 coordinates, bit for bit.  Two forms:
+//   F64 = true (and STEDR/STLCSS): the test itself runs on the FP64
+//     coordinates; counts are carried in FP32 (exact below 2^24) and the
+//     update selects with the compare's lane mask:
+//       EDR  match ? diag : min3(diag, up, left) + 1 (unit-cost edit
+//            distances satisfy diag <= min(up, left) + 1),
+//       LCSS match ? diag + 1 : max(up, left).
 //   F64 = false: the test runs on the FP32 copy against a guard band
 //     [thr_lo, thr_hi) that provably contains every FP32 squared distance
 //     whose FP64 counterpart could fall on the other side of thr (the host
 //     derives it from a bound on |coordinate|); a lane that meets a value
 //     inside the band recomputes its whole row step with the FP64 test.
-template <int M> struct RecCount {
-    using AT = float;
+//     Counts are int32 and the match is a bit, not a lane mask: with
+//     0 <= thr_lo <= thr_hi (as the host guarantees) and d2 >= 0, the FP32
+//     bit patterns order as the values, so t = bits(d2) - bits(thr_lo) has
+//     its sign bit set exactly when d2 < thr_lo, and one unsigned compare
+//     t < bits(thr_hi) - bits(thr_lo) finds the band.  A compare writes a
+//     scalar mask and the chip issues fewer of those than plain VALU
+//     operations (a cell-chain test: 5.35e12 cells/s with the bit against
+//     4.77e12 with two compares and a select, 4.39e12 with a carry-in add);
+//     the update is then an add and one v_max3_i32:
+//       EDR keeps N[i][j] = (i + j) - E[i][j] instead of the edit distance
+//         E.  Substituting E = (i + j) - N into E = min(diag + 1 - match,
+//         up + 1, left + 1) gives N = max(N_diag + 1 + match, N_up,
+//         N_left), exactly.  The reference's E is zero on both boundaries,
+//         so N[0][j] = j and N[i][0] = i; the count is na + nb - N at the
+//         last cell.
+//       LCSS: match ? diag + 1 : max(up, left) equals max(up, left, diag +
+//         match), because LCSS values of prefixes never decrease and grow
+//         by at most one per added point (diag <= up, left <= diag + 1).
+template <int M, typename AT_> struct RecCount {
+    using AT = AT_;
+    static constexpr bool kInt = std::is_same<AT, int32_t>::value;
     static constexpr bool kGap = false;
     static constexpr bool kEdit = M == kEDR || M == kSTEDR;  // EDR form, else LCSS form
-    __device__ static AT top(AT) { return 0.0f; }
-    __device__ static AT left0(AT) { return 0.0f; }
-    // EDR: min(diag + sub, up + 1, left + 1) with sub in {0, 1} equals
-    // (match ? diag : min3(diag, up, left) + 1), since unit-cost edit
-    // distances satisfy diag <= min(up, left) + 1.  LCSS: match ? diag + 1 :
-    // max(up, left).
+    // the int EDR form's N has the boundaries N[0][j] = j, N[i][0] = i
+    // (core/edr.py:30-32 zeroes E's); the others keep zero boundaries
+    static constexpr bool kIdxBnd = kEdit && kInt;
+    __device__ static AT top(AT) { return 0; }
+    __device__ static AT left0(AT) { return 0; }
+    // int forms, m in {0, 1}
+    __device__ static AT pick_bit(int32_t m, AT diag, AT up, AT left) {
+        if constexpr (kEdit) return max(max(diag + 1 + m, up), left);
+        else return max(max(diag + m, up), left);
+    }
     __device__ static AT pick(bool match, AT diag, AT up, AT left) {
-        if constexpr (kEdit) return match ? diag : min3f(diag, up, left) + 1.0f;
+        if constexpr (kInt) return pick_bit((int32_t)match, diag, up, left);
+        else if constexpr (kEdit) return match ? diag : min3f(diag, up, left) + 1.0f;
         else return match ? diag + 1.0f : fmaxf(up, left);
     }
     __device__ static double finish(AT v, int64_t na, int64_t nb, int32_t *cnt) {
-        int32_t c = (int32_t)v;
+        const int32_t c = kIdxBnd ? (int32_t)(na + nb) - (int32_t)v : (int32_t)v;
         if (cnt) *cnt = c;
         if constexpr (kEdit) return (double)c / (double)(na > nb ? na : nb);
         else return 1.0 - (double)c / (double)(na < nb ? na : nb);
     }
 };
-template <int M> struct RecCount64 : RecCount<M> {
+template <int M> struct RecCount64 : RecCount<M, float> {
     using CT = double;
     static constexpr int kWMax = WMaxOf<CT>::value;
     static constexpr bool kTime = false;
     static constexpr bool kBand = false;
     __device__ static float cell(CT ax, CT ay, float, CT bx, CT by, float, float diag, float up, float left,
                                  double thr) {
-        return RecCount<M>::pick(dist2_f64(ax, ay, bx, by) < thr, diag, up, left);
+        return RecCount<M, float>::pick(dist2_f64(ax, ay, bx, by) < thr, diag, up, left);
     }
 };
-template <int M> struct RecCount32 : RecCount<M> {
+template <int M> struct RecCount32 : RecCount<M, int32_t> {
     using CT = float;
     static constexpr int kWMax = WMaxOf<CT>::value;
     static constexpr bool kTime = false;
@@ -246,7 +277,7 @@ template <int M> struct RecCount32 : RecCount<M> {
 // STEDR / STLCSS (core/stedr.py:44-46, core/stlcss.py:41-42): a match also
 // needs |t1 - t2| < delta, tested in FP64 like the spatial part.  Columns
 // carry a third FP64 value, so at most 16 per lane keep the registers.
-template <int M> struct RecST : RecCount<M> {
+template <int M> struct RecST : RecCount<M, float> {
     using CT = double;
     static constexpr int kWMax = 16;
     static constexpr bool kTime = true;
@@ -254,7 +285,7 @@ template <int M> struct RecST : RecCount<M> {
     __device__ static float cell_t(CT ax, CT ay, CT at, CT bx, CT by, CT bt, float diag, float up, float left,
                                    double thr, double delta) {
         const bool m = (dist2_f64(ax, ay, bx, by) < thr) && (fabs(at - bt) < delta);
-        return RecCount<M>::pick(m, diag, up, left);
+        return RecCount<M, float>::pick(m, diag, up, left);
     }
 };
 template <> struct Rec<kSTEDR, true> : RecST<kSTEDR> {};
@@ -328,6 +359,9 @@ template <int L> __device__ __forceinline__ float shl_in(float v) {
     if constexpr (L > 16) return shr1_wave(v);
     else return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), 0x111, 0xF, 0xF, true));
 }
+template <int L> __device__ __forceinline__ int32_t shl_in(int32_t v) {
+    return __builtin_bit_cast(int32_t, shl_in<L>(__builtin_bit_cast(float, v)));
+}
 template <int L> __device__ __forceinline__ double shl_in(double v) {
     if constexpr (L > 16) return shr1_wave(v);
     else return shr1(v);
@@ -373,6 +407,7 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
     if ((uint32_t)r < nrow_s) {
         AT left_in;
         if constexpr (kBnd) left_in = lane0 ? cur_bl : recv;
+        else if constexpr (R::kIdxBnd) left_in = lane0 ? (AT)(r + 1) : recv;  // D[r + 1][0] = r + 1
         else left_in = lane0 ? left_fixed : recv;
         AT diag = lp_in;  // D[r][col0]: the corner value before the first row
         lp_out = left_in;
@@ -383,12 +418,14 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
             // FP32 match test; `amb` collects the lanes that met a squared
             // distance inside the guard band (wave-uniform mask)
             uint64_t amb = 0;
+            const uint32_t lob = __builtin_bit_cast(uint32_t, a.thr_lo);
+            const uint32_t span = __builtin_bit_cast(uint32_t, a.thr_hi) - lob;
 #pragma unroll
             for (int c = 0; c < W; ++c) {
                 const float d2 = dist2_f32(cur.x, cur.y, bx[c], by[c]);
-                const bool m = d2 < a.thr_lo;
-                amb |= __builtin_amdgcn_ballot_w64(d2 < a.thr_hi) & ~__builtin_amdgcn_ballot_w64(m);
-                const AT v = R::pick(m, diag, src[c], lf);
+                const uint32_t t = __builtin_bit_cast(uint32_t, d2) - lob;  // sign bit: d2 < thr_lo
+                amb |= __builtin_amdgcn_ballot_w64(t < span);                // thr_lo <= d2 < thr_hi
+                const AT v = R::pick_bit((int32_t)(t >> 31), diag, src[c], lf);
                 diag = src[c];
                 dst[c] = v;
                 lf = v;
@@ -512,10 +549,10 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
             else gb[c] = 0;
             if constexpr (R::kTime) bt[c] = a.t64[ocol + col];
             else bt[c] = 0;
-            A[c] = topv;
-            B[c] = topv;
+            A[c] = R::kIdxBnd ? (AT)(col0 + c + 1) : topv;  // D[0][col0 + c + 1]
+            B[c] = A[c];
         }
-        AT lpA = (col0 == 0) ? (AT)0 : topv;  // D[0][col0]
+        AT lpA = R::kIdxBnd ? (AT)col0 : (col0 == 0) ? (AT)0 : topv;  // D[0][col0]
         AT lpB = lpA;
         const bool write_bnd = (j == L - 1) && (strip + 1 < nstrips);
         const uint32_t nrow_s = in_strip ? (uint32_t)nrow : 0u;

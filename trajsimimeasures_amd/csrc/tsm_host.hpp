@@ -185,7 +185,9 @@ inline bool finite64(double v) {
 // the band [thr (1 - 3 rel), thr (1 + 3 rel)) rounded outwards to FP32, a
 // value below it is a match and a value at or above it is not, exactly as
 // in FP64.  Returns false when no useful band exists (unknown or huge
-// coordinates relative to eps): then the test runs in FP64.
+// coordinates relative to eps): then the test runs in FP64.  A band has
+// 0 <= lo <= hi (the kernel orders FP32 bit patterns, which holds only for
+// non-negative values).
 // coord_max bounds the FP32 copy's coordinates, which may be taken relative
 // to an origin: fl32(fl64(x - o)).  The FP64 subtraction adds at most
 // 2^-53 |x - o| to the u |x - o| of the FP32 rounding; U is widened by
