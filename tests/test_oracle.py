@@ -192,6 +192,64 @@ def test_edr_short_update_rule(rng):
         assert _edr_short_rule(a, b, thr) == O.count("edr", a, b, eps)
 
 
+def _int_forms(a, b, thr):
+    """EDR and LCSS counts with the FP32 kernel's integer updates:
+    N = max(N_diag + 1 + m, N_up, N_left) on N = (i + j) - E with the
+    boundaries N[0][j] = j, N[i][0] = i (the count is n1 + n2 - N), and
+    L = max(L_up, L_left, L_diag + m)."""
+    n1, n2 = len(a), len(b)
+    N = np.zeros((n1 + 1, n2 + 1), dtype=np.int64)
+    N[0, :] = np.arange(n2 + 1)
+    N[:, 0] = np.arange(n1 + 1)
+    L = np.zeros((n1 + 1, n2 + 1), dtype=np.int64)
+    for i in range(1, n1 + 1):
+        for j in range(1, n2 + 1):
+            d2 = (a[i - 1, 0] - b[j - 1, 0]) ** 2 + (a[i - 1, 1] - b[j - 1, 1]) ** 2
+            m = int(d2 < thr)
+            N[i, j] = max(N[i - 1, j - 1] + 1 + m, N[i - 1, j], N[i, j - 1])
+            L[i, j] = max(L[i - 1, j - 1] + m, L[i - 1, j], L[i, j - 1])
+    return n1 + n2 - int(N[n1, n2]), int(L[n1, n2])
+
+
+def test_int_update_forms(rng):
+    """The FP32 kernel's integer EDR (shifted) and LCSS (max3) updates give
+    the oracle's counts, at match densities from none to all."""
+    for _ in range(300):
+        n1, n2 = rng.integers(1, 12, 2)
+        a, b = rnd(rng, n1), rnd(rng, n2)
+        eps = rng.random() * 1.2
+        thr = O.eps_threshold(eps)
+        e, l = _int_forms(a, b, thr)
+        assert e == O.count("edr", a, b, eps)
+        assert l == O.count("lcss", a, b, eps)
+
+
+def test_fp32_bit_pattern_match_and_band():
+    """For non-negative FP32 d2 and 0 <= lo <= hi: the sign bit of
+    bits(d2) - bits(lo) is d2 < lo, and bits(d2) - bits(lo) < bits(hi) -
+    bits(lo) (unsigned) is lo <= d2 < hi (zero, subnormals, the band's
+    edges, huge values and infinity included)."""
+    f32 = np.float32
+    rng = np.random.default_rng(5)
+    d2 = np.concatenate([
+        np.array([0.0, 1e-45, 1e-40, 1e-38, 6.25e-6, 1.0, 3.4e38, np.inf], dtype=f32),
+        rng.random(20000).astype(f32) * f32(2e-5),
+        (rng.random(2000) * 1e30).astype(f32),
+    ])
+    bands = [(0.0, 0.0), (np.inf, np.inf), (6.2e-6, 6.3e-6), (1e-45, 1e-40), (1.0, 1.0), (0.0, 3.4e38)]
+    for lo, hi in bands:
+        lo, hi = f32(lo), f32(hi)
+        # the band's own edges and their neighbours
+        edges = np.array([lo, hi, np.nextafter(lo, f32(0)), np.nextafter(lo, f32(np.inf)),
+                          np.nextafter(hi, f32(0)), np.nextafter(hi, f32(np.inf))], dtype=f32)
+        x = np.concatenate([d2, edges])
+        xb = x.view(np.uint32)
+        lob, hib = np.array([lo], dtype=f32).view(np.uint32)[0], np.array([hi], dtype=f32).view(np.uint32)[0]
+        t = (xb - lob).astype(np.uint32)
+        assert np.array_equal((t >> 31) == 1, x < lo), (lo, hi)
+        assert np.array_equal(t < np.uint32(hib - lob), (x >= lo) & (x < hi)), (lo, hi)
+
+
 @pytest.mark.parametrize("n1,n2", [(1, 1), (1, 3), (3, 2), (3, 3), (4, 3)])
 def test_st_bruteforce(rng, n1, n2):
     """STEDR / STLCSS (core/stedr.py:44-46, core/stlcss.py:41-42): a cell
