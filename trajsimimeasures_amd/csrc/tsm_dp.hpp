@@ -48,6 +48,7 @@ constexpr int kRowKeys = TSM_ROW_KEYS;
 static_assert(kRowKeys >= 16 && kRowKeys <= 128 && (kRowKeys & (kRowKeys - 1)) == 0, "row keys: power of two <= 128");
 constexpr int kMaxSeg = 16;     // segments per wave at L = 4 (the strip scratch; 2-lane pairs never need strips)
 constexpr int kMaxBuckets = 96;  // the planner's bucket capacity
+constexpr int kLongRowCols16 = 128;  // long-row pairs: 16 lanes up to this many columns, 64 beyond
 constexpr int kTabMax = 257;    // bucket table covers column counts 1..256
 
 // Register columns per lane: 24 with FP32 coordinates (TSM_WMAX32; measured
@@ -107,6 +108,9 @@ struct BucketTable {
     uint8_t lanes[kMaxBuckets];  // L of used bucket k
     uint8_t kid[kMaxBuckets];    // (L, W) variant of used bucket k (bucket_L / bucket_W)
     uint8_t w64[8];      // with wide64: used-bucket number of L = 64, W = WM - 4 i (i < WM / 4)
+    uint8_t w16[32];     // with long_rows: used-bucket number of L = 16, W = w (1 <= w <= kLongRowCols16 / 16)
+    int long_rows;       // nonzero: a pair with more rows and at most wide16 columns takes 16 lanes up to
+                         // kLongRowCols16 columns and 64 beyond (w64[bucket_w64(wm, ncol)])
     int nb;              // used buckets
     int ncmax;           // largest column count the table covers
     int wide;            // used bucket for ncol > ncmax (16 lanes, W = WM; strips when wider than wide16)

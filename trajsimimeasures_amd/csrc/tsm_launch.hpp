@@ -43,10 +43,25 @@ inline int model_lanes(int WM, int nc) {
 // of 16 * WM columns), or with wide_lanes = 64, 16 lanes up to one such
 // strip and a whole wavefront beyond (W = 12, 16, 20 or 24: the smallest
 // that holds the pair, then W = WM in strips of 64 * WM).
+// Long rows: the choice above prices a pair as if its rows were about its
+// columns.  A pair with far more rows is a chain of rows + L - 1 steps of W
+// cells that no other wave can shorten, and in a launch of few pairs such
+// chains outlast the rest of the work (1,000 queries x 100 candidates of
+// lognormal lengths: 0.7 % of the pairs had over 500 rows, and the launch
+// took 0.47 ms against 0.31 ms without them).  So in the launches that use
+// 64-lane segments (no lanes hint), a pair with more than kLongRows rows
+// takes 16 lanes (W = ceil(nc / 16)) up to kLongRowCols16 columns and a
+// whole wavefront (W = 4, 8, ...) beyond: for 900 x 30 points a chain of
+// 915 steps of 2 cells instead of 901 of 15, for 900 x 300 one of 963
+// steps of 8 cells instead of 915 of 19, at most doubling that pair's lane
+// work.
+constexpr int kLongRows = 384;
+
 // Only the variants that the set's lengths (shorter sides up to max_len)
 // can reach become buckets of the launch, numbered in schedule order: the
 // 64-lane ones, then by W and L descending (per step, a task's time grows
-// with W; per task, its steps with L).
+// with W; per task, its steps with L); with long rows, the 16-lane buckets
+// before the narrower ones (they hold the long chains).
 inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint, int wide_lanes, int64_t max_len) {
     BucketTable t{};
     t.ncmax = 16 * WM < kTabMax - 1 ? 16 * WM : kTabMax - 1;
@@ -65,6 +80,12 @@ inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint, int wide
         var[nc] = (uint8_t)bucket_of(WM, L, (nc + L - 1) / L);
         if (nc <= max_len || nc == 1) used[var[nc]] = true;
     }
+    t.long_rows = (t.wide64 && lanes_hint == 0 && max_len > kLongRows) ? kLongRows : 0;
+    if (t.long_rows) {
+        for (int W = 1; W <= kLongRowCols16 / 16 && 16 * (W - 1) < max_len; ++W) used[bucket_of(WM, 16, W)] = true;
+        for (int64_t nc = kLongRowCols16 + 1; nc <= max_len && nc <= t.wide16; nc = ((nc - 1) / 256 + 1) * 256 + 1)
+            used[bucket_w64(WM, nc)] = true;  // each 64-lane W up to wide16 columns
+    }
     const int wide_id = bucket_of(WM, 16, WM);
     if (max_len > t.ncmax) {
         used[wide_id] = true;
@@ -77,9 +98,11 @@ inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint, int wide
     int order[kMaxBuckets], n = 0;
     for (int b = 0; b < bucket_count(WM); ++b)
         if (used[b]) order[n++] = b;
-    auto before = [WM](int x, int y) {
+    const bool l16_first = t.long_rows != 0;
+    auto before = [WM, l16_first](int x, int y) {
         const int lx = bucket_L(WM, x), ly = bucket_L(WM, y);
         if ((lx == 64) != (ly == 64)) return lx == 64;
+        if (l16_first && (lx == 16) != (ly == 16)) return lx == 16;
         if (bucket_W(WM, x) != bucket_W(WM, y)) return bucket_W(WM, x) > bucket_W(WM, y);
         return lx > ly;
     };
@@ -98,6 +121,13 @@ inline BucketTable bucket_table(int WM, int coord_bits, int lanes_hint, int wide
     t.b[0] = t.b[1];
     t.wide = used[wide_id] ? num[wide_id] : 0;
     for (int i = 0; i < bucket_n64(WM); ++i) t.w64[i] = used[i] ? num[i] : 0;
+    if (t.long_rows) {
+        for (int W = 1; W <= kLongRowCols16 / 16; ++W) {
+            const int id = bucket_of(WM, 16, W);
+            t.w16[W] = used[id] ? num[id] : 0;
+        }
+        t.w16[0] = t.w16[1];
+    }
     return t;
 }
 

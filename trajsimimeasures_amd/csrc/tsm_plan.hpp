@@ -55,7 +55,9 @@ __device__ __forceinline__ int bin_of(const PlanArgs &a, int64_t na, int64_t nb)
     const int64_t nrow = na > nb ? na : nb;
     const int64_t ncol = na > nb ? nb : na;
     int b;
-    if (ncol <= a.tab.ncmax) b = (int)a.tab.b[ncol];
+    if (a.tab.long_rows && nrow > a.tab.long_rows && ncol <= a.tab.wide16)
+        b = ncol <= kLongRowCols16 ? (int)a.tab.w16[(ncol + 15) / 16] : (int)a.tab.w64[bucket_w64(a.tab.wm, ncol)];
+    else if (ncol <= a.tab.ncmax) b = (int)a.tab.b[ncol];
     else if (!a.tab.wide64 || ncol <= a.tab.wide16) b = a.tab.wide;
     else b = a.tab.w64[bucket_w64(a.tab.wm, ncol)];
     const int rk = (int)((nrow < 511 ? nrow : 511) * kRowKeys / 512);
