@@ -247,3 +247,49 @@ def test_random_launch_shapes(gpu, seed):
         assert np.array_equal(cnt, rcnt)
     else:
         assert np.max(np.abs(got - ref) / np.maximum(ref, 1e-300)) <= 1e-5
+
+
+@pytest.mark.parametrize("measure", ["dtw", "edr", "lcss", "erp", "frechet", "stedr", "stlcss"])
+def test_long_row_buckets(gpu, measure):
+    """Planned launches of up to 2^20 pairs put pairs with more than 384 rows
+    on 16 lanes (up to 128 columns) or a whole wavefront (up to 384): pairs of
+    385 .. 1,200 rows with columns at every boundary of that rule, against
+    the oracle and, bit for bit (ERP within its boundary-sum rounding),
+    against the same call with a 16-lane hint, which keeps the column-count
+    buckets."""
+    from trajsimimeasures_amd import engine
+    from trajsimimeasures_amd.data import with_timestamps
+    rng = np.random.default_rng(31)
+    cols = [1, 2, 15, 16, 17, 31, 100, 127, 128, 129, 200, 255, 256, 257, 300, 383, 384, 385, 500]
+    rows = [385, 386, 400, 700, 1200]
+    la = [int(r) for r in rows for _ in cols] + [384, 384, 60]
+    lb = [int(c) for _ in rows for c in cols] + [30, 300, 1000]
+    swap = rng.random(len(la)) < 0.5  # the longer side on either trajectory
+    la, lb = [b if s else a for a, b, s in zip(la, lb, swap)], [a if s else b for a, b, s in zip(la, lb, swap)]
+    print(f"[case] long-row buckets measure={measure} pairs={len(la)}", flush=True)
+    ts, pa, pb = fixed_lengths(la, lb, seed=32)
+    st = measure in ("stedr", "stlcss")
+    if st:
+        ts = with_timestamps(np.random.default_rng(33), ts)
+    d = engine.DeviceTrajSet(ts)
+    g = ts.xy.mean(0)
+    kw = dict(eps=EPS, delta=DELTA) if st else dict(eps=EPS, gpoint=g)
+    _set_direct(0)
+    try:
+        got, cnt = _run(measure, d, pa, pb, **kw)
+        hint, hcnt = _run(measure, d, pa, pb, lanes=16, **kw)
+    finally:
+        _set_direct(-1)
+    if measure == "erp":
+        assert np.max(np.abs(got - hint) / np.maximum(np.abs(hint), 1e-300)) <= 1e-6
+    else:
+        assert np.array_equal(got, hint), (measure, np.nonzero(got != hint)[0][:5])
+    if st:
+        ref, _ = O.pairs_st(measure, ts.xy, ts.t, ts.offsets, pa, pb, EPS, DELTA)
+        assert np.array_equal(got, ref)
+    else:
+        ref, rcnt = O.pairs(measure, ts.xy, ts.offsets, pa, pb, eps=EPS, gpoint=g)
+        if measure in ("edr", "lcss"):
+            assert np.array_equal(cnt, rcnt)
+        else:
+            assert np.max(np.abs(got - ref) / np.maximum(ref, 1e-300)) <= 1e-5
