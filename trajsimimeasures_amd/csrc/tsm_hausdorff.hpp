@@ -11,7 +11,7 @@
 // excluded from the max.
 //
 // Mapping: one wavefront per pair.  For each direction the wave computes
-// the other side's segment constants (start point, direction, 1/|s|^2)
+// the other side's segment constants (start point, direction, direction / |s|^2)
 // into LDS, then lanes hold up to 8 points each and sweep a share of the
 // segments, keeping running minima in VGPRs (hd_layout); the minima meet
 // across lane groups, a wave max-reduction and one square root finish the
@@ -86,41 +86,50 @@ __device__ __forceinline__ T p2t_sq(T px, T py, const T *__restrict__ Y, int ny)
 }
 
 // FP32 form.  Per segment k of Y the wave stages in LDS its start point and
-// direction G[k] = (x1, y1, dx, dy) and R[k] = 1/|s|^2 rounded up (0 for a
-// degenerate segment).  One clamped projection gives the reference's value
-// in every case: with c = 0 if u < 1e-5, else min(u, 1), the residual
-// |(p - s1) - c (s2 - s1)|^2 is
+// direction G[k] = (x1, y1, dx, dy) and Q[k] = (dx, dy) r', with r' the
+// reciprocal of |s|^2 rounded up and widened by 2^-20 (0 for a degenerate
+// segment).  One clamped projection gives the reference's value in every
+// case: with u = (p - s1) . Q and c = 0 if u < 1e-5, else min(u, 1), the
+// residual |(p - s1) - c (s2 - s1)|^2 is
 //   - |p - s1|^2 exactly when u < 1e-5 (c = 0; the reference's
 //     min(dps1, dps2) is dps1 there, since u < 1/2 means s1 is nearer), and
-//     for a degenerate segment (R = 0 gives u = 0), as the reference;
+//     for a degenerate segment (Q = 0 gives u = 0), as the reference;
 //   - |p - s2|^2 up to rounding when u > 1 (c = 1; u > 1 means s2 is
 //     nearer, so min(dps1, dps2) = dps2);
 //   - the distance to the foot of the perpendicular otherwise.
-// 1/|s|^2 rounded up makes r |s|^2 >= 1 exactly: a point equal to s2 has
-// u1 = |s|^2 (the same expression on the same values), hence u >= 1, c = 1
-// and a residual of exactly 0, as the reference's FP64 u = 1 gives.
-__device__ __forceinline__ void p2s_eval(float px, float py, float4 g, float r, float &best) {
+// A point equal to s2 has p - s1 = (dx, dy) bit for bit, and then u >= 1:
+// with rounding unit e = 2^-24, r |s|^2 >= 1 for the computed |s|^2, which
+// is at most (dx^2 + dy^2)(1 + e)^2, and the two products and the FMA of u
+// lose at most (1 - e)^3, so u >= (1 + 2^-20)(1 - e)^6 / (1 + e)^2 > 1:
+// c = 1 and the residual is exactly 0, as the reference's FP64 u = 1
+// gives.  Elsewhere the widening moves u by at most 2^-20 relative, which
+// moves the foot of the perpendicular along the segment and the squared
+// distance by a second-order amount.
+__device__ __forceinline__ void p2s_eval(float px, float py, float4 g, float2 q, float &best) {
     const float ex = px - g.x, ey = py - g.y;
-    const float u = fmaf(ey, g.w, ex * g.z) * r;
+    const float u = fmaf(ey, q.y, ex * q.x);
     const float c = u < 0.00001f ? 0.0f : fminf(u, 1.0f);
     const float gx = fmaf(-c, g.z, ex), gy = fmaf(-c, g.w, ey);
     best = fminf(best, fmaf(gy, gy, gx * gx));
 }
 
-__device__ __forceinline__ void seg_consts(float x1, float y1, float x2, float y2, float4 &g, float &r) {
+__device__ __forceinline__ void seg_consts(float x1, float y1, float x2, float y2, float4 &g, float2 &q) {
     const float xd = x2 - x1, yd = y2 - y1;
     g = make_float4(x1, y1, xd, yd);
-    r = 0.0f;
+    q = make_float2(0.0f, 0.0f);
     if (!(x1 == x2 && y1 == y2)) {
         const float segl = fmaf(yd, yd, xd * xd);
-        r = 1.0f / segl;
+        float r = 1.0f / segl;
         if (fmaf(r, segl, -1.0f) < 0.0f)  // r > 0 finite: the next float up
             r = __builtin_bit_cast(float, __builtin_bit_cast(unsigned int, r) + 1u);
+        r *= 1.00000095367431640625f;  // 1 + 2^-20
+        q = make_float2(xd * r, yd * r);
     }
 }
 
-// Segments staged per LDS chunk: 20 KB per wave, so 8 waves share a CU.
-constexpr int kHChunk = 1024;
+// Segments staged per LDS chunk: 24 B each, 19,968 B per wave, so 8 waves
+// share a CU.
+constexpr int kHChunk = 832;
 
 // Lane layout of one direction (points of X against the segments of Y):
 // Ps groups of Lg = 64 / Ps lanes; lane t of group g holds points
@@ -151,7 +160,7 @@ __device__ __forceinline__ void hd_layout(int nx, int nseg, int &M, int &Ps) {
 // this lane's share (the wave max of the results is the direction's value).
 template <int M>
 __device__ __forceinline__ float hd_dir_f32(const float *__restrict__ X, int nx, const float *__restrict__ Y, int ny,
-                                         int Ps, float4 *sg, float *sr) {
+                                         int Ps, float4 *sg, float2 *sq) {
     const int lane = threadIdx.x;
     const int Lg = kWave / Ps;
     const int grp = lane / Lg, t = lane % Lg;
@@ -172,15 +181,15 @@ __device__ __forceinline__ float hd_dir_f32(const float *__restrict__ X, int nx,
             __syncthreads();  // the previous chunk's reads are done
             for (int k = lane; k < nc; k += kWave) {
                 const float *s = Y + 2 * (c0 + k);
-                seg_consts(s[0], s[1], s[2], s[3], sg[k], sr[k]);
+                seg_consts(s[0], s[1], s[2], s[3], sg[k], sq[k]);
             }
             __syncthreads();
 #pragma unroll 4
             for (int k = grp; k < nc; k += Ps) {
                 const float4 g = sg[k];
-                const float r = sr[k];
+                const float2 d = sq[k];
 #pragma unroll
-                for (int q = 0; q < M; ++q) p2s_eval(px[q], py[q], g, r, best[q]);
+                for (int q = 0; q < M; ++q) p2s_eval(px[q], py[q], g, d, best[q]);
             }
         }
 #pragma unroll
@@ -193,18 +202,18 @@ __device__ __forceinline__ float hd_dir_f32(const float *__restrict__ X, int nx,
 }
 
 __device__ __forceinline__ float hd_dir_dispatch(const float *X, int nx, const float *Y, int ny, float4 *sg,
-                                                 float *sr) {
+                                                 float2 *sq) {
     int M, Ps;
     hd_layout(nx, ny - 1, M, Ps);
     switch (M) {
-    case 1: return hd_dir_f32<1>(X, nx, Y, ny, Ps, sg, sr);
-    case 2: return hd_dir_f32<2>(X, nx, Y, ny, Ps, sg, sr);
-    case 3: return hd_dir_f32<3>(X, nx, Y, ny, Ps, sg, sr);
-    case 4: return hd_dir_f32<4>(X, nx, Y, ny, Ps, sg, sr);
-    case 5: return hd_dir_f32<5>(X, nx, Y, ny, Ps, sg, sr);
-    case 6: return hd_dir_f32<6>(X, nx, Y, ny, Ps, sg, sr);
-    case 7: return hd_dir_f32<7>(X, nx, Y, ny, Ps, sg, sr);
-    default: return hd_dir_f32<8>(X, nx, Y, ny, Ps, sg, sr);
+    case 1: return hd_dir_f32<1>(X, nx, Y, ny, Ps, sg, sq);
+    case 2: return hd_dir_f32<2>(X, nx, Y, ny, Ps, sg, sq);
+    case 3: return hd_dir_f32<3>(X, nx, Y, ny, Ps, sg, sq);
+    case 4: return hd_dir_f32<4>(X, nx, Y, ny, Ps, sg, sq);
+    case 5: return hd_dir_f32<5>(X, nx, Y, ny, Ps, sg, sq);
+    case 6: return hd_dir_f32<6>(X, nx, Y, ny, Ps, sg, sq);
+    case 7: return hd_dir_f32<7>(X, nx, Y, ny, Ps, sg, sq);
+    default: return hd_dir_f32<8>(X, nx, Y, ny, Ps, sg, sq);
     }
 }
 
@@ -213,7 +222,7 @@ __global__ __launch_bounds__(kWave) void hausdorff_kernel(HArgs a) {
     const int lane = threadIdx.x;
     const T *xy = HPt<T>::base(a);
     __shared__ float4 sg[sizeof(T) == 4 ? kHChunk : 1];
-    __shared__ float sr[sizeof(T) == 4 ? kHChunk : 1];
+    __shared__ float2 sq[sizeof(T) == 4 ? kHChunk : 1];
     for (int64_t p = blockIdx.x; p < a.npairs; p += gridDim.x) {
         int64_t ia, ib;
         decode_pair(a.ps, p, ia, ib);
@@ -232,7 +241,7 @@ __global__ __launch_bounds__(kWave) void hausdorff_kernel(HArgs a) {
             const int ny = dir == 0 ? nb : na;
             if (ny < 2) continue;
             if constexpr (sizeof(T) == 4) {
-                const float d = hd_dir_dispatch(X, nx, Y, ny, sg, sr);
+                const float d = hd_dir_dispatch(X, nx, Y, ny, sg, sq);
                 m = d > m ? d : m;
             } else {
                 for (int base = 0; base < nx; base += kWave) {
