@@ -438,7 +438,12 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
                 lf = v;
             }
             if (amb != 0) {
-                if ((amb >> (threadIdx.x & (kWave - 1))) & 1ull) {
+                // the empty asm keeps this lane's test (three VALU
+                // instructions) inside the rarely taken branch; hoisted above
+                // it, every row step paid for it
+                uint64_t am = amb;
+                asm volatile("" : "+s"(am));
+                if ((am >> (threadIdx.x & (kWave - 1))) & 1ull) {
                     // redo this lane's row step with the FP64 test; src (the
                     // previous row), the corner and the left input are intact
                     const double2 pr = reinterpret_cast<const double2 *>(a.xy64)[orow + r];
