@@ -384,7 +384,7 @@ template <typename CT, typename AT> struct RowPt {
 // row from `src` and writing the new one to `dst`.  `cur` holds the point of
 // row r, loaded one step earlier; `nxt` receives the point of the lane's
 // next row.  (Loading two steps ahead measured slower.)
-template <int M, bool F64, int L, int W, bool kBnd>
+template <int M, bool F64, int L, int W, bool kBnd, bool kLds>
 __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t nrow_s, int rlast, int64_t orow,
                                         int64_t ocol, int col0, int ncol, bool lane0, typename Rec<M, F64>::AT left_fixed,
                                         typename Rec<M, F64>::AT *bnd, bool write_bnd,
@@ -397,7 +397,8 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
                                         RowPt<typename Rec<M, F64>::CT, typename Rec<M, F64>::AT> &nxt,
                                         typename Rec<M, F64>::AT &lp_in, typename Rec<M, F64>::AT &lp_out,
                                         typename Rec<M, F64>::AT cur_bl, typename Rec<M, F64>::AT &nxt_bl,
-                                        typename Rec<M, F64>::CT gx, typename Rec<M, F64>::CT gy) {
+                                        typename Rec<M, F64>::CT gx, typename Rec<M, F64>::CT gy,
+                                        const typename Rec<M, F64>::CT *srow, const double *strow) {
     using R = Rec<M, F64>;
     using CT = typename R::CT;
     using AT = typename R::AT;
@@ -405,8 +406,14 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
     // next row of this lane, kept inside the trajectory: a lane whose next
     // row is out of range does not use the point, so any row will do
     const uint32_t rn = min((uint32_t)(s + 1 - j), (uint32_t)rlast);
-    Pt<CT>::load(a, orow + rn, nxt.x, nxt.y);
-    if constexpr (R::kTime) nxt.t = a.t64[orow + rn];
+    if constexpr (kLds) {  // the row trajectory staged in LDS
+        nxt.x = srow[2 * rn];
+        nxt.y = srow[2 * rn + 1];
+        if constexpr (R::kTime) nxt.t = strow[rn];
+    } else {
+        Pt<CT>::load(a, orow + rn, nxt.x, nxt.y);
+        if constexpr (R::kTime) nxt.t = a.t64[orow + rn];
+    }
     if constexpr (kBnd) {
         if (lane0) nxt_bl = bnd[rn];
     }
@@ -495,7 +502,22 @@ __device__ __forceinline__ void dp_step(const DPArgs &a, int s, int j, uint32_t 
 }
 
 // One task: 64/L segments, each sweeping one pair with W columns per lane.
-template <int M, bool F64, int L, int W>
+// Dynamic LDS of direct launches: the row trajectory's points (and
+// timestamps) of the wave's pair.
+extern __shared__ __attribute__((aligned(16))) unsigned char tsm_dp_lds[];
+
+// Bytes of dynamic LDS a direct launch of measure (M, F64) passes: up to
+// 64 WM rows, the longest trajectory a direct launch admits.
+template <int M, bool F64> constexpr size_t direct_lds_bytes() {
+    using R = Rec<M, F64>;
+    return (size_t)64 * R::kWMax * (2 * sizeof(typename R::CT) + (R::kTime ? sizeof(double) : 0));
+}
+
+// kDirect: a direct launch's task (L = 64, the wave's own pair, no column
+// strips): the wave first copies its row trajectory into LDS, so each row
+// step reads its point there instead of waiting on a global load that, with
+// at most one wave per SIMD, nothing else would cover.
+template <int M, bool F64, int L, int W, bool kDirect = false>
 __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t wave_id) {
     using R = Rec<M, F64>;
     using CT = typename R::CT;
@@ -527,6 +549,23 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
     }
     const AT topv = R::top(sb);
     const AT leftv = R::left0(sa);
+
+    CT *srow = nullptr;
+    double *strow = nullptr;
+    if constexpr (kDirect) {
+        constexpr int kRows = 64 * R::kWMax;  // the host admits rows <= 64 WM
+        srow = reinterpret_cast<CT *>(tsm_dp_lds);
+        strow = reinterpret_cast<double *>(tsm_dp_lds + 2 * kRows * sizeof(CT));
+        const int nst = nrow < kRows ? nrow : kRows;
+        for (int i = lane; i < nst; i += kWave) {
+            CT x, y;
+            Pt<CT>::load(a, orow + i, x, y);
+            srow[2 * i] = x;
+            srow[2 * i + 1] = y;
+            if constexpr (R::kTime) strow[i] = a.t64[orow + i];
+        }
+        __syncthreads();
+    }
 
     // a pair needing strips longer than the scratch column (max_len was
     // understated by the caller) is not computed; its result is NaN and its
@@ -570,27 +609,33 @@ __device__ __forceinline__ void run_task(const DPArgs &a, int64_t slot0, int64_t
         const uint32_t nrow_s = in_strip ? (uint32_t)nrow : 0u;
         RowPt<CT, AT> p0, p1;  // points of the lane's rows at even / odd steps
         const uint32_t r0 = min((uint32_t)(-j), (uint32_t)rlast);
-        Pt<CT>::load(a, orow + r0, p0.x, p0.y);
         p0.t = 0;
         p1.t = 0;
-        if constexpr (R::kTime) p0.t = a.t64[orow + r0];
-        AT bl0 = 0, bl1 = 0;
-        if (strip == 0) {
-            for (int s = 0; s < nsteps; s += 2) {
-                dp_step<M, F64, L, W, false>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb, bt, A,
-                                          B, p0, p1, lpA, lpB, bl0, bl1, gx, gy);
-                if (s + 1 < nsteps)
-                    dp_step<M, F64, L, W, false>(a, s + 1, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by,
-                                              gb, bt, B, A, p1, p0, lpB, lpA, bl1, bl0, gx, gy);
-            }
+        if constexpr (kDirect) {
+            p0.x = srow[2 * r0];
+            p0.y = srow[2 * r0 + 1];
+            if constexpr (R::kTime) p0.t = strow[r0];
         } else {
+            Pt<CT>::load(a, orow + r0, p0.x, p0.y);
+            if constexpr (R::kTime) p0.t = a.t64[orow + r0];
+        }
+        AT bl0 = 0, bl1 = 0;
+        if (kDirect || strip == 0) {
+            for (int s = 0; s < nsteps; s += 2) {
+                dp_step<M, F64, L, W, false, kDirect>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb,
+                                                   bt, A, B, p0, p1, lpA, lpB, bl0, bl1, gx, gy, srow, strow);
+                if (s + 1 < nsteps)
+                    dp_step<M, F64, L, W, false, kDirect>(a, s + 1, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx,
+                                                       by, gb, bt, B, A, p1, p0, lpB, lpA, bl1, bl0, gx, gy, srow, strow);
+            }
+        } else if constexpr (!kDirect) {
             if (lane0) bl0 = bnd[r0];
             for (int s = 0; s < nsteps; s += 2) {
-                dp_step<M, F64, L, W, true>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb, bt, A,
-                                         B, p0, p1, lpA, lpB, bl0, bl1, gx, gy);
+                dp_step<M, F64, L, W, true, false>(a, s, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by, gb, bt,
+                                                A, B, p0, p1, lpA, lpB, bl0, bl1, gx, gy, srow, strow);
                 if (s + 1 < nsteps)
-                    dp_step<M, F64, L, W, true>(a, s + 1, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by,
-                                             gb, bt, B, A, p1, p0, lpB, lpA, bl1, bl0, gx, gy);
+                    dp_step<M, F64, L, W, true, false>(a, s + 1, j, nrow_s, rlast, orow, ocol, col0, ncol, lane0, leftv, bnd, write_bnd, bx, by,
+                                                    gb, bt, B, A, p1, p0, lpB, lpA, bl1, bl0, gx, gy, srow, strow);
             }
         }
 
@@ -625,6 +670,19 @@ __device__ __forceinline__ void dispatch_bucket(const DPArgs &a, int bk, int64_t
     }
 }
 
+// A direct launch's 64-lane variants (b < WM / 4), with the row trajectory
+// in LDS.
+template <int M, bool F64, int WM, int b>
+__device__ __forceinline__ void dispatch_direct(const DPArgs &a, int bk, int64_t wave_id) {
+    if constexpr (b < bucket_n64(WM)) {
+        if (bk == b) {
+            run_task<M, F64, 64, bucket_W(WM, b), true>(a, wave_id, wave_id);
+            return;
+        }
+        dispatch_direct<M, F64, WM, b + 1>(a, bk, wave_id);
+    }
+}
+
 // Waves of the DP kernel that clear the planner's counters first.
 constexpr int kPlanZeroWaves = 16;
 
@@ -641,8 +699,9 @@ template <int M, bool F64> __device__ __forceinline__ void dp_body(const DPArgs 
     const bool direct = a.slots == nullptr;
     bool first = true;
     // one call site of the bucket dispatch (every (L, W) variant is inlined
-    // there): a direct launch passes through the loop once, a persistent
-    // wave until the task queue is drained
+    // there) for persistent waves, which loop until the task queue is
+    // drained; a direct launch's wave takes its 64-lane variant (row points
+    // in LDS) once
     while (true) {
         int kid;
         int64_t slot0;
@@ -664,7 +723,8 @@ template <int M, bool F64> __device__ __forceinline__ void dp_body(const DPArgs 
             const int64_t na = a.offs[ia + 1] - a.offs[ia], nb = a.offs[ib + 1] - a.offs[ib];
             const int64_t nc = na < nb ? na : nb;
             kid = __builtin_amdgcn_readfirstlane(bucket_w64(WM, nc < 1 ? 1 : nc));
-            slot0 = wave_id;
+            dispatch_direct<M, F64, WM, 0>(a, kid, wave_id);
+            break;
         } else {
             unsigned int t = 0;
             if (lane == 0) t = atomicAdd(a.counter, 1u);

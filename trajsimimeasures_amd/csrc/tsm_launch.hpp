@@ -177,9 +177,10 @@ int run_dp(Workspace &c, int lanes_hint, const tsm_trajset &set, const PairSpace
                                    ? g_ctx.direct_pairs
                                    : kDirectRounds * (int64_t)g_ctx.num_cu * g_ctx.waves_per_cu;
     if (lanes_hint == 0 && npairs <= direct_max && npairs <= 0x7fffffff && set.max_len <= 64 * (int64_t)WM) {
-        a.slots = nullptr;  // direct: wave w computes pair w
+        a.slots = nullptr;  // direct: wave w computes pair w, its row trajectory in dynamic LDS
         const ProfEvents pe;
-        hipExtLaunchKernelGGL(kern, dim3((unsigned)npairs), dim3(kWave), 0, st, pe.start, pe.stop, 0, a);
+        hipExtLaunchKernelGGL(kern, dim3((unsigned)npairs), dim3(kWave), (uint32_t)direct_lds_bytes<M, F64>(), st, pe.start,
+                              pe.stop, 0, a);
         TSM_HIP(hipGetLastError());
         return TSM_OK;
     }
