@@ -12,8 +12,8 @@
 // longer trajectory on the rows (streamed) and the shorter on the columns
 // (held in registers).
 //
-// Layout of the work: a "segment" of L lanes (L = 4, 8, 16 or 64, so 16, 8,
-// 4 or 1 segments per wavefront) owns one pair.  Lane j of a segment holds W
+// Layout of the work: a "segment" of L lanes (L = 2, 4, 8, 16 or 64, so 32,
+// 16, 8, 4 or 1 segments per wavefront) owns one pair.  Lane j of a segment holds W
 // consecutive columns -- their points in VGPRs and the previous row of D for
 // them -- and sweeps the rows in skewed order: at step s it computes row
 // s - j.  The value of D at the lane's left edge comes from lane j-1,
@@ -31,6 +31,7 @@
 // pairs by (bucket, rows) with a counting sort and packs them 64/L to a task,
 // so all segments of a task share (L, W) and have similar row counts; one
 // persistent kernel pulls tasks from an atomic counter, largest W first.
+// Small batches skip all of it (direct launches: wave w, pair w).
 #pragma once
 
 #include <type_traits>
