@@ -69,6 +69,10 @@ def _lens_every_w(wm: int, seed: int):
 @pytest.mark.parametrize("measure", ["dtw", "edr", "lcss", "erp", "frechet"])
 @pytest.mark.parametrize("precision", ["fp32", "fp64"])
 def test_direct_equals_planned_every_w(gpu, measure, precision):
+    """Sets up to 40 points longer than a direct launch admits (64 x 24): the
+    default call is planned as well (its wide pairs on 64-lane segments,
+    column strips beyond 1,536 columns), equal to the forced planned launch
+    and to the oracle; test_direct_launch_every_w covers direct launches."""
     la, lb = _lens_every_w(24, seed=7)
     print(f"[case] direct vs planned measure={measure} precision={precision} pairs={len(la)}", flush=True)
     ts, pa, pb = fixed_lengths(la, lb, seed=8)
@@ -293,3 +297,62 @@ def test_long_row_buckets(gpu, measure):
             assert np.array_equal(cnt, rcnt)
         else:
             assert np.max(np.abs(got - ref) / np.maximum(ref, 1e-300)) <= 1e-5
+
+
+def _lens_direct(wm: int, seed: int):
+    """Like _lens_every_w, with every length within a direct launch's limit
+    (64 wm points), so the call is a direct launch: every 64-lane W, rows up
+    to the limit (the row trajectory staged in LDS)."""
+    la, lb = _lens_every_w(wm, seed)
+    cap = 64 * wm
+    return [min(x, cap) for x in la], [min(x, cap) for x in lb]
+
+
+@pytest.mark.parametrize("measure", ["dtw", "edr", "lcss", "erp", "frechet"])
+@pytest.mark.parametrize("precision", ["fp32", "fp64"])
+def test_direct_launch_every_w(gpu, measure, precision):
+    """Direct launches (lengths up to 64 x 24) in every 64-lane W, rows up to
+    1,536 points read from LDS: equal to the planned launch (ERP within its
+    boundary-sum rounding) and to the oracle."""
+    from trajsimimeasures_amd import engine
+    la, lb = _lens_direct(24, seed=17)
+    print(f"[case] direct every W measure={measure} precision={precision} pairs={len(la)}", flush=True)
+    ts, pa, pb = fixed_lengths(la, lb, seed=18)
+    assert ts.lengths.max() <= 64 * 24
+    d = engine.DeviceTrajSet(ts)
+    g = ts.xy.mean(0)
+    kw = dict(eps=EPS, gpoint=g, precision=precision)
+    (dv, dc), (pv, pc) = _both(measure, d, pa, pb, **kw)
+    if measure == "erp":
+        tol = 1e-6 if precision == "fp32" else 1e-13
+        assert np.max(np.abs(dv - pv) / np.maximum(np.abs(pv), 1e-300)) <= tol
+    else:
+        assert np.array_equal(dv, pv), (measure, np.nonzero(dv != pv)[0][:5])
+    if dc is not None:
+        assert np.array_equal(dc, pc)
+    ref, rcnt = O.pairs(measure, ts.xy, ts.offsets, pa, pb, eps=EPS, gpoint=g)
+    if measure in ("edr", "lcss"):
+        assert np.array_equal(dc, rcnt) and np.array_equal(dv, ref)
+    else:
+        rtol = 1e-5 if precision == "fp32" else 1e-13
+        rel = np.abs(dv - ref) / np.maximum(np.abs(ref), 1e-300)
+        assert rel.max() <= rtol, (measure, precision, float(rel.max()))
+
+
+@pytest.mark.parametrize("measure", ["stedr", "stlcss"])
+def test_direct_launch_st(gpu, measure):
+    """STEDR/STLCSS direct launches (lengths up to 64 x 16, points and
+    timestamps staged in LDS) in every 64-lane W: equal to the planned
+    launch and to the oracle."""
+    from trajsimimeasures_amd import engine
+    from trajsimimeasures_amd.data import with_timestamps
+    la, lb = _lens_direct(16, seed=19)
+    print(f"[case] direct ST measure={measure} pairs={len(la)}", flush=True)
+    ts, pa, pb = fixed_lengths(la, lb, seed=20)
+    ts = with_timestamps(np.random.default_rng(21), ts)
+    assert ts.lengths.max() <= 64 * 16
+    d = engine.DeviceTrajSet(ts)
+    (dv, _), (pv, _) = _both(measure, d, pa, pb, eps=EPS, delta=DELTA)
+    assert np.array_equal(dv, pv)
+    ref, _ = O.pairs_st(measure, ts.xy, ts.t, ts.offsets, pa, pb, EPS, DELTA)
+    assert np.array_equal(dv, ref)
