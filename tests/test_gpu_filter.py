@@ -398,3 +398,46 @@ def test_dist_loss_kernel_matches_reference(gpu):
         lr_, dEr, _ = R.dist_loss(E.astype(np.float64), Y.astype(np.float64), B, K)
         assert abs(float(loss.mean()) - lr_) <= 1e-4 * abs(lr_) + 1e-7, (B, K)
         assert _rel(dE.cpu().numpy().astype(np.float64), dEr) < 1e-4, (B, K)
+
+
+@pytest.mark.parametrize("C", [10, 100, 256, 300])
+def test_select_candidates_matches_sort(gpu, C):
+    """tsm_nn_select_candidates against a lexicographic sort of (value,
+    column): rows of C - 1 and cap + 1 keys flagged, rows of C .. cap keys
+    (the bitonic sort up to 512 keys, the radix select beyond, when C <= 256)
+    giving their C smallest ascending; values repeat (ties broken by column)
+    and include negatives."""
+    import torch
+
+    from trajsimimeasures_amd import filter as F
+    rng = np.random.default_rng(23 + C)
+    cap = 4096
+    counts = [C - 1, C, C + 1, 511, 512, 513, 700, 1000, 2048, 3000, cap, cap + 1]
+    counts = [c for c in counts if c >= C - 1]
+    nq = len(counts)
+    key = np.zeros((nq, cap), dtype=np.float32)
+    idx = np.zeros((nq, cap), dtype=np.int32)
+    for r, c in enumerate(counts):
+        m = min(c, cap)
+        v = rng.normal(0.0, 1.0, m).astype(np.float32)
+        v[rng.random(m) < 0.3] = np.float32(0.25)  # a large group of equal values
+        v[rng.random(m) < 0.1] = np.float32(-0.5)
+        key[r, :m] = v
+        idx[r, :m] = rng.permutation(1_000_000)[:m].astype(np.int32)
+    dev = "cuda"
+    cnt_t = torch.as_tensor(np.array(counts, dtype=np.uint32).view(np.int32), device=dev)
+    key_t = torch.as_tensor(key, device=dev)
+    idx_t = torch.as_tensor(idx, device=dev)
+    out = torch.full((nq, C), -7, dtype=torch.int32, device=dev)
+    flag = torch.full((nq,), -7, dtype=torch.int32, device=dev)
+    from trajsimimeasures_amd import _lib
+    _lib.check(F._nn().tsm_nn_select_candidates(F._p(cnt_t), F._p(key_t), F._p(idx_t), nq, cap, C, F._p(out),
+                                                F._p(flag), F._s()))
+    out, flag = out.cpu().numpy(), flag.cpu().numpy()
+    for r, c in enumerate(counts):
+        if c < C or c > cap:
+            assert flag[r] == 1 and np.all(out[r] == 0), (r, c)
+            continue
+        assert flag[r] == 0, (r, c)
+        order = np.lexsort((idx[r, :c], key[r, :c]))[:C]
+        assert np.array_equal(out[r], idx[r, order]), (C, c)

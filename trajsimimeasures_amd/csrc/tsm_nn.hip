@@ -488,16 +488,60 @@ __global__ __launch_bounds__(NT, NT / 256) void search_filter_kernel(GemmArgs g,
     search_flush(hq, hq.n, f, NT);
 }
 
-// The C smallest candidates of each row: keys (order-preserving bits of the
-// value, then the column) sorted by a bitonic network in LDS.  A row whose
-// list is short (fewer than C) or overflowed (more than cap) is flagged for
-// the caller's exact fallback and gets column 0 in every slot (a valid
-// index, so work queued on it before the fallback reads no garbage).  One
-// 256-thread block per row; cap <= 4096.
+// The C smallest candidates of each row, ascending: keys (order-preserving
+// bits of the value, then the column; unique within a row, ties broken by
+// position all the same).  A row whose list is short (fewer than C) or
+// overflowed (more than cap) is flagged for the caller's exact fallback and
+// gets column 0 in every slot (a valid index, so work queued on it before
+// the fallback reads no garbage).  One 256-thread block per row; cap <= 4096.
+// Rows of up to kSelSort keys (or C > 256) sort them all with a bitonic
+// network in LDS, a barrier per stage (45 at 512 keys, 66 at 2,048).  The
+// launch lasts as long as its longest row, so longer rows select instead:
+// four 8-bit radix passes over the value bits find the value V of the C-th
+// key (three barriers a pass), four more over the column bits of the keys
+// of value V when the C-th key splits them, the C keys at or below it are
+// gathered, and they are ordered by counting ranks.
+constexpr int kSelSort = 512;
+
+// The bin d where the cumulative count of the 256 bins reaches rank k
+// (1-based): pick = {d, k minus the counts below d, d's count}.  Wave 0
+// scans 4 bins per lane.
+__device__ __forceinline__ void radix_pick(const uint32_t *hist, uint32_t k, uint32_t *pick) {
+    const int lane = threadIdx.x;
+    uint32_t c[4], sum = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        c[i] = hist[4 * lane + i];
+        sum += c[i];
+    }
+    uint32_t incl = sum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += v;
+    }
+    uint32_t before = incl - sum;
+    if (before < k && k <= incl) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            if (before < k && k <= before + c[i]) {
+                pick[0] = (uint32_t)(4 * lane + i);
+                pick[1] = k - before;
+                pick[2] = c[i];
+            }
+            before += c[i];
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void select_candidates_kernel(const uint32_t *cnt, const float *key,
                                                                 const int32_t *idx, int cap, int C, int32_t *out,
                                                                 int32_t *flag) {
     __shared__ uint64_t sk[4096];
+    __shared__ uint64_t sel[256];
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t pick[3];
+    __shared__ uint32_t nsel;
     const int64_t row = blockIdx.x;
     const uint32_t n = cnt[row];
     if (n < (uint32_t)C || n > (uint32_t)cap) {
@@ -506,6 +550,68 @@ __global__ __launch_bounds__(256) void select_candidates_kernel(const uint32_t *
         return;
     }
     if (threadIdx.x == 0) flag[row] = 0;
+    if (n > (uint32_t)kSelSort && C <= 256) {  // block-uniform
+        for (int t = threadIdx.x; t < (int)n; t += blockDim.x) {
+            const uint32_t u = __builtin_bit_cast(uint32_t, key[row * cap + t]);
+            const uint32_t k = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+            sk[t] = ((uint64_t)k << 32) | (uint32_t)idx[row * cap + t];
+        }
+        // radix passes over the value bits (hi), then, when the C-th key's
+        // value is shared by more keys than the rank needs, over the column
+        // bits (lo) of those keys
+        uint32_t vhi = 0, vlo = 0, kk = (uint32_t)C, group = 0;
+        bool split = false;  // the tie group is split by column
+        for (int word = 1; word >= 0; --word) {
+            if (word == 0) {
+                if (group == kk) break;  // block-uniform: the whole tie group is needed
+                split = true;
+            }
+            uint32_t prefix = 0, pmask = 0;
+            for (int shift = 24; shift >= 0; shift -= 8) {
+                hist[threadIdx.x] = 0u;
+                __syncthreads();  // keys stored (first pass), bins cleared, last pick read
+                for (int t = threadIdx.x; t < (int)n; t += blockDim.x) {
+                    const uint64_t me = sk[t];
+                    const uint32_t h = (uint32_t)(me >> 32);
+                    if (word == 1) {
+                        if ((h & pmask) == prefix) atomicAdd(&hist[(h >> shift) & 255u], 1u);
+                    } else {
+                        const uint32_t l = (uint32_t)me;
+                        if (h == vhi && (l & pmask) == prefix) atomicAdd(&hist[(l >> shift) & 255u], 1u);
+                    }
+                }
+                __syncthreads();
+                if (threadIdx.x < 64) radix_pick(hist, kk, pick);
+                __syncthreads();
+                prefix |= pick[0] << shift;
+                pmask |= 255u << shift;
+                kk = pick[1];
+                group = pick[2];
+            }
+            if (word == 1) vhi = prefix;
+            else vlo = prefix;
+        }
+        // the C keys: value below vhi, or value vhi and (the whole group was
+        // needed, or column at most vlo)
+        if (threadIdx.x == 0) nsel = 0u;
+        __syncthreads();
+        for (int t = threadIdx.x; t < (int)n; t += blockDim.x) {
+            const uint64_t me = sk[t];
+            const uint32_t h = (uint32_t)(me >> 32);
+            if (h < vhi || (h == vhi && (!split || (uint32_t)me <= vlo))) {
+                const uint32_t slot = atomicAdd(&nsel, 1u);  // < C with unique keys; bounded all the same
+                if (slot < (uint32_t)C) sel[slot] = me;
+            }
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < C; t += blockDim.x) {
+            const uint64_t me = sel[t];
+            int r = 0;
+            for (int j = 0; j < C; ++j) r += (sel[j] < me) | ((sel[j] == me) & (j < t));
+            out[row * C + r] = (int32_t)(uint32_t)me;
+        }
+        return;
+    }
     int P = 1;
     while (P < (int)n) P <<= 1;
     for (int t = threadIdx.x; t < P; t += blockDim.x) {
