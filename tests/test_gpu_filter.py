@@ -404,9 +404,9 @@ def test_dist_loss_kernel_matches_reference(gpu):
 def test_select_candidates_matches_sort(gpu, C):
     """tsm_nn_select_candidates against a lexicographic sort of (value,
     column): rows of C - 1 and cap + 1 keys flagged, rows of C .. cap keys
-    (the bitonic sort up to 512 keys, the radix select beyond, when C <= 256)
-    giving their C smallest ascending; values repeat (ties broken by column)
-    and include negatives."""
+    (the radix select when C <= 256, the bitonic sort at C = 300) giving
+    their C smallest ascending; values repeat (ties broken by column) and
+    include negatives."""
     import torch
 
     from trajsimimeasures_amd import filter as F

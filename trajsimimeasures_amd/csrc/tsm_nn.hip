@@ -494,14 +494,14 @@ __global__ __launch_bounds__(NT, NT / 256) void search_filter_kernel(GemmArgs g,
 // overflowed (more than cap) is flagged for the caller's exact fallback and
 // gets column 0 in every slot (a valid index, so work queued on it before
 // the fallback reads no garbage).  One 256-thread block per row; cap <= 4096.
-// Rows of up to kSelSort keys (or C > 256) sort them all with a bitonic
-// network in LDS, a barrier per stage (45 at 512 keys, 66 at 2,048).  The
-// launch lasts as long as its longest row, so longer rows select instead:
-// four 8-bit radix passes over the value bits find the value V of the C-th
-// key (three barriers a pass), four more over the column bits of the keys
-// of value V when the C-th key splits them, the C keys at or below it are
-// gathered, and they are ordered by counting ranks.
-constexpr int kSelSort = 512;
+// With C <= 256, four 8-bit radix passes over the value bits find the
+// value V of the C-th key (three barriers a pass), four more over the
+// column bits of the keys of value V when the C-th key splits them, the C
+// keys at or below it are gathered, and they are ordered by counting ranks.
+// Sorting every key with a bitonic network instead (kept for C > 256) takes
+// a barrier per stage, 45 at 512 keys and 66 at 2,048, and the launch lasts
+// as long as its longest row: 72 µs against 17-18 µs per 1,000 rows of
+// about 400 keys.
 
 // The bin d where the cumulative count of the 256 bins reaches rank k
 // (1-based): pick = {d, k minus the counts below d, d's count}.  Wave 0
@@ -550,7 +550,7 @@ __global__ __launch_bounds__(256) void select_candidates_kernel(const uint32_t *
         return;
     }
     if (threadIdx.x == 0) flag[row] = 0;
-    if (n > (uint32_t)kSelSort && C <= 256) {  // block-uniform
+    if (C <= 256) {  // block-uniform
         for (int t = threadIdx.x; t < (int)n; t += blockDim.x) {
             const uint32_t u = __builtin_bit_cast(uint32_t, key[row * cap + t]);
             const uint32_t k = (u & 0x80000000u) ? ~u : (u | 0x80000000u);
