@@ -43,9 +43,19 @@ int tsm_nn_gemm_nt_acc(int M, int N, int K, const uint16_t *A, int lda, const ui
                        int ldc, float alpha, void *stream);
 /* The C smallest candidates of each row (sorted ascending; ties by column):
  * out[rows, C].  flag[row] = 1 when the row had fewer than C or more than cap
- * candidates (then out is not written for it), else 0.  cap <= 4096. */
+ * candidates (then its out row holds column 0 in every slot), else 0.
+ * cap <= 4096. */
 int tsm_nn_select_candidates(const uint32_t *cnt, const float *key, const int32_t *idx, int rows, int cap, int C,
                              int32_t *out, int32_t *flag, void *stream);
+/* The same, and the exact re-rank's pairs of each slot s = row * C + r:
+ * pa[s] = row_ids[row], pb[s] = col_ids[out[s]] (-1 for a column outside
+ * [0, ncol_ids)), so the re-rank needs no separate gather.  The reference's
+ * learned query (test_knn_time.py:179-201) stops at the embedding search's
+ * ids; these pairs are those ids' exact-distance pairs for the re-rank. */
+int tsm_nn_select_candidates_pairs(const uint32_t *cnt, const float *key, const int32_t *idx, int rows, int cap,
+                                   int C, int32_t *out, int32_t *flag, const int32_t *row_ids,
+                                   const int32_t *col_ids, int64_t ncol_ids, int32_t *pa, int32_t *pb,
+                                   void *stream);
 /* FP32 [rows, cols] -> BF16 (transpose != 0: dst[c][r]). */
 int tsm_nn_cast_bf16(const float *src, int rows, int cols, int ld_src, uint16_t *dst, int ld_dst, int transpose,
                      void *stream);

@@ -441,3 +441,58 @@ def test_select_candidates_matches_sort(gpu, C):
         assert flag[r] == 0, (r, c)
         order = np.lexsort((idx[r, :c], key[r, :c]))[:C]
         assert np.array_equal(out[r], idx[r, order]), (C, c)
+
+
+@pytest.mark.parametrize("C", [100, 300])
+def test_select_candidates_pairs(gpu, C):
+    """tsm_nn_select_candidates_pairs: out and flag as the plain entry point
+    gives them, and the re-rank pairs pa = row id repeated C times, pb = the
+    column's id (flagged rows: column 0's id; a column past the id table:
+    -1), against the torch construction the search used to do."""
+    import torch
+
+    from trajsimimeasures_amd import _lib
+    from trajsimimeasures_amd import filter as F
+    rng = np.random.default_rng(91 + C)
+    cap, ncols = 4096, 5000
+    counts = [C - 1, C, 700, 3000, cap + 1, 900]
+    nq = len(counts)
+    key = np.zeros((nq, cap), dtype=np.float32)
+    idx = np.zeros((nq, cap), dtype=np.int32)
+    for r, c in enumerate(counts):
+        m = min(c, cap)
+        v = rng.normal(0.0, 1.0, m).astype(np.float32)
+        v[rng.random(m) < 0.2] = np.float32(0.5)
+        key[r, :m] = v
+        idx[r, :m] = rng.permutation(ncols)[:m].astype(np.int32)
+    idx[5, 0], key[5, 0] = ncols + 3, np.float32(-9.0)  # a column outside the id table, ranked first
+    dev = "cuda"
+    cnt_t = torch.as_tensor(np.array(counts, dtype=np.uint32).view(np.int32), device=dev)
+    key_t, idx_t = torch.as_tensor(key, device=dev), torch.as_tensor(idx, device=dev)
+    row_ids = torch.as_tensor(rng.permutation(10_000)[:nq].astype(np.int32), device=dev)
+    col_ids = torch.as_tensor(rng.permutation(1_000_000)[:ncols].astype(np.int32), device=dev)
+    L = F._nn()
+    out0 = torch.full((nq, C), -7, dtype=torch.int32, device=dev)
+    flag0 = torch.full((nq,), -7, dtype=torch.int32, device=dev)
+    _lib.check(L.tsm_nn_select_candidates(F._p(cnt_t), F._p(key_t), F._p(idx_t), nq, cap, C, F._p(out0),
+                                          F._p(flag0), F._s()))
+    out = torch.full((nq, C), -7, dtype=torch.int32, device=dev)
+    flag = torch.full((nq,), -7, dtype=torch.int32, device=dev)
+    pa = torch.full((nq * C,), -7, dtype=torch.int32, device=dev)
+    pb = torch.full((nq * C,), -7, dtype=torch.int32, device=dev)
+    _lib.check(L.tsm_nn_select_candidates_pairs(F._p(cnt_t), F._p(key_t), F._p(idx_t), nq, cap, C, F._p(out),
+                                                F._p(flag), F._p(row_ids), F._p(col_ids), ncols, F._p(pa),
+                                                F._p(pb), F._s()))
+    assert torch.equal(out, out0) and torch.equal(flag, flag0)
+    o = out.cpu().numpy()
+    assert list(flag.cpu().numpy()) == [1, 0, 0, 0, 1, 0]
+    assert o[5, 0] == ncols + 3
+    ids = col_ids.cpu().numpy()
+    want_pb = np.where(o < ncols, ids[np.minimum(o, ncols - 1)], -1).reshape(-1)
+    assert np.array_equal(pb.cpu().numpy(), want_pb)
+    assert np.array_equal(pa.cpu().numpy(), np.repeat(row_ids.cpu().numpy(), C))
+    # argument checks: missing id tables are refused before any launch
+    with pytest.raises(_lib.TsmError):
+        _lib.check(L.tsm_nn_select_candidates_pairs(F._p(cnt_t), F._p(key_t), F._p(idx_t), nq, cap, C, F._p(out),
+                                                    F._p(flag), None, F._p(col_ids), ncols, F._p(pa), F._p(pb),
+                                                    F._s()))

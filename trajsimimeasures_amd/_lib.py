@@ -40,7 +40,7 @@ NN_EXPORTS = ("tsm_nn_gemm_nt", "tsm_nn_cast_bf16", "tsm_nn_resample", "tsm_nn_c
               "tsm_nn_gru_h0", "tsm_nn_gru_h0_bwd", "tsm_nn_gru_step_fwd", "tsm_nn_gru_step_bwd", "tsm_nn_colsum",
               "tsm_nn_rank_loss", "tsm_nn_dist_loss", "tsm_nn_adam", "tsm_nn_rownorm_bf16", "tsm_nn_topk_f32", "tsm_nn_topk_f64",
               "tsm_nn_linear_f32", "tsm_nn_split_bf16", "tsm_nn_gemm_nt_filter", "tsm_nn_select_candidates",
-              "tsm_nn_gemm_nt_acc", "tsm_nn_gru_forward", "tsm_nn_kth_smallest_f32")
+              "tsm_nn_select_candidates_pairs", "tsm_nn_gemm_nt_acc", "tsm_nn_gru_forward", "tsm_nn_kth_smallest_f32")
 # include/tsm_cluster.h
 CLUSTER_EXPORTS = ("tsm_square_from_triu", "tsm_kmedoids_fasterpam")
 

@@ -534,9 +534,28 @@ __device__ __forceinline__ void radix_pick(const uint32_t *hist, uint32_t k, uin
     }
 }
 
+// Optional pair output of the selection: slot s = row * C + r of the
+// candidate list also gives the pair (row_ids[row], col_ids[column]) that
+// an exact re-rank of the list computes (-1 for a column past ncol_ids,
+// which the DP planner turns into a NaN result).  pa == nullptr: none.
+struct SelectPairs {
+    const int32_t *row_ids, *col_ids;
+    int64_t ncol_ids;
+    int32_t *pa, *pb;
+};
+
+__device__ __forceinline__ void select_emit(const SelectPairs &sp, int32_t *out, int64_t row, int64_t s,
+                                            int32_t col) {
+    out[s] = col;
+    if (sp.pa) {
+        sp.pa[s] = sp.row_ids[row];
+        sp.pb[s] = (col >= 0 && col < sp.ncol_ids) ? sp.col_ids[col] : -1;
+    }
+}
+
 __global__ __launch_bounds__(256) void select_candidates_kernel(const uint32_t *cnt, const float *key,
                                                                 const int32_t *idx, int cap, int C, int32_t *out,
-                                                                int32_t *flag) {
+                                                                int32_t *flag, SelectPairs sp) {
     __shared__ uint64_t sk[4096];
     __shared__ uint64_t sel[256];
     __shared__ uint32_t hist[256];
@@ -546,7 +565,7 @@ __global__ __launch_bounds__(256) void select_candidates_kernel(const uint32_t *
     const uint32_t n = cnt[row];
     if (n < (uint32_t)C || n > (uint32_t)cap) {
         if (threadIdx.x == 0) flag[row] = 1;
-        for (int t = threadIdx.x; t < C; t += blockDim.x) out[row * C + t] = 0;
+        for (int t = threadIdx.x; t < C; t += blockDim.x) select_emit(sp, out, row, row * C + t, 0);
         return;
     }
     if (threadIdx.x == 0) flag[row] = 0;
@@ -608,7 +627,7 @@ __global__ __launch_bounds__(256) void select_candidates_kernel(const uint32_t *
             const uint64_t me = sel[t];
             int r = 0;
             for (int j = 0; j < C; ++j) r += (sel[j] < me) | ((sel[j] == me) & (j < t));
-            out[row * C + r] = (int32_t)(uint32_t)me;
+            select_emit(sp, out, row, row * C + r, (int32_t)(uint32_t)me);
         }
         return;
     }
@@ -640,7 +659,7 @@ __global__ __launch_bounds__(256) void select_candidates_kernel(const uint32_t *
             __syncthreads();
         }
     }
-    for (int t = threadIdx.x; t < C; t += blockDim.x) out[row * C + t] = (int32_t)(uint32_t)sk[t];
+    for (int t = threadIdx.x; t < C; t += blockDim.x) select_emit(sp, out, row, row * C + t, (int32_t)(uint32_t)sk[t]);
 }
 
 // FP32 [rows, cols] (ld_src) -> BF16, optionally transposed: dst[c][r].
@@ -1551,7 +1570,22 @@ int tsm_nn_select_candidates(const uint32_t *cnt, const float *key, const int32_
         return fail(TSM_E_ARG, "select_candidates: need 1 <= C <= cap <= 4096");
     if (rows == 0) return TSM_OK;
     hipLaunchKernelGGL(select_candidates_kernel, dim3(rows), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), cnt,
-                       key, idx, cap, C, out, flag);
+                       key, idx, cap, C, out, flag, SelectPairs{nullptr, nullptr, 0, nullptr, nullptr});
+    TSM_HIP(hipGetLastError());
+    return TSM_OK;
+}
+
+int tsm_nn_select_candidates_pairs(const uint32_t *cnt, const float *key, const int32_t *idx, int rows, int cap,
+                                   int C, int32_t *out, int32_t *flag, const int32_t *row_ids,
+                                   const int32_t *col_ids, int64_t ncol_ids, int32_t *pa, int32_t *pb,
+                                   void *stream) {
+    if (rows < 0 || cap < 1 || cap > 4096 || C < 1 || C > cap || !cnt || !key || !idx || !out || !flag)
+        return fail(TSM_E_ARG, "select_candidates_pairs: need 1 <= C <= cap <= 4096");
+    if (!row_ids || !col_ids || ncol_ids < 1 || !pa || !pb)
+        return fail(TSM_E_ARG, "select_candidates_pairs: need row_ids, col_ids (ncol_ids >= 1), pa and pb");
+    if (rows == 0) return TSM_OK;
+    hipLaunchKernelGGL(select_candidates_kernel, dim3(rows), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), cnt,
+                       key, idx, cap, C, out, flag, SelectPairs{row_ids, col_ids, ncol_ids, pa, pb});
     TSM_HIP(hipGetLastError());
     return TSM_OK;
 }

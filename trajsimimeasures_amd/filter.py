@@ -78,13 +78,14 @@ def _nn():
         L.tsm_nn_split_bf16.argtypes = [P, I64, i, i, P, P, P]
         L.tsm_nn_gemm_nt_filter.argtypes = [i, i, i, P, i, P, i, f, P, P, i, P, P, P, P]
         L.tsm_nn_select_candidates.argtypes = [P, P, P, i, i, i, P, P, P]
+        L.tsm_nn_select_candidates_pairs.argtypes = [P, P, P, i, i, i, P, P, P, P, I64, P, P, P]
         L.tsm_nn_gemm_nt_acc.argtypes = [i, i, i, P, i, P, i, P, i, f, P]
         L.tsm_nn_gru_forward.argtypes = [P, i, i, i, P, P, P, P, P, P, P, P, P, i, P]
         L.tsm_nn_kth_smallest_f32.argtypes = [P, I64, I64, I64, i, P, P]
         for n in ("gemm_nt", "cast_bf16", "resample", "conv_fwd", "conv_bwd", "gru_h0", "gru_h0_bwd",
                   "gru_step_fwd", "gru_step_bwd", "colsum", "rank_loss", "dist_loss", "adam", "rownorm_bf16", "topk_f32",
                   "topk_f64", "linear_f32", "split_bf16", "gemm_nt_filter", "select_candidates",
-                  "gemm_nt_acc", "gru_forward", "kth_smallest_f32"):
+                  "select_candidates_pairs", "gemm_nt_acc", "gru_forward", "kth_smallest_f32"):
             getattr(L, "tsm_nn_" + n).restype = ctypes.c_int
         L._nn_declared = True
     return L
@@ -338,7 +339,7 @@ class FilterIndex:
     def __init__(self, enc: Encoder, dset: DeviceTrajSet, db_ids):
         self.enc, self.dset = enc, dset
         self.last_redo = 0
-        self.db_ids = torch.as_tensor(db_ids, dtype=torch.int32, device=enc.device)
+        self.db_ids = torch.as_tensor(db_ids, dtype=torch.int32, device=enc.device).contiguous()
         E = enc.embed(dset, self.db_ids)
         self.norm = torch.empty(E.shape[0], dtype=torch.float32, device=E.device)
         self.B = self._split(E, query_side=False, norm=self.norm)
@@ -352,7 +353,7 @@ class FilterIndex:
         return out
 
     def search(self, q_ids, C: int = DEFAULT_C, *, cap: int = 4096, sample: int = 16384, ks: int | None = None,
-               defer_redo: bool = False):
+               defer_redo: bool = False, pairs: bool = False):
         """Candidate database positions [nq, C] by embedding distance (the C
         smallest |e_d|^2 - 2 e_q . e_d of each query, ascending).
 
@@ -368,14 +369,22 @@ class FilterIndex:
         defer_redo=True returns (candidates, flags, query operand) without
         that redo and without any host synchronisation: flagged rows hold
         column 0, and the caller redoes them (``redo_rows``) once it
-        synchronises anyway."""
+        synchronises anyway.  pairs=True (with defer_redo) also returns the
+        exact re-rank's pairs: (q_ids repeated C times, the candidates'
+        database ids), both int32 [nq * C], written by the selection kernel
+        on the filtered path."""
+        if pairs and not defer_redo:
+            raise ValueError("pairs=True needs defer_redo=True")
         Eq = self.enc.embed(self.dset, q_ids)
         A = self._split(Eq, query_side=True)
         nq, nd = A.shape[0], self.B.shape[0]
         if nd < 4 * sample or C > cap // 2:
             cand = self._search_full(A, C)
             if defer_redo:
-                return cand, torch.zeros(nq, dtype=torch.int32, device=A.device), A
+                flag = torch.zeros(nq, dtype=torch.int32, device=A.device)
+                if pairs:
+                    return (cand, flag, A) + self._rerank_pairs(q_ids, cand)
+                return cand, flag, A
             return cand
         stride = nd // sample
         if ks is None:
@@ -401,6 +410,14 @@ class FilterIndex:
                                            _p(self.norm), _p(tau), cap, _p(cnt), _p(key), _p(idx), _s()))
         cand = torch.empty((nq, C), dtype=torch.int32, device=A.device)
         flag = torch.empty(nq, dtype=torch.int32, device=A.device)
+        if pairs:
+            qi = torch.as_tensor(q_ids, dtype=torch.int32, device=A.device).contiguous()
+            pa = torch.empty(nq * C, dtype=torch.int32, device=A.device)
+            pb = torch.empty(nq * C, dtype=torch.int32, device=A.device)
+            _lib.check(L.tsm_nn_select_candidates_pairs(_p(cnt), _p(key), _p(idx), nq, cap, C, _p(cand), _p(flag),
+                                                        _p(qi), _p(self.db_ids), self.db_ids.numel(), _p(pa),
+                                                        _p(pb), _s()))
+            return cand, flag, A, pa, pb
         _lib.check(L.tsm_nn_select_candidates(_p(cnt), _p(key), _p(idx), nq, cap, C, _p(cand), _p(flag), _s()))
         if defer_redo:
             return cand, flag, A
@@ -409,6 +426,12 @@ class FilterIndex:
         if self.last_redo:
             cand[redo] = self._search_full(A[redo].contiguous(), C)
         return cand
+
+    def _rerank_pairs(self, q_ids, cand: torch.Tensor):
+        """The exact re-rank's pairs of a candidate list [nq, C]: each query id
+        C times, and the candidates' database ids."""
+        qi = torch.as_tensor(q_ids, dtype=torch.int32, device=cand.device)
+        return qi.repeat_interleave(cand.shape[1]), self.db_ids[cand.reshape(-1).long()]
 
     def _search_full(self, A: torch.Tensor, C: int, chunk: int = 1 << 20) -> torch.Tensor:
         """The C smallest of every full distance row (dense GEMM + radix select)."""
@@ -445,11 +468,12 @@ def exact_knn(dset: DeviceTrajSet, q0: int, nq: int, d0: int, nd: int, k: int = 
     return out, time.perf_counter() - t0
 
 
-def _rerank(index: FilterIndex, qids: torch.Tensor, cand: torch.Tensor, k: int) -> torch.Tensor:
-    """Exact DTW of each query against its candidates, the k smallest."""
+def _rerank(index: FilterIndex, qids: torch.Tensor, cand: torch.Tensor, k: int, pa=None, pb=None) -> torch.Tensor:
+    """Exact DTW of each query against its candidates, the k smallest.  pa,
+    pb: the pairs when the search already wrote them."""
     nq, C = cand.shape
-    pa = qids.repeat_interleave(C)
-    pb = index.db_ids[cand.reshape(-1).long()]
+    if pa is None:
+        pa, pb = index._rerank_pairs(qids, cand)
     d = dp_pairs("dtw", index.dset, pa, pb, validate=False).view(nq, C)
     idx, _ = topk_smallest(d, k)
     return torch.gather(cand, 1, idx.long())
@@ -463,8 +487,8 @@ def filtered_knn(index: FilterIndex, q0: int, nq: int, k: int = 10, C: int = DEF
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     qids = torch.arange(q0, q0 + nq, dtype=torch.int32, device="cuda")
-    cand, flag, A = index.search(qids, C, defer_redo=True)
-    best = _rerank(index, qids, cand, k)
+    cand, flag, A, pa, pb = index.search(qids, C, defer_redo=True, pairs=True)
+    best = _rerank(index, qids, cand, k, pa, pb)
     redo = torch.nonzero(flag).reshape(-1)  # synchronises, after everything is queued
     index.last_redo = int(redo.numel())
     if index.last_redo:
