@@ -272,6 +272,9 @@ class Encoder:
         if ids is None:
             ids = torch.arange(dset.n, dtype=torch.int32, device=self.device)
         ids = torch.as_tensor(ids, dtype=torch.int32, device=self.device)
+        if 0 < ids.numel() <= batch:  # one batch: the forward pass's own output, no copy
+            X, lenf = self.resample(dset, ids, ids.numel())
+            return self.forward(X, lenf)
         out = torch.empty((ids.numel(), D), dtype=torch.float32, device=self.device)
         for o in range(0, ids.numel(), batch):
             sl = ids[o:o + batch]
@@ -343,6 +346,7 @@ class FilterIndex:
         E = enc.embed(dset, self.db_ids)
         self.norm = torch.empty(E.shape[0], dtype=torch.float32, device=E.device)
         self.B = self._split(E, query_side=False, norm=self.norm)
+        self._sample_norm = {}  # (stride, sample) -> norms of the threshold sample rows
 
     @staticmethod
     def _split(E: torch.Tensor, query_side: bool, norm=None) -> torch.Tensor:
@@ -396,8 +400,10 @@ class FilterIndex:
             ks = max(int(round(4 * C * sample / nd)), 12)
         ks = max(1, min(ks, sample))
         Bs = self.B[::stride][:sample]
-        ns = torch.empty(sample, dtype=torch.float32, device=A.device)
-        ns.copy_(self.norm[::stride][:sample])
+        ns = self._sample_norm.get((stride, sample))
+        if ns is None:  # the sample rows' norms: index data, gathered once
+            ns = self.norm[::stride][:sample].contiguous()
+            self._sample_norm[(stride, sample)] = ns
         dist_s = torch.empty((nq, sample), dtype=torch.float32, device=A.device)
         gemm_nt(A, Bs, dist_s, alpha=-2.0, bias=ns)
         tau = kth_smallest(dist_s, ks)
