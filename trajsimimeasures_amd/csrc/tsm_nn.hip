@@ -144,22 +144,25 @@ __device__ __forceinline__ void tile_load(const uint16_t *base, int ld, int row0
 // `rows` read row rows - 1 (their products land in columns nobody keeps).
 // Native vectors (not the uint4 struct), so the values stay in registers.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
-template <int ROWS, int NT>
+// KW: the K width of the tile (KW / 8 chunks of 16 bytes per row).
+template <int ROWS, int NT, int KW = kBK>
 __device__ __forceinline__ void tile_load_rows(const uint16_t *base, int ld, int row0, int rows, int k0,
-                                               u32x4 (&r)[ROWS * 8 / NT]) {
+                                               u32x4 (&r)[ROWS * (KW / 8) / NT]) {
+    constexpr int CH = KW / 8;
 #pragma unroll
-    for (int i = 0; i < ROWS * 8 / NT; ++i) {
+    for (int i = 0; i < ROWS * CH / NT; ++i) {
         const int q = (int)threadIdx.x + NT * i;
-        const int row = min(row0 + (q >> 3), rows - 1), k = k0 + (q & 7) * 8;
+        const int row = min(row0 + q / CH, rows - 1), k = k0 + (q % CH) * 8;
         r[i] = *reinterpret_cast<const u32x4 *>(base + (int64_t)row * ld + k);
     }
 }
-template <int ROWS, int NT, int ST = kLdsStride>
-__device__ __forceinline__ void tile_store_v(uint16_t *s, const u32x4 (&r)[ROWS * 8 / NT]) {
+template <int ROWS, int NT, int ST = kLdsStride, int KW = kBK>
+__device__ __forceinline__ void tile_store_v(uint16_t *s, const u32x4 (&r)[ROWS * (KW / 8) / NT]) {
+    constexpr int CH = KW / 8;
 #pragma unroll
-    for (int i = 0; i < ROWS * 8 / NT; ++i) {
+    for (int i = 0; i < ROWS * CH / NT; ++i) {
         const int q = (int)threadIdx.x + NT * i;
-        *reinterpret_cast<u32x4 *>(s + (q >> 3) * ST + (q & 7) * 8) = r[i];
+        *reinterpret_cast<u32x4 *>(s + (q / CH) * ST + (q % CH) * 8) = r[i];
     }
 }
 template <int ROWS, int NT> __device__ __forceinline__ void tile_store(uint16_t *s, const uint4 (&r)[ROWS * 8 / NT]) {
@@ -280,10 +283,10 @@ __global__ __launch_bounds__(2 * BM) void gemm_nt_lds_kernel(GemmArgs g, FilterA
 constexpr int kSearchRows = 128;
 // LDS row strides of the search kernel (BF16 elements): A rows 32 KC + 48
 // (216 dwords at K = 384, 152 at K = 256: 24 mod 32 both), database rows
-// kLdsStride.  Measured on the 1,000 x 1M
-// search: strides of 8 or 24 dwords mod 32 ran it 9 % faster than 4 or 16
-// (at 4, SQ_LDS_BANK_CONFLICT was 43 % of the LDS-active cycles).
-constexpr int kSearchAPad = 48, kSearchBStride = kLdsStride;
+// KW + 16 for a K step of KW (40 or 72 dwords: 8 mod 32).  Measured on the
+// 1,000 x 1M search: strides of 8 or 24 dwords mod 32 ran it 9 % faster than
+// 4 or 16 (at 4, SQ_LDS_BANK_CONFLICT was 43 % of the LDS-active cycles).
+constexpr int kSearchAPad = 48;
 
 // The one-tile-ahead register ring of the search kernel: six named tiles
 // picked at compile time (an array here ends up in scratch memory).  1024
@@ -296,6 +299,15 @@ constexpr int kSearchAPad = 48, kSearchBStride = kLdsStride;
 // 49 % of the cycles and the matrix cores 38 %: each step's barrier, not
 // LDS bandwidth, sets the pace.
 constexpr int kSearchThreads = 1024, kSearchColTiles = 2;
+// K width of a search step at K = 256 (64 or 128): a 128-wide step halves the
+// block barriers per database tile, its database buffers taking the LDS of
+// half the hit queue (163,556 of 163,840 bytes, 126 VGPRs).  Measured on the
+// 1,000 x 1M search (random embeddings, 731 hits per row, identical hit
+// sets): 719 / 715 us mean against 735 / 731 with 64-wide steps.
+#ifndef TSM_SEARCH_KW256
+#define TSM_SEARCH_KW256 128
+#endif
+constexpr int kSearchKW256 = TSM_SEARCH_KW256;
 template <int RV> struct SearchRingT {
     u32x4 r0[RV], r1[RV], r2[RV], r3[RV], r4[RV], r5[RV];
     template <int S> __device__ __forceinline__ u32x4 (&at())[RV] {
@@ -308,39 +320,46 @@ template <int RV> struct SearchRingT {
     }
 };
 
-// One K step s of tile t (NS steps of 64 per tile).  The fragments of step
-// s come from Bs[cur] (stored and made visible in the step before); then the
-// ring slot of the next step goes to Bs[cur ^ 1] (last read in the step
-// before, finished at its barrier) and is reloaded from tile tn (tn2 for the
-// next tile's step 0); then the products; then the step's one barrier.  No
-// wave waits on its own LDS store before reading.  A wave owns 32 rows x
-// 16 JT columns (2 x JT MFMA tiles).
-template <int s, int NS, int NT, int JT>
-__device__ __forceinline__ void search_step(const GemmArgs &g, SearchRingT<kBN * 8 / NT> &ring, const uint16_t *As,
-                                           int AS, uint16_t (*Bs)[kBN * kSearchBStride], int &cur, int tn, int tn2,
-                                           int wm, int wn, int r16, int kb, f32x4 (&acc)[2][JT]) {
-    bf16x8 af[2][2], bfr[2][JT];
-#pragma unroll
-    for (int kk = 0; kk < 2; ++kk) {
-#pragma unroll
-        for (int i = 0; i < 2; ++i)
-            af[kk][i] = __builtin_bit_cast(
-                bf16x8, *reinterpret_cast<const uint4 *>(&As[(wm + 16 * i + r16) * AS + 64 * s + 32 * kk + kb]));
-#pragma unroll
-        for (int j = 0; j < JT; ++j)
-            bfr[kk][j] = __builtin_bit_cast(
-                bf16x8, *reinterpret_cast<const uint4 *>(&Bs[cur][(wn + 16 * j + r16) * kSearchBStride + 32 * kk + kb]));
-    }
+// One K step s of tile t (NS steps of KW per tile, KW = 64 or 128).  The
+// fragments of step s come from Bs[cur] (stored and made visible in the step
+// before), 64 of K at a time; after the first 64's fragment reads, the ring
+// slot of the next step goes to Bs[cur ^ 1] (last read in the step before,
+// finished at its barrier) and is reloaded from tile tn (tn2 for the next
+// tile's step 0); then the products; then the step's one barrier.  No wave
+// waits on its own LDS store before reading.  A wave owns 32 rows x 16 JT
+// columns (2 x JT MFMA tiles); K chunks of 32 are multiplied in order.
+template <int s, int NS, int NT, int JT, int KW>
+__device__ __forceinline__ void search_step(const GemmArgs &g, SearchRingT<kBN * (KW / 8) / NT> &ring,
+                                           const uint16_t *As, int AS, uint16_t (*Bs)[kBN * (KW + 16)], int &cur,
+                                           int tn, int tn2, int wm, int wn, int r16, int kb, f32x4 (&acc)[2][JT]) {
+    constexpr int BST = KW + 16;  // database row stride in LDS: 40 or 72 dwords, 8 mod 32
     constexpr int sn = (s + 1) % NS;
-    tile_store_v<kBN, NT, kSearchBStride>(Bs[cur ^ 1], ring.template at<sn>());
-    tile_load_rows<kBN, NT>(g.B, g.ldb, (s + 1 < NS ? tn : tn2) * kBN, g.N, sn * kBK, ring.template at<sn>());
 #pragma unroll
-    for (int kk = 0; kk < 2; ++kk)
+    for (int h = 0; h < KW / 64; ++h) {
+        bf16x8 af[2][2], bfr[2][JT];
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+        for (int kk = 0; kk < 2; ++kk) {
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                af[kk][i] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4 *>(
+                                                           &As[(wm + 16 * i + r16) * AS + KW * s + 64 * h + 32 * kk + kb]));
 #pragma unroll
             for (int j = 0; j < JT; ++j)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[kk][i], bfr[kk][j], acc[i][j], 0, 0, 0);
+                bfr[kk][j] = __builtin_bit_cast(
+                    bf16x8, *reinterpret_cast<const uint4 *>(&Bs[cur][(wn + 16 * j + r16) * BST + 64 * h + 32 * kk + kb]));
+        }
+        if (h == 0) {
+            tile_store_v<kBN, NT, BST, KW>(Bs[cur ^ 1], ring.template at<sn>());
+            tile_load_rows<kBN, NT, KW>(g.B, g.ldb, (s + 1 < NS ? tn : tn2) * kBN, g.N, sn * KW, ring.template at<sn>());
+        }
+#pragma unroll
+        for (int kk = 0; kk < 2; ++kk)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < JT; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[kk][i], bfr[kk][j], acc[i][j], 0, 0, 0);
+    }
     __syncthreads();
     cur ^= 1;
 }
@@ -370,15 +389,17 @@ __device__ __forceinline__ void search_flush(const SearchQueue<QN> &q, unsigned 
     }
 }
 
-template <int KC, int NT, int JT>
+template <int KC, int NT, int JT, int KW>
 __global__ __launch_bounds__(NT, NT / 256) void search_filter_kernel(GemmArgs g, FilterArgs f, int qblocks,
                                                                           int chunks, int tiles_per_chunk) {
-    static_assert(KC == 8 || KC == 12, "the register ring holds four or six K steps of 64");
-    constexpr int NS = KC / 2;  // K steps of 64 per tile
+    static_assert(KC == 8 || KC == 12, "K = 256 or 384");
+    static_assert(KW == 64 || (KW == 128 && KC == 8), "K steps of 64, or of 128 at K = 256");
+    constexpr int NS = 32 * KC / KW;  // K steps per tile: the register ring holds two to six
     constexpr int AS = 32 * KC + kSearchAPad;  // A row stride in LDS (BF16), padded
-    constexpr int QN = KC == 8 ? 2048 : 512;  // queue slots within the 160 KB of LDS
+    // queue slots within the 160 KB of LDS (A 76 KB at K = 256, B 40 or 72 KB)
+    constexpr int QN = KC == 12 ? 512 : KW == 64 ? 2048 : 1000;
     __shared__ __attribute__((aligned(16))) uint16_t As[kSearchRows * AS];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][kBN * kSearchBStride];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][kBN * (KW + 16)];
     __shared__ SearchQueue<QN> hq;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r16 = lane & 15, kb = (lane >> 4) * 8;
@@ -408,19 +429,21 @@ __global__ __launch_bounds__(NT, NT / 256) void search_filter_kernel(GemmArgs g,
             const float tv = f.tau[row < g.M ? row : g.M - 1];
             taur[i][e] = row < g.M ? tv : -__builtin_huge_valf();
         }
-    SearchRingT<kBN * 8 / NT> ring;  // step s of the next tile
-    tile_load_rows<kBN, NT>(g.B, g.ldb, t0 * kBN, g.N, 0 * kBK, ring.template at<0>());
-    tile_load_rows<kBN, NT>(g.B, g.ldb, t0 * kBN, g.N, 1 * kBK, ring.template at<1>());
-    tile_load_rows<kBN, NT>(g.B, g.ldb, t0 * kBN, g.N, 2 * kBK, ring.template at<2>());
-    tile_load_rows<kBN, NT>(g.B, g.ldb, t0 * kBN, g.N, 3 * kBK, ring.template at<3>());
+    SearchRingT<kBN * (KW / 8) / NT> ring;  // step s of the next tile
+    tile_load_rows<kBN, NT, KW>(g.B, g.ldb, t0 * kBN, g.N, 0 * KW, ring.template at<0>());
+    tile_load_rows<kBN, NT, KW>(g.B, g.ldb, t0 * kBN, g.N, 1 * KW, ring.template at<1>());
+    if constexpr (NS > 2) {
+        tile_load_rows<kBN, NT, KW>(g.B, g.ldb, t0 * kBN, g.N, 2 * KW, ring.template at<2>());
+        tile_load_rows<kBN, NT, KW>(g.B, g.ldb, t0 * kBN, g.N, 3 * KW, ring.template at<3>());
+    }
     if constexpr (NS > 4) {
-        tile_load_rows<kBN, NT>(g.B, g.ldb, t0 * kBN, g.N, 4 * kBK, ring.template at<4>());
-        tile_load_rows<kBN, NT>(g.B, g.ldb, t0 * kBN, g.N, 5 * kBK, ring.template at<5>());
+        tile_load_rows<kBN, NT, KW>(g.B, g.ldb, t0 * kBN, g.N, 4 * KW, ring.template at<4>());
+        tile_load_rows<kBN, NT, KW>(g.B, g.ldb, t0 * kBN, g.N, 5 * KW, ring.template at<5>());
     }
     // step 0 of tile t0 into Bs[0], its ring slot reloaded with tile t0 + 1
     int cur = 0;
-    tile_store_v<kBN, NT, kSearchBStride>(Bs[0], ring.template at<0>());
-    tile_load_rows<kBN, NT>(g.B, g.ldb, (t0 + 1 < t1 ? t0 + 1 : t0) * kBN, g.N, 0, ring.template at<0>());
+    tile_store_v<kBN, NT, KW + 16, KW>(Bs[0], ring.template at<0>());
+    tile_load_rows<kBN, NT, KW>(g.B, g.ldb, (t0 + 1 < t1 ? t0 + 1 : t0) * kBN, g.N, 0, ring.template at<0>());
     __syncthreads();
     for (int t = t0; t < t1; ++t) {
         f32x4 acc[2][JT];
@@ -439,16 +462,18 @@ __global__ __launch_bounds__(NT, NT / 256) void search_filter_kernel(GemmArgs g,
         }
         // the last tile reloads itself (unused)
         const int tn = t + 1 < t1 ? t + 1 : t, tn2 = t + 2 < t1 ? t + 2 : t1 - 1;
-        search_step<0, NS, NT, JT>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
+        search_step<0, NS, NT, JT, KW>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
         // the queue count as every wave sees it: no wave adds to the queue
         // between step 0's barrier and the last step's
         const unsigned qn = hq.n;
-        search_step<1, NS, NT, JT>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
-        search_step<2, NS, NT, JT>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
-        search_step<3, NS, NT, JT>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
+        search_step<1, NS, NT, JT, KW>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
+        if constexpr (NS > 2) {
+            search_step<2, NS, NT, JT, KW>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
+            search_step<3, NS, NT, JT, KW>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
+        }
         if constexpr (NS > 4) {
-            search_step<4, NS, NT, JT>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
-            search_step<5, NS, NT, JT>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
+            search_step<4, NS, NT, JT, KW>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
+            search_step<5, NS, NT, JT, KW>(g, ring, As, AS, Bs, cur, tn, tn2, wm, wn, r16, kb, acc);
         }
         if (qn >= (unsigned)QN / 2) {  // block-uniform
             search_flush(hq, qn, f, NT);
@@ -1520,11 +1545,11 @@ int tsm_nn_gemm_nt_filter(int M, int N, int K, const uint16_t *A, int lda, const
         if (chunks < 8) chunks = 8;
         const int tpc = (ntiles + chunks - 1) / chunks;
         if (K == 256)
-            hipLaunchKernelGGL((search_filter_kernel<8, kSearchThreads, kSearchColTiles>), dim3(qblocks * chunks),
+            hipLaunchKernelGGL((search_filter_kernel<8, kSearchThreads, kSearchColTiles, kSearchKW256>), dim3(qblocks * chunks),
                                dim3(kSearchThreads), 0, reinterpret_cast<hipStream_t>(stream), g, fa, qblocks, chunks,
                                tpc);
         else
-            hipLaunchKernelGGL((search_filter_kernel<12, kSearchThreads, kSearchColTiles>), dim3(qblocks * chunks),
+            hipLaunchKernelGGL((search_filter_kernel<12, kSearchThreads, kSearchColTiles, 64>), dim3(qblocks * chunks),
                                dim3(kSearchThreads), 0, reinterpret_cast<hipStream_t>(stream), g, fa, qblocks, chunks,
                                tpc);
     } else if (M > kBM) {  // 256 query rows per block: the database is streamed half as often
