@@ -1425,10 +1425,24 @@ __global__ __launch_bounds__(256) void topk_small_kernel(const T *V, int64_t row
 // byte, so its histogram atomics serialise within each wave.)
 constexpr int kKthSmallK = 32;
 
+// Minimum over the 64 lanes (all active), uniform: DPP moves instead of LDS
+// permutes.  quad_perm xor 1 and xor 2, then the half-row and row mirrors,
+// leave every lane of a 16-lane row holding the row's minimum; row_bcast:15
+// folds rows 0 into 1 and 2 into 3, row_bcast:31 folds row 1 into rows 2
+// and 3, so lane 63 holds the wave's minimum.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ __forceinline__ uint32_t dpp_min_u32(uint32_t v) {
+    const uint32_t o = (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, ROW_MASK, 0xf, false);
+    return min(v, o);
+}
 __device__ __forceinline__ uint32_t wave_min_u32(uint32_t v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, kWave));
-    return v;
+    v = dpp_min_u32<0xb1>(v);        // quad_perm [1, 0, 3, 2]
+    v = dpp_min_u32<0x4e>(v);        // quad_perm [2, 3, 0, 1]
+    v = dpp_min_u32<0x141>(v);       // row_half_mirror
+    v = dpp_min_u32<0x140>(v);       // row_mirror
+    v = dpp_min_u32<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
+    v = dpp_min_u32<0x143, 0xc>(v);  // row_bcast:31 into rows 2 and 3
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
 __global__ __launch_bounds__(kKthThreads) void kth_small_k_kernel(const float *V, int64_t cols, int64_t ld, int k,
